@@ -1,0 +1,230 @@
+/* phg.h — C ABI of the MI355X-native BNEEPT hot path of Photon-v2.
+ *
+ * This is the thin host->device boundary that the reference's renderer seam would call.
+ * The reference has no device layer; each entry point below names the reference
+ * interface whose work it takes over (paths relative to the Photon-v2 source tree,
+ * Engine/Engine/Source/...).
+ *
+ * Conventions
+ *   - Plain C types only; no exceptions cross the ABI; every call returns phg_status.
+ *   - A context owns all device memory it allocates; host arrays passed in are borrowed
+ *     for the duration of the call only.
+ *   - One host thread drives a context (mirrors the blocking phRender model,
+ *     Engine/CEngine/Include/ph_c_core.h:7-21).  phg_get_stats may be called from another
+ *     thread while no other call on the same context runs.
+ *   - Raster (0,0) is the lower-left image corner; films are stored row by row from the
+ *     bottom row up (HdrRgbFilm sensor order, Core/Filmic/HdrRgbFilm.cpp:103-121).
+ */
+#ifndef PHG_H
+#define PHG_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum phg_status {
+    PHG_OK = 0,
+    PHG_ERR_INVALID_ARG = 1,
+    PHG_ERR_HIP = 2,          /* a HIP runtime call failed; see phg_last_error */
+    PHG_ERR_OUT_OF_MEMORY = 3,
+    PHG_ERR_NO_SCENE = 4,     /* render/intersect before phg_upload_scene */
+    PHG_ERR_NO_DEVICE = 5,    /* no usable gfx950 device */
+    PHG_ERR_UNSUPPORTED = 6   /* e.g. BVH too deep for the traversal stack */
+} phg_status;
+
+/* Surface optics (Core/SurfaceBehavior/SurfaceOptics): one record per material. */
+enum {
+    PHG_MAT_ABSORBER = 0,   /* IdealAbsorber: area-light surfaces (Actor/Light/AAreaLight.cpp:30-34) */
+    PHG_MAT_LAMBERT = 1,    /* LambertianReflector via MatteOpaque */
+    PHG_MAT_CONDUCTOR = 2,  /* OpaqueMicrofacet + Schlick conductor Fresnel via AbradedOpaque */
+    PHG_MAT_DIELECTRIC = 3  /* TranslucentMicrofacet + exact dielectric Fresnel via AbradedTranslucent */
+};
+
+typedef struct phg_material {
+    uint32_t type;
+    float rgb[3];        /* albedo (Lambert) or F0 (conductor) */
+    float roughness;     /* GGX alpha = max(roughness^2, 0.001) */
+    float ior_outer;     /* dielectric only */
+    float ior_inner;
+    float pad;
+} phg_material;          /* 32 bytes */
+
+/* Area-light emitter group (DiffuseSurfaceEmitter / GroupedDiffuseSurfaceEmitter):
+ * constant front-face radiance over triangles emitter_tris[tri_begin, tri_begin+tri_count). */
+typedef struct phg_emitter {
+    float radiance[3];
+    uint32_t tri_begin;
+    uint32_t tri_count;
+    uint32_t pad[3];
+} phg_emitter;           /* 32 bytes */
+
+/* Pinhole camera (Core/Receiver/PinholeCamera.cpp:26-69), horizontal FOV
+ * (EngineEnv/Observer/SingleLensObserver.cpp:70-81).  right/up/forward orthonormal. */
+typedef struct phg_camera {
+    float pos[3];
+    float right[3];
+    float up[3];
+    float forward[3];
+    float tan_half_fov;
+    float pad[3];
+} phg_camera;            /* 64 bytes */
+
+/* Flattened, world-space scene (what a renderer seam derives from the cooked VisualWorld,
+ * World/VisualWorld.cpp:150-198).  Triangles carry face normals only (PTriangle,
+ * Core/Intersection/PTriangle.cpp:19-35). */
+typedef struct phg_scene_desc {
+    uint32_t num_tris;
+    uint32_t num_materials;
+    uint32_t num_emitters;
+    uint32_t num_emitter_tris;
+    const float* verts;            /* 9 floats per triangle: A, B, C */
+    const uint32_t* tri_material;  /* per triangle */
+    const int32_t* tri_emitter;    /* per triangle; -1 if not emissive */
+    const phg_material* materials;
+    const phg_emitter* emitters;
+    const uint32_t* emitter_tris;  /* triangle ids of all emitter groups, concatenated */
+    phg_camera camera;
+} phg_scene_desc;
+
+typedef struct phg_device_opts {
+    int32_t device;        /* HIP device ordinal */
+    uint32_t flags;        /* reserved, 0 */
+} phg_device_opts;
+
+typedef struct phg_bvh_opts {
+    uint32_t max_leaf_tris;   /* 1..8, default 4 */
+    uint32_t sah_bins;        /* 8..64, default 16 */
+    uint32_t nthreads;        /* host build threads, 0 = all cores */
+    uint32_t pad;
+} phg_bvh_opts;
+
+/* One ray: origin, minT, direction, maxT (TLineSegment, Math/Geometry/TLineSegment.ipp). */
+typedef struct phg_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} phg_ray;                /* 32 bytes */
+
+/* Closest-hit record.  prim = input triangle id (-1 on a miss); b, c are the barycentric
+ * weights of vertices B and C as computed by the watertight test. */
+typedef struct phg_hit {
+    float t;
+    int32_t prim;
+    float b;
+    float c;
+} phg_hit;                /* 16 bytes */
+
+enum { PHG_PTR_HOST = 0, PHG_PTR_DEVICE = 1 };
+enum { PHG_QUERY_CLOSEST = 0, PHG_QUERY_ANY = 1 };
+
+typedef struct phg_render_opts {
+    uint32_t width, height;
+    uint32_t spp;              /* samples per pixel rendered by this call */
+    uint32_t spp_begin;        /* index of the first sample (stream key), for progressive passes */
+    uint32_t max_bounces;      /* bounce-loop cap; 10000 reproduces the reference's MAX_RAY_BOUNCES */
+    uint32_t spp_per_pass;     /* samples per pixel per wavefront pass; 0 = automatic */
+    uint64_t seed;
+    uint32_t flags;            /* PHG_RENDER_* */
+    uint32_t pad;
+} phg_render_opts;
+
+enum { PHG_RENDER_COUNT_TRAVERSAL = 1 };  /* count node visits / triangle tests (slower) */
+
+typedef struct phg_stats {
+    uint64_t paths;             /* completed camera paths (samples) */
+    uint64_t rays_primary;
+    uint64_t rays_extension;
+    uint64_t rays_shadow;
+    uint64_t node_visits;       /* only with PHG_RENDER_COUNT_TRAVERSAL / counted intersect */
+    uint64_t tri_tests;
+    double render_ms;           /* wall time of the last phg_render call */
+    double trace_ms;            /* GPU time in closest-hit traversal kernels (last call) */
+    double shadow_ms;           /* GPU time in any-hit traversal kernels (last call) */
+    double shade_ms;            /* GPU time in generate/shade/film kernels (last call) */
+    uint32_t bvh_nodes;
+    uint32_t bvh_depth;
+    uint32_t bvh_stack_need;
+    uint32_t pad;
+    double bvh_build_ms;
+} phg_stats;
+
+/* Context lifetime.  Replaces nothing in the reference: the CPU engine needs no device. */
+phg_status phg_create(void** out_ctx, const phg_device_opts* opts);
+phg_status phg_destroy(void* ctx);
+const char* phg_last_error(void* ctx);
+
+/* Scene upload + acceleration structure build.
+ * Replaces VisualWorld's top-level accelerator build, TBinaryBvhIntersector::update ->
+ * TBvhBuilder<2>::buildInformativeBvh -> TLinearDepthFirstBinaryBvh::build
+ * (Core/Intersection/BVH/TBinaryBvhIntersector.ipp:66-97, Math/Algorithm/BVH/TBvhBuilder.ipp:34-86),
+ * and ESPowerFavoring::update (Core/Emitter/Sampler/ESPowerFavoring.cpp:21-49). */
+phg_status phg_upload_scene(void* ctx, const phg_scene_desc* scene, const phg_bvh_opts* bvh);
+
+/* Batched closest-hit / any-hit ray queries.
+ * Replaces Scene::isIntersecting / Scene::isOccluding (World/Scene.cpp:36-68) over
+ * TBinaryBvhIntersector::isIntersecting (TBinaryBvhIntersector.ipp:25-51) and
+ * PTriangle::isIntersecting -> TWatertightTriangle::isIntersecting
+ * (PTriangle.cpp:37-49, Math/Geometry/TWatertightTriangle.ipp:16-149).
+ * For PHG_QUERY_ANY, hits[i].prim is 1 if occluded and -1 otherwise.
+ * ptr_kind selects whether rays/hits are host or device pointers. */
+phg_status phg_intersect(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits,
+                         int query, int ptr_kind);
+
+/* Same as phg_intersect(PHG_QUERY_CLOSEST) but also returns per-ray node visits and
+ * triangle tests (for the algorithmic-byte roofline). */
+phg_status phg_intersect_counted(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits,
+                                 uint32_t* node_visits, uint32_t* tri_tests, int ptr_kind);
+
+/* Film.  Replaces HdrRgbFilm's FP64 RadianceSensor storage (Core/Filmic/HdrRgbFilm.cpp:62-122,
+ * Core/Receiver/RadianceSensor.h:10-21) with a box filter: 8 doubles per pixel =
+ * sum r, g, b, sample weight, sum r^2, g^2, b^2, 0 (the squares feed per-pixel CIs). */
+phg_status phg_film_clear(void* ctx, uint32_t width, uint32_t height);
+phg_status phg_film_read(void* ctx, double* out, uint64_t out_len);
+
+/* Render spp samples per pixel with BNEEPT into the film.
+ * Replaces EqualSamplingRenderer::doRender (Core/Renderer/Sampling/EqualSamplingRenderer.cpp:119-203)
+ * -> ReceiverSamplingWork::doWork (ReceiverSamplingWork.cpp:61-138)
+ * -> BNEEPTEstimator::estimate (Core/Estimator/BNEEPTEstimator.cpp:38-262)
+ * -> HdrRgbFilm::addSample (HdrRgbFilm.cpp:62-122). */
+phg_status phg_render(void* ctx, const phg_render_opts* opts);
+
+/* Statistics of the last render/intersect (Renderer::asyncQueryRenderStats,
+ * Core/Renderer/Sampling/EqualSamplingRenderer.cpp:258-280, reports paths/second). */
+phg_status phg_get_stats(void* ctx, phg_stats* out);
+
+/* ---- Procedural scene generation (host).  Not a reference interface: produces the
+ * benchmark workloads as flattened scenes.  The returned arrays are owned by the library
+ * until phg_scene_free. */
+enum {
+    PHG_GEN_CORNELL = 1,        /* unit-cube Cornell box, 34 + 2 light triangles */
+    PHG_GEN_PROCEDURAL = 2,     /* displaced icospheres + ground + area quads */
+    PHG_GEN_OCCLUDED_ROOM = 3   /* room lit through a slit by a hidden quad light */
+};
+
+typedef struct phg_gen_opts {
+    uint32_t kind;
+    uint32_t seed;
+    uint32_t target_tris;      /* PROCEDURAL/ROOM: total triangle count target */
+    uint32_t num_instances;    /* PROCEDURAL: icosphere instances */
+    uint32_t num_lights;       /* PROCEDURAL: area-light quads */
+    uint32_t subdiv;           /* PROCEDURAL: icosphere subdivision level */
+    uint32_t width, height;    /* for the camera aspect */
+    float fov_deg;             /* horizontal */
+    uint32_t pad;
+} phg_gen_opts;
+
+typedef struct phg_scene_host {
+    phg_scene_desc desc;       /* pointers into library-owned arrays */
+} phg_scene_host;
+
+phg_status phg_scene_generate(const phg_gen_opts* opts, phg_scene_host** out);
+void phg_scene_free(phg_scene_host* scene);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHG_H */

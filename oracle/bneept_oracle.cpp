@@ -1,0 +1,1283 @@
+// CPU ORACLE FOR THE PHOTON-V2 BNEEPT HOT PATH — TEST INFRASTRUCTURE ONLY.
+//
+// This file restates, in plain scalar C++, the reference renderer's BNEEPT path so that
+// the GPU implementation can be checked against it.  Only tests/, the smoke check in
+// __graft_entry__.py and bench.py's CPU-baseline leg may load the library built from it;
+// the product (photon-v2_amd/) never links or calls it.
+//
+// Build: oracle/Makefile, with -ffp-contract=off so that every float operation is
+// rounded exactly where the C++ source says (the reference's default build has no
+// fast-math and no FMA contraction: CMakeLists.txt:141-143).
+//
+// Each function cites the reference file:line it follows (paths relative to the
+// Photon-v2 source tree).  Parity status of each part is listed in oracle/README.md.
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+#include "bneept_oracle.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------------------------------
+// Vector math in the reference's operation order (Math/TVector3.ipp, Math/math.ipp).
+// ---------------------------------------------------------------------------------------
+struct V3 { float x, y, z; };
+
+static inline V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+static inline V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+// math.ipp:79-87 length_squared / TVector3 dot: x*x' + y*y' + z*z' left to right
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float absdot(V3 a, V3 b) { return std::fabs(dot(a, b)); }
+// TVector3.ipp:75-80
+static inline V3 cross(V3 a, V3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+// math.ipp normalize(TSpan): rcpLen = 1/length, then multiply
+static inline V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
+static inline bool is_zero(V3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+static inline bool is_finite(V3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+// TVectorNBase.ipp:66-71
+static inline V3 safe_normalize(V3 a, V3 fallback) {
+    const V3 n = normalize(a);
+    return (!is_zero(n) && is_finite(n)) ? n : fallback;
+}
+// TVector3.ipp:106-110: this - normal * (2 * normal.dot(this))
+static inline V3 reflect(V3 v, V3 n) { return v - n * (2.0f * dot(n, v)); }
+// TVector3.ipp:19-25 weightedSum: vA*wA + vB*wB + vC*wC
+static inline V3 wsum(V3 a, float wa, V3 b, float wb, V3 c, float wc) { return a * wa + b * wb + c * wc; }
+
+struct RGB { float r, g, b; };
+static inline RGB rgb(float v) { return RGB{v, v, v}; }
+static inline bool rgb_zero(RGB c) { return c.r == 0.0f && c.g == 0.0f && c.b == 0.0f; }
+static inline bool rgb_finite(RGB c) { return std::isfinite(c.r) && std::isfinite(c.g) && std::isfinite(c.b); }
+static inline RGB mulc(RGB a, RGB b) { return RGB{a.r * b.r, a.g * b.g, a.b * b.b}; }
+static inline RGB muls(RGB a, float s) { return RGB{a.r * s, a.g * s, a.b * s}; }
+static inline RGB divs(RGB a, float s) { return RGB{a.r / s, a.g / s, a.b / s}; }
+// math.h:98-111 safe_clamp: non-finite -> lower bound
+static inline float safe_clamp(float v, float lo, float hi) { return std::isfinite(v) ? std::clamp(v, lo, hi) : lo; }
+// BNEEPTEstimator.cpp:264-268 rationalClamp
+static inline RGB rational_clamp(RGB c) {
+    return RGB{safe_clamp(c.r, 0.0f, 1e9f), safe_clamp(c.g, 0.0f, 1e9f), safe_clamp(c.b, 0.0f, 1e9f)};
+}
+// Linear-sRGB relative luminance: the Y row of the linear-sRGB -> CIE XYZ matrix
+// (Math/Color/color_spaces.ipp:129, used through TSpectrumBase.ipp:124).
+static inline float luminance(RGB c) {
+    return c.r * 0.21263900587151027f + c.g * 0.715168678767756f + c.b * 0.07219231536073371f;
+}
+
+static constexpr float kPi = 3.14159265358979323846f;
+static constexpr float kTwoPi = 6.28318530717958647692f;
+static constexpr float kRcpPi = 0.31830988618379067154f;
+static constexpr float kRcpSqrt2 = 0.70710678118654752440f;
+
+// ---------------------------------------------------------------------------------------
+// PCG32 (Math/Random/Pcg32.h:72-124) and the moremur mixer (Math/hash.ipp:104-115).
+// ---------------------------------------------------------------------------------------
+static inline uint64_t moremur(uint64_t v) {
+    v ^= v >> 27; v *= 0x3C79AC492BA7B653ULL;
+    v ^= v >> 33; v *= 0x1C69B3F74AC4AE35ULL;
+    v ^= v >> 27;
+    return v;
+}
+
+struct Pcg32 {
+    uint64_t state = 0x853C49E6748FEA9BULL;
+    uint64_t inc = 0xDA3E39CB94B95BDBULL;
+    static constexpr uint64_t kMul = 0x5851F42D4C957F2DULL;
+    Pcg32() = default;
+    Pcg32(uint64_t seq, uint64_t init_state) {
+        state = 0u; inc = (seq << 1u) | 1u;
+        next(); state += init_state; next();
+    }
+    uint32_t next() {
+        const uint64_t old = state;
+        state = old * kMul + inc;
+        const uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = static_cast<uint32_t>(old >> 59u);
+        return (xs >> rot) | (xs << ((~rot + 1u) & 31));
+    }
+};
+
+// Jump-ahead exactly as Pcg32.h:93-112 (LCG power by squaring, Brown 1994).
+static inline void pcg_jump(Pcg32& r, uint64_t distance) {
+    uint64_t cur_mult = Pcg32::kMul, cur_plus = r.inc, acc_mult = 1u, acc_plus = 0u;
+    uint64_t delta = distance;
+    while (delta > 0) {
+        if (delta & 1) { acc_mult *= cur_mult; acc_plus = acc_plus * cur_mult + cur_plus; }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+        delta /= 2;
+    }
+    r.state = acc_mult * r.state + acc_plus;
+}
+
+// Sample stream of one camera path.  The stream is this project's own definition (the
+// reference draws its first 5 dimensions from the sample generator and the rest from a
+// thread-local PCG32, SampleFlow.h:157-169): one PCG32 per path, sequence = path key,
+// initial state = moremur(key) as in the single-argument Pcg32 constructor (Pcg32.h:72-74).
+// A 32-bit draw maps to the odd multiples of 2^-24, i.e. strictly inside (0, 1).
+struct Stream {
+    Pcg32 rng;
+    explicit Stream(uint64_t key) : rng(key, moremur(key)) {}
+    float u() { const uint32_t b = rng.next(); return static_cast<float>(((b >> 9) << 1) | 1u) * 0x1p-24f; }
+};
+
+static inline uint64_t path_key(uint64_t seed, uint64_t pixel, uint64_t sample, uint64_t npix) {
+    return (seed << 40) ^ (sample * npix + pixel);
+}
+
+// ---------------------------------------------------------------------------------------
+// Watertight ray/triangle test (Math/Geometry/TWatertightTriangle.ipp:16-149).
+// ---------------------------------------------------------------------------------------
+static bool wt_intersect(V3 va, V3 vb, V3 vc, V3 org, V3 dir, float minT, float maxT,
+                         float* out_t, V3* out_bary) {
+    V3 d = dir;
+    V3 a = va - org, b = vb - org, c = vc - org;
+    if (std::fabs(d.x) > std::fabs(d.y)) {
+        if (std::fabs(d.x) > std::fabs(d.z)) {
+            d = mk(d.y, d.z, d.x); a = mk(a.y, a.z, a.x); b = mk(b.y, b.z, b.x); c = mk(c.y, c.z, c.x);
+        }
+    } else {
+        if (std::fabs(d.y) > std::fabs(d.z)) {
+            d = mk(d.z, d.x, d.y); a = mk(a.z, a.x, a.y); b = mk(b.z, b.x, b.y); c = mk(c.z, c.x, c.y);
+        }
+    }
+    const float rz = 1.0f / d.z;
+    const float sx = -d.x * rz;
+    const float sy = -d.y * rz;
+    const float sz = rz;
+    a.x += sx * a.z; a.y += sy * a.z;
+    b.x += sx * b.z; b.y += sy * b.z;
+    c.x += sx * c.z; c.y += sy * c.z;
+    float ea = b.x * c.y - b.y * c.x;
+    float eb = c.x * a.y - c.y * a.x;
+    float ec = a.x * b.y - a.y * b.x;
+    if (ea == 0.0f || eb == 0.0f || ec == 0.0f) {
+        const double ea64 = (double)b.x * (double)c.y - (double)b.y * (double)c.x;
+        const double eb64 = (double)c.x * (double)a.y - (double)c.y * (double)a.x;
+        const double ec64 = (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+        ea = (float)ea64; eb = (float)eb64; ec = (float)ec64;
+    }
+    if ((ea < 0.0f || eb < 0.0f || ec < 0.0f) && (ea > 0.0f || eb > 0.0f || ec > 0.0f)) return false;
+    const float det = ea + eb + ec;
+    if (det == 0.0f || !std::isfinite(det)) return false;
+    a.z *= sz; b.z *= sz; c.z *= sz;
+    const float ts = ea * a.z + eb * b.z + ec * c.z;
+    if (det > 0.0f) {
+        if (ts < minT * det || ts > maxT * det) return false;
+    } else {
+        if (ts > minT * det || ts < maxT * det) return false;
+    }
+    const float rd = 1.0f / det;
+    *out_bary = mk(ea * rd, eb * rd, ec * rd);
+    *out_t = ts * rd;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// Robust slab test (Math/Geometry/TAABB3D.ipp:390-420, Ize 2013).
+// ---------------------------------------------------------------------------------------
+static inline bool slab_robust(const float* bmin, const float* bmax, V3 org, V3 dir, V3 rcp,
+                               float minT, float maxT) {
+    const float o[3] = {org.x, org.y, org.z}, d[3] = {dir.x, dir.y, dir.z}, r[3] = {rcp.x, rcp.y, rcp.z};
+    float tmin = minT, tmax = maxT;
+    for (int i = 0; i < 3; ++i) {
+        const float dist[2] = {bmin[i] - o[i], bmax[i] - o[i]};
+        const int isneg = d[i] < 0.0f;
+        const float mind = dist[isneg] * r[i];
+        const float maxd = dist[1 - isneg] * r[i];
+        tmin = mind > tmin ? mind : tmin;
+        tmax = maxd < tmax ? maxd : tmax;
+    }
+    constexpr float mult = FLT_EPSILON * 2 + 1;
+    return tmin <= tmax * mult;
+}
+
+// ---------------------------------------------------------------------------------------
+// Scene view (the flattened scene format shared with the C ABI of the product).
+// ---------------------------------------------------------------------------------------
+struct Scene {
+    const orc_scene* s;
+    std::vector<float> emit_cdf;     // TPwcDistribution1D CDF over emitters (size n+1)
+    std::vector<float> emit_pick;    // pdfDiscrete per emitter
+    std::vector<float> tri_area;
+    // reference binary BVH
+    struct Node { float bmin[3], bmax[3]; uint32_t child_or_item; uint32_t nitems; uint8_t axis; uint8_t leaf; };
+    std::vector<Node> nodes;
+    std::vector<uint32_t> items;
+
+    V3 vtx(uint32_t t, int k) const { const float* p = s->verts + 9 * (size_t)t + 3 * k; return mk(p[0], p[1], p[2]); }
+};
+
+// PTriangle face normal (PTriangle.cpp:28-31, TBasicTriangle.ipp:116-121, 268-275)
+static V3 face_normal(V3 a, V3 b, V3 c) {
+    const V3 eab = b - a, eac = c - a;
+    const V3 cr = cross(eab, eac);
+    const float cf = dot(cr, cr);
+    const bool degenerate = !(cf > 0.0f) || std::isinf(cf);
+    return !degenerate ? safe_normalize(cr, mk(0, 1, 0)) : mk(0, 1, 0);
+}
+static float tri_area(V3 a, V3 b, V3 c) { return length(cross(b - a, c - a)) * 0.5f; }
+
+// ---- Reference binary BVH: TBvhBuilder.ipp:112-293 / 350-382 and
+//      TLinearDepthFirstBinaryBvh.ipp:16-41,182-225.  maxNodeItems = 1, split axis = max
+//      extent of the centroid bounds, equal-item (median) split, which is what the builder
+//      falls back to in practice (its SAH bucket scan selects a zero cost past the end).
+static void bvh_build_rec(Scene& S, std::vector<uint32_t>& idx, size_t lo, size_t hi,
+                          const std::vector<float>& cen) {
+    Scene::Node node{};
+    for (int k = 0; k < 3; ++k) { node.bmin[k] = INFINITY; node.bmax[k] = -INFINITY; }
+    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = lo; i < hi; ++i) {
+        const uint32_t t = idx[i];
+        for (int v = 0; v < 3; ++v) {
+            const float* p = S.s->verts + 9 * (size_t)t + 3 * v;
+            for (int k = 0; k < 3; ++k) { node.bmin[k] = std::min(node.bmin[k], p[k]); node.bmax[k] = std::max(node.bmax[k], p[k]); }
+        }
+        for (int k = 0; k < 3; ++k) { cmin[k] = std::min(cmin[k], cen[3 * t + k]); cmax[k] = std::max(cmax[k], cen[3 * t + k]); }
+    }
+    const size_t me = S.nodes.size();
+    if (hi - lo <= 1) {
+        node.leaf = 1; node.child_or_item = (uint32_t)S.items.size(); node.nitems = (uint32_t)(hi - lo);
+        for (size_t i = lo; i < hi; ++i) S.items.push_back(idx[i]);
+        S.nodes.push_back(node);
+        return;
+    }
+    int axis = 0;
+    const float ext[3] = {cmax[0] - cmin[0], cmax[1] - cmin[1], cmax[2] - cmin[2]};
+    if (ext[1] > ext[axis]) axis = 1;
+    if (ext[2] > ext[axis]) axis = 2;
+    const size_t mid = lo + (hi - lo) / 2;
+    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
+                     [&](uint32_t a, uint32_t b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
+    node.leaf = 0; node.axis = (uint8_t)axis;
+    S.nodes.push_back(node);
+    bvh_build_rec(S, idx, lo, mid, cen);
+    S.nodes[me].child_or_item = (uint32_t)S.nodes.size();
+    bvh_build_rec(S, idx, mid, hi, cen);
+}
+
+static void bvh_build(Scene& S) {
+    const uint32_t n = S.s->num_tris;
+    S.nodes.clear(); S.items.clear();
+    if (n == 0) return;
+    std::vector<float> cen(3 * (size_t)n);
+    std::vector<uint32_t> idx(n);
+    for (uint32_t t = 0; t < n; ++t) {
+        idx[t] = t;
+        const V3 a = S.vtx(t, 0), b = S.vtx(t, 1), c = S.vtx(t, 2);
+        cen[3 * t + 0] = (a.x + b.x + c.x) / 3.0f;
+        cen[3 * t + 1] = (a.y + b.y + c.y) / 3.0f;
+        cen[3 * t + 2] = (a.z + b.z + c.z) / 3.0f;
+    }
+    S.nodes.reserve(2 * (size_t)n);
+    S.items.reserve(n);
+    bvh_build_rec(S, idx, 0, n, cen);
+}
+
+struct Hit { int32_t tri = -1; float t = 0; V3 bary{0, 0, 0}; };
+
+// TLinearDepthFirstBinaryBvh.ipp:43-139 nearestTraversal + the per-item tester of
+// TBinaryBvhIntersector.ipp:25-51: the segment's maxT shrinks to each accepted hit.
+static bool bvh_nearest(const Scene& S, V3 org, V3 dir, float minT, float maxT, Hit* out) {
+    if (S.nodes.empty()) return false;
+    uint32_t stack[64]; int sp = 0;
+    uint32_t cur = 0;
+    bool has = false;
+    float segMax = maxT;
+    const int isneg[3] = {dir.x < 0.0f, dir.y < 0.0f, dir.z < 0.0f};
+    const V3 rcp = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    while (true) {
+        const Scene::Node& nd = S.nodes[cur];
+        if (slab_robust(nd.bmin, nd.bmax, org, dir, rcp, minT, segMax)) {
+            if (nd.leaf) {
+                for (uint32_t i = 0; i < nd.nitems; ++i) {
+                    const uint32_t t = S.items[nd.child_or_item + i];
+                    float ht; V3 hb;
+                    if (wt_intersect(S.vtx(t, 0), S.vtx(t, 1), S.vtx(t, 2), org, dir, minT, segMax, &ht, &hb)) {
+                        segMax = ht; has = true; out->tri = (int32_t)t; out->t = ht; out->bary = hb;
+                    }
+                }
+                if (sp == 0) break;
+                cur = stack[--sp];
+            } else {
+                if (isneg[nd.axis]) { stack[sp++] = cur + 1; cur = nd.child_or_item; }
+                else { stack[sp++] = nd.child_or_item; cur = cur + 1; }
+            }
+        } else {
+            if (sp == 0) break;
+            cur = stack[--sp];
+        }
+    }
+    return has;
+}
+
+static inline bool scene_intersect(const Scene& S, V3 org, V3 dir, float minT, float maxT, Hit* out) {
+    return bvh_nearest(S, org, dir, minT, maxT, out);
+}
+
+// ---------------------------------------------------------------------------------------
+// Hit records and bases.
+// ---------------------------------------------------------------------------------------
+struct SurfaceHit {
+    int32_t tri;
+    V3 P, Ng, Ns;
+    V3 bx, by, bz;   // shading basis, y = Ns
+    V3 incident_dir; // direction of the ray that found this hit
+};
+
+// TOrthonormalBasis3.ipp:16-39 makeFromUnitY
+static void basis_from_unit_y(V3 y, V3* bx, V3* bz) {
+    V3 x;
+    if (std::fabs(y.y) < kRcpSqrt2) {
+        x = mk(-y.z, 0.0f, y.x);
+        x = x * (1.0f / std::sqrt(x.x * x.x + x.z * x.z));
+    } else {
+        x = mk(y.y, -y.x, 0.0f);
+        x = x * (1.0f / std::sqrt(x.x * x.x + x.y * x.y));
+    }
+    *bx = x;
+    *bz = cross(x, y);
+}
+
+// PTriangle::calcHitDetail (PTriangle.cpp:66-135) with HitInfo::computeBases
+// (HitInfo.h:140-170).  Vertex normals all equal the face normal, so dNdU = 0 and the
+// shading basis always comes from makeFromUnitY(Ns).
+static SurfaceHit make_hit(const Scene& S, int32_t tri, V3 bary, V3 incident_dir) {
+    SurfaceHit h;
+    h.tri = tri;
+    const V3 a = S.vtx(tri, 0), b = S.vtx(tri, 1), c = S.vtx(tri, 2);
+    h.P = wsum(a, bary.x, b, bary.y, c, bary.z);
+    h.Ng = face_normal(a, b, c);
+    h.Ns = normalize(wsum(h.Ng, bary.x, h.Ng, bary.y, h.Ng, bary.z));
+    h.by = h.Ns;
+    basis_from_unit_y(h.Ns, &h.bx, &h.bz);
+    h.incident_dir = incident_dir;
+    return h;
+}
+static inline V3 to_local(const SurfaceHit& h, V3 w) { return mk(dot(h.bx, w), dot(h.by, w), dot(h.bz, w)); }
+static inline V3 to_world(const SurfaceHit& h, V3 l) { return h.bx * l.x + h.by * l.y + h.bz * l.z; }
+
+// SidednessAgreement.h:104-283, Strict policy
+static inline bool side_agreed(const SurfaceHit& X, V3 v) { return dot(X.Ng, v) * dot(X.Ns, v) > 0.0f; }
+static inline bool same_hemi(const SurfaceHit& X, V3 a, V3 b) {
+    return side_agreed(X, a) && side_agreed(X, b) && dot(a, X.Ng) * dot(b, X.Ng) > 0.0f;
+}
+static inline bool opposite_hemi(const SurfaceHit& X, V3 a, V3 b) {
+    return side_agreed(X, a) && side_agreed(X, b) && dot(a, X.Ng) * dot(b, X.Ng) < 0.0f;
+}
+
+// ---------------------------------------------------------------------------------------
+// Materials: Lambertian, GGX conductor (Schlick), GGX dielectric (exact Fresnel).
+// ---------------------------------------------------------------------------------------
+static inline float mat_alpha(const orc_material& m) {
+    // RoughnessToAlphaMapping.h:55-58 (squared) + IsoTrowbridgeReitzConstant.h:23-30 (>= 0.001)
+    const float a = m.roughness * m.roughness;
+    return a > 0.001f ? a : 0.001f;
+}
+
+// IsoTrowbridgeReitz.cpp:20-33
+static inline float ggx_lambda(V3 N, V3 w, float alpha) {
+    const float a2 = alpha * alpha;
+    const float nod = dot(N, w);
+    const float nod2 = std::min(nod * nod, 1.0f);
+    return 0.5f * (-1.0f + std::sqrt(1.0f + a2 * (1.0f / nod2 - 1.0f)));
+}
+// IsoTrowbridgeReitz.cpp:35-55
+static inline float ggx_D(V3 N, V3 H, float alpha) {
+    const float noh = dot(N, H);
+    const float a2 = alpha * alpha;
+    const float noh2 = noh * noh;
+    const float inner = noh2 * (a2 - 1.0f) + 1.0f;
+    const float den = kPi * inner * inner;
+    return a2 / den;
+}
+// ShapeInvariantMicrofacet.cpp:20-52 (height-correlated) + Microfacet.h:131-145
+static inline float ggx_G(V3 N, V3 H, V3 L, V3 V, float alpha) {
+    const float nol = dot(N, L), nov = dot(N, V), hol = dot(H, L), hov = dot(H, V);
+    if (!(hol * nol > 0.0f && hov * nov > 0.0f)) return 0.0f;
+    const float ll = ggx_lambda(N, L, alpha), lv = ggx_lambda(N, V, alpha);
+    return 1.0f / (1.0f + ll + lv);
+}
+// ShapeInvariantMicrofacet.h:124-138 visibleDistribution
+static inline float ggx_vndf_pdf(V3 N, V3 H, V3 V, float alpha) {
+    const float lv = ggx_lambda(N, V, alpha);
+    const float g1v = 1.0f / (1.0f + lv);
+    const float ndf = ggx_D(N, H, alpha);
+    return g1v * ndf * std::fabs(dot(H, V) / dot(N, V));
+}
+// TrowbridgeReitz.cpp:19-54 + 67-82 (Dupuy & Benyoub 2023 spherical-cap VNDF sampling)
+static V3 ggx_sample_vh(const SurfaceHit& X, V3 N, V3 V, float alpha, float u0, float u1) {
+    const V3 upperV = dot(V, N) >= 0.0f ? V : neg(V);
+    const V3 lv = to_local(X, upperV);
+    const V3 sv = normalize(mk(lv.x * alpha, lv.y, lv.z * alpha));
+    const float phi = kTwoPi * u0;
+    const float y = std::fma(1.0f - u1, 1.0f + sv.y, -sv.y);
+    const float st = std::sqrt(std::clamp(1.0f - y * y, 0.0f, 1.0f));
+    const float z = st * std::cos(phi);
+    const float x = st * std::sin(phi);
+    const V3 h = mk(x, y, z) + sv;
+    const V3 lh = mk(h.x * alpha, h.y, h.z * alpha);
+    return safe_normalize(to_world(X, lh), N);
+}
+
+// SchlickApproxConductorFresnel.cpp:35-43 (std::pow(float, int) computes in double)
+static inline RGB fresnel_schlick(const orc_material& m, float cosI) {
+    const float c = std::fabs(cosI);
+    const float p = static_cast<float>(std::pow(1.0f - c, 5));
+    return RGB{(1.0f - m.rgb[0]) * p + m.rgb[0], (1.0f - m.rgb[1]) * p + m.rgb[1], (1.0f - m.rgb[2]) * p + m.rgb[2]};
+}
+// ExactDielectricFresnel.cpp:12-41
+static inline float fresnel_dielectric(float etaOuter, float etaInner, float cosThetaI) {
+    float etaI = etaOuter, etaT = etaInner, cosI = cosThetaI;
+    if (cosThetaI < 0.0f) { std::swap(etaI, etaT); cosI = std::fabs(cosI); }
+    const float ratio = etaI / etaT;
+    const float sinT2 = (1.0f - cosI * cosI) * ratio * ratio;
+    if (sinT2 >= 1.0f) return 1.0f;
+    const float cosT = std::sqrt(1.0f - sinT2);
+    const float rpar = (etaT * cosI - etaI * cosT) / (etaT * cosI + etaI * cosT);
+    const float rper = (etaI * cosI - etaT * cosT) / (etaI * cosI + etaT * cosT);
+    return 0.5f * (rpar * rpar + rper * rper);
+}
+// DielectricFresnel.cpp:18-42
+static bool refract_dir(float etaOuter, float etaInner, V3 I, V3 N, V3* out) {
+    const float ion = dot(I, N);
+    const float sgn = ion < 0.0f ? -1.0f : 1.0f;
+    const float ratio = sgn < 0.0f ? etaInner / etaOuter : etaOuter / etaInner;
+    const float sq = 1.0f - ratio * ratio * (1.0f - ion * ion);
+    if (sq <= 0.0f) return false;
+    const float nf = ratio * ion - sgn * std::sqrt(sq);
+    const float iff = -ratio;
+    *out = normalize(N * nf + I * iff);
+    return true;
+}
+// TranslucentMicrofacet.cpp:362-366
+static inline float reflect_prob(float F) { return std::clamp(F, 0.0001f, 1.0f - 0.0001f); }
+
+// BsdfHelper.h:14-45
+static inline bool half_vec_same_hemi(V3 L, V3 V, V3 N, V3* H) {
+    V3 h = L + V;
+    if (is_zero(h)) return false;
+    h = normalize(h);
+    const float nh = dot(N, h);
+    const float s = (float)((0.0f < nh) - (0.0f > nh));
+    h = h * s;
+    *H = h;
+    return !is_zero(h);
+}
+
+struct BsdfEval { bool ok; RGB f; };
+struct BsdfSample { bool ok; V3 L; RGB w; float relIor; };
+
+// SurfaceOptics::calcBsdf (SurfaceOptics.cpp:17-30) + material calcBsdf
+static BsdfEval bsdf_eval(const orc_material& m, const SurfaceHit& X, V3 L, V3 V) {
+    BsdfEval e{false, rgb(0)};
+    if (!side_agreed(X, L) || !side_agreed(X, V)) return e;
+    const V3 N = X.Ns;
+    if (m.type == ORC_MAT_LAMBERT) {  // LambertianReflector.cpp:39-52
+        if (!same_hemi(X, L, V)) return e;
+        e.f = muls(RGB{m.rgb[0], m.rgb[1], m.rgb[2]}, kRcpPi);
+        e.ok = rgb_finite(e.f);
+    } else if (m.type == ORC_MAT_CONDUCTOR) {  // OpaqueMicrofacet.cpp:59-91
+        const float nol = dot(N, L), nov = dot(N, V);
+        if (!same_hemi(X, L, V)) return e;
+        V3 H;
+        if (!half_vec_same_hemi(L, V, N, &H)) return e;
+        const float hol = dot(H, L);
+        const float a = mat_alpha(m);
+        const RGB F = fresnel_schlick(m, hol);
+        const float D = ggx_D(N, H, a);
+        const float G = ggx_G(N, H, L, V, a);
+        e.f = muls(F, D * G / (4.0f * std::fabs(nov * nol)));
+        e.ok = rgb_finite(e.f);
+    } else if (m.type == ORC_MAT_DIELECTRIC) {  // TranslucentMicrofacet.cpp:62-152
+        const float nol = dot(N, L), nov = dot(N, V);
+        const float nolnov = nol * nov;
+        if (nolnov == 0.0f) return e;
+        const float a = mat_alpha(m);
+        if (same_hemi(X, L, V)) {
+            V3 H;
+            if (!half_vec_same_hemi(L, V, N, &H)) return e;
+            const float hol = dot(H, L);
+            const float F = fresnel_dielectric(m.ior_outer, m.ior_inner, hol);
+            const float D = ggx_D(N, H, a);
+            const float G = ggx_G(N, H, L, V, a);
+            e.f = rgb(F * (D * G / (4.0f * std::fabs(nolnov))));
+            e.ok = rgb_finite(e.f);
+        } else if (opposite_hemi(X, L, V)) {
+            float etaI = m.ior_outer, etaT = m.ior_inner;
+            if (nol < 0.0f) std::swap(etaI, etaT);
+            V3 H = L * (-etaI) + V * (-etaT);
+            if (is_zero(H)) return e;
+            H = normalize(H);
+            if (dot(N, H) < 0.0f) H = H * -1.0f;
+            const float hov = dot(H, V);
+            const float hol = dot(H, L);
+            const float F = 1.0f - fresnel_dielectric(m.ior_outer, m.ior_inner, hol);
+            const float D = ggx_D(N, H, a);
+            const float G = ggx_G(N, H, L, V, a);
+            const float transport = etaT / etaI;
+            const float iorTerm = transport * etaI / (etaI * hol + etaT * hov);
+            if (!std::isfinite(iorTerm)) return e;
+            const float dotTerm = std::fabs(hol * hov / nolnov);
+            e.f = rgb(F * (D * G * dotTerm * (iorTerm * iorTerm)));
+            e.ok = rgb_finite(e.f);
+        }
+    }
+    return e;
+}
+
+// SurfaceOptics::calcBsdfPdf (SurfaceOptics.cpp:54-67) + material calcBsdfPdf.
+// Returns the pdf value; callers treat a result that is not finite and > 0 as "no pdf".
+static float bsdf_pdf(const orc_material& m, const SurfaceHit& X, V3 L, V3 V) {
+    if (!side_agreed(X, L) || !side_agreed(X, V)) return 0.0f;
+    const V3 N = X.Ns;
+    if (m.type == ORC_MAT_LAMBERT) {  // LambertianReflector.cpp:86-99
+        if (!same_hemi(X, L, V)) return 0.0f;
+        return absdot(L, N) * kRcpPi;
+    } else if (m.type == ORC_MAT_CONDUCTOR) {  // OpaqueMicrofacet.cpp:125-153
+        if (!same_hemi(X, L, V)) return 0.0f;
+        V3 H;
+        if (!half_vec_same_hemi(L, V, N, &H)) return 0.0f;
+        const float pdf = ggx_vndf_pdf(N, H, V, mat_alpha(m));
+        return pdf / (4.0f * absdot(H, L));
+    } else if (m.type == ORC_MAT_DIELECTRIC) {  // TranslucentMicrofacet.cpp:275-360
+        const float a = mat_alpha(m);
+        if (same_hemi(X, L, V)) {
+            V3 H;
+            if (!half_vec_same_hemi(L, V, N, &H)) return 0.0f;
+            const float F = fresnel_dielectric(m.ior_outer, m.ior_inner, dot(H, L));
+            const float rp = reflect_prob(F);
+            const float pdf = ggx_vndf_pdf(N, H, V, a);
+            return pdf / (4.0f * absdot(H, L)) * rp;
+        } else if (opposite_hemi(X, L, V)) {
+            const float nol = dot(N, L);
+            float etaI = m.ior_outer, etaT = m.ior_inner;
+            if (nol < 0.0f) std::swap(etaI, etaT);
+            V3 H = L * (-etaI) + V * (-etaT);
+            if (is_zero(H)) return 0.0f;
+            H = normalize(H);
+            if (dot(N, H) < 0.0f) H = H * -1.0f;
+            const float hov = dot(H, V), hol = dot(H, L);
+            const float iorTerm = etaI * hol + etaT * hov;
+            const float mult = std::fabs(etaI * etaI * hol) / (iorTerm * iorTerm);
+            const float F = fresnel_dielectric(m.ior_outer, m.ior_inner, hol);
+            const float tp = 1.0f - reflect_prob(F);
+            const float pdf = ggx_vndf_pdf(N, H, V, a);
+            return pdf * mult * tp;
+        }
+        return 0.0f;
+    }
+    return 0.0f;
+}
+static inline bool pdf_ok(float p) { return std::isfinite(p) && p > 0.0f; }
+
+// BsdfSampleOutput::setPdfAppliedBsdfCos (BsdfSampleQuery.h:196-213)
+static inline bool measurable(RGB w, float cosv) { return rgb_finite(w) && 0.0f < cosv && cosv < 1.1f; }
+
+// SurfaceOptics::genBsdfSample (SurfaceOptics.cpp:32-52) + material genBsdfSample.
+// u[0], u[1]: direction; u[2]: reflect/transmit pick (dielectric only).
+static BsdfSample bsdf_sample(const orc_material& m, const SurfaceHit& X, V3 V, const float* u) {
+    BsdfSample s{false, mk(0, 0, 0), rgb(0), 1.0f};
+    if (!side_agreed(X, V)) return s;
+    const V3 N = X.Ns;
+    if (m.type == ORC_MAT_LAMBERT) {  // LambertianReflector.cpp:54-84, THemisphere.h:104-119
+        const float phi = kTwoPi * u[0];
+        const float yv = std::sqrt(u[1]);
+        const float yr = std::sqrt(1.0f - yv * yv);
+        V3 L = mk(std::sin(phi) * yr, yv, std::cos(phi) * yr);
+        L = normalize(to_world(X, L));
+        if (dot(V, N) < 0.0f) L = L * -1.0f;
+        s.w = RGB{m.rgb[0], m.rgb[1], m.rgb[2]};
+        s.ok = measurable(s.w, absdot(N, L));
+        s.L = L;
+    } else if (m.type == ORC_MAT_CONDUCTOR) {  // OpaqueMicrofacet.cpp:93-123
+        const float a = mat_alpha(m);
+        const V3 H = ggx_sample_vh(X, N, V, a, u[0], u[1]);
+        const V3 L = normalize(reflect(neg(V), H));
+        const float nov = dot(N, V);
+        const float hol = dot(H, L);
+        const float dotTerms = std::fabs(hol / nov);
+        const RGB F = fresnel_schlick(m, hol);
+        const float G = ggx_G(N, H, L, V, a);
+        const float D = ggx_D(N, H, a);
+        const float pdf = ggx_vndf_pdf(N, H, V, a);
+        s.w = muls(F, G * D * dotTerms / pdf);
+        s.ok = measurable(s.w, absdot(N, L));
+        s.L = L;
+    } else if (m.type == ORC_MAT_DIELECTRIC) {  // TranslucentMicrofacet.cpp:154-273
+        const float a = mat_alpha(m);
+        const V3 H = ggx_sample_vh(X, N, V, a, u[0], u[1]);
+        float F = fresnel_dielectric(m.ior_outer, m.ior_inner, dot(H, V));
+        const float rp = reflect_prob(F);
+        V3 L;
+        // SampleFlow::unflowedPick with a fresh sample (sample.h:41-48)
+        if (u[2] < rp) {
+            L = normalize(reflect(neg(V), H));
+            if (!same_hemi(X, V, L)) return s;
+            F = F / rp;
+        } else {
+            V3 R;
+            if (!refract_dir(m.ior_outer, m.ior_inner, V, H, &R) || !opposite_hemi(X, V, R)) return s;
+            L = R;
+            F = 1.0f - fresnel_dielectric(m.ior_outer, m.ior_inner, dot(H, L));
+            float etaI = m.ior_outer, etaT = m.ior_inner;
+            if (dot(N, L) < 0.0f) std::swap(etaI, etaT);
+            const float rel = etaT / etaI;
+            s.relIor = rel;
+            F = F * (rel * rel);
+            F = F / (1.0f - rp);
+        }
+        const float pdf = ggx_vndf_pdf(N, H, V, a);
+        const float nov = dot(N, V), hov = dot(H, V);
+        const float dotTerms = std::fabs(hov / nov);
+        const float G = ggx_G(N, H, L, V, a);
+        const float D = ggx_D(N, H, a);
+        s.w = rgb(F * (G * D * dotTerms / pdf));
+        s.ok = measurable(s.w, absdot(N, L));
+        s.L = L;
+    } else {
+        return s;  // absorber (IdealAbsorber.cpp:32-39): never measurable
+    }
+    if (s.ok && !side_agreed(X, s.L)) s.ok = false;
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Self-intersection avoidance: SurfaceHitRefinery iterative mode
+// (SurfaceHitRefinery.cpp:62-263, SurfaceHitRefinery.h:200-473), 1 refinement,
+// error factors 5e-8 / 2.5e-5 (PTriangle.cpp:128-129).
+// ---------------------------------------------------------------------------------------
+static constexpr float kMeanErr = 5e-8f;
+static constexpr float kMaxErr = 5e-8f * 5e2f;
+static constexpr int kMaxEscapeIters = 256;  // bound for the unbounded search loops
+
+static inline bool reintersect(const Scene& S, int32_t tri, V3 org, V3 dir, float minT, float maxT, float* t) {
+    V3 b;
+    return wt_intersect(S.vtx(tri, 0), S.vtx(tri, 1), S.vtx(tri, 2), org, dir, minT, maxT, t, &b);
+}
+
+struct EscapeResult { V3 offset; V3 unit_dir; float max_dist; };
+
+static EscapeResult iterative_offset(const Scene& S, const SurfaceHit& X, V3 dir) {
+    const V3 N = X.Ng;
+    const V3 offsetDir = dot(N, dir) > 0.0f ? N : neg(N);
+    const V3 escDir = normalize(dir);
+    const float escCos = std::clamp(dot(escDir, offsetDir), 0.0f, 1.0f);
+    float init = length(X.P) * kMeanErr;
+    if (!std::isfinite(init)) init = std::max(length(X.P) * 1e-6f, 2e-4f);
+    init = init + FLT_MIN;
+    float maxD = init, minD = 0.0f;
+    for (int it = 0; it < kMaxEscapeIters; ++it) {
+        float t;
+        if (!reintersect(S, X.tri, X.P + offsetDir * maxD, escDir, 0.0f, FLT_MAX, &t)) break;
+        maxD += t * escCos;
+        minD = maxD;
+        maxD *= 2.0f;
+    }
+    int nref = 0;
+    for (int it = 0; it < kMaxEscapeIters && nref < 1; ++it) {
+        const float mid = (minD + maxD) * 0.5f;
+        if (!(minD < mid && mid < maxD)) break;
+        float t;
+        if (!reintersect(S, X.tri, X.P + offsetDir * mid, escDir, 0.0f, FLT_MAX, &t)) { maxD = mid; ++nref; }
+        else minD = std::clamp(mid + t * escCos, mid, maxD);
+    }
+    return EscapeResult{offsetDir * maxD, escDir, FLT_MAX};
+}
+
+// tryEscapeIteratively + iterativeMutualOffset.  Returns false when no valid ray exists.
+static bool mutual_escape(const Scene& S, const SurfaceHit& X, const SurfaceHit& X2,
+                          V3* org, V3* dir, float* maxT) {
+    const V3 x2x = X2.P - X.P;
+    const EscapeResult rx = iterative_offset(S, X, x2x);
+    float init = length(X2.P) * kMaxErr;
+    if (!std::isfinite(init)) init = std::max(length(X2.P) * 1e-3f, 2e-4f);
+    init = init + FLT_MIN;
+    const V3 o = X.P + rx.offset;
+    const float dist = length(o - X2.P);
+    float maxD = init, minD = 0.0f;
+    for (int it = 0; it < kMaxEscapeIters && dist > maxD; ++it) {
+        float t;
+        if (!reintersect(S, X2.tri, o, rx.unit_dir, 0.0f, dist - maxD, &t)) break;
+        maxD = std::max(dist - t, maxD);
+        minD = maxD;
+        maxD *= 2.0f;
+    }
+    int nref = 0;
+    for (int it = 0; it < kMaxEscapeIters && dist > maxD && nref < 1; ++it) {
+        const float mid = (minD + maxD) * 0.5f;
+        if (!(minD < mid && mid < maxD)) break;
+        float t;
+        if (!reintersect(S, X2.tri, o, rx.unit_dir, 0.0f, dist - mid, &t)) { maxD = mid; ++nref; }
+        else minD = std::clamp(dist - t, mid, maxD);
+    }
+    const float len = std::max(dist - maxD, 0.0f);
+    if (!(len > 0.0f && std::isfinite(len))) return false;
+    *org = o; *dir = rx.unit_dir; *maxT = len;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// Emitters: ESPowerFavoring over area-light groups (ESPowerFavoring.cpp:21-97),
+// GroupedDiffuseSurfaceEmitter (uniform triangle pick, :58-84), DiffuseSurfaceEmitterBase
+// (:34-89), SurfaceEmitter (front-face emission, :20-61), lta::pdfA_to_pdfW (lta.h:17-25).
+// ---------------------------------------------------------------------------------------
+static inline float pdfA_to_pdfW(float pdfA, V3 posToTarget, V3 N) {
+    const float d2 = dot(posToTarget, posToTarget);
+    const float s = pdfA / dot(normalize(posToTarget), N) * d2;
+    return std::isfinite(s) ? std::fabs(s) : 0.0f;
+}
+
+static void emitters_prepare(Scene& S) {
+    const orc_scene* s = S.s;
+    S.tri_area.resize(s->num_tris);
+    for (uint32_t t = 0; t < s->num_tris; ++t) S.tri_area[t] = tri_area(S.vtx(t, 0), S.vtx(t, 1), S.vtx(t, 2));
+    // weights = approximate radiant flux = luminance(radiance) * area * pi per triangle,
+    // summed over the group (GroupedDiffuseSurfaceEmitter.cpp:110-120)
+    const uint32_t n = s->num_emitters;
+    std::vector<float> w(n, 0.0f);
+    for (uint32_t e = 0; e < n; ++e) {
+        const orc_emitter& em = s->emitters[e];
+        float total = 0.0f;
+        for (uint32_t k = 0; k < em.tri_count; ++k) {
+            const uint32_t t = s->emitter_tris[em.tri_begin + k];
+            const float flux = luminance(RGB{em.radiance[0], em.radiance[1], em.radiance[2]}) * S.tri_area[t] * kPi;
+            total += flux;
+        }
+        w[e] = total;
+    }
+    // TPwcDistribution1D.ipp (min 0, max 1): cdf[i] = cdf[i-1] + w*delta, normalised
+    S.emit_cdf.assign(n + 1, 0.0f);
+    S.emit_pick.assign(n, 0.0f);
+    if (n == 0) return;
+    const float delta = 1.0f / (float)n;
+    for (uint32_t i = 1; i <= n; ++i) S.emit_cdf[i] = S.emit_cdf[i - 1] + w[i - 1] * delta;
+    const float rcpSum = 1.0f / S.emit_cdf[n];
+    S.emit_cdf[0] = 0.0f; S.emit_cdf[n] = 1.0f;
+    if (std::isfinite(rcpSum)) { for (uint32_t i = 1; i < n; ++i) S.emit_cdf[i] *= rcpSum; }
+    else { for (uint32_t i = 1; i < n; ++i) S.emit_cdf[i] = (float)i / (float)n; }
+    for (uint32_t i = 0; i < n; ++i) S.emit_pick[i] = S.emit_cdf[i + 1] - S.emit_cdf[i];
+}
+
+// TPwcDistribution1D.ipp:99-107 sampleDiscrete (lower_bound over the CDF)
+static uint32_t cdf_sample(const Scene& S, float u) {
+    const auto& c = S.emit_cdf;
+    auto it = std::lower_bound(c.begin(), c.end(), u);
+    if (it == c.begin()) {
+        for (uint32_t i = 0; i + 1 < c.size(); ++i) if ((c[i + 1] - c[i]) > 0.0f) return i;
+        return 0;
+    }
+    return (uint32_t)(it - c.begin() - 1);
+}
+
+struct NeeSample { bool ok; SurfaceHit Xe; RGB Le; float pdfW; };
+
+// TDirectLightEstimator::neeSampleEmission (TDirectLightEstimator.ipp:70-102) through
+// Scene::genDirectSample; u[0]: emitter, u[1]: triangle in group, u[2..3]: position.
+static NeeSample nee_sample(const Scene& S, const SurfaceHit& X, const float* u) {
+    NeeSample r{};
+    r.ok = false;
+    const orc_scene* s = S.s;
+    if (s->num_emitters == 0) return r;
+    const uint32_t ei = cdf_sample(S, u[0]);
+    const float pickPdf = S.emit_pick[ei];
+    const orc_emitter& em = s->emitters[ei];
+    uint32_t k = (uint32_t)(u[1] * (float)em.tri_count);
+    if (k >= em.tri_count) k = em.tri_count - 1;
+    const uint32_t tri = s->emitter_tris[em.tri_begin + k];
+    // PTriangle::genPosSample (PTriangle.cpp:269-293), Osada (TBasicTriangle.ipp:236-259)
+    const float A = std::sqrt(u[2]);
+    const float B = u[3];
+    const V3 bary = mk(1.0f - A, A * (1.0f - B), B * A);
+    const float pdfA = 1.0f / S.tri_area[tri];
+    const V3 va = S.vtx(tri, 0), vb = S.vtx(tri, 1), vc = S.vtx(tri, 2);
+    const V3 pos = wsum(va, bary.x, vb, bary.y, vc, bary.z);
+    const SurfaceHit Xe = make_hit(S, (int32_t)tri, bary, pos - X.P);
+    const V3 e2t = X.P - pos;
+    if (!(dot(e2t, Xe.Ns) > 0.0f)) return r;  // canEmit, front face
+    float pdf = pdfA_to_pdfW(pdfA, e2t, Xe.Ns);
+    if (em.tri_count >= 2) pdf = pdf * (1.0f / (float)em.tri_count);
+    pdf = pdf * pickPdf;
+    if (!pdf_ok(pdf)) return r;
+    if (!side_agreed(X, pos - X.P)) return r;
+    V3 o, d; float mt;
+    if (!mutual_escape(S, X, Xe, &o, &d, &mt)) return r;
+    Hit occ;
+    if (scene_intersect(S, o, d, 0.0f, mt, &occ)) return r;
+    r.ok = true; r.Xe = Xe; r.Le = RGB{em.radiance[0], em.radiance[1], em.radiance[2]}; r.pdfW = pdf;
+    return r;
+}
+
+// neeSamplePdfWUnoccluded (TDirectLightEstimator.ipp:218-230) -> ESPowerFavoring::calcDirectPdf
+static float nee_pdf(const Scene& S, const SurfaceHit& X, const SurfaceHit& Xe) {
+    const int32_t ei = S.s->tri_emitter[Xe.tri];
+    if (ei < 0) return 0.0f;
+    const orc_emitter& em = S.s->emitters[ei];
+    const V3 e2t = X.P - Xe.P;
+    if (!(dot(e2t, Xe.Ns) > 0.0f)) return 0.0f;
+    const float pdfA = 1.0f / S.tri_area[Xe.tri];
+    float pdfW = pdfA_to_pdfW(pdfA, e2t, Xe.Ns);
+    const float groupPick = em.tri_count >= 2 ? 1.0f / (float)em.tri_count : 1.0f;
+    pdfW = pdfW * groupPick;
+    if (!pdf_ok(pdfW)) return 0.0f;
+    const float pdf = pdfW * S.emit_pick[ei];
+    return pdf_ok(pdf) ? pdf : 0.0f;
+}
+
+// DiffuseSurfaceEmitterBase::evalEnergyFromSurface (:34-49)
+static RGB eval_emission(const Scene& S, const SurfaceHit& Xe) {
+    const int32_t ei = S.s->tri_emitter[Xe.tri];
+    if (ei < 0) return rgb(0);
+    const V3 emitDir = neg(Xe.incident_dir);
+    if (!(dot(emitDir, Xe.Ns) > 0.0f)) return rgb(0);
+    const orc_emitter& em = S.s->emitters[ei];
+    return RGB{em.radiance[0], em.radiance[1], em.radiance[2]};
+}
+
+// TMIS.h:32-35 power heuristic
+static inline float mis_power(float p0, float p1) { return (p0 * p0) / (p0 * p0 + p1 * p1); }
+
+// ---------------------------------------------------------------------------------------
+// Camera: pinhole with horizontal FOV (PinholeCamera.cpp:26-69, SingleLensObserver.cpp:27-85).
+// Raster (0,0) is the lower-left corner; the traced ray is the reversed sensed ray.
+// ---------------------------------------------------------------------------------------
+static void camera_ray(const orc_camera& c, uint32_t W, uint32_t H, float rx, float ry, V3* org, V3* dir) {
+    const V3 pos = mk(c.pos[0], c.pos[1], c.pos[2]);
+    const V3 right = mk(c.right[0], c.right[1], c.right[2]);
+    const V3 up = mk(c.up[0], c.up[1], c.up[2]);
+    const V3 fwd = mk(c.forward[0], c.forward[1], c.forward[2]);
+    const float cx = (2.0f * (rx / (float)W) - 1.0f) * c.tan_half_fov;
+    const float cy = (2.0f * (ry / (float)H) - 1.0f) * c.tan_half_fov * ((float)H / (float)W);
+    *org = pos;
+    *dir = normalize(right * cx + up * cy + fwd);
+}
+
+// ---------------------------------------------------------------------------------------
+// BNEEPTEstimator::estimate (BNEEPTEstimator.cpp:38-262) with a bounce cap.
+// ---------------------------------------------------------------------------------------
+struct PathStats { uint64_t rays_primary = 0, rays_ext = 0, rays_shadow = 0; };
+
+static RGB estimate(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, PathStats* st) {
+    RGB energy = rgb(0), T = rgb(1);
+    float rrScale = 1.0f;
+    Hit h;
+    st->rays_primary++;
+    if (!scene_intersect(S, org, dir, 0.0f, FLT_MAX, &h)) return energy;
+    SurfaceHit X = make_hit(S, h.tri, h.bary, dir);
+    if (!side_agreed(X, dir)) return energy;
+    V3 curDir = dir;
+    // zero-bounce emission (SurfaceTracer.h:239-259)
+    {
+        const RGB Le = eval_emission(S, X);
+        if (S.s->tri_emitter[X.tri] >= 0) { energy.r += Le.r; energy.g += Le.g; energy.b += Le.b; }
+    }
+    for (uint32_t nb = 0; nb < max_bounces; ++nb) {
+        const orc_material& m = S.s->materials[S.s->tri_material[X.tri]];
+        float u[8];
+        for (int k = 0; k < 8; ++k) u[k] = sf.u();
+        if (m.type == ORC_MAT_ABSORBER) break;  // absorber: no BSDF, path ends
+        const V3 V = curDir * -1.0f;
+        // NEE with MIS
+        {
+            NeeSample ns = nee_sample(S, X, u);
+            st->rays_shadow += 1;
+            if (ns.ok) {
+                const V3 L = normalize(ns.Xe.P - X.P);
+                const BsdfEval be = bsdf_eval(m, X, L, V);
+                if (be.ok) {
+                    float bsdfPdf = bsdf_pdf(m, X, L, V);
+                    if (!pdf_ok(bsdfPdf)) bsdfPdf = 0.0f;
+                    const float w = mis_power(ns.pdfW, bsdfPdf);
+                    const V3 N = X.Ns;
+                    RGB weight = muls(be.f, absdot(N, L));
+                    weight = mulc(weight, T);
+                    weight = muls(weight, w / ns.pdfW);
+                    weight = rational_clamp(weight);
+                    energy.r += ns.Le.r * weight.r; energy.g += ns.Le.g * weight.g; energy.b += ns.Le.b * weight.b;
+                }
+            }
+        }
+        // BSDF sample and extension
+        const BsdfSample bs = bsdf_sample(m, X, V, u + 4);
+        if (!bs.ok) break;
+        const V3 L = bs.L;
+        const EscapeResult er = iterative_offset(S, X, L);
+        const V3 no = X.P + er.offset;
+        st->rays_ext++;
+        Hit nh;
+        if (!scene_intersect(S, no, er.unit_dir, 0.0f, FLT_MAX, &nh)) break;
+        const SurfaceHit nX = make_hit(S, nh.tri, nh.bary, er.unit_dir);
+        if (!side_agreed(nX, er.unit_dir)) break;
+        if (S.s->tri_emitter[nX.tri] >= 0) {
+            const RGB Le = eval_emission(S, nX);
+            if (!rgb_zero(Le)) {
+                const float neePdf = nee_pdf(S, X, nX);
+                const float bsdfPdf = bsdf_pdf(m, X, L, V);
+                if (pdf_ok(bsdfPdf)) {
+                    const float w = mis_power(bsdfPdf, neePdf);
+                    RGB weight = mulc(bs.w, T);
+                    weight = muls(weight, w);
+                    weight = rational_clamp(weight);
+                    energy.r += Le.r * weight.r; energy.g += Le.g * weight.g; energy.b += Le.b * weight.b;
+                }
+            }
+        }
+        T = mulc(T, bs.w);
+        rrScale /= bs.relIor * bs.relIor;
+        if (nb >= 3) {
+            const float surv = safe_clamp(luminance(muls(T, rrScale)), 0.0f, 0.95f);
+            if (u[7] < surv) T = muls(T, 1.0f / surv);
+            else break;
+        }
+        if (rgb_zero(T)) break;
+        X = nX;
+        curDir = er.unit_dir;
+    }
+    return energy;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+// ---------------------------------------------------------------------------------------
+// C ABI (ctypes entry points for tests and the CPU baseline).
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+uint32_t orc_version(void) { return 1; }
+
+void orc_pcg32_generate(uint64_t seq, uint64_t init_state, int use_default, uint64_t jump,
+                        uint32_t n, uint32_t* out) {
+    Pcg32 r = use_default ? Pcg32() : Pcg32(seq, init_state);
+    pcg_jump(r, jump);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
+}
+
+uint64_t orc_moremur(uint64_t v) { return moremur(v); }
+
+void orc_stream_samples(uint64_t key, uint32_t n, float* out) {
+    Stream s(key);
+    for (uint32_t i = 0; i < n; ++i) out[i] = s.u();
+}
+
+int orc_intersect_triangle(const float* tri9, const float* ray8, float* out_t, float* out_bary3) {
+    const V3 a = mk(tri9[0], tri9[1], tri9[2]), b = mk(tri9[3], tri9[4], tri9[5]), c = mk(tri9[6], tri9[7], tri9[8]);
+    const V3 o = mk(ray8[0], ray8[1], ray8[2]), d = mk(ray8[4], ray8[5], ray8[6]);
+    float t; V3 bb;
+    const bool hit = wt_intersect(a, b, c, o, d, ray8[3], ray8[7], &t, &bb);
+    if (hit) { *out_t = t; out_bary3[0] = bb.x; out_bary3[1] = bb.y; out_bary3[2] = bb.z; }
+    return hit ? 1 : 0;
+}
+
+// Brute-force closest hit, triangles tested in index order with a shrinking segment
+// (the per-leaf semantics of TLinearDepthFirstBinaryBvh.ipp:79-92).  Flags per ray:
+// bit0 = another triangle's hit lies within 4 ulp of the closest t (tie),
+// bit1 = closest hit's smallest barycentric coordinate is below 1e-6 (edge-near).
+void orc_intersect_brute(const float* verts, uint32_t ntri, const float* rays, uint32_t nray,
+                         int32_t* out_tri, float* out_t, float* out_bary, uint32_t* out_flags,
+                         uint32_t nthreads) {
+    auto work = [&](uint32_t r0, uint32_t r1) {
+        for (uint32_t r = r0; r < r1; ++r) {
+            const float* ry = rays + 8 * (size_t)r;
+            const V3 o = mk(ry[0], ry[1], ry[2]), d = mk(ry[4], ry[5], ry[6]);
+            const float minT = ry[3], maxT = ry[7];
+            float segMax = maxT; int32_t best = -1; float bt = 0; V3 bb{0, 0, 0};
+            float t1 = INFINITY, t2 = INFINITY;  // smallest and second-smallest t over all hits
+            for (uint32_t t = 0; t < ntri; ++t) {
+                const float* p = verts + 9 * (size_t)t;
+                const V3 a = mk(p[0], p[1], p[2]), b = mk(p[3], p[4], p[5]), c = mk(p[6], p[7], p[8]);
+                float ht; V3 hb;
+                if (wt_intersect(a, b, c, o, d, minT, maxT, &ht, &hb)) {
+                    if (ht < t1) { t2 = t1; t1 = ht; } else if (ht < t2) { t2 = ht; }
+                }
+                if (wt_intersect(a, b, c, o, d, minT, segMax, &ht, &hb)) { segMax = ht; best = (int32_t)t; bt = ht; bb = hb; }
+            }
+            uint32_t flags = 0;
+            if (best >= 0) {
+                const float lim = std::nextafter(std::nextafter(std::nextafter(std::nextafter(t1, INFINITY), INFINITY), INFINITY), INFINITY);
+                if (t2 <= lim) flags |= 1u;
+                if (std::min(bb.x, std::min(bb.y, bb.z)) < 1e-6f) flags |= 2u;
+            }
+            out_tri[r] = best; out_t[r] = bt;
+            out_bary[3 * (size_t)r + 0] = bb.x; out_bary[3 * (size_t)r + 1] = bb.y; out_bary[3 * (size_t)r + 2] = bb.z;
+            out_flags[r] = flags;
+        }
+    };
+    if (nthreads <= 1 || nray < 1024) { work(0, nray); return; }
+    std::vector<std::thread> th;
+    const uint32_t chunk = (nray + nthreads - 1) / nthreads;
+    for (uint32_t i = 0; i < nthreads; ++i) {
+        const uint32_t a = i * chunk, b = std::min(nray, a + chunk);
+        if (a < b) th.emplace_back(work, a, b);
+    }
+    for (auto& t : th) t.join();
+}
+
+// Closest hit through the reference's binary BVH (median split, depth-first layout).
+void orc_intersect_refbvh(const float* verts, uint32_t ntri, const float* rays, uint32_t nray,
+                          int32_t* out_tri, float* out_t, uint32_t nthreads) {
+    orc_scene sc{};
+    sc.num_tris = ntri; sc.verts = verts;
+    Scene S; S.s = &sc;
+    bvh_build(S);
+    auto work = [&](uint32_t r0, uint32_t r1) {
+        for (uint32_t r = r0; r < r1; ++r) {
+            const float* ry = rays + 8 * (size_t)r;
+            Hit h;
+            if (bvh_nearest(S, mk(ry[0], ry[1], ry[2]), mk(ry[4], ry[5], ry[6]), ry[3], ry[7], &h)) { out_tri[r] = h.tri; out_t[r] = h.t; }
+            else { out_tri[r] = -1; out_t[r] = 0; }
+        }
+    };
+    if (nthreads <= 1) { work(0, nray); return; }
+    std::vector<std::thread> th;
+    const uint32_t chunk = (nray + nthreads - 1) / nthreads;
+    for (uint32_t i = 0; i < nthreads; ++i) {
+        const uint32_t a = i * chunk, b = std::min(nray, a + chunk);
+        if (a < b) th.emplace_back(work, a, b);
+    }
+    for (auto& t : th) t.join();
+}
+
+// Surface-optics queries at a synthetic flat hit with normal N (Ng = Ns = N).
+static SurfaceHit synthetic_hit(const float* N3) {
+    SurfaceHit X{};
+    X.tri = 0; X.P = mk(0, 0, 0);
+    X.Ng = X.Ns = X.by = mk(N3[0], N3[1], N3[2]);
+    basis_from_unit_y(X.Ns, &X.bx, &X.bz);
+    X.incident_dir = mk(0, 0, 0);
+    return X;
+}
+
+int orc_bsdf_eval(const orc_material* m, const float* N3, const float* L3, const float* V3_, float* out_f3) {
+    const SurfaceHit X = synthetic_hit(N3);
+    const BsdfEval e = bsdf_eval(*m, X, mk(L3[0], L3[1], L3[2]), mk(V3_[0], V3_[1], V3_[2]));
+    out_f3[0] = e.f.r; out_f3[1] = e.f.g; out_f3[2] = e.f.b;
+    return e.ok ? 1 : 0;
+}
+
+float orc_bsdf_pdf(const orc_material* m, const float* N3, const float* L3, const float* V3_) {
+    const SurfaceHit X = synthetic_hit(N3);
+    return bsdf_pdf(*m, X, mk(L3[0], L3[1], L3[2]), mk(V3_[0], V3_[1], V3_[2]));
+}
+
+int orc_bsdf_sample(const orc_material* m, const float* N3, const float* V3_, const float* u3,
+                    float* out_L3, float* out_w3, float* out_relior) {
+    const SurfaceHit X = synthetic_hit(N3);
+    const BsdfSample s = bsdf_sample(*m, X, mk(V3_[0], V3_[1], V3_[2]), u3);
+    out_L3[0] = s.L.x; out_L3[1] = s.L.y; out_L3[2] = s.L.z;
+    out_w3[0] = s.w.r; out_w3[1] = s.w.g; out_w3[2] = s.w.b;
+    *out_relior = s.relIor;
+    return s.ok ? 1 : 0;
+}
+
+float orc_ggx_D(const float* N3, const float* H3, float alpha) { return ggx_D(mk(N3[0], N3[1], N3[2]), mk(H3[0], H3[1], H3[2]), alpha); }
+float orc_ggx_G(const float* N3, const float* H3, const float* L3, const float* V3_, float alpha) {
+    return ggx_G(mk(N3[0], N3[1], N3[2]), mk(H3[0], H3[1], H3[2]), mk(L3[0], L3[1], L3[2]), mk(V3_[0], V3_[1], V3_[2]), alpha);
+}
+float orc_ggx_vndf_pdf(const float* N3, const float* H3, const float* V3_, float alpha) {
+    return ggx_vndf_pdf(mk(N3[0], N3[1], N3[2]), mk(H3[0], H3[1], H3[2]), mk(V3_[0], V3_[1], V3_[2]), alpha);
+}
+void orc_ggx_sample_vh(const float* N3, const float* V3_, float alpha, float u0, float u1, float* out_H3) {
+    const SurfaceHit X = synthetic_hit(N3);
+    const V3 H = ggx_sample_vh(X, X.Ns, mk(V3_[0], V3_[1], V3_[2]), alpha, u0, u1);
+    out_H3[0] = H.x; out_H3[1] = H.y; out_H3[2] = H.z;
+}
+float orc_fresnel_dielectric(float eta_outer, float eta_inner, float cos_i) { return fresnel_dielectric(eta_outer, eta_inner, cos_i); }
+void orc_fresnel_schlick(const float* f0_3, float cos_i, float* out3) {
+    orc_material m{}; m.rgb[0] = f0_3[0]; m.rgb[1] = f0_3[1]; m.rgb[2] = f0_3[2];
+    const RGB F = fresnel_schlick(m, cos_i);
+    out3[0] = F.r; out3[1] = F.g; out3[2] = F.b;
+}
+int orc_half_vector(const float* L3, const float* V3_, float* out_H3) {
+    V3 h = mk(L3[0], L3[1], L3[2]) + mk(V3_[0], V3_[1], V3_[2]);
+    if (is_zero(h)) return 0;
+    h = normalize(h);
+    out_H3[0] = h.x; out_H3[1] = h.y; out_H3[2] = h.z;
+    return 1;
+}
+void orc_basis_from_y(const float* y3, float* out_x3, float* out_z3) {
+    V3 x, z;
+    basis_from_unit_y(mk(y3[0], y3[1], y3[2]), &x, &z);
+    out_x3[0] = x.x; out_x3[1] = x.y; out_x3[2] = x.z;
+    out_z3[0] = z.x; out_z3[1] = z.y; out_z3[2] = z.z;
+}
+
+// Emitter pick distribution (TPwcDistribution1D over given weights): writes cdf (n+1) and
+// pdfDiscrete (n); returns index picked for each of nu samples.
+void orc_pwc_distribution(const float* weights, uint32_t n, const float* us, uint32_t nu,
+                          float* out_cdf, float* out_pdf, uint32_t* out_pick) {
+    orc_scene sc{}; Scene S; S.s = &sc;
+    const float delta = 1.0f / (float)n;
+    S.emit_cdf.assign(n + 1, 0.0f);
+    for (uint32_t i = 1; i <= n; ++i) S.emit_cdf[i] = S.emit_cdf[i - 1] + weights[i - 1] * delta;
+    const float rcpSum = 1.0f / S.emit_cdf[n];
+    S.emit_cdf[0] = 0.0f; S.emit_cdf[n] = 1.0f;
+    if (std::isfinite(rcpSum)) { for (uint32_t i = 1; i < n; ++i) S.emit_cdf[i] *= rcpSum; }
+    else { for (uint32_t i = 1; i < n; ++i) S.emit_cdf[i] = (float)i / (float)n; }
+    for (uint32_t i = 0; i <= n; ++i) out_cdf[i] = S.emit_cdf[i];
+    for (uint32_t i = 0; i < n; ++i) out_pdf[i] = S.emit_cdf[i + 1] - S.emit_cdf[i];
+    for (uint32_t i = 0; i < nu; ++i) out_pick[i] = cdf_sample(S, us[i]);
+}
+
+// NEE query at a given surface point X (position, normal) of triangle x_tri: returns the
+// sampled emitter point, pdfW and visibility using the full scene.
+int orc_nee_sample(const orc_scene* scene, int32_t x_tri, const float* bary3, const float* u4,
+                   float* out_pos3, float* out_pdfw, float* out_le3) {
+    Scene S; S.s = scene; bvh_build(S); emitters_prepare(S);
+    const SurfaceHit X = make_hit(S, x_tri, mk(bary3[0], bary3[1], bary3[2]), mk(0, 0, 0));
+    const NeeSample ns = nee_sample(S, X, u4);
+    if (!ns.ok) return 0;
+    out_pos3[0] = ns.Xe.P.x; out_pos3[1] = ns.Xe.P.y; out_pos3[2] = ns.Xe.P.z;
+    *out_pdfw = ns.pdfW;
+    out_le3[0] = ns.Le.r; out_le3[1] = ns.Le.g; out_le3[2] = ns.Le.b;
+    return 1;
+}
+
+float orc_nee_pdf(const orc_scene* scene, int32_t x_tri, const float* x_bary3, int32_t e_tri, const float* e_bary3) {
+    Scene S; S.s = scene; emitters_prepare(S);
+    const SurfaceHit X = make_hit(S, x_tri, mk(x_bary3[0], x_bary3[1], x_bary3[2]), mk(0, 0, 0));
+    const SurfaceHit Xe = make_hit(S, e_tri, mk(e_bary3[0], e_bary3[1], e_bary3[2]), mk(0, 0, 0));
+    return nee_pdf(S, X, Xe);
+}
+
+void orc_camera_rays(const orc_camera* cam, uint32_t W, uint32_t H, const float* raster2, uint32_t n, float* out_rays8) {
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 o, d;
+        camera_ray(*cam, W, H, raster2[2 * i], raster2[2 * i + 1], &o, &d);
+        float* r = out_rays8 + 8 * (size_t)i;
+        r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = 0.0f; r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = FLT_MAX;
+    }
+}
+
+// Render [spp_begin, spp_begin + spp) samples per pixel into an FP64 film of 8 doubles per
+// pixel: sum r, g, b, sample count, sum r^2, g^2, b^2, 0.  Box filter: every sample falls in
+// its own pixel (TSamplingFilm.ipp:103-134 with a unit-size filter).  Work is split into
+// 128x128 tiles over nthreads threads (EqualSamplingRenderer.cpp:61).
+int orc_render(const orc_scene* scene, const orc_render_opts* opt, double* film, orc_render_stats* stats) {
+    Scene S; S.s = scene;
+    bvh_build(S);
+    emitters_prepare(S);
+    const uint32_t W = opt->width, H = opt->height;
+    const uint64_t npix = (uint64_t)W * H;
+    const uint32_t TS = 128;
+    const uint32_t tx = (W + TS - 1) / TS, ty = (H + TS - 1) / TS;
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint64_t> n_prim{0}, n_ext{0}, n_shadow{0};
+    const uint32_t y_limit = opt->row_limit ? std::min(opt->row_limit, H) : H;
+    auto worker = [&]() {
+        PathStats st;
+        while (true) {
+            const uint32_t tile = next.fetch_add(1);
+            if (tile >= tx * ty) break;
+            const uint32_t x0 = (tile % tx) * TS, y0 = (tile / tx) * TS;
+            for (uint32_t y = y0; y < std::min(y0 + TS, y_limit); ++y)
+                for (uint32_t x = x0; x < std::min(x0 + TS, W); ++x) {
+                    const uint64_t pix = (uint64_t)y * W + x;
+                    double* f = film + 8 * pix;
+                    for (uint32_t s = opt->spp_begin; s < opt->spp_begin + opt->spp; ++s) {
+                        Stream sf(path_key(opt->seed, pix, s, npix));
+                        const float rx = (float)x + sf.u();
+                        const float ry = (float)y + sf.u();
+                        V3 o, d;
+                        camera_ray(scene->camera, W, H, rx, ry, &o, &d);
+                        const RGB L = estimate(S, o, d, sf, opt->max_bounces, &st);
+                        f[0] += L.r; f[1] += L.g; f[2] += L.b; f[3] += 1.0;
+                        f[4] += (double)L.r * L.r; f[5] += (double)L.g * L.g; f[6] += (double)L.b * L.b;
+                    }
+                }
+        }
+        n_prim += st.rays_primary; n_ext += st.rays_ext; n_shadow += st.rays_shadow;
+    };
+    const uint32_t nt = std::max(1u, opt->nthreads);
+    std::vector<std::thread> th;
+    for (uint32_t i = 0; i < nt; ++i) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    if (stats) { stats->rays_primary = n_prim; stats->rays_extension = n_ext; stats->rays_shadow = n_shadow; }
+    return 0;
+}
+
+// Render the paths of an explicit list of (pixel, sample) pairs; writes RGB per path.
+void orc_render_paths(const orc_scene* scene, const orc_render_opts* opt, const uint32_t* pix,
+                      const uint32_t* smp, uint32_t n, float* out_rgb) {
+    Scene S; S.s = scene; bvh_build(S); emitters_prepare(S);
+    const uint32_t W = opt->width, H = opt->height;
+    const uint64_t npix = (uint64_t)W * H;
+    PathStats st;
+    for (uint32_t i = 0; i < n; ++i) {
+        Stream sf(path_key(opt->seed, pix[i], smp[i], npix));
+        const float rx = (float)(pix[i] % W) + sf.u();
+        const float ry = (float)(pix[i] / W) + sf.u();
+        V3 o, d;
+        camera_ray(scene->camera, W, H, rx, ry, &o, &d);
+        const RGB L = estimate(S, o, d, sf, opt->max_bounces, &st);
+        out_rgb[3 * i] = L.r; out_rgb[3 * i + 1] = L.g; out_rgb[3 * i + 2] = L.b;
+    }
+}
+
+// General filtered splat into an FP64 film (HdrRgbFilm::addRgbSample, HdrRgbFilm.cpp:82-122)
+// and develop (HdrRgbFilm.cpp:167-207), for the reference's film known-answer tests.
+// filter: 0 box (size 1), 1 gaussian (sigma 0.5, size 4, submerged), 2 Blackman-Harris
+// (radius 2), 3 unit horizontal Heaviside step (size 1).
+static double filter_eval(int filter, double x, double y) {
+    if (filter == 0) return 1.0;
+    if (filter == 1) {
+        auto g = [](double a, double b) { return std::exp(-(a * a) / (2 * 0.25) - (b * b) / (2 * 0.25)); };
+        const double edge = g(2.0, 2.0);
+        return std::max(g(x, y) - edge, 0.0);
+    }
+    if (filter == 2) {
+        auto bh = [](double v) {
+            const double r = 2.0;
+            if (std::fabs(v) >= r) return 0.0;
+            const double n = (v + r) / (2 * r);
+            const double pi = 3.14159265358979323846;
+            return 0.35875 - 0.48829 * std::cos(2 * pi * n) + 0.14128 * std::cos(4 * pi * n) - 0.01168 * std::cos(6 * pi * n);
+        };
+        return bh(x) * bh(y);
+    }
+    if (filter == 3) return x >= 0.0 ? 1.0 : 0.0;
+    return 0.0;
+}
+static double filter_half(int filter) { return filter == 0 ? 0.5 : filter == 3 ? 0.5 : 2.0; }
+
+void orc_film_splat(int filter, int64_t W, int64_t H, const double* xy, const float* rgb3, uint32_t n, double* sensors4) {
+    const double hs = filter_half(filter);
+    for (uint32_t i = 0; i < n; ++i) {
+        const double xp = xy[2 * i], yp = xy[2 * i + 1];
+        double fminx = std::max(xp - hs, 0.0), fminy = std::max(yp - hs, 0.0);
+        double fmaxx = std::min(xp + hs, (double)W), fmaxy = std::min(yp + hs, (double)H);
+        const int64_t x0 = (int64_t)std::ceil(fminx - 0.5), y0 = (int64_t)std::ceil(fminy - 0.5);
+        const int64_t x1 = (int64_t)std::floor(fmaxx - 0.5) + 1, y1 = (int64_t)std::floor(fmaxy - 0.5) + 1;
+        for (int64_t y = y0; y < y1; ++y)
+            for (int64_t x = x0; x < x1; ++x) {
+                const double fx = x - (xp - 0.5), fy = y - (yp - 0.5);
+                const double w = filter_eval(filter, fx, fy);
+                double* s = sensors4 + 4 * (y * W + x);
+                s[0] += rgb3[3 * i] * w; s[1] += rgb3[3 * i + 1] * w; s[2] += rgb3[3 * i + 2] * w; s[3] += w;
+            }
+    }
+}
+
+void orc_film_develop(int64_t W, int64_t H, const double* sensors4, float* out_rgb) {
+    for (int64_t i = 0; i < W * H; ++i) {
+        const double w = sensors4[4 * i + 3];
+        const double rw = w == 0.0 ? 0.0 : 1.0 / w;
+        for (int k = 0; k < 3; ++k) out_rgb[3 * i + k] = (float)(sensors4[4 * i + k] * rw);
+    }
+}
+
+}  // extern "C"

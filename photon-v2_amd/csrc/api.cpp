@@ -1,0 +1,450 @@
+// C ABI of the BNEEPT hot path (include/phg.h): context, scene upload, batched ray
+// queries, wavefront render loop, FP64 film, statistics.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../include/phg.h"
+#include "bvh_build.h"
+#include "kernels.h"
+
+namespace phg {
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;  // elements
+    void free_() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    hipError_t ensure(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        free_();
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return e;
+        p = static_cast<T*>(q); n = std::max<size_t>(count, 1);
+        return hipSuccess;
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool have_scene = false;
+    // scene
+    DevBuf<WideNode> nodes;
+    DevBuf<TriRec> tris;
+    DevBuf<phg_material> mats;
+    DevBuf<EmitterRec> emitters;
+    DevBuf<float> emit_cdf;
+    DevBuf<EmitTri> emit_tris;
+    SceneDev sc{};
+    uint32_t stack_need = 0, bvh_depth = 0, bvh_nodes = 0;
+    double bvh_build_ms = 0;
+    // film
+    DevBuf<double> film;
+    uint32_t film_w = 0, film_h = 0;
+    // wavefront buffers
+    DevBuf<uint64_t> rng;
+    DevBuf<float4> T, Wem, prevP, L, hits, shadow_c;
+    DevBuf<phg_ray> ext0, ext1, shadow;
+    DevBuf<uint32_t> ext_pid0, ext_pid1, shadow_pid;
+    DevBuf<uint32_t> counts;
+    DevBuf<unsigned long long> tcount;
+    DevBuf<uint32_t> errflags;
+    // intersect staging
+    DevBuf<phg_ray> q_rays;
+    DevBuf<phg_hit> q_hits;
+    DevBuf<uint32_t> q_nvis, q_ntri;
+    // stats
+    phg_stats stats{};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t cu_count = 256;
+};
+
+#define PHG_HIP(ctx, expr)                                                                        \
+    do {                                                                                          \
+        const hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                                   \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                       \
+            return e_ == hipErrorOutOfMemory ? PHG_ERR_OUT_OF_MEMORY : PHG_ERR_HIP;               \
+        }                                                                                         \
+    } while (0)
+
+// ---- host-side float math identical to the device (built with -ffp-contract=off) --------
+struct H3 { float x, y, z; };
+static inline H3 hsub(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline H3 hcross(H3 a, H3 b) { return H3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+static inline float hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline float htri_area(H3 a, H3 b, H3 c) { return std::sqrt(hdot(hcross(hsub(b, a), hsub(c, a)), hcross(hsub(b, a), hsub(c, a)))) * 0.5f; }
+static inline float hluminance(const float* c) {
+    return c[0] * 0.21263900587151027f + c[1] * 0.715168678767756f + c[2] * 0.07219231536073371f;
+}
+
+static phg_status set_err(Ctx* c, phg_status s, const std::string& msg) { c->err = msg; return s; }
+
+}  // namespace
+}  // namespace phg
+
+using namespace phg;
+
+extern "C" {
+
+phg_status phg_create(void** out_ctx, const phg_device_opts* opts) {
+    if (!out_ctx) return PHG_ERR_INVALID_ARG;
+    *out_ctx = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PHG_ERR_NO_DEVICE;
+    const int dev = opts ? opts->device : 0;
+    if (dev < 0 || dev >= ndev) return PHG_ERR_INVALID_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return PHG_ERR_NO_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PHG_ERR_NO_DEVICE;
+    Ctx* c = new Ctx();
+    c->device = dev;
+    c->cu_count = (uint32_t)prop.multiProcessorCount;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PHG_ERR_HIP;
+    }
+    for (auto& e : c->ev) (void)hipEventCreate(&e);
+    *out_ctx = c;
+    return PHG_OK;
+}
+
+phg_status phg_destroy(void* ctx) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c) return PHG_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->nodes.free_(); c->tris.free_(); c->mats.free_(); c->emitters.free_(); c->emit_cdf.free_(); c->emit_tris.free_();
+    c->film.free_(); c->rng.free_(); c->T.free_(); c->Wem.free_(); c->prevP.free_(); c->L.free_(); c->hits.free_();
+    c->shadow_c.free_(); c->ext0.free_(); c->ext1.free_(); c->shadow.free_(); c->ext_pid0.free_(); c->ext_pid1.free_();
+    c->shadow_pid.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
+    c->q_nvis.free_(); c->q_ntri.free_();
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PHG_OK;
+}
+
+const char* phg_last_error(void* ctx) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    return c ? c->err.c_str() : "null context";
+}
+
+phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_opts* bo) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !s) return PHG_ERR_INVALID_ARG;
+    if (s->num_tris == 0 || !s->verts || !s->tri_material || !s->tri_emitter || !s->materials || s->num_materials == 0)
+        return set_err(c, PHG_ERR_INVALID_ARG, "scene needs triangles, per-triangle material/emitter ids and materials");
+    if (s->num_emitters > 0 && (!s->emitters || !s->emitter_tris))
+        return set_err(c, PHG_ERR_INVALID_ARG, "emitters declared without emitter arrays");
+    for (uint32_t t = 0; t < s->num_tris; ++t) {
+        if (s->tri_material[t] >= s->num_materials) return set_err(c, PHG_ERR_INVALID_ARG, "triangle material id out of range");
+        if (s->tri_emitter[t] >= (int32_t)s->num_emitters) return set_err(c, PHG_ERR_INVALID_ARG, "triangle emitter id out of range");
+    }
+    for (uint32_t m = 0; m < s->num_materials; ++m)
+        if (s->materials[m].type > PHG_MAT_DIELECTRIC) return set_err(c, PHG_ERR_INVALID_ARG, "unknown material type");
+    for (uint32_t e = 0; e < s->num_emitters; ++e) {
+        const phg_emitter& em = s->emitters[e];
+        if (em.tri_count == 0 || (uint64_t)em.tri_begin + em.tri_count > s->num_emitter_tris)
+            return set_err(c, PHG_ERR_INVALID_ARG, "emitter triangle range out of bounds");
+        for (uint32_t k = 0; k < em.tri_count; ++k)
+            if (s->emitter_tris[em.tri_begin + k] >= s->num_tris) return set_err(c, PHG_ERR_INVALID_ARG, "emitter triangle id out of range");
+    }
+    PHG_HIP(c, hipSetDevice(c->device));
+    BvhBuildParams bp;
+    if (bo) {
+        if (bo->max_leaf_tris) bp.max_leaf_tris = bo->max_leaf_tris;
+        if (bo->sah_bins) bp.sah_bins = bo->sah_bins;
+        bp.nthreads = bo->nthreads;
+    }
+    BvhBuildResult bvh;
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+        bvh = build_wide_bvh(s->verts, s->num_tris, bp);
+    } catch (const std::exception& ex) {
+        return set_err(c, PHG_ERR_INVALID_ARG, ex.what());
+    }
+    c->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (bvh.stack_need > 64)
+        return set_err(c, PHG_ERR_UNSUPPORTED, "BVH needs a traversal stack deeper than 64 entries");
+    // triangles in leaf order
+    std::vector<TriRec> tr(s->num_tris);
+    std::vector<uint32_t> pos_of(s->num_tris);
+    for (uint32_t k = 0; k < s->num_tris; ++k) {
+        const uint32_t t = bvh.tri_order[k];
+        pos_of[t] = k;
+        TriRec& r = tr[k];
+        std::memcpy(r.a, s->verts + 9 * (size_t)t, 9 * sizeof(float));
+        r.prim = (int32_t)t;
+        r.mat = s->tri_material[t];
+        r.emitter = s->tri_emitter[t];
+    }
+    // emitters: power-favouring CDF (ESPowerFavoring::update, TPwcDistribution1D ctor)
+    const uint32_t ne = s->num_emitters;
+    std::vector<EmitterRec> er(std::max(ne, 1u));
+    std::vector<float> cdf(ne + 1, 0.0f);
+    std::vector<EmitTri> etris(std::max(s->num_emitter_tris, 1u));
+    for (uint32_t i = 0; i < s->num_emitter_tris; ++i) {
+        const uint32_t t = s->emitter_tris[i];
+        const float* v = s->verts + 9 * (size_t)t;
+        etris[i].tri = pos_of[t];
+        etris[i].area = htri_area(H3{v[0], v[1], v[2]}, H3{v[3], v[4], v[5]}, H3{v[6], v[7], v[8]});
+    }
+    if (ne > 0) {
+        std::vector<float> w(ne, 0.0f);
+        for (uint32_t e = 0; e < ne; ++e) {
+            const phg_emitter& em = s->emitters[e];
+            float total = 0.0f;
+            for (uint32_t k = 0; k < em.tri_count; ++k) {
+                const float flux = hluminance(em.radiance) * etris[em.tri_begin + k].area * 3.14159265358979323846f;
+                total += flux;
+            }
+            w[e] = total;
+        }
+        const float delta = 1.0f / (float)ne;
+        for (uint32_t i = 1; i <= ne; ++i) cdf[i] = cdf[i - 1] + w[i - 1] * delta;
+        const float rcpSum = 1.0f / cdf[ne];
+        cdf[0] = 0.0f; cdf[ne] = 1.0f;
+        if (std::isfinite(rcpSum)) { for (uint32_t i = 1; i < ne; ++i) cdf[i] *= rcpSum; }
+        else { for (uint32_t i = 1; i < ne; ++i) cdf[i] = (float)i / (float)ne; }
+        for (uint32_t e = 0; e < ne; ++e) {
+            const phg_emitter& em = s->emitters[e];
+            EmitterRec& r = er[e];
+            std::memset(&r, 0, sizeof(r));
+            r.radiance[0] = em.radiance[0]; r.radiance[1] = em.radiance[1]; r.radiance[2] = em.radiance[2];
+            r.pick_pdf = cdf[e + 1] - cdf[e];
+            r.tri_begin = em.tri_begin; r.tri_count = em.tri_count;
+        }
+    }
+    PHG_HIP(c, c->nodes.ensure(bvh.nodes.size()));
+    PHG_HIP(c, c->tris.ensure(tr.size()));
+    PHG_HIP(c, c->mats.ensure(s->num_materials));
+    PHG_HIP(c, c->emitters.ensure(er.size()));
+    PHG_HIP(c, c->emit_cdf.ensure(cdf.size()));
+    PHG_HIP(c, c->emit_tris.ensure(etris.size()));
+    PHG_HIP(c, hipMemcpy(c->nodes.p, bvh.nodes.data(), bvh.nodes.size() * sizeof(WideNode), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->tris.p, tr.data(), tr.size() * sizeof(TriRec), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->mats.p, s->materials, s->num_materials * sizeof(phg_material), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->emitters.p, er.data(), er.size() * sizeof(EmitterRec), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->emit_cdf.p, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->emit_tris.p, etris.data(), etris.size() * sizeof(EmitTri), hipMemcpyHostToDevice));
+    c->sc.nodes = c->nodes.p;
+    c->sc.tris = c->tris.p;
+    c->sc.mats = c->mats.p;
+    c->sc.emitters = c->emitters.p;
+    c->sc.emit_cdf = c->emit_cdf.p;
+    c->sc.emit_tris = c->emit_tris.p;
+    c->sc.num_emitters = ne;
+    c->sc.cam = s->camera;
+    c->stack_need = bvh.stack_need;
+    c->bvh_depth = bvh.depth;
+    c->bvh_nodes = (uint32_t)bvh.nodes.size();
+    c->have_scene = true;
+    c->stats.bvh_nodes = c->bvh_nodes;
+    c->stats.bvh_depth = c->bvh_depth;
+    c->stats.bvh_stack_need = c->stack_need;
+    c->stats.bvh_build_ms = c->bvh_build_ms;
+    return PHG_OK;
+}
+
+static phg_status do_intersect(Ctx* c, const phg_ray* rays, uint64_t n, phg_hit* hits, int query, int ptr_kind,
+                               uint32_t* nvis, uint32_t* ntri) {
+    if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
+    if (n == 0) return PHG_OK;
+    if (!rays || !hits || (query != PHG_QUERY_CLOSEST && query != PHG_QUERY_ANY) ||
+        (ptr_kind != PHG_PTR_HOST && ptr_kind != PHG_PTR_DEVICE))
+        return set_err(c, PHG_ERR_INVALID_ARG, "bad intersect arguments");
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, c->errflags.ensure(1));
+    PHG_HIP(c, hipMemsetAsync(c->errflags.p, 0, sizeof(uint32_t), c->stream));
+    const phg_ray* d_rays = rays;
+    phg_hit* d_hits = hits;
+    uint32_t* d_nvis = nullptr;
+    uint32_t* d_ntri = nullptr;
+    if (ptr_kind == PHG_PTR_HOST) {
+        PHG_HIP(c, c->q_rays.ensure(n));
+        PHG_HIP(c, c->q_hits.ensure(n));
+        PHG_HIP(c, hipMemcpyAsync(c->q_rays.p, rays, n * sizeof(phg_ray), hipMemcpyHostToDevice, c->stream));
+        d_rays = c->q_rays.p;
+        d_hits = c->q_hits.p;
+        if (nvis) {
+            PHG_HIP(c, c->q_nvis.ensure(n));
+            PHG_HIP(c, c->q_ntri.ensure(n));
+            d_nvis = c->q_nvis.p; d_ntri = c->q_ntri.p;
+        }
+    } else {
+        d_nvis = nvis; d_ntri = ntri;
+    }
+    PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    PHG_HIP(c, launch_intersect(c->sc, d_rays, n, d_hits, query == PHG_QUERY_ANY, d_nvis, d_ntri, c->errflags.p,
+                                c->stack_need, c->stream));
+    PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    if (ptr_kind == PHG_PTR_HOST) {
+        PHG_HIP(c, hipMemcpyAsync(hits, d_hits, n * sizeof(phg_hit), hipMemcpyDeviceToHost, c->stream));
+        if (nvis) {
+            PHG_HIP(c, hipMemcpyAsync(nvis, d_nvis, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            PHG_HIP(c, hipMemcpyAsync(ntri, d_ntri, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    uint32_t flags = 0;
+    PHG_HIP(c, hipMemcpyAsync(&flags, c->errflags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (query == PHG_QUERY_ANY) c->stats.shadow_ms = ms; else c->stats.trace_ms = ms;
+    if (flags) return set_err(c, PHG_ERR_UNSUPPORTED, "traversal stack overflow");
+    return PHG_OK;
+}
+
+phg_status phg_intersect(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits, int query, int ptr_kind) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c) return PHG_ERR_INVALID_ARG;
+    return do_intersect(c, rays, n, hits, query, ptr_kind, nullptr, nullptr);
+}
+
+phg_status phg_intersect_counted(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits, uint32_t* node_visits,
+                                 uint32_t* tri_tests, int ptr_kind) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c) return PHG_ERR_INVALID_ARG;
+    if (!node_visits || !tri_tests) return set_err(c, PHG_ERR_INVALID_ARG, "counter outputs required");
+    return do_intersect(c, rays, n, hits, PHG_QUERY_CLOSEST, ptr_kind, node_visits, tri_tests);
+}
+
+phg_status phg_film_clear(void* ctx, uint32_t width, uint32_t height) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || width == 0 || height == 0) return PHG_ERR_INVALID_ARG;
+    PHG_HIP(c, hipSetDevice(c->device));
+    const uint64_t n = 8ull * width * height;
+    PHG_HIP(c, c->film.ensure(n));
+    PHG_HIP(c, launch_fill_zero(c->film.p, n, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    c->film_w = width; c->film_h = height;
+    return PHG_OK;
+}
+
+phg_status phg_film_read(void* ctx, double* out, uint64_t out_len) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !out) return PHG_ERR_INVALID_ARG;
+    const uint64_t n = 8ull * c->film_w * c->film_h;
+    if (n == 0) return set_err(c, PHG_ERR_INVALID_ARG, "film not initialised");
+    if (out_len < n) return set_err(c, PHG_ERR_INVALID_ARG, "output buffer too small");
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, hipMemcpyAsync(out, c->film.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    return PHG_OK;
+}
+
+phg_status phg_render(void* ctx, const phg_render_opts* o) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !o) return PHG_ERR_INVALID_ARG;
+    if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
+    if (o->width != c->film_w || o->height != c->film_h)
+        return set_err(c, PHG_ERR_INVALID_ARG, "film size differs from render size; call phg_film_clear first");
+    const uint64_t npix = (uint64_t)o->width * o->height;
+    if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "image size out of range");
+    if (o->seed >= (1ull << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "seed must be below 2^24");
+    if ((uint64_t)o->spp_begin + o->spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "sample index range too large");
+    if (o->spp == 0) return PHG_OK;
+    PHG_HIP(c, hipSetDevice(c->device));
+    // samples per pass: fill ~4M paths in flight
+    uint32_t k = o->spp_per_pass;
+    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, (4ull << 20) / npix);
+    k = std::min(k, o->spp);
+    uint64_t cap = npix * k;
+    if (cap > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
+    PHG_HIP(c, c->rng.ensure(cap));
+    PHG_HIP(c, c->T.ensure(cap)); PHG_HIP(c, c->Wem.ensure(cap)); PHG_HIP(c, c->prevP.ensure(cap)); PHG_HIP(c, c->L.ensure(cap));
+    PHG_HIP(c, c->hits.ensure(cap)); PHG_HIP(c, c->shadow_c.ensure(cap));
+    PHG_HIP(c, c->ext0.ensure(cap)); PHG_HIP(c, c->ext1.ensure(cap)); PHG_HIP(c, c->shadow.ensure(cap));
+    PHG_HIP(c, c->ext_pid0.ensure(cap)); PHG_HIP(c, c->ext_pid1.ensure(cap)); PHG_HIP(c, c->shadow_pid.ensure(cap));
+    PHG_HIP(c, c->counts.ensure(2 * kCountSlots));
+    PHG_HIP(c, c->tcount.ensure(8));
+    PHG_HIP(c, c->errflags.ensure(1));
+    PHG_HIP(c, hipMemsetAsync(c->tcount.p, 0, 8 * sizeof(unsigned long long), c->stream));
+    PHG_HIP(c, hipMemsetAsync(c->errflags.p, 0, sizeof(uint32_t), c->stream));
+    PHG_HIP(c, launch_clear_counts(c->counts.p, 2 * kCountSlots, c->stream));
+    PathBuffers pb{};
+    pb.rng = c->rng.p; pb.T = c->T.p; pb.Wem = c->Wem.p; pb.prevP = c->prevP.p; pb.L = c->L.p;
+    pb.ext[0] = c->ext0.p; pb.ext[1] = c->ext1.p; pb.ext_pid[0] = c->ext_pid0.p; pb.ext_pid[1] = c->ext_pid1.p;
+    pb.hits = c->hits.p; pb.shadow = c->shadow.p; pb.shadow_pid = c->shadow_pid.p; pb.shadow_c = c->shadow_c.p;
+    pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = c->film.p;
+    const uint32_t count_trav = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) ? 1u : 0u;
+    const uint32_t grid_trace = c->cu_count * 8;   // persistent-style grid-stride launches
+    const uint32_t grid_shade = c->cu_count * 4;
+    double t_trace = 0, t_shadow = 0, t_shade = 0;
+    uint64_t rays_primary = 0;
+    const auto wall0 = std::chrono::steady_clock::now();
+    for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
+        const uint32_t kk = std::min(k, o->spp - s0);
+        PassParams pp{};
+        pp.width = o->width; pp.height = o->height; pp.npix = (uint32_t)npix;
+        pp.paths = (uint32_t)(npix * kk); pp.sample_begin = o->spp_begin + s0; pp.max_bounces = o->max_bounces;
+        pp.seed = o->seed;
+        rays_primary += pp.paths;
+        PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        PHG_HIP(c, launch_generate(c->sc, pp, pb, c->stream));
+        PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        float ms = 0; PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
+        for (uint32_t b = 0;; ++b) {
+            PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+            PHG_HIP(c, launch_extend(c->sc, pb, b, grid_trace, c->stack_need, count_trav, c->stream));
+            PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+            PHG_HIP(c, launch_shade(c->sc, pp, pb, b, grid_shade, c->stream));
+            PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
+            PHG_HIP(c, launch_shadow(c->sc, pb, b, grid_trace, c->stack_need, count_trav, c->stream));
+            PHG_HIP(c, hipEventRecord(c->ev[3], c->stream));
+            uint32_t next = 0;
+            PHG_HIP(c, hipMemcpyAsync(&next, c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            PHG_HIP(c, hipStreamSynchronize(c->stream));
+            float a = 0, bb = 0, cc = 0;
+            PHG_HIP(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+            PHG_HIP(c, hipEventElapsedTime(&bb, c->ev[1], c->ev[2]));
+            PHG_HIP(c, hipEventElapsedTime(&cc, c->ev[2], c->ev[3]));
+            t_trace += a; t_shade += bb; t_shadow += cc;
+            if (next == 0 || b >= o->max_bounces) break;
+        }
+        PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        PHG_HIP(c, launch_film(pp, pb, kk, c->stream));
+        PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
+    }
+    uint32_t flags = 0;
+    unsigned long long tc[8];
+    PHG_HIP(c, hipMemcpyAsync(&flags, c->errflags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, hipMemcpyAsync(tc, c->tcount.p, sizeof(tc), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    c->stats.render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    c->stats.paths = npix * (uint64_t)o->spp;
+    c->stats.rays_primary = rays_primary;
+    c->stats.rays_extension = tc[4];
+    c->stats.rays_shadow = tc[5];
+    c->stats.node_visits = tc[0] + tc[2];
+    c->stats.tri_tests = tc[1] + tc[3];
+    c->stats.trace_ms = t_trace;
+    c->stats.shadow_ms = t_shadow;
+    c->stats.shade_ms = t_shade;
+    if (flags) return set_err(c, PHG_ERR_UNSUPPORTED, "traversal stack overflow");
+    return PHG_OK;
+}
+
+phg_status phg_get_stats(void* ctx, phg_stats* out) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !out) return PHG_ERR_INVALID_ARG;
+    *out = c->stats;
+    return PHG_OK;
+}
+
+}  // extern "C"

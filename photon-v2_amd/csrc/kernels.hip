@@ -1,0 +1,535 @@
+// gfx950 kernels of the BNEEPT path: 4-wide BVH traversal (closest and any hit), and the
+// wavefront stages generate -> extend -> shade -> shadow -> film.
+//
+// Traversal: one ray per lane, 128-byte node fetches (8 x 16-byte loads), child boxes
+// tested against the shrinking segment with the reference's robust slab test, a per-lane
+// traversal stack in LDS laid out [depth][lane] (bank-conflict free), near-first order.
+// Queues between stages are compacted with a wave64 ballot + popcount prefix and one
+// atomic per wave.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "pt_device.h"
+
+namespace phg {
+
+// ---------------------------------------------------------------------------------------
+// launch geometry helpers
+// ---------------------------------------------------------------------------------------
+// Blocks are handed to the 8 XCDs round robin.  Remap so that consecutive logical blocks
+// (coherent rays) run on the same XCD and share its L2.
+__device__ __forceinline__ uint32_t xcd_block_id() {
+    const uint32_t b = blockIdx.x, n = gridDim.x;
+    if ((n & 7u) != 0u) return b;
+    return (b & 7u) * (n >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
+    uint32_t base = 0;
+    if (pred && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    const uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64u - lane)));
+    return base + (uint32_t)__popcll(below);
+}
+
+// ---------------------------------------------------------------------------------------
+// 4-wide BVH traversal
+// ---------------------------------------------------------------------------------------
+struct TraceResult { float t; int32_t tri; float a, b, c; };
+
+template <int STACK, bool ANY, bool COUNT>
+__device__ __forceinline__ TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris,
+                                                V3 o, V3 d, float tmin, float tmax, int32_t* __restrict__ stk,
+                                                uint32_t stride, uint32_t* nvis, uint32_t* ntri, uint32_t* overflow) {
+    TraceResult r; r.t = 0.0f; r.tri = -1; r.a = 0.0f; r.b = 0.0f; r.c = 0.0f;
+    const V3 rcp = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool nx = d.x < 0.0f, ny = d.y < 0.0f, nz = d.z < 0.0f;
+    constexpr float mult = 1.1920928955078125e-07f * 2.0f + 1.0f;  // TAABB3D.ipp:417
+    float tb = tmax;
+    int sp = 0;
+    int32_t cur = 0;
+    while (true) {
+        if (cur >= 0) {
+            const float4* p = reinterpret_cast<const float4*>(nodes + cur);
+            const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+            const int4 ch = reinterpret_cast<const int4*>(p)[6];
+            if (COUNT) (*nvis)++;
+            const float bl[4][6] = {{lx.x, hx.x, ly.x, hy.x, lz.x, hz.x}, {lx.y, hx.y, ly.y, hy.y, lz.y, hz.y},
+                                    {lx.z, hx.z, ly.z, hy.z, lz.z, hz.z}, {lx.w, hx.w, ly.w, hy.w, lz.w, hz.w}};
+            const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+            float key[4];
+            int32_t kid[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                // robust slab (TAABB3D.ipp:390-420) against [tmin, tb]
+                float t0 = tmin, t1 = tb;
+                {
+                    const float mind = ((nx ? bl[k][1] : bl[k][0]) - o.x) * rcp.x;
+                    const float maxd = ((nx ? bl[k][0] : bl[k][1]) - o.x) * rcp.x;
+                    t0 = mind > t0 ? mind : t0; t1 = maxd < t1 ? maxd : t1;
+                }
+                {
+                    const float mind = ((ny ? bl[k][3] : bl[k][2]) - o.y) * rcp.y;
+                    const float maxd = ((ny ? bl[k][2] : bl[k][3]) - o.y) * rcp.y;
+                    t0 = mind > t0 ? mind : t0; t1 = maxd < t1 ? maxd : t1;
+                }
+                {
+                    const float mind = ((nz ? bl[k][5] : bl[k][4]) - o.z) * rcp.z;
+                    const float maxd = ((nz ? bl[k][4] : bl[k][5]) - o.z) * rcp.z;
+                    t0 = mind > t0 ? mind : t0; t1 = maxd < t1 ? maxd : t1;
+                }
+                const bool hit = (t0 <= t1 * mult) && cc[k] != kEmptyChild;
+                key[k] = hit ? t0 : INFINITY;
+                kid[k] = cc[k];
+            }
+            // sorting network on (key, kid), ascending
+#define PHG_CSWAP(i, j)                                                   \
+    if (key[j] < key[i]) {                                                \
+        const float tk = key[i]; key[i] = key[j]; key[j] = tk;            \
+        const int32_t ti = kid[i]; kid[i] = kid[j]; kid[j] = ti;          \
+    }
+            PHG_CSWAP(0, 1) PHG_CSWAP(2, 3) PHG_CSWAP(0, 2) PHG_CSWAP(1, 3) PHG_CSWAP(1, 2)
+#undef PHG_CSWAP
+            if (key[0] == INFINITY) {
+                if (sp == 0) break;
+                cur = stk[(--sp) * stride];
+                continue;
+            }
+#pragma unroll
+            for (int k = 3; k >= 1; --k) {
+                if (key[k] != INFINITY) {
+                    if (sp < STACK) stk[(sp++) * stride] = kid[k];
+                    else *overflow = 1u;
+                }
+            }
+            cur = kid[0];
+        } else {
+            const uint32_t first = (uint32_t)cur & ((1u << kLeafFirstBits) - 1u);
+            const uint32_t cnt = (((uint32_t)cur >> kLeafFirstBits) & 7u) + 1u;
+            if (COUNT) (*ntri) += cnt;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                V3 a, b, c;
+                load_tri(tris, first + i, &a, &b, &c);
+                float t, ba, bb, bc;
+                if (wt_intersect(a, b, c, o, d, tmin, tb, &t, &ba, &bb, &bc)) {
+                    tb = t;
+                    r.t = t; r.tri = (int32_t)(first + i); r.a = ba; r.b = bb; r.c = bc;
+                    if (ANY) return r;
+                }
+            }
+            if (sp == 0) break;
+            cur = stk[(--sp) * stride];
+        }
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched ray queries (phg_intersect): rays as phg_ray, hits as phg_hit with input ids.
+// ---------------------------------------------------------------------------------------
+template <int STACK, bool ANY, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_intersect(SceneDev sc, const phg_ray* __restrict__ rays, uint64_t n,
+                                                          phg_hit* __restrict__ hits, uint32_t* __restrict__ nvis_out,
+                                                          uint32_t* __restrict__ ntri_out, uint32_t* __restrict__ flags) {
+    __shared__ int32_t stack_mem[STACK * kTraceBlock];
+    const uint64_t i = (uint64_t)xcd_block_id() * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    const float4 r0 = rp[0], r1 = rp[1];
+    uint32_t nvis = 0, ntri = 0, ovf = 0;
+    const TraceResult tr = traverse<STACK, ANY, COUNT>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                                       stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+    phg_hit h;
+    if (tr.tri >= 0) {
+        h.t = ANY ? 0.0f : tr.t;
+        h.prim = ANY ? 1 : sc.tris[tr.tri].prim;
+        h.b = ANY ? 0.0f : tr.b;
+        h.c = ANY ? 0.0f : tr.c;
+    } else {
+        h.t = 0.0f; h.prim = -1; h.b = 0.0f; h.c = 0.0f;
+    }
+    hits[i] = h;
+    if (COUNT) { nvis_out[i] = nvis; ntri_out[i] = ntri; }
+    if (ovf) atomicOr(flags, 1u);
+}
+
+// ---------------------------------------------------------------------------------------
+// Wavefront BNEEPT
+// ---------------------------------------------------------------------------------------
+enum : uint32_t { kFlagDieAfterEmission = 1u };
+
+
+
+__device__ __forceinline__ void path_ids(const PassParams& pp, uint32_t pid, uint32_t* pixel, uint32_t* sample) {
+    *pixel = pid % pp.npix;
+    *sample = pp.sample_begin + pid / pp.npix;
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, PassParams pp, PathBuffers pb) {
+    const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
+    if (i == 0) pb.counts[0] = pp.paths;
+    if (i >= pp.paths) return;
+    uint32_t pixel, sample;
+    path_ids(pp, i, &pixel, &sample);
+    Stream s = stream_init(path_key(pp.seed, pixel, sample, pp.npix));
+    const float rx = (float)(pixel % pp.width) + s.u();
+    const float ry = (float)(pixel / pp.width) + s.u();
+    // PinholeCamera::genReceiveRayDir with the horizontal-FOV sensor of SingleLensObserver
+    const phg_camera& c = sc.cam;
+    const float cx = (2.0f * (rx / (float)pp.width) - 1.0f) * c.tan_half_fov;
+    const float cy = (2.0f * (ry / (float)pp.height) - 1.0f) * c.tan_half_fov * ((float)pp.height / (float)pp.width);
+    const V3 right = mk(c.right[0], c.right[1], c.right[2]);
+    const V3 up = mk(c.up[0], c.up[1], c.up[2]);
+    const V3 fwd = mk(c.forward[0], c.forward[1], c.forward[2]);
+    const V3 d = normalize(right * cx + up * cy + fwd);
+    float4* rq = reinterpret_cast<float4*>(pb.ext[0] + i);
+    rq[0] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
+    rq[1] = make_float4(d.x, d.y, d.z, kFltMax);
+    pb.ext_pid[0][i] = i;
+    pb.rng[i] = s.state;
+    pb.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    pb.Wem[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pb.prevP[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+    pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
+    __shared__ int32_t stack_mem[STACK * kTraceBlock];
+    const uint32_t n = pb.counts[bounce % kCountSlots];
+    // the counters the shade stage of this bounce pushes to start from zero
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.counts[(bounce + 1) % kCountSlots] = 0u;
+        pb.counts[kCountSlots + bounce % kCountSlots] = 0u;
+        if (bounce > 0) atomicAdd(&pb.traversal_counters[4], (unsigned long long)n);
+    }
+    const uint32_t q = bounce & 1u;
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    uint32_t nvis = 0, ntri = 0, ovf = 0;
+    for (uint32_t i = xcd_block_id() * kTraceBlock + threadIdx.x; i < n; i += stride) {
+        const float4* rp = reinterpret_cast<const float4*>(pb.ext[q] + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        TraceResult tr;
+        if (count_traversal)
+            tr = traverse<STACK, false, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                              stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+        else
+            tr = traverse<STACK, false, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                               stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+        pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
+    }
+    if (count_traversal) {
+        atomicAdd(&pb.traversal_counters[0], (unsigned long long)nvis);
+        atomicAdd(&pb.traversal_counters[1], (unsigned long long)ntri);
+    }
+    if (ovf) atomicOr(pb.error_flags, 1u);
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
+    __shared__ int32_t stack_mem[STACK * kTraceBlock];
+    const uint32_t n = pb.counts[kCountSlots + bounce % kCountSlots];
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&pb.traversal_counters[5], (unsigned long long)n);
+    uint32_t nvis = 0, ntri = 0, ovf = 0;
+    for (uint32_t i = xcd_block_id() * kTraceBlock + threadIdx.x; i < n; i += stride) {
+        const float4* rp = reinterpret_cast<const float4*>(pb.shadow + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        TraceResult tr;
+        if (count_traversal)
+            tr = traverse<STACK, true, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                             stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+        else
+            tr = traverse<STACK, true, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                              stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+        if (tr.tri < 0) {
+            const uint32_t pid = pb.shadow_pid[i];
+            const float4 c = pb.shadow_c[i];
+            float4 L = pb.L[pid];
+            L.x += c.x; L.y += c.y; L.z += c.z;
+            pb.L[pid] = L;
+        }
+    }
+    if (count_traversal) {
+        atomicAdd(&pb.traversal_counters[2], (unsigned long long)nvis);
+        atomicAdd(&pb.traversal_counters[3], (unsigned long long)ntri);
+    }
+    if (ovf) atomicOr(pb.error_flags, 1u);
+}
+
+
+// One bounce-loop iteration of BNEEPTEstimator::estimate (BNEEPTEstimator.cpp:84-255) for
+// every path whose extension ray of this bounce is queued: emission found by the previous
+// BSDF sample (MIS against NEE), then NEE at this vertex (shadow ray queued with its
+// contribution), then the BSDF sample that extends the path, throughput update and
+// Russian roulette.  Whole waves iterate together because the queue pushes use ballots.
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams pp, PathBuffers pb, uint32_t bounce) {
+    const uint32_t n = pb.counts[bounce % kCountSlots];
+    const uint32_t qin = bounce & 1u, qout = qin ^ 1u;
+    const uint32_t stride = gridDim.x * kShadeBlock;
+    const uint32_t iend = (n + 63u) & ~63u;
+    for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < iend; i += stride) {
+        bool push_ext = false, push_shadow = false;
+        uint32_t pid = 0;
+        V3 ext_o = mk(0.0f, 0.0f, 0.0f), ext_d = mk(0.0f, 0.0f, 0.0f);
+        V3 sh_o = mk(0.0f, 0.0f, 0.0f), sh_d = mk(0.0f, 0.0f, 0.0f);
+        float sh_t = 0.0f;
+        RGB sh_c = rgb1(0.0f);
+        if (i < n) {
+            pid = pb.ext_pid[qin][i];
+            const float4 h = pb.hits[i];
+            const int32_t tri = __float_as_int(h.x);
+            const float4 rd = reinterpret_cast<const float4*>(pb.ext[qin] + i)[1];
+            const V3 dir = mk(rd.x, rd.y, rd.z);
+            bool stop = tri < 0;
+            float4 Lacc = pb.L[pid];
+            bool l_dirty = false;
+            SurfaceHit X;
+            TriRec tr;
+            TriVerts tv;
+            if (!stop) {
+                tr = sc.tris[tri];
+                tv.a = mk(tr.a[0], tr.a[1], tr.a[2]);
+                tv.b = mk(tr.b[0], tr.b[1], tr.b[2]);
+                tv.c = mk(tr.c[0], tr.c[1], tr.c[2]);
+                X = make_hit(tv.a, tv.b, tv.c, (uint32_t)tri, h.y, h.z, h.w);
+                if (!side_agreed(X, dir)) stop = true;  // SurfaceTracer.h:126-143
+            }
+            if (!stop && tr.emitter >= 0) {
+                // DiffuseSurfaceEmitterBase::evalEnergyFromSurface: front face only
+                const EmitterRec em = sc.emitters[tr.emitter];
+                const bool front = dot(neg(dir), X.Ns) > 0.0f;
+                const RGB Le = front ? rgb3(em.radiance[0], em.radiance[1], em.radiance[2]) : rgb1(0.0f);
+                if (bounce == 0) {
+                    // zero-bounce emission (SurfaceTracer.h:239-259)
+                    Lacc.x += Le.r; Lacc.y += Le.g; Lacc.z += Le.b; l_dirty = true;
+                } else if (!rgb_zero(Le)) {
+                    // BSDF-sampled emitter hit, MIS against NEE (BNEEPTEstimator.cpp:178-217)
+                    const float4 prevP = pb.prevP[pid];
+                    const float4 wem = pb.Wem[pid];
+                    const V3 Xp = mk(prevP.x, prevP.y, prevP.z);
+                    // neeSamplePdfWUnoccluded -> ESPowerFavoring::calcDirectPdf
+                    float neePdf = 0.0f;
+                    const V3 e2t = Xp - X.P;
+                    if (dot(e2t, X.Ns) > 0.0f) {
+                        const float pdfA = 1.0f / tri_area(tv.a, tv.b, tv.c);
+                        float pdfW = pdfA_to_pdfW(pdfA, e2t, X.Ns);
+                        const float gp = em.tri_count >= 2 ? 1.0f / (float)em.tri_count : 1.0f;
+                        pdfW = pdfW * gp;
+                        if (pdf_ok(pdfW)) {
+                            const float pdf = pdfW * em.pick_pdf;
+                            neePdf = pdf_ok(pdf) ? pdf : 0.0f;
+                        }
+                    }
+                    const float bsdfPdf = wem.w;
+                    if (pdf_ok(bsdfPdf)) {
+                        const float w = mis_power(bsdfPdf, neePdf);
+                        RGB weight = muls(rgb3(wem.x, wem.y, wem.z), w);
+                        weight = rational_clamp(weight);
+                        Lacc.x += Le.r * weight.r; Lacc.y += Le.g * weight.g; Lacc.z += Le.b * weight.b;
+                        l_dirty = true;
+                    }
+                }
+            }
+            if (!stop && bounce > 0) {
+                const uint32_t flags = __float_as_uint(pb.prevP[pid].w);
+                if (flags & kFlagDieAfterEmission) stop = true;
+            }
+            if (!stop && bounce >= pp.max_bounces) stop = true;
+            if (!stop) {
+                uint32_t pixel, sample;
+                path_ids(pp, pid, &pixel, &sample);
+                const uint64_t key = path_key(pp.seed, pixel, sample, pp.npix);
+                Stream s = stream_resume(key, pb.rng[pid]);
+                float u[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) u[k] = s.u();
+                const phg_material m = sc.mats[tr.mat];
+                if (m.type == PHG_MAT_ABSORBER) stop = true;
+                if (!stop) {
+                    const float4 T4 = pb.T[pid];
+                    const RGB T = rgb3(T4.x, T4.y, T4.z);
+                    float rrScale = T4.w;
+                    const V3 V = dir * -1.0f;
+                    // ---- NEE (TDirectLightEstimator.ipp:70-102, BNEEPTEstimator.cpp:103-149)
+                    if (sc.num_emitters > 0) {
+                        const uint32_t ei = cdf_sample(sc.emit_cdf, sc.num_emitters, u[0]);
+                        const EmitterRec em = sc.emitters[ei];
+                        uint32_t k = (uint32_t)(u[1] * (float)em.tri_count);
+                        if (k >= em.tri_count) k = em.tri_count - 1u;
+                        const EmitTri et = sc.emit_tris[em.tri_begin + k];
+                        TriVerts te;
+                        load_tri(sc.tris, et.tri, &te.a, &te.b, &te.c);
+                        const float A = sqrtf(u[2]);
+                        const float B = u[3];
+                        const float e0 = 1.0f - A, e1 = A * (1.0f - B), e2 = B * A;
+                        const float pdfA = 1.0f / et.area;
+                        const SurfaceHit Xe = make_hit(te.a, te.b, te.c, et.tri, e0, e1, e2);
+                        const V3 e2t = X.P - Xe.P;
+                        if (dot(e2t, Xe.Ns) > 0.0f) {
+                            float pdf = pdfA_to_pdfW(pdfA, e2t, Xe.Ns);
+                            if (em.tri_count >= 2) pdf = pdf * (1.0f / (float)em.tri_count);
+                            pdf = pdf * em.pick_pdf;
+                            V3 so, sd; float st;
+                            if (pdf_ok(pdf) && side_agreed(X, Xe.P - X.P) && mutual_escape(X, tv, Xe, te, &so, &sd, &st)) {
+                                const V3 Ld = normalize(Xe.P - X.P);
+                                const BsdfEval be = bsdf_eval(m, X, Ld, V);
+                                if (be.ok) {
+                                    float bp = bsdf_pdf(m, X, Ld, V);
+                                    if (!pdf_ok(bp)) bp = 0.0f;
+                                    const float w = mis_power(pdf, bp);
+                                    RGB weight = muls(be.f, absdot(X.Ns, Ld));
+                                    weight = mulc(weight, T);
+                                    weight = muls(weight, w / pdf);
+                                    weight = rational_clamp(weight);
+                                    sh_c = rgb3(em.radiance[0] * weight.r, em.radiance[1] * weight.g, em.radiance[2] * weight.b);
+                                    if (!rgb_zero(sh_c)) { push_shadow = true; sh_o = so; sh_d = sd; sh_t = st; }
+                                }
+                            }
+                        }
+                    }
+                    // ---- BSDF sample + extension (BNEEPTEstimator.cpp:151-255)
+                    const BsdfSample bs = bsdf_sample(m, X, V, u[4], u[5], u[6]);
+                    if (bs.ok) {
+                        V3 off, ud;
+                        iterative_offset(X, tv, bs.L, &off, &ud);
+                        const float bsdfPdf = bsdf_pdf(m, X, bs.L, V);
+                        const RGB wem = mulc(bs.w, T);
+                        RGB Tn = mulc(T, bs.w);
+                        rrScale /= bs.relIor * bs.relIor;
+                        bool die = false;
+                        if (bounce >= 3) {  // RussianRoulette.h:30-55
+                            const float surv = safe_clamp(luminance(muls(Tn, rrScale)), 0.0f, 0.95f);
+                            if (u[7] < surv) Tn = muls(Tn, 1.0f / surv);
+                            else die = true;
+                        }
+                        if (!die && rgb_zero(Tn)) die = true;
+                        pb.T[pid] = make_float4(Tn.r, Tn.g, Tn.b, rrScale);
+                        pb.Wem[pid] = make_float4(wem.r, wem.g, wem.b, bsdfPdf);
+                        pb.prevP[pid] = make_float4(X.P.x, X.P.y, X.P.z, __uint_as_float(die ? kFlagDieAfterEmission : 0u));
+                        pb.rng[pid] = s.state;
+                        push_ext = true;
+                        ext_o = X.P + off;
+                        ext_d = ud;
+                    }
+                }
+            }
+            if (l_dirty) pb.L[pid] = Lacc;
+        }
+        const uint32_t slot_e = wave_push(&pb.counts[(bounce + 1) % kCountSlots], push_ext);
+        if (push_ext) {
+            float4* rq = reinterpret_cast<float4*>(pb.ext[qout] + slot_e);
+            rq[0] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0f);
+            rq[1] = make_float4(ext_d.x, ext_d.y, ext_d.z, kFltMax);
+            pb.ext_pid[qout][slot_e] = pid;
+        }
+        const uint32_t slot_s = wave_push(&pb.counts[kCountSlots + bounce % kCountSlots], push_shadow);
+        if (push_shadow) {
+            float4* rq = reinterpret_cast<float4*>(pb.shadow + slot_s);
+            rq[0] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
+            rq[1] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_t);
+            pb.shadow_pid[slot_s] = pid;
+            pb.shadow_c[slot_s] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.0f);
+        }
+    }
+}
+
+// FP64 film accumulation, box filter, owner computes: each pixel sums its own samples of
+// this pass in sample order (HdrRgbFilm::addRgbSample, HdrRgbFilm.cpp:82-122).
+__global__ __launch_bounds__(kShadeBlock) void k_film(PassParams pp, PathBuffers pb, uint32_t spp_in_pass) {
+    const uint32_t p = blockIdx.x * kShadeBlock + threadIdx.x;
+    if (p >= pp.npix) return;
+    double2* f = reinterpret_cast<double2*>(pb.film + 8 * (size_t)p);
+    double2 f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
+    for (uint32_t s = 0; s < spp_in_pass; ++s) {
+        const float4 L = pb.L[(size_t)s * pp.npix + p];
+        f0.x += L.x; f0.y += L.y; f1.x += L.z; f1.y += 1.0;
+        f2.x += (double)L.x * L.x; f2.y += (double)L.y * L.y; f3.x += (double)L.z * L.z;
+    }
+    f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
+}
+
+__global__ void k_clear_counts(uint32_t* counts, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counts[i] = 0u;
+}
+
+__global__ void k_fill_double(double* p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------------------
+// host launch wrappers
+// ---------------------------------------------------------------------------------------
+static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+template <int STACK>
+static hipError_t launch_intersect_t(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
+                                     uint32_t* nvis, uint32_t* ntri, uint32_t* flags, hipStream_t st) {
+    const uint32_t nb = blocks_for(n, kTraceBlock);
+    if (nvis) {
+        hipLaunchKernelGGL((k_intersect<STACK, false, true>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags);
+    } else if (any) {
+        hipLaunchKernelGGL((k_intersect<STACK, true, false>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags);
+    } else {
+        hipLaunchKernelGGL((k_intersect<STACK, false, false>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
+                            uint32_t* nvis, uint32_t* ntri, uint32_t* flags, uint32_t stack, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (stack <= 32) return launch_intersect_t<32>(sc, rays, n, hits, any, nvis, ntri, flags, st);
+    if (stack <= 64) return launch_intersect_t<64>(sc, rays, n, hits, any, nvis, ntri, flags, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_generate(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, hipStream_t st) {
+    hipLaunchKernelGGL(k_generate, dim3(blocks_for(pp.paths, kShadeBlock)), dim3(kShadeBlock), 0, st, sc, pp, pb);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
+                         uint32_t count, hipStream_t st) {
+    if (stack <= 32) hipLaunchKernelGGL(k_extend<32>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
+    else if (stack <= 64) hipLaunchKernelGGL(k_extend<64>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
+                         uint32_t count, hipStream_t st) {
+    if (stack <= 32) hipLaunchKernelGGL(k_shadow<32>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
+    else if (stack <= 64) hipLaunchKernelGGL(k_shadow<64>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
+    return hipGetLastError();
+}
+
+hipError_t launch_film(const PassParams& pp, const PathBuffers& pb, uint32_t spp_in_pass, hipStream_t st) {
+    hipLaunchKernelGGL(k_film, dim3(blocks_for(pp.npix, kShadeBlock)), dim3(kShadeBlock), 0, st, pp, pb, spp_in_pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_counts(uint32_t* counts, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_counts, dim3(blocks_for(n, 256)), dim3(256), 0, st, counts, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(double* p, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_double, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace phg

@@ -1,0 +1,347 @@
+// Procedural workloads as flattened scenes (phg_scene_generate).
+//
+// Cornell box: 5 walls + 2 boxes (34 triangles) + a 0.25 x 0.25 ceiling light (2 triangles).
+// Procedural: displaced icosphere instances + a tessellated ground + area-light quads.
+// Occluded room: a room lit only through a 0.1-wide slit in an interior wall.
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <new>
+#include <utility>
+#include <vector>
+
+#include "../../include/phg.h"
+
+namespace {
+
+struct F3 { float x, y, z; };
+static F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+static F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static F3 mul(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+static F3 crs(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static F3 nrm(F3 a) { const float l = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); return mul(a, 1.0f / l); }
+
+// splitmix64-based deterministic RNG for scene parameters
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ULL + 0x1234567ULL) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double u() { return (double)(next() >> 11) * 0x1p-53; }
+    float uf(float a, float b) { return (float)(a + (b - a) * u()); }
+};
+
+// 3-D gradient noise on an integer lattice with hashed gradients, and its fractal sum.
+static uint32_t hash3(int x, int y, int z, uint32_t seed) {
+    uint32_t h = seed * 0x27d4eb2dU;
+    h ^= (uint32_t)x * 0x8da6b343U; h = (h << 13) | (h >> 19);
+    h ^= (uint32_t)y * 0xd8163841U; h = (h << 13) | (h >> 19);
+    h ^= (uint32_t)z * 0xcb1ab31fU;
+    h ^= h >> 16; h *= 0x85ebca6bU; h ^= h >> 13; h *= 0xc2b2ae35U; h ^= h >> 16;
+    return h;
+}
+static float grad(uint32_t h, float x, float y, float z) {
+    switch (h % 12u) {
+        case 0: return x + y; case 1: return -x + y; case 2: return x - y; case 3: return -x - y;
+        case 4: return x + z; case 5: return -x + z; case 6: return x - z; case 7: return -x - z;
+        case 8: return y + z; case 9: return -y + z; case 10: return y - z; default: return -y - z;
+    }
+}
+static float fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+static float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+static float gnoise(F3 p, uint32_t seed) {
+    const int xi = (int)std::floor(p.x), yi = (int)std::floor(p.y), zi = (int)std::floor(p.z);
+    const float xf = p.x - xi, yf = p.y - yi, zf = p.z - zi;
+    const float u = fade(xf), v = fade(yf), w = fade(zf);
+    float c[8];
+    for (int k = 0; k < 8; ++k) {
+        const int dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+        c[k] = grad(hash3(xi + dx, yi + dy, zi + dz, seed), xf - dx, yf - dy, zf - dz);
+    }
+    const float x00 = lerpf(c[0], c[1], u), x10 = lerpf(c[2], c[3], u);
+    const float x01 = lerpf(c[4], c[5], u), x11 = lerpf(c[6], c[7], u);
+    return lerpf(lerpf(x00, x10, v), lerpf(x01, x11, v), w);
+}
+static float fbm(F3 p, uint32_t seed, int octaves) {
+    float sum = 0.0f, amp = 0.5f;
+    for (int o = 0; o < octaves; ++o) {
+        sum += amp * gnoise(p, seed + (uint32_t)o * 101u);
+        p = mul(p, 2.0f);
+        amp *= 0.5f;
+    }
+    return sum;
+}
+
+struct Icosphere {
+    std::vector<F3> v;
+    std::vector<uint32_t> f;  // 3 per face
+};
+
+static Icosphere make_icosphere(uint32_t subdiv) {
+    Icosphere s;
+    const float t = (1.0f + std::sqrt(5.0f)) / 2.0f;
+    const F3 base[12] = {f3(-1, t, 0), f3(1, t, 0), f3(-1, -t, 0), f3(1, -t, 0), f3(0, -1, t), f3(0, 1, t),
+                         f3(0, -1, -t), f3(0, 1, -t), f3(t, 0, -1), f3(t, 0, 1), f3(-t, 0, -1), f3(-t, 0, 1)};
+    for (const F3& p : base) s.v.push_back(nrm(p));
+    const uint32_t faces[60] = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2,
+                                10, 7, 6, 7, 1, 8, 3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5,
+                                2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+    s.f.assign(faces, faces + 60);
+    for (uint32_t it = 0; it < subdiv; ++it) {
+        std::map<std::pair<uint32_t, uint32_t>, uint32_t> mid;
+        auto midpoint = [&](uint32_t a, uint32_t b) {
+            const auto key = a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+            auto found = mid.find(key);
+            if (found != mid.end()) return found->second;
+            const uint32_t id = (uint32_t)s.v.size();
+            s.v.push_back(nrm(mul(add(s.v[a], s.v[b]), 0.5f)));
+            mid[key] = id;
+            return id;
+        };
+        std::vector<uint32_t> nf;
+        nf.reserve(s.f.size() * 4);
+        for (size_t i = 0; i < s.f.size(); i += 3) {
+            const uint32_t a = s.f[i], b = s.f[i + 1], c = s.f[i + 2];
+            const uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+            const uint32_t q[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+            nf.insert(nf.end(), q, q + 12);
+        }
+        s.f.swap(nf);
+    }
+    return s;
+}
+
+struct SceneBuf {
+    phg_scene_host host;
+    std::vector<float> verts;
+    std::vector<uint32_t> tri_mat;
+    std::vector<int32_t> tri_emit;
+    std::vector<phg_material> mats;
+    std::vector<phg_emitter> emits;
+    std::vector<uint32_t> emit_tris;
+
+    uint32_t add_material(uint32_t type, F3 c, float rough, float ior_out, float ior_in) {
+        phg_material m{};
+        m.type = type; m.rgb[0] = c.x; m.rgb[1] = c.y; m.rgb[2] = c.z;
+        m.roughness = rough; m.ior_outer = ior_out; m.ior_inner = ior_in;
+        mats.push_back(m);
+        return (uint32_t)mats.size() - 1;
+    }
+    void tri(F3 a, F3 b, F3 c, uint32_t mat, int32_t emitter = -1) {
+        const float v[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+        verts.insert(verts.end(), v, v + 9);
+        tri_mat.push_back(mat);
+        tri_emit.push_back(emitter);
+        if (emitter >= 0) emit_tris.push_back((uint32_t)tri_mat.size() - 1);
+    }
+    // quad p0 p1 p2 p3 (counter-clockwise seen from the side its normal points to)
+    void quad(F3 p0, F3 p1, F3 p2, F3 p3, uint32_t mat, int32_t emitter = -1) {
+        tri(p0, p1, p2, mat, emitter);
+        tri(p0, p2, p3, mat, emitter);
+    }
+    // tessellated quad with nu x nv cells
+    void grid(F3 p0, F3 eu, F3 ev, uint32_t nu, uint32_t nv, uint32_t mat) {
+        for (uint32_t j = 0; j < nv; ++j)
+            for (uint32_t i = 0; i < nu; ++i) {
+                const F3 a = add(p0, add(mul(eu, (float)i / nu), mul(ev, (float)j / nv)));
+                const F3 b = add(p0, add(mul(eu, (float)(i + 1) / nu), mul(ev, (float)j / nv)));
+                const F3 c = add(p0, add(mul(eu, (float)(i + 1) / nu), mul(ev, (float)(j + 1) / nv)));
+                const F3 d = add(p0, add(mul(eu, (float)i / nu), mul(ev, (float)(j + 1) / nv)));
+                quad(a, b, c, d, mat);
+            }
+    }
+    // axis-aligned light quad facing -y at height y
+    void light_down(float x0, float x1, float z0, float z1, float y, F3 radiance, uint32_t mat) {
+        const int32_t e = (int32_t)emits.size();
+        phg_emitter em{};
+        em.radiance[0] = radiance.x; em.radiance[1] = radiance.y; em.radiance[2] = radiance.z;
+        em.tri_begin = (uint32_t)emit_tris.size();
+        em.tri_count = 2;
+        emits.push_back(em);
+        // (x1-x0, 0, 0) x (0, 0, z1-z0) points to -y
+        tri(f3(x0, y, z0), f3(x1, y, z0), f3(x0, y, z1), mat, e);
+        tri(f3(x1, y, z0), f3(x1, y, z1), f3(x0, y, z1), mat, e);
+    }
+    // box of size (sx, sy, sz) standing on y = 0, rotated by deg around y, centred at (cx, cz)
+    void box(float cx, float cz, float sx, float sy, float sz, float deg, uint32_t mat) {
+        const float r = deg * 3.14159265358979f / 180.0f, c = std::cos(r), s = std::sin(r);
+        auto P = [&](float x, float y, float z) {
+            const float px = x - sx * 0.5f, pz = z - sz * 0.5f;
+            return f3(cx + c * px + s * pz, y, cz - s * px + c * pz);
+        };
+        const F3 v000 = P(0, 0, 0), v100 = P(sx, 0, 0), v010 = P(0, sy, 0), v110 = P(sx, sy, 0);
+        const F3 v001 = P(0, 0, sz), v101 = P(sx, 0, sz), v011 = P(0, sy, sz), v111 = P(sx, sy, sz);
+        quad(v010, v011, v111, v110, mat);  // top (+y)
+        quad(v000, v100, v101, v001, mat);  // bottom (-y)
+        quad(v000, v010, v110, v100, mat);  // -z
+        quad(v001, v101, v111, v011, mat);  // +z
+        quad(v000, v001, v011, v010, mat);  // -x
+        quad(v100, v110, v111, v101, mat);  // +x
+    }
+    void camera(F3 pos, F3 target, F3 up_hint, float fov_deg) {
+        const F3 fwd = nrm(sub(target, pos));
+        const F3 right = nrm(crs(fwd, up_hint));
+        const F3 up = crs(right, fwd);
+        phg_camera& c = host.desc.camera;
+        std::memset(&c, 0, sizeof(c));
+        c.pos[0] = pos.x; c.pos[1] = pos.y; c.pos[2] = pos.z;
+        c.right[0] = right.x; c.right[1] = right.y; c.right[2] = right.z;
+        c.up[0] = up.x; c.up[1] = up.y; c.up[2] = up.z;
+        c.forward[0] = fwd.x; c.forward[1] = fwd.y; c.forward[2] = fwd.z;
+        c.tan_half_fov = (float)std::tan((double)fov_deg * 3.14159265358979323846 / 360.0);
+    }
+    void finish() {
+        phg_scene_desc& d = host.desc;
+        d.num_tris = (uint32_t)tri_mat.size();
+        d.num_materials = (uint32_t)mats.size();
+        d.num_emitters = (uint32_t)emits.size();
+        d.num_emitter_tris = (uint32_t)emit_tris.size();
+        d.verts = verts.data();
+        d.tri_material = tri_mat.data();
+        d.tri_emitter = tri_emit.data();
+        d.materials = mats.data();
+        d.emitters = emits.data();
+        d.emitter_tris = emit_tris.data();
+    }
+};
+
+static void gen_cornell(SceneBuf& s, const phg_gen_opts& o) {
+    const uint32_t absorber = s.add_material(PHG_MAT_ABSORBER, f3(0, 0, 0), 0, 1, 1);
+    const uint32_t white = s.add_material(PHG_MAT_LAMBERT, f3(0.73f, 0.73f, 0.73f), 0, 1, 1);
+    const uint32_t red = s.add_material(PHG_MAT_LAMBERT, f3(0.65f, 0.05f, 0.05f), 0, 1, 1);
+    const uint32_t green = s.add_material(PHG_MAT_LAMBERT, f3(0.12f, 0.45f, 0.15f), 0, 1, 1);
+    // walls of the unit cube, normals facing inwards
+    s.quad(f3(0, 0, 0), f3(0, 0, 1), f3(1, 0, 1), f3(1, 0, 0), white);  // floor y=0 (+y)
+    s.quad(f3(0, 1, 0), f3(1, 1, 0), f3(1, 1, 1), f3(0, 1, 1), white);  // ceiling y=1 (-y)
+    s.quad(f3(0, 0, 1), f3(0, 1, 1), f3(1, 1, 1), f3(1, 0, 1), white);  // back z=1 (-z)
+    s.quad(f3(1, 0, 0), f3(1, 0, 1), f3(1, 1, 1), f3(1, 1, 0), red);    // x=1 (-x), left in view
+    s.quad(f3(0, 0, 0), f3(0, 1, 0), f3(0, 1, 1), f3(0, 0, 1), green);  // x=0 (+x), right in view
+    s.box(0.35f, 0.35f, 0.3f, 0.3f, 0.3f, -17.0f, white);
+    s.box(0.65f, 0.65f, 0.3f, 0.6f, 0.3f, 17.0f, white);
+    s.light_down(0.375f, 0.625f, 0.375f, 0.625f, 1.0f - 1e-4f, f3(17, 12, 4), absorber);
+    const float fov = o.fov_deg > 0 ? o.fov_deg : 40.0f;
+    s.camera(f3(0.5f, 0.5f, -1.374f), f3(0.5f, 0.5f, 0.0f), f3(0, 1, 0), fov);
+}
+
+static void add_displaced_sphere(SceneBuf& s, const Icosphere& ico, F3 c, float radius, float amp,
+                                 uint32_t noise_seed, uint32_t mat) {
+    std::vector<F3> p(ico.v.size());
+    for (size_t i = 0; i < ico.v.size(); ++i) {
+        const F3 n = ico.v[i];
+        const float d = fbm(mul(n, 2.0f), noise_seed, 4);
+        p[i] = add(c, mul(n, radius + amp * d));
+    }
+    for (size_t i = 0; i < ico.f.size(); i += 3) s.tri(p[ico.f[i]], p[ico.f[i + 1]], p[ico.f[i + 2]], mat);
+}
+
+static void gen_procedural(SceneBuf& s, const phg_gen_opts& o) {
+    Rng rng(o.seed ? o.seed : 2025);
+    const uint32_t ninst = o.num_instances ? o.num_instances : 64;
+    const uint32_t nlights = o.num_lights ? o.num_lights : 16;
+    const uint32_t subdiv = o.subdiv ? o.subdiv : 4;
+    const uint32_t target = o.target_tris ? o.target_tris : (1u << 20);
+    const Icosphere ico = make_icosphere(subdiv);
+    const uint32_t absorber = s.add_material(PHG_MAT_ABSORBER, f3(0, 0, 0), 0, 1, 1);
+    const uint32_t ground = s.add_material(PHG_MAT_LAMBERT, f3(0.6f, 0.6f, 0.6f), 0, 1, 1);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const F3 c = f3(rng.uf(-10, 10), rng.uf(-10, 10), rng.uf(-10, 10));
+        const float radius = rng.uf(0.5f, 1.5f);
+        const float rough = rng.uf(0.05f, 0.6f);
+        uint32_t mat;
+        if (i % 2 == 0) {
+            mat = s.add_material(PHG_MAT_CONDUCTOR, f3(rng.uf(0.5f, 1), rng.uf(0.5f, 1), rng.uf(0.5f, 1)), rough, 1, 1);
+        } else {
+            mat = s.add_material(PHG_MAT_DIELECTRIC, f3(1, 1, 1), rough, 1.0f, rng.uf(1.3f, 1.7f));
+        }
+        add_displaced_sphere(s, ico, c, radius, 0.1f, (uint32_t)(o.seed * 977u + i), mat);
+    }
+    // ground plane at y = -12 tessellated so that spheres + ground reach the triangle target
+    // (2 per cell); the light quads come on top
+    const uint32_t have = (uint32_t)s.tri_mat.size();
+    uint32_t cells = target > have ? (target - have) / 2 : 1;
+    uint32_t nz = (uint32_t)std::sqrt((double)cells * 512.0 / 704.0);
+    if (nz == 0) nz = 1;
+    uint32_t nx = cells / nz;
+    if (nx == 0) nx = 1;
+    // (ex, 0, 0) x (0, 0, ez) -> -y; build with eu = +z, ev = +x so the normal is +y
+    s.grid(f3(-30, -12, -30), f3(0, 0, 60), f3(60, 0, 0), nz, nx, ground);
+    for (uint32_t l = 0; l < nlights; ++l) {
+        const float x = rng.uf(-12, 10), z = rng.uf(-12, 10);
+        const float r = rng.uf(1, 20);
+        s.light_down(x, x + 2.0f, z, z + 2.0f, 13.0f, f3(r, r, r), absorber);
+    }
+    s.camera(f3(0, 4, -34), f3(0, -2, 0), f3(0, 1, 0), o.fov_deg > 0 ? o.fov_deg : 50.0f);
+}
+
+static void gen_room(SceneBuf& s, const phg_gen_opts& o) {
+    Rng rng(o.seed ? o.seed : 7);
+    const uint32_t target = o.target_tris ? o.target_tris : 200000;
+    const uint32_t absorber = s.add_material(PHG_MAT_ABSORBER, f3(0, 0, 0), 0, 1, 1);
+    const uint32_t wall = s.add_material(PHG_MAT_CONDUCTOR, f3(0.75f, 0.75f, 0.75f), rng.uf(0.3f, 0.5f), 1, 1);
+    const uint32_t floor_m = s.add_material(PHG_MAT_CONDUCTOR, f3(0.6f, 0.55f, 0.5f), rng.uf(0.2f, 0.5f), 1, 1);
+    const uint32_t inner = s.add_material(PHG_MAT_CONDUCTOR, f3(0.7f, 0.7f, 0.7f), rng.uf(0.3f, 0.5f), 1, 1);
+    const uint32_t tess = 40;
+    // room [0,4] x [0,3] x [0,4]; normals facing inwards
+    s.grid(f3(0, 0, 0), f3(0, 0, 4), f3(4, 0, 0), tess, tess, floor_m);   // floor (+y)
+    s.grid(f3(0, 3, 0), f3(4, 0, 0), f3(0, 0, 4), tess, tess, wall);      // ceiling (-y)
+    s.grid(f3(0, 0, 4), f3(0, 3, 0), f3(4, 0, 0), tess, tess, wall);      // z = 4 (-z)
+    s.grid(f3(0, 0, 0), f3(4, 0, 0), f3(0, 3, 0), tess, tess, wall);      // z = 0 (+z)
+    s.grid(f3(0, 0, 0), f3(0, 3, 0), f3(0, 0, 4), tess, tess, wall);      // x = 0 (+x)
+    s.grid(f3(4, 0, 0), f3(0, 0, 4), f3(0, 3, 0), tess, tess, wall);      // x = 4 (-x)
+    // interior wall at x in [2.95, 3.05] with a full-height slit z in [1.95, 2.05]
+    const float wx0 = 2.95f, wx1 = 3.05f, g0 = 1.95f, g1 = 2.05f;
+    s.grid(f3(wx0, 0, 0), f3(0, 3, 0), f3(0, 0, g0), 8, 8, inner);       // face towards the room, part 1
+    s.grid(f3(wx0, 0, g1), f3(0, 3, 0), f3(0, 0, 4 - g1), 8, 8, inner);  // part 2
+    s.grid(f3(wx1, 0, 0), f3(0, 0, g0), f3(0, 3, 0), 8, 8, inner);       // chamber side
+    s.grid(f3(wx1, 0, g1), f3(0, 0, 4 - g1), f3(0, 3, 0), 8, 8, inner);
+    s.grid(f3(wx0, 0, g0), f3(wx1 - wx0, 0, 0), f3(0, 3, 0), 1, 8, inner);  // slit jambs
+    s.grid(f3(wx0, 0, g1), f3(0, 3, 0), f3(wx1 - wx0, 0, 0), 8, 1, inner);
+    // the hidden light: 0.5 x 0.5, radiance 50, chamber ceiling, facing down
+    s.light_down(3.275f, 3.775f, 0.75f, 1.25f, 2.999f, f3(50, 50, 50), absorber);
+    // furniture: displaced spheres to reach the triangle target
+    const Icosphere ico = make_icosphere(4);
+    uint32_t i = 0;
+    while (s.tri_mat.size() + ico.f.size() / 3 <= target) {
+        const F3 c = f3(rng.uf(0.5f, 2.5f), rng.uf(0.3f, 1.2f), rng.uf(0.5f, 3.5f));
+        const float rough = rng.uf(0.1f, 0.5f);
+        const uint32_t mat = s.add_material(PHG_MAT_CONDUCTOR, f3(rng.uf(0.4f, 0.95f), rng.uf(0.4f, 0.95f), rng.uf(0.4f, 0.95f)), rough, 1, 1);
+        add_displaced_sphere(s, ico, c, rng.uf(0.1f, 0.3f), 0.02f, 31u * i + o.seed, mat);
+        ++i;
+    }
+    s.camera(f3(0.3f, 1.6f, 0.3f), f3(3.0f, 1.2f, 2.4f), f3(0, 1, 0), o.fov_deg > 0 ? o.fov_deg : 70.0f);
+}
+
+}  // namespace
+
+extern "C" phg_status phg_scene_generate(const phg_gen_opts* opts, phg_scene_host** out) {
+    if (!opts || !out) return PHG_ERR_INVALID_ARG;
+    SceneBuf* s = new (std::nothrow) SceneBuf();
+    if (!s) return PHG_ERR_OUT_OF_MEMORY;
+    std::memset(&s->host, 0, sizeof(s->host));
+    try {
+        switch (opts->kind) {
+            case PHG_GEN_CORNELL: gen_cornell(*s, *opts); break;
+            case PHG_GEN_PROCEDURAL: gen_procedural(*s, *opts); break;
+            case PHG_GEN_OCCLUDED_ROOM: gen_room(*s, *opts); break;
+            default: delete s; return PHG_ERR_INVALID_ARG;
+        }
+    } catch (const std::bad_alloc&) {
+        delete s;
+        return PHG_ERR_OUT_OF_MEMORY;
+    }
+    s->finish();
+    *out = &s->host;  // host is the first member: the pointer converts back in phg_scene_free
+    return PHG_OK;
+}
+
+extern "C" void phg_scene_free(phg_scene_host* scene) {
+    if (!scene) return;
+    delete reinterpret_cast<SceneBuf*>(scene);
+}

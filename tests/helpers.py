@@ -1,0 +1,115 @@
+"""Shared inputs for the parity tests: random ray batches, triangle soups, per-pixel CIs."""
+import numpy as np
+
+FLT_MAX = float(np.finfo(np.float32).max)
+
+
+def random_soup(n, seed, spread=10.0, size=1.0):
+    """n random triangles in a cube of half-width `spread`."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, (n, 1, 3))
+    v = c + rng.normal(0, size, (n, 3, 3))
+    return v.reshape(n, 9).astype(np.float32)
+
+
+def rays_towards(verts, n, seed, jitter=0.0):
+    """Rays from random origins aimed at random points of random triangles (mostly hits)."""
+    rng = np.random.default_rng(seed)
+    v = verts.reshape(-1, 3, 3).astype(np.float64)
+    tri = rng.integers(0, v.shape[0], n)
+    w = rng.dirichlet((1, 1, 1), n)
+    target = np.einsum("nk,nkd->nd", w, v[tri])
+    lo, hi = v.reshape(-1, 3).min(0), v.reshape(-1, 3).max(0)
+    ext = hi - lo
+    org = rng.uniform(lo - 0.5 * ext, hi + 0.5 * ext, (n, 3))
+    d = target - org + rng.normal(0, jitter, (n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = org
+    r[:, 3] = 0.0
+    r[:, 4:7] = d
+    r[:, 7] = FLT_MAX
+    return r
+
+
+def random_rays(verts, n, seed):
+    """Rays with uniformly random directions from points around the scene."""
+    rng = np.random.default_rng(seed)
+    v = verts.reshape(-1, 3)
+    lo, hi = v.min(0), v.max(0)
+    org = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((n, 8), np.float32)
+    r[:, 0:3] = org
+    r[:, 3] = 0.0
+    r[:, 4:7] = d
+    r[:, 7] = FLT_MAX
+    return r
+
+
+def pixel_z(film_a, film_b):
+    """Per-pixel, per-channel z statistic of the difference of two independent-sample
+    means, from FP64 films holding sum, count and sum of squares."""
+    def mv(f):
+        n = f[..., 3:4]
+        m = f[..., 0:3] / n
+        var = (f[..., 4:7] - n * m * m) / np.maximum(n - 1, 1)
+        return m, np.maximum(var, 0) / n
+    ma, va = mv(film_a)
+    mb, vb = mv(film_b)
+    se = np.sqrt(va + vb)
+    d = ma - mb
+    z = np.where(se > 0, d / np.where(se > 0, se, 1), np.where(np.abs(d) > 1e-6 * (np.abs(ma) + 1e-12), np.inf, 0.0))
+    return z
+
+
+def scene_from_tris(verts, mat_type=1, rgb=(0.5, 0.5, 0.5), roughness=0.3, ior=1.5, camera=None):
+    """phg.Scene with one material for all triangles and no emitters."""
+    import phg
+    n = verts.shape[0]
+    mats = np.zeros(1, phg.Scene.MAT_DTYPE)
+    mats["type"] = mat_type
+    mats["rgb"] = rgb
+    mats["roughness"] = roughness
+    mats["ior_outer"] = 1.0
+    mats["ior_inner"] = ior
+    cam = camera or {"pos": [0, 0, -5], "right": [1, 0, 0], "up": [0, 1, 0], "forward": [0, 0, 1],
+                     "tan_half_fov": 0.5}
+    return phg.Scene(np.ascontiguousarray(verts, np.float32), np.zeros(n, np.uint32), np.full(n, -1, np.int32),
+                     mats, np.zeros(0, phg.Scene.EMIT_DTYPE), np.zeros(0, np.uint32), cam)
+
+
+def furnace_scene(albedo=0.5, radiance=0.5, mat_type=1, roughness=0.4):
+    """Closed unit cube whose six inward-facing walls are all emitters with constant
+    radiance and all share one material; with a Lambertian albedo rho every pixel's expected
+    value is radiance / (1 - rho) (unbounded path length)."""
+    import phg
+    P = lambda x, y, z: np.array([x, y, z], np.float32)
+    quads = [  # counter-clockwise seen from inside the cube
+        (P(0, 0, 0), P(0, 0, 1), P(1, 0, 1), P(1, 0, 0)),  # floor, +y
+        (P(0, 1, 0), P(1, 1, 0), P(1, 1, 1), P(0, 1, 1)),  # ceiling, -y
+        (P(0, 0, 1), P(0, 1, 1), P(1, 1, 1), P(1, 0, 1)),  # z=1, -z
+        (P(0, 0, 0), P(1, 0, 0), P(1, 1, 0), P(0, 1, 0)),  # z=0, +z
+        (P(0, 0, 0), P(0, 1, 0), P(0, 1, 1), P(0, 0, 1)),  # x=0, +x
+        (P(1, 0, 0), P(1, 0, 1), P(1, 1, 1), P(1, 1, 0)),  # x=1, -x
+    ]
+    verts, emit = [], []
+    for qi, (a, b, c, d) in enumerate(quads):
+        verts += [np.concatenate([a, b, c]), np.concatenate([a, c, d])]
+        emit += [qi, qi]
+    verts = np.array(verts, np.float32)
+    n = len(verts)
+    mats = np.zeros(1, phg.Scene.MAT_DTYPE)
+    mats["type"] = mat_type
+    mats["rgb"] = (albedo, albedo, albedo)
+    mats["roughness"] = roughness
+    mats["ior_outer"] = 1.0
+    mats["ior_inner"] = 1.5
+    emitters = np.zeros(6, phg.Scene.EMIT_DTYPE)
+    emitters["radiance"] = radiance
+    emitters["tri_begin"] = np.arange(6) * 2
+    emitters["tri_count"] = 2
+    cam = {"pos": [0.5, 0.5, 0.1], "right": [-1, 0, 0], "up": [0, 1, 0], "forward": [0, 0, 1], "tan_half_fov": 0.8}
+    return phg.Scene(verts, np.zeros(n, np.uint32), np.array(emit, np.int32), mats, emitters,
+                     np.arange(n, dtype=np.uint32), cam)

@@ -1,0 +1,115 @@
+"""End-to-end parity of the GPU BNEEPT wavefront against the CPU oracle restatement of
+BNEEPTEstimator::estimate (Core/Estimator/BNEEPTEstimator.cpp:38-262).
+
+Both sides draw every sample from the same per-path PCG32 stream, so most paths agree
+bit for bit; transcendental functions (libm vs the device math library) make some paths
+diverge, so the image-level bar is statistical: per-pixel 99% confidence intervals."""
+import numpy as np
+import pytest
+
+import phg
+from oracle import oracle as orc
+from helpers import furnace_scene, pixel_z
+
+pytestmark = pytest.mark.gpu
+
+Z99 = 2.5758
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = phg.Context(0)
+    yield c
+    c.close()
+
+
+def gpu_render(ctx, scene, W, H, spp, bounces, seed, **kw):
+    ctx.upload(scene)
+    ctx.film_clear(W, H)
+    ctx.render(spp, max_bounces=bounces, seed=seed, **kw)
+    return ctx.film()
+
+
+def check_parity(gpu, ref, name, min_identical):
+    assert np.all(gpu[..., 3] == ref[..., 3])
+    ident = np.all(np.isclose(gpu[..., :3], ref[..., :3], rtol=1e-6, atol=1e-9), axis=-1).mean()
+    z = pixel_z(gpu, ref)
+    frac_out = np.mean(np.abs(z) > Z99)
+    mg, mr = gpu[..., :3].sum() / gpu[..., 3].sum(), ref[..., :3].sum() / ref[..., 3].sum()
+    print(f"[render] case={name} identical_pixels={ident:.4f} outside_99CI={frac_out:.4f} "
+          f"mean_gpu={mg:.6f} mean_oracle={mr:.6f}", flush=True)
+    assert ident >= min_identical, f"{name}: only {ident:.3f} of pixels identical"
+    assert frac_out <= 0.01 + 3 * np.sqrt(0.01 * 0.99 / z.size), f"{name}: {frac_out:.4f} of pixels outside 99% CI"
+    assert abs(mg - mr) <= 0.01 * abs(mr) + 1e-6
+
+
+def test_cornell_matches_oracle(ctx):
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    W = H = 64
+    print("[render] case=cornell start", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, 16, 5, 11)
+    ref, _ = orc.render(sc, W, H, 16, max_bounces=5, seed=11)
+    check_parity(gpu, ref, "cornell", 0.9)
+
+
+def test_procedural_ggx_matches_oracle(ctx):
+    sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=4, target_tris=30_000, num_instances=12, num_lights=6, subdiv=3)
+    W = H = 48
+    print("[render] case=procedural start", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, 8, 8, 3)
+    ref, _ = orc.render(sc, W, H, 8, max_bounces=8, seed=3)
+    check_parity(gpu, ref, "procedural", 0.5)
+
+
+def test_unbounded_paths_match_oracle(ctx):
+    # reference semantics: no bounce cap (MAX_RAY_BOUNCES 10000), Russian roulette ends paths
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    W = H = 32
+    print("[render] case=cornell_unbounded start", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, 16, 10000, 5)
+    ref, _ = orc.render(sc, W, H, 16, max_bounces=10000, seed=5)
+    check_parity(gpu, ref, "cornell_unbounded", 0.85)
+
+
+@pytest.mark.parametrize("albedo,radiance", [(0.5, 0.5), (0.8, 0.2)])
+def test_furnace_closed_form(ctx, albedo, radiance):
+    sc = furnace_scene(albedo=albedo, radiance=radiance)
+    W = H = 32
+    spp = 64
+    print(f"[render] case=furnace albedo={albedo} start", flush=True)
+    film = gpu_render(ctx, sc, W, H, spp, 10000, 1)
+    expect = radiance / (1 - albedo)
+    n = film[..., 3].sum()
+    mean = film[..., 0].sum() / n
+    var = film[..., 4].sum() / n - mean * mean
+    se = np.sqrt(var / n)
+    print(f"[render] case=furnace albedo={albedo} mean={mean:.6f} expect={expect:.6f} se={se:.2e}", flush=True)
+    assert abs(mean - expect) < 4 * se + 1e-4
+
+
+def test_progressive_passes_equal_single_pass(ctx):
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    W = H = 32
+    ctx.upload(sc)
+    ctx.film_clear(W, H)
+    ctx.render(8, max_bounces=5, seed=9)
+    once = ctx.film()
+    ctx.film_clear(W, H)
+    ctx.render(3, max_bounces=5, seed=9)
+    ctx.render(5, max_bounces=5, seed=9, spp_begin=3)
+    twice = ctx.film()
+    ctx.film_clear(W, H)
+    ctx.render(8, max_bounces=5, seed=9, spp_per_pass=1)
+    per1 = ctx.film()
+    assert np.array_equal(once, twice)
+    assert np.array_equal(once, per1)
+
+
+def test_stats_report_paths_and_rays(ctx):
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    gpu_render(ctx, sc, 32, 32, 4, 5, 2, count_traversal=True)
+    s = ctx.stats()
+    assert s["paths"] == 32 * 32 * 4
+    assert s["rays_primary"] == 32 * 32 * 4
+    assert s["rays_extension"] > 0 and s["rays_shadow"] > 0
+    assert s["node_visits"] > 0 and s["tri_tests"] > 0
