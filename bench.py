@@ -1,0 +1,160 @@
+"""Benchmark of the GPU BNEEPT hot path (one JSON line on stdout).
+
+Workload (one step): plain BNEEPT over the 1M-triangle procedural scene — 64 displaced
+icospheres + tessellated ground (2^20 triangles) + 16 area-light quads, GGX conductors and
+dielectrics — at 1024x1024, 64 samples per pixel, 8 bounces, FP32 shading, FP64 film.
+Inputs are resident on the GPU before timing starts (scene and BVH uploaded once).
+
+Reported beside the throughput:
+  roofline      the closest-hit traversal kernel: algorithmic bytes
+                (48 per ray + 128 per node visited + 48 per triangle tested, counted by the
+                same kernel in a counted render of the same paths) over its summed HIP-event
+                launch durations, against the 8 TB/s HBM peak.
+  cpu_baseline  the CPU oracle (scalar C++ restatement of the reference path) on this
+                host's cores, over a bounded sample of the same image (every k-th row).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "photon-v2_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "Msamples/s at 1 GPU; time to relMSE 0.01, guided vs plain GPU BNEEPT"
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--height", type=int, default=1024)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--bounces", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import phg
+
+    W, H, spp, B = args.width, args.height, args.spp, args.bounces
+    t0 = time.time()
+    scene = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=2025)
+    gen_s = time.time() - t0
+    print(f"[bench] scene: {scene.num_tris} triangles, {len(scene.emitters)} lights ({gen_s:.1f}s)",
+          file=sys.stderr, flush=True)
+    ctx = phg.Context(0)
+    t0 = time.time()
+    ctx.upload(scene)
+    upload_s = time.time() - t0
+    st0 = ctx.stats()
+    print(f"[bench] BVH: {st0['bvh_nodes']} nodes, depth {st0['bvh_depth']}, build {st0['bvh_build_ms']:.0f} ms, "
+          f"upload {upload_s:.1f}s", file=sys.stderr, flush=True)
+
+    # counted render of one sample per pixel: node visits / triangle tests per traversal
+    ctx.film_clear(W, H)
+    ctx.render(1, max_bounces=B, seed=1, count_traversal=True)
+    cs = ctx.stats()
+    rays_closest_1 = cs["rays_primary"] + cs["rays_extension"]
+    nodes_per_ray = cs["trace_node_visits"] / rays_closest_1
+    tris_per_ray = cs["trace_tri_tests"] / rays_closest_1
+
+    for _ in range(args.warmup):
+        ctx.film_clear(W, H)
+        ctx.render(spp, max_bounces=B, seed=1)
+    times, trace_ms, trace_launch, rays = [], 0.0, 0, [0, 0, 0]
+    for s in range(args.steps):
+        ctx.film_clear(W, H)
+        t = time.perf_counter()
+        ctx.render(spp, max_bounces=B, seed=1)
+        times.append(time.perf_counter() - t)
+        st = ctx.stats()
+        trace_ms += st["trace_ms"]
+        trace_launch += st["trace_launches"]
+        rays[0] += st["rays_primary"]
+        rays[1] += st["rays_extension"]
+        rays[2] += st["rays_shadow"]
+        print(f"[bench] step {s}: {times[-1] * 1e3:.1f} ms, trace {st['trace_ms']:.1f} ms, "
+              f"shadow {st['shadow_ms']:.1f} ms, shade {st['shade_ms']:.1f} ms", file=sys.stderr, flush=True)
+    total_s = sum(times)
+    samples = W * H * spp * args.steps
+    msps = samples / total_s / 1e6
+    rays_closest = rays[0] + rays[1]
+    # algorithmic bytes of the closest-hit traversal kernel over all its launches
+    bytes_closest = rays_closest * (32 + 16 + nodes_per_ray * 128 + tris_per_ray * 48)
+    achieved = bytes_closest / (trace_ms / 1e3) / 1e9
+    img = phg.develop(ctx.film())
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        threads = os.cpu_count() or 1
+        # probe, then size the row sample to about args.cpu_seconds of CPU work
+        step = 64
+        _, pst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=threads, row_step=256)
+        per_path_s = pst["render_ms"] / 1e3 / max(pst["paths"], 1)
+        want_paths = args.cpu_seconds / max(per_path_s, 1e-9)
+        step = int(max(1, min(256, round(W * H * spp / max(want_paths, 1.0)))))
+        _, cst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=threads, row_step=step)
+        cpu = {"value": cst["paths"] / (cst["render_ms"] / 1e3) / 1e6, "unit": "Msamples/s", "cores": threads,
+               "kind": "port",
+               "sample": f"every {step}th row of the same {W}x{H} image at {spp} spp ({cst['paths']} paths, "
+                         f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(msps, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_s / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"plain BNEEPT, procedural {scene.num_tris}-triangle scene, {W}x{H}, {spp} spp, "
+                               f"{B} bounces", "width": W, "height": H, "spp": spp, "max_bounces": B,
+                   "triangles": scene.num_tris, "lights": int(len(scene.emitters))},
+        "mrays_per_s": round(sum(rays) / total_s / 1e6, 3),
+        "mrays_per_s_by_class": {"primary": round(rays[0] / total_s / 1e6, 3),
+                                 "extension": round(rays[1] / total_s / 1e6, 3),
+                                 "shadow": round(rays[2] / total_s / 1e6, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "k_extend (closest-hit traversal)",
+                     "launches": trace_launch, "avg_launch_ms": round(trace_ms / max(trace_launch, 1), 4),
+                     "bytes_per_launch": round(bytes_closest / max(trace_launch, 1)),
+                     "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
+        "cpu_baseline": cpu,
+        "image_mean": [round(float(x), 6) for x in img.mean(axis=(0, 1))],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

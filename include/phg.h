@@ -150,6 +150,12 @@ typedef struct phg_stats {
     uint32_t bvh_stack_need;
     uint32_t pad;
     double bvh_build_ms;
+    uint64_t trace_node_visits;   /* closest-hit kernels, with PHG_RENDER_COUNT_TRAVERSAL */
+    uint64_t trace_tri_tests;
+    uint64_t shadow_node_visits;  /* any-hit kernels, with PHG_RENDER_COUNT_TRAVERSAL */
+    uint64_t shadow_tri_tests;
+    uint32_t trace_launches;      /* closest-hit kernel launches in the last render */
+    uint32_t shadow_launches;
 } phg_stats;
 
 /* Context lifetime.  Replaces nothing in the reference: the CPU engine needs no device. */

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -1084,6 +1085,44 @@ int orc_bsdf_sample(const orc_material* m, const float* N3, const float* V3_, co
     return s.ok ? 1 : 0;
 }
 
+// Batched surface-optics queries at a flat hit with normal N: n samples (u: 3 per sample),
+// or n evaluations (L: 3 per direction) of bsdf and pdf.
+void orc_bsdf_sample_batch(const orc_material* m, const float* N3, const float* V3_, uint32_t n, const float* u,
+                           float* out_L, float* out_w, float* out_relior, int32_t* out_ok) {
+    const SurfaceHit X = synthetic_hit(N3);
+    const V3 V = mk(V3_[0], V3_[1], V3_[2]);
+    for (uint32_t i = 0; i < n; ++i) {
+        const BsdfSample s = bsdf_sample(*m, X, V, u + 3 * (size_t)i);
+        out_L[3 * i] = s.L.x; out_L[3 * i + 1] = s.L.y; out_L[3 * i + 2] = s.L.z;
+        out_w[3 * i] = s.w.r; out_w[3 * i + 1] = s.w.g; out_w[3 * i + 2] = s.w.b;
+        out_relior[i] = s.relIor; out_ok[i] = s.ok ? 1 : 0;
+    }
+}
+
+void orc_bsdf_eval_batch(const orc_material* m, const float* N3, const float* V3_, uint32_t n, const float* L,
+                         float* out_f, float* out_pdf, int32_t* out_ok) {
+    const SurfaceHit X = synthetic_hit(N3);
+    const V3 V = mk(V3_[0], V3_[1], V3_[2]);
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3 l = mk(L[3 * i], L[3 * i + 1], L[3 * i + 2]);
+        const BsdfEval e = bsdf_eval(*m, X, l, V);
+        out_f[3 * i] = e.f.r; out_f[3 * i + 1] = e.f.g; out_f[3 * i + 2] = e.f.b;
+        out_ok[i] = e.ok ? 1 : 0;
+        out_pdf[i] = bsdf_pdf(*m, X, l, V);
+    }
+}
+
+void orc_ggx_batch(const float* N3, const float* V3_, float alpha, uint32_t n, const float* H, float* out_D,
+                   float* out_vndf, float* out_lambda) {
+    const V3 N = mk(N3[0], N3[1], N3[2]), V = mk(V3_[0], V3_[1], V3_[2]);
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3 h = mk(H[3 * i], H[3 * i + 1], H[3 * i + 2]);
+        out_D[i] = ggx_D(N, h, alpha);
+        out_vndf[i] = ggx_vndf_pdf(N, h, V, alpha);
+        out_lambda[i] = ggx_lambda(N, h, alpha);
+    }
+}
+
 float orc_ggx_D(const float* N3, const float* H3, float alpha) { return ggx_D(mk(N3[0], N3[1], N3[2]), mk(H3[0], H3[1], H3[2]), alpha); }
 float orc_ggx_G(const float* N3, const float* H3, const float* L3, const float* V3_, float alpha) {
     return ggx_G(mk(N3[0], N3[1], N3[2]), mk(H3[0], H3[1], H3[2]), mk(L3[0], L3[1], L3[2]), mk(V3_[0], V3_[1], V3_[2]), alpha);
@@ -1176,15 +1215,16 @@ int orc_render(const orc_scene* scene, const orc_render_opts* opt, double* film,
     const uint32_t TS = 128;
     const uint32_t tx = (W + TS - 1) / TS, ty = (H + TS - 1) / TS;
     std::atomic<uint32_t> next{0};
-    std::atomic<uint64_t> n_prim{0}, n_ext{0}, n_shadow{0};
-    const uint32_t y_limit = opt->row_limit ? std::min(opt->row_limit, H) : H;
+    std::atomic<uint64_t> n_prim{0}, n_ext{0}, n_shadow{0}, n_paths{0};
+    const uint32_t row_step = opt->row_step ? opt->row_step : 1;
     auto worker = [&]() {
         PathStats st;
         while (true) {
             const uint32_t tile = next.fetch_add(1);
             if (tile >= tx * ty) break;
             const uint32_t x0 = (tile % tx) * TS, y0 = (tile / tx) * TS;
-            for (uint32_t y = y0; y < std::min(y0 + TS, y_limit); ++y)
+            for (uint32_t y = y0; y < std::min(y0 + TS, H); ++y) {
+                if (y % row_step != 0) continue;
                 for (uint32_t x = x0; x < std::min(x0 + TS, W); ++x) {
                     const uint64_t pix = (uint64_t)y * W + x;
                     double* f = film + 8 * pix;
@@ -1197,16 +1237,23 @@ int orc_render(const orc_scene* scene, const orc_render_opts* opt, double* film,
                         const RGB L = estimate(S, o, d, sf, opt->max_bounces, &st);
                         f[0] += L.r; f[1] += L.g; f[2] += L.b; f[3] += 1.0;
                         f[4] += (double)L.r * L.r; f[5] += (double)L.g * L.g; f[6] += (double)L.b * L.b;
+                        n_paths.fetch_add(1, std::memory_order_relaxed);
                     }
                 }
+            }
         }
         n_prim += st.rays_primary; n_ext += st.rays_ext; n_shadow += st.rays_shadow;
     };
     const uint32_t nt = std::max(1u, opt->nthreads);
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<std::thread> th;
     for (uint32_t i = 0; i < nt; ++i) th.emplace_back(worker);
     for (auto& t : th) t.join();
-    if (stats) { stats->rays_primary = n_prim; stats->rays_extension = n_ext; stats->rays_shadow = n_shadow; }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) {
+        stats->rays_primary = n_prim; stats->rays_extension = n_ext; stats->rays_shadow = n_shadow;
+        stats->paths = n_paths; stats->render_ms = ms;
+    }
     return 0;
 }
 

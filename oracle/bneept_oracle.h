@@ -58,12 +58,14 @@ typedef struct orc_render_opts {
     uint32_t max_bounces;
     uint32_t nthreads;
     uint64_t seed;
-    uint32_t row_limit; /* 0 = all rows; else render rows [0, row_limit) only */
+    uint32_t row_step;  /* 0 or 1 = all rows; k = only rows y with y % k == 0 */
     uint32_t pad;
 } orc_render_opts;
 
 typedef struct orc_render_stats {
     uint64_t rays_primary, rays_extension, rays_shadow;
+    uint64_t paths;
+    double render_ms;   /* path tracing only, after the BVH build */
 } orc_render_stats;
 
 #ifdef __cplusplus

@@ -43,11 +43,12 @@ class orc_scene(C.Structure):
 class orc_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp_begin", C.c_uint32), ("spp", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("nthreads", C.c_uint32), ("seed", C.c_uint64),
-                ("row_limit", C.c_uint32), ("pad", C.c_uint32)]
+                ("row_step", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class orc_render_stats(C.Structure):
-    _fields_ = [("rays_primary", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64)]
+    _fields_ = [("rays_primary", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("paths", C.c_uint64), ("render_ms", C.c_double)]
 
 
 _lib = None
@@ -76,6 +77,9 @@ def lib() -> C.CDLL:
     L.orc_bsdf_pdf.argtypes = [vp, vp, vp, vp]
     L.orc_bsdf_pdf.restype = f
     L.orc_bsdf_sample.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.orc_bsdf_sample_batch.argtypes = [vp, vp, vp, u32, vp, vp, vp, vp, vp]
+    L.orc_bsdf_eval_batch.argtypes = [vp, vp, vp, u32, vp, vp, vp, vp]
+    L.orc_ggx_batch.argtypes = [vp, vp, f, u32, vp, vp, vp, vp]
     L.orc_ggx_D.argtypes = [vp, vp, f]
     L.orc_ggx_D.restype = f
     L.orc_ggx_G.argtypes = [vp, vp, vp, vp, f]
@@ -141,15 +145,16 @@ class SceneView:
         self.s = s
 
 
-def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_limit=0):
+def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_step=1):
     """FP64 film (H, W, 8) of the oracle BNEEPT render, plus ray counts."""
     L = lib()
     sv = SceneView(scene)
     film = np.zeros((height, width, 8), np.float64)
-    o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, row_limit, 0)
+    o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, row_step, 0)
     st = orc_render_stats()
     L.orc_render(C.byref(sv.s), C.byref(o), film.ctypes.data, C.byref(st))
-    return film, {"rays_primary": st.rays_primary, "rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow}
+    return film, {"rays_primary": st.rays_primary, "rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow,
+                  "paths": st.paths, "render_ms": st.render_ms}
 
 
 def render_paths(scene, width, height, pixels, samples, max_bounces=10000, seed=0):
@@ -187,3 +192,37 @@ def intersect_refbvh(verts, rays, nthreads=None):
     L.orc_intersect_refbvh(verts.ctypes.data, verts.shape[0], rays.ctypes.data, n, tri.ctypes.data, t.ctypes.data,
                            nthreads or os.cpu_count() or 1)
     return tri, t
+
+
+def bsdf_sample_batch(m, N, V, u):
+    L = lib()
+    u = f32(u).reshape(-1, 3)
+    n = u.shape[0]
+    Ls, w = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+    rel, ok = np.zeros(n, np.float32), np.zeros(n, np.int32)
+    N, V = f32(N), f32(V)
+    L.orc_bsdf_sample_batch(C.byref(m), N.ctypes.data, V.ctypes.data, n, u.ctypes.data, Ls.ctypes.data,
+                            w.ctypes.data, rel.ctypes.data, ok.ctypes.data)
+    return Ls, w, rel, ok.astype(bool)
+
+
+def bsdf_eval_batch(m, N, V, Ls):
+    L = lib()
+    Ls = f32(Ls).reshape(-1, 3)
+    n = Ls.shape[0]
+    fv, pdf, ok = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.int32)
+    N, V = f32(N), f32(V)
+    L.orc_bsdf_eval_batch(C.byref(m), N.ctypes.data, V.ctypes.data, n, Ls.ctypes.data, fv.ctypes.data,
+                          pdf.ctypes.data, ok.ctypes.data)
+    return fv, pdf, ok.astype(bool)
+
+
+def ggx_batch(N, V, alpha, H):
+    L = lib()
+    H = f32(H).reshape(-1, 3)
+    n = H.shape[0]
+    D, vndf, lam = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+    N, V = f32(N), f32(V)
+    L.orc_ggx_batch(N.ctypes.data, V.ctypes.data, alpha, n, H.ctypes.data, D.ctypes.data, vndf.ctypes.data,
+                    lam.ctypes.data)
+    return D, vndf, lam

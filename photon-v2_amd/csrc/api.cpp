@@ -384,6 +384,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     const uint32_t grid_shade = c->cu_count * 4;
     double t_trace = 0, t_shadow = 0, t_shade = 0;
     uint64_t rays_primary = 0;
+    uint32_t n_trace = 0, n_shadow = 0;
     const auto wall0 = std::chrono::steady_clock::now();
     for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
         const uint32_t kk = std::min(k, o->spp - s0);
@@ -413,6 +414,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
             PHG_HIP(c, hipEventElapsedTime(&bb, c->ev[1], c->ev[2]));
             PHG_HIP(c, hipEventElapsedTime(&cc, c->ev[2], c->ev[3]));
             t_trace += a; t_shade += bb; t_shadow += cc;
+            ++n_trace; ++n_shadow;
             if (next == 0 || b >= o->max_bounces) break;
         }
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
@@ -433,6 +435,12 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     c->stats.rays_shadow = tc[5];
     c->stats.node_visits = tc[0] + tc[2];
     c->stats.tri_tests = tc[1] + tc[3];
+    c->stats.trace_node_visits = tc[0];
+    c->stats.trace_tri_tests = tc[1];
+    c->stats.shadow_node_visits = tc[2];
+    c->stats.shadow_tri_tests = tc[3];
+    c->stats.trace_launches = n_trace;
+    c->stats.shadow_launches = n_shadow;
     c->stats.trace_ms = t_trace;
     c->stats.shadow_ms = t_shadow;
     c->stats.shade_ms = t_shade;

@@ -75,7 +75,10 @@ class phg_stats(C.Structure):
                 ("rays_shadow", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("render_ms", C.c_double), ("trace_ms", C.c_double), ("shadow_ms", C.c_double),
                 ("shade_ms", C.c_double), ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32),
-                ("bvh_stack_need", C.c_uint32), ("pad", C.c_uint32), ("bvh_build_ms", C.c_double)]
+                ("bvh_stack_need", C.c_uint32), ("pad", C.c_uint32), ("bvh_build_ms", C.c_double),
+                ("trace_node_visits", C.c_uint64), ("trace_tri_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+                ("trace_launches", C.c_uint32), ("shadow_launches", C.c_uint32)]
 
 
 class phg_gen_opts(C.Structure):
