@@ -9,44 +9,46 @@
 
 #include "kernels.h"
 
+#define PHG_HD __host__ __device__ inline
+
 namespace phg {
 
 // ---- vector math in the reference's operation order (Math/TVector3.ipp) ----------------
 struct V3 { float x, y, z; };
-__device__ __forceinline__ V3 mk(float x, float y, float z) { V3 v; v.x = x; v.y = y; v.z = z; return v; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float absdot(V3 a, V3 b) { return fabsf(dot(a, b)); }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+PHG_HD V3 mk(float x, float y, float z) { V3 v; v.x = x; v.y = y; v.z = z; return v; }
+PHG_HD V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+PHG_HD V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+PHG_HD V3 operator*(V3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+PHG_HD V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+PHG_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PHG_HD float absdot(V3 a, V3 b) { return fabsf(dot(a, b)); }
+PHG_HD V3 cross(V3 a, V3 b) {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
-__device__ __forceinline__ bool is_zero(V3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
-__device__ __forceinline__ bool is_finite(V3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
-__device__ __forceinline__ V3 safe_normalize(V3 a, V3 fb) {
+PHG_HD float length(V3 a) { return sqrtf(dot(a, a)); }
+PHG_HD V3 normalize(V3 a) { const float r = 1.0f / length(a); return a * r; }
+PHG_HD bool is_zero(V3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+PHG_HD bool is_finite(V3 a) { return __builtin_isfinite(a.x) && __builtin_isfinite(a.y) && __builtin_isfinite(a.z); }
+PHG_HD V3 safe_normalize(V3 a, V3 fb) {
     const V3 n = normalize(a);
     return (!is_zero(n) && is_finite(n)) ? n : fb;
 }
-__device__ __forceinline__ V3 reflect(V3 v, V3 n) { return v - n * (2.0f * dot(n, v)); }
-__device__ __forceinline__ V3 wsum(V3 a, float wa, V3 b, float wb, V3 c, float wc) { return a * wa + b * wb + c * wc; }
-__device__ __forceinline__ float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
-__device__ __forceinline__ float safe_clamp(float v, float lo, float hi) { return isfinite(v) ? clampf(v, lo, hi) : lo; }
+PHG_HD V3 reflect(V3 v, V3 n) { return v - n * (2.0f * dot(n, v)); }
+PHG_HD V3 wsum(V3 a, float wa, V3 b, float wb, V3 c, float wc) { return a * wa + b * wb + c * wc; }
+PHG_HD float clampf(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+PHG_HD float safe_clamp(float v, float lo, float hi) { return __builtin_isfinite(v) ? clampf(v, lo, hi) : lo; }
 
 struct RGB { float r, g, b; };
-__device__ __forceinline__ RGB rgb1(float v) { RGB c; c.r = v; c.g = v; c.b = v; return c; }
-__device__ __forceinline__ RGB rgb3(float r, float g, float b) { RGB c; c.r = r; c.g = g; c.b = b; return c; }
-__device__ __forceinline__ bool rgb_zero(RGB c) { return c.r == 0.0f && c.g == 0.0f && c.b == 0.0f; }
-__device__ __forceinline__ bool rgb_finite(RGB c) { return isfinite(c.r) && isfinite(c.g) && isfinite(c.b); }
-__device__ __forceinline__ RGB mulc(RGB a, RGB b) { return rgb3(a.r * b.r, a.g * b.g, a.b * b.b); }
-__device__ __forceinline__ RGB muls(RGB a, float s) { return rgb3(a.r * s, a.g * s, a.b * s); }
-__device__ __forceinline__ RGB rational_clamp(RGB c) {  // BNEEPTEstimator.cpp:264-268
+PHG_HD RGB rgb1(float v) { RGB c; c.r = v; c.g = v; c.b = v; return c; }
+PHG_HD RGB rgb3(float r, float g, float b) { RGB c; c.r = r; c.g = g; c.b = b; return c; }
+PHG_HD bool rgb_zero(RGB c) { return c.r == 0.0f && c.g == 0.0f && c.b == 0.0f; }
+PHG_HD bool rgb_finite(RGB c) { return __builtin_isfinite(c.r) && __builtin_isfinite(c.g) && __builtin_isfinite(c.b); }
+PHG_HD RGB mulc(RGB a, RGB b) { return rgb3(a.r * b.r, a.g * b.g, a.b * b.b); }
+PHG_HD RGB muls(RGB a, float s) { return rgb3(a.r * s, a.g * s, a.b * s); }
+PHG_HD RGB rational_clamp(RGB c) {  // BNEEPTEstimator.cpp:264-268
     return rgb3(safe_clamp(c.r, 0.0f, 1e9f), safe_clamp(c.g, 0.0f, 1e9f), safe_clamp(c.b, 0.0f, 1e9f));
 }
-__device__ __forceinline__ float luminance(RGB c) {  // linear-sRGB Y (color_spaces.ipp:129)
+PHG_HD float luminance(RGB c) {  // linear-sRGB Y (color_spaces.ipp:129)
     return c.r * 0.21263900587151027f + c.g * 0.715168678767756f + c.b * 0.07219231536073371f;
 }
 
@@ -58,7 +60,7 @@ constexpr float kFltMax = 3.40282346638528859812e+38f;
 constexpr float kFltMin = 1.17549435082228750797e-38f;
 
 // ---- sample stream: PCG32 (Math/Random/Pcg32.h:72-124), moremur (Math/hash.ipp:104-115) --
-__device__ __forceinline__ uint64_t moremur(uint64_t v) {
+PHG_HD uint64_t moremur(uint64_t v) {
     v ^= v >> 27; v *= 0x3C79AC492BA7B653ULL;
     v ^= v >> 33; v *= 0x1C69B3F74AC4AE35ULL;
     v ^= v >> 27;
@@ -68,7 +70,7 @@ constexpr uint64_t kPcgMul = 0x5851F42D4C957F2DULL;
 
 struct Stream {
     uint64_t state, inc;
-    __device__ __forceinline__ uint32_t next() {
+    PHG_HD uint32_t next() {
         const uint64_t old = state;
         state = old * kPcgMul + inc;
         const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
@@ -76,22 +78,22 @@ struct Stream {
         return (xs >> rot) | (xs << ((~rot + 1u) & 31));
     }
     // odd multiples of 2^-24: strictly inside (0, 1)
-    __device__ __forceinline__ float u() { const uint32_t b = next(); return (float)(((b >> 9) << 1) | 1u) * 0x1p-24f; }
+    PHG_HD float u() { const uint32_t b = next(); return (float)(((b >> 9) << 1) | 1u) * 0x1p-24f; }
 };
-__device__ __forceinline__ uint64_t path_key(uint64_t seed, uint64_t pixel, uint64_t sample, uint64_t npix) {
+PHG_HD uint64_t path_key(uint64_t seed, uint64_t pixel, uint64_t sample, uint64_t npix) {
     return (seed << 40) ^ (sample * npix + pixel);
 }
-__device__ __forceinline__ Stream stream_init(uint64_t key) {
+PHG_HD Stream stream_init(uint64_t key) {
     Stream s; s.state = 0u; s.inc = (key << 1u) | 1u;
     s.next(); s.state += moremur(key); s.next();
     return s;
 }
-__device__ __forceinline__ Stream stream_resume(uint64_t key, uint64_t state) {
+PHG_HD Stream stream_resume(uint64_t key, uint64_t state) {
     Stream s; s.state = state; s.inc = (key << 1u) | 1u; return s;
 }
 
 // ---- watertight ray/triangle (Math/Geometry/TWatertightTriangle.ipp:16-149) -------------
-__device__ __forceinline__ bool wt_intersect(V3 va, V3 vb, V3 vc, V3 org, V3 dir, float minT, float maxT,
+PHG_HD bool wt_intersect(V3 va, V3 vb, V3 vc, V3 org, V3 dir, float minT, float maxT,
                                              float* out_t, float* out_a, float* out_b, float* out_c) {
     V3 d = dir;
     V3 a = va - org, b = vb - org, c = vc - org;
@@ -122,7 +124,7 @@ __device__ __forceinline__ bool wt_intersect(V3 va, V3 vb, V3 vc, V3 org, V3 dir
     }
     if ((ea < 0.0f || eb < 0.0f || ec < 0.0f) && (ea > 0.0f || eb > 0.0f || ec > 0.0f)) return false;
     const float det = ea + eb + ec;
-    if (det == 0.0f || !isfinite(det)) return false;
+    if (det == 0.0f || !__builtin_isfinite(det)) return false;
     a.z *= sz; b.z *= sz; c.z *= sz;
     const float ts = ea * a.z + eb * b.z + ec * c.z;
     if (det > 0.0f) {
@@ -136,7 +138,7 @@ __device__ __forceinline__ bool wt_intersect(V3 va, V3 vb, V3 vc, V3 org, V3 dir
     return true;
 }
 
-__device__ __forceinline__ void load_tri(const TriRec* tris, uint32_t i, V3* a, V3* b, V3* c) {
+PHG_HD void load_tri(const TriRec* tris, uint32_t i, V3* a, V3* b, V3* c) {
     const float4* p = reinterpret_cast<const float4*>(tris + i);
     const float4 q0 = p[0], q1 = p[1], q2 = p[2];
     *a = mk(q0.x, q0.y, q0.z); *b = mk(q0.w, q1.x, q1.y); *c = mk(q1.z, q1.w, q2.x);
@@ -148,16 +150,16 @@ struct SurfaceHit {
     V3 P, Ng, Ns, bx, bz;  // shading basis: x = bx, y = Ns, z = bz
 };
 
-__device__ __forceinline__ V3 face_normal(V3 a, V3 b, V3 c) {  // TBasicTriangle.ipp:116-121, 268-275
+PHG_HD V3 face_normal(V3 a, V3 b, V3 c) {  // TBasicTriangle.ipp:116-121, 268-275
     const V3 cr = cross(b - a, c - a);
     const float cf = dot(cr, cr);
-    const bool degenerate = !(cf > 0.0f) || isinf(cf);
+    const bool degenerate = !(cf > 0.0f) || __builtin_isinf(cf);
     return !degenerate ? safe_normalize(cr, mk(0.0f, 1.0f, 0.0f)) : mk(0.0f, 1.0f, 0.0f);
 }
-__device__ __forceinline__ float tri_area(V3 a, V3 b, V3 c) { return length(cross(b - a, c - a)) * 0.5f; }
+PHG_HD float tri_area(V3 a, V3 b, V3 c) { return length(cross(b - a, c - a)) * 0.5f; }
 
 // TOrthonormalBasis3.ipp:16-39 makeFromUnitY
-__device__ __forceinline__ void basis_from_unit_y(V3 y, V3* bx, V3* bz) {
+PHG_HD void basis_from_unit_y(V3 y, V3* bx, V3* bz) {
     V3 x;
     if (fabsf(y.y) < kRcpSqrt2) {
         x = mk(-y.z, 0.0f, y.x);
@@ -170,7 +172,7 @@ __device__ __forceinline__ void basis_from_unit_y(V3 y, V3* bx, V3* bz) {
     *bz = cross(x, y);
 }
 
-__device__ __forceinline__ SurfaceHit make_hit(V3 a, V3 b, V3 c, uint32_t tri, float ba, float bb, float bc) {
+PHG_HD SurfaceHit make_hit(V3 a, V3 b, V3 c, uint32_t tri, float ba, float bb, float bc) {
     SurfaceHit h;
     h.tri = tri;
     h.P = wsum(a, ba, b, bb, c, bc);
@@ -179,30 +181,30 @@ __device__ __forceinline__ SurfaceHit make_hit(V3 a, V3 b, V3 c, uint32_t tri, f
     basis_from_unit_y(h.Ns, &h.bx, &h.bz);
     return h;
 }
-__device__ __forceinline__ V3 to_local(const SurfaceHit& h, V3 w) { return mk(dot(h.bx, w), dot(h.Ns, w), dot(h.bz, w)); }
-__device__ __forceinline__ V3 to_world(const SurfaceHit& h, V3 l) { return h.bx * l.x + h.Ns * l.y + h.bz * l.z; }
+PHG_HD V3 to_local(const SurfaceHit& h, V3 w) { return mk(dot(h.bx, w), dot(h.Ns, w), dot(h.bz, w)); }
+PHG_HD V3 to_world(const SurfaceHit& h, V3 l) { return h.bx * l.x + h.Ns * l.y + h.bz * l.z; }
 
 // Strict sidedness (Core/LTA/SidednessAgreement.h:104-283)
-__device__ __forceinline__ bool side_agreed(const SurfaceHit& X, V3 v) { return dot(X.Ng, v) * dot(X.Ns, v) > 0.0f; }
-__device__ __forceinline__ bool same_hemi(const SurfaceHit& X, V3 a, V3 b) {
+PHG_HD bool side_agreed(const SurfaceHit& X, V3 v) { return dot(X.Ng, v) * dot(X.Ns, v) > 0.0f; }
+PHG_HD bool same_hemi(const SurfaceHit& X, V3 a, V3 b) {
     return side_agreed(X, a) && side_agreed(X, b) && dot(a, X.Ng) * dot(b, X.Ng) > 0.0f;
 }
-__device__ __forceinline__ bool opposite_hemi(const SurfaceHit& X, V3 a, V3 b) {
+PHG_HD bool opposite_hemi(const SurfaceHit& X, V3 a, V3 b) {
     return side_agreed(X, a) && side_agreed(X, b) && dot(a, X.Ng) * dot(b, X.Ng) < 0.0f;
 }
 
 // ---- GGX (Core/SurfaceBehavior/Property/{IsoTrowbridgeReitz,TrowbridgeReitz,ShapeInvariantMicrofacet}) --
-__device__ __forceinline__ float mat_alpha(const phg_material& m) {
+PHG_HD float mat_alpha(const phg_material& m) {
     const float a = m.roughness * m.roughness;
     return a > 0.001f ? a : 0.001f;
 }
-__device__ __forceinline__ float ggx_lambda(V3 N, V3 w, float alpha) {
+PHG_HD float ggx_lambda(V3 N, V3 w, float alpha) {
     const float a2 = alpha * alpha;
     const float nod = dot(N, w);
     const float nod2 = fminf(nod * nod, 1.0f);
     return 0.5f * (-1.0f + sqrtf(1.0f + a2 * (1.0f / nod2 - 1.0f)));
 }
-__device__ __forceinline__ float ggx_D(V3 N, V3 H, float alpha) {
+PHG_HD float ggx_D(V3 N, V3 H, float alpha) {
     const float noh = dot(N, H);
     const float a2 = alpha * alpha;
     const float noh2 = noh * noh;
@@ -210,19 +212,19 @@ __device__ __forceinline__ float ggx_D(V3 N, V3 H, float alpha) {
     const float den = kPi * inner * inner;
     return a2 / den;
 }
-__device__ __forceinline__ float ggx_G(V3 N, V3 H, V3 L, V3 V, float alpha) {
+PHG_HD float ggx_G(V3 N, V3 H, V3 L, V3 V, float alpha) {
     const float nol = dot(N, L), nov = dot(N, V), hol = dot(H, L), hov = dot(H, V);
     if (!(hol * nol > 0.0f && hov * nov > 0.0f)) return 0.0f;
     const float ll = ggx_lambda(N, L, alpha), lv = ggx_lambda(N, V, alpha);
     return 1.0f / (1.0f + ll + lv);
 }
-__device__ __forceinline__ float ggx_vndf_pdf(V3 N, V3 H, V3 V, float alpha) {
+PHG_HD float ggx_vndf_pdf(V3 N, V3 H, V3 V, float alpha) {
     const float lv = ggx_lambda(N, V, alpha);
     const float g1v = 1.0f / (1.0f + lv);
     const float ndf = ggx_D(N, H, alpha);
     return g1v * ndf * fabsf(dot(H, V) / dot(N, V));
 }
-__device__ __forceinline__ V3 ggx_sample_vh(const SurfaceHit& X, V3 N, V3 V, float alpha, float u0, float u1) {
+PHG_HD V3 ggx_sample_vh(const SurfaceHit& X, V3 N, V3 V, float alpha, float u0, float u1) {
     const V3 upperV = dot(V, N) >= 0.0f ? V : neg(V);
     const V3 lv = to_local(X, upperV);
     const V3 sv = normalize(mk(lv.x * alpha, lv.y, lv.z * alpha));
@@ -238,12 +240,12 @@ __device__ __forceinline__ V3 ggx_sample_vh(const SurfaceHit& X, V3 N, V3 V, flo
 
 // Fresnel (SchlickApproxConductorFresnel.cpp:35-43, ExactDielectricFresnel.cpp:12-41,
 // DielectricFresnel.cpp:18-42)
-__device__ __forceinline__ RGB fresnel_schlick(const phg_material& m, float cosI) {
+PHG_HD RGB fresnel_schlick(const phg_material& m, float cosI) {
     const float c = fabsf(cosI);
     const float p = (float)pow((double)(1.0f - c), 5.0);
     return rgb3((1.0f - m.rgb[0]) * p + m.rgb[0], (1.0f - m.rgb[1]) * p + m.rgb[1], (1.0f - m.rgb[2]) * p + m.rgb[2]);
 }
-__device__ __forceinline__ float fresnel_dielectric(float etaOuter, float etaInner, float cosThetaI) {
+PHG_HD float fresnel_dielectric(float etaOuter, float etaInner, float cosThetaI) {
     float etaI = etaOuter, etaT = etaInner, cosI = cosThetaI;
     if (cosThetaI < 0.0f) { const float t = etaI; etaI = etaT; etaT = t; cosI = fabsf(cosI); }
     const float ratio = etaI / etaT;
@@ -254,7 +256,7 @@ __device__ __forceinline__ float fresnel_dielectric(float etaOuter, float etaInn
     const float rper = (etaI * cosI - etaT * cosT) / (etaI * cosI + etaT * cosT);
     return 0.5f * (rpar * rpar + rper * rper);
 }
-__device__ __forceinline__ bool refract_dir(float etaOuter, float etaInner, V3 I, V3 N, V3* out) {
+PHG_HD bool refract_dir(float etaOuter, float etaInner, V3 I, V3 N, V3* out) {
     const float ion = dot(I, N);
     const float sgn = ion < 0.0f ? -1.0f : 1.0f;
     const float ratio = sgn < 0.0f ? etaInner / etaOuter : etaOuter / etaInner;
@@ -265,10 +267,10 @@ __device__ __forceinline__ bool refract_dir(float etaOuter, float etaInner, V3 I
     *out = normalize(N * nf + I * iff);
     return true;
 }
-__device__ __forceinline__ float reflect_prob(float F) { return clampf(F, 0.0001f, 1.0f - 0.0001f); }
+PHG_HD float reflect_prob(float F) { return clampf(F, 0.0001f, 1.0f - 0.0001f); }
 
 // BsdfHelper.h:14-45
-__device__ __forceinline__ bool half_vec_same_hemi(V3 L, V3 V, V3 N, V3* H) {
+PHG_HD bool half_vec_same_hemi(V3 L, V3 V, V3 N, V3* H) {
     V3 h = L + V;
     if (is_zero(h)) return false;
     h = normalize(h);
@@ -283,7 +285,7 @@ struct BsdfEval { bool ok; RGB f; };
 struct BsdfSample { bool ok; V3 L; RGB w; float relIor; };
 
 // SurfaceOptics::calcBsdf + LambertianReflector/OpaqueMicrofacet/TranslucentMicrofacet::calcBsdf
-__device__ inline BsdfEval bsdf_eval(const phg_material& m, const SurfaceHit& X, V3 L, V3 V) {
+PHG_HD BsdfEval bsdf_eval(const phg_material& m, const SurfaceHit& X, V3 L, V3 V) {
     BsdfEval e; e.ok = false; e.f = rgb1(0.0f);
     if (!side_agreed(X, L) || !side_agreed(X, V)) return e;
     const V3 N = X.Ns;
@@ -331,7 +333,7 @@ __device__ inline BsdfEval bsdf_eval(const phg_material& m, const SurfaceHit& X,
             const float G = ggx_G(N, H, L, V, a);
             const float transport = etaT / etaI;
             const float iorTerm = transport * etaI / (etaI * hol + etaT * hov);
-            if (!isfinite(iorTerm)) return e;
+            if (!__builtin_isfinite(iorTerm)) return e;
             const float dotTerm = fabsf(hol * hov / nolnov);
             e.f = rgb1(F * (D * G * dotTerm * (iorTerm * iorTerm)));
             e.ok = rgb_finite(e.f);
@@ -342,7 +344,7 @@ __device__ inline BsdfEval bsdf_eval(const phg_material& m, const SurfaceHit& X,
 
 // SurfaceOptics::calcBsdfPdf + material calcBsdfPdf; a result that is not finite and > 0
 // means "no pdf" (lta::PDF::operator bool, Core/LTA/PDF.h).
-__device__ inline float bsdf_pdf(const phg_material& m, const SurfaceHit& X, V3 L, V3 V) {
+PHG_HD float bsdf_pdf(const phg_material& m, const SurfaceHit& X, V3 L, V3 V) {
     if (!side_agreed(X, L) || !side_agreed(X, V)) return 0.0f;
     const V3 N = X.Ns;
     if (m.type == PHG_MAT_LAMBERT) {
@@ -383,11 +385,11 @@ __device__ inline float bsdf_pdf(const phg_material& m, const SurfaceHit& X, V3 
     }
     return 0.0f;
 }
-__device__ __forceinline__ bool pdf_ok(float p) { return isfinite(p) && p > 0.0f; }
-__device__ __forceinline__ bool measurable(RGB w, float cosv) { return rgb_finite(w) && 0.0f < cosv && cosv < 1.1f; }
+PHG_HD bool pdf_ok(float p) { return __builtin_isfinite(p) && p > 0.0f; }
+PHG_HD bool measurable(RGB w, float cosv) { return rgb_finite(w) && 0.0f < cosv && cosv < 1.1f; }
 
 // SurfaceOptics::genBsdfSample + material genBsdfSample.  u0, u1: direction; u2: lobe pick.
-__device__ inline BsdfSample bsdf_sample(const phg_material& m, const SurfaceHit& X, V3 V, float u0, float u1, float u2) {
+PHG_HD BsdfSample bsdf_sample(const phg_material& m, const SurfaceHit& X, V3 V, float u0, float u1, float u2) {
     BsdfSample s; s.ok = false; s.L = mk(0.0f, 0.0f, 0.0f); s.w = rgb1(0.0f); s.relIor = 1.0f;
     if (!side_agreed(X, V)) return s;
     const V3 N = X.Ns;
@@ -457,20 +459,20 @@ constexpr float kMeanErr = 5e-8f;
 constexpr float kMaxErr = 5e-8f * 5e2f;
 constexpr int kMaxEscapeIters = 256;
 
-__device__ __forceinline__ bool reintersect(V3 a, V3 b, V3 c, V3 org, V3 dir, float minT, float maxT, float* t) {
+PHG_HD bool reintersect(V3 a, V3 b, V3 c, V3 org, V3 dir, float minT, float maxT, float* t) {
     float x, y, z;
     return wt_intersect(a, b, c, org, dir, minT, maxT, t, &x, &y, &z);
 }
 
 struct TriVerts { V3 a, b, c; };
 
-__device__ inline void iterative_offset(const SurfaceHit& X, const TriVerts& tv, V3 dir, V3* out_offset, V3* out_dir) {
+PHG_HD void iterative_offset(const SurfaceHit& X, const TriVerts& tv, V3 dir, V3* out_offset, V3* out_dir) {
     const V3 N = X.Ng;
     const V3 offsetDir = dot(N, dir) > 0.0f ? N : neg(N);
     const V3 escDir = normalize(dir);
     const float escCos = clampf(dot(escDir, offsetDir), 0.0f, 1.0f);
     float init = length(X.P) * kMeanErr;
-    if (!isfinite(init)) init = fmaxf(length(X.P) * 1e-6f, 2e-4f);
+    if (!__builtin_isfinite(init)) init = fmaxf(length(X.P) * 1e-6f, 2e-4f);
     init = init + kFltMin;
     float maxD = init, minD = 0.0f;
     for (int it = 0; it < kMaxEscapeIters; ++it) {
@@ -492,13 +494,13 @@ __device__ inline void iterative_offset(const SurfaceHit& X, const TriVerts& tv,
     *out_dir = escDir;
 }
 
-__device__ inline bool mutual_escape(const SurfaceHit& X, const TriVerts& tx, const SurfaceHit& X2, const TriVerts& t2,
+PHG_HD bool mutual_escape(const SurfaceHit& X, const TriVerts& tx, const SurfaceHit& X2, const TriVerts& t2,
                                      V3* org, V3* dir, float* maxT) {
     const V3 x2x = X2.P - X.P;
     V3 off, ud;
     iterative_offset(X, tx, x2x, &off, &ud);
     float init = length(X2.P) * kMaxErr;
-    if (!isfinite(init)) init = fmaxf(length(X2.P) * 1e-3f, 2e-4f);
+    if (!__builtin_isfinite(init)) init = fmaxf(length(X2.P) * 1e-3f, 2e-4f);
     init = init + kFltMin;
     const V3 o = X.P + off;
     const float dist = length(o - X2.P);
@@ -519,20 +521,20 @@ __device__ inline bool mutual_escape(const SurfaceHit& X, const TriVerts& tx, co
         else minD = clampf(dist - t, mid, maxD);
     }
     const float len = fmaxf(dist - maxD, 0.0f);
-    if (!(len > 0.0f && isfinite(len))) return false;
+    if (!(len > 0.0f && __builtin_isfinite(len))) return false;
     *org = o; *dir = ud; *maxT = len;
     return true;
 }
 
 // ---- emitters (ESPowerFavoring.cpp:60-97, GroupedDiffuseSurfaceEmitter.cpp:58-84,
 //      DiffuseSurfaceEmitterBase.cpp:34-89, SurfaceEmitter.cpp:20-61, lta.h:17-25) ----------
-__device__ __forceinline__ float pdfA_to_pdfW(float pdfA, V3 posToTarget, V3 N) {
+PHG_HD float pdfA_to_pdfW(float pdfA, V3 posToTarget, V3 N) {
     const float d2 = dot(posToTarget, posToTarget);
     const float s = pdfA / dot(normalize(posToTarget), N) * d2;
-    return isfinite(s) ? fabsf(s) : 0.0f;
+    return __builtin_isfinite(s) ? fabsf(s) : 0.0f;
 }
 // TPwcDistribution1D::sampleDiscrete: lower_bound over cdf[0..n]
-__device__ __forceinline__ uint32_t cdf_sample(const float* cdf, uint32_t n, float u) {
+PHG_HD uint32_t cdf_sample(const float* cdf, uint32_t n, float u) {
     uint32_t lo = 0, len = n + 1;
     while (len > 0) {
         const uint32_t half = len >> 1;
@@ -545,6 +547,6 @@ __device__ __forceinline__ uint32_t cdf_sample(const float* cdf, uint32_t n, flo
     }
     return lo - 1;
 }
-__device__ __forceinline__ float mis_power(float p0, float p1) { return (p0 * p0) / (p0 * p0 + p1 * p1); }
+PHG_HD float mis_power(float p0, float p1) { return (p0 * p0) / (p0 * p0 + p1 * p1); }
 
 }  // namespace phg

@@ -277,7 +277,7 @@ static void gen_procedural(SceneBuf& s, const phg_gen_opts& o) {
         const float r = rng.uf(1, 20);
         s.light_down(x, x + 2.0f, z, z + 2.0f, 13.0f, f3(r, r, r), absorber);
     }
-    s.camera(f3(0, 4, -34), f3(0, -2, 0), f3(0, 1, 0), o.fov_deg > 0 ? o.fov_deg : 50.0f);
+    s.camera(f3(0, 12, -24), f3(0, -6, 0), f3(0, 1, 0), o.fov_deg > 0 ? o.fov_deg : 60.0f);
 }
 
 static void gen_room(SceneBuf& s, const phg_gen_opts& o) {

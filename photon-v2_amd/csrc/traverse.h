@@ -1,0 +1,109 @@
+// 4-wide BVH traversal of one ray (host and device): 128-byte node fetches, child boxes
+// tested against the shrinking segment with the reference's robust slab test
+// (Math/Geometry/TAABB3D.ipp:390-420) widened by the watertight test's error bound,
+// near-first order with a per-ray stack addressed as
+// stk[depth * stride] (in LDS on the device: [depth][lane], bank-conflict free), leaves
+// tested with the watertight triangle test.  Closest hit keeps shrinking the segment to the
+// accepted hit as TLinearDepthFirstBinaryBvh::nearestTraversal does
+// (Math/Algorithm/BVH/TLinearDepthFirstBinaryBvh.ipp:43-139); any hit returns at the first.
+#pragma once
+
+#include "pt_device.h"
+
+namespace phg {
+
+struct TraceResult { float t; int32_t tri; float a, b, c; };
+
+template <int STACK, bool ANY, bool COUNT>
+PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris,
+                                                V3 o, V3 d, float tmin, float tmax, int32_t* __restrict__ stk,
+                                                uint32_t stride, uint32_t* nvis, uint32_t* ntri, uint32_t* overflow) {
+    TraceResult r; r.t = 0.0f; r.tri = -1; r.a = 0.0f; r.b = 0.0f; r.c = 0.0f;
+    const V3 rcp = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool nx = d.x < 0.0f, ny = d.y < 0.0f, nz = d.z < 0.0f;
+    constexpr float mult = 1.1920928955078125e-07f * 2.0f + 1.0f;  // TAABB3D.ipp:417
+    // dominant axis of the direction, chosen as the watertight test permutes it
+    const float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
+    const int kdom = adx > ady ? (adx > adz ? 0 : 2) : (ady > adz ? 1 : 2);
+    float tb = tmax;
+    int sp = 0;
+    int32_t cur = 0;
+    while (true) {
+        if (cur >= 0) {
+            const float4* p = reinterpret_cast<const float4*>(nodes + cur);
+            const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+            const int4 ch = reinterpret_cast<const int4*>(p)[6];
+            if (COUNT) (*nvis)++;
+            const float bl[4][6] = {{lx.x, hx.x, ly.x, hy.x, lz.x, hz.x}, {lx.y, hx.y, ly.y, hy.y, lz.y, hz.y},
+                                    {lx.z, hx.z, ly.z, hy.z, lz.z, hz.z}, {lx.w, hx.w, ly.w, hy.w, lz.w, hz.w}};
+            const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
+            float key[4];
+            int32_t kid[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                // robust slab (TAABB3D.ipp:390-420): entry t0 over [tmin, ...), exit t1
+                float t0 = tmin, t1 = kFltMax;
+                const float mx = ((nx ? bl[k][1] : bl[k][0]) - o.x) * rcp.x;
+                const float Mx = ((nx ? bl[k][0] : bl[k][1]) - o.x) * rcp.x;
+                const float my = ((ny ? bl[k][3] : bl[k][2]) - o.y) * rcp.y;
+                const float My = ((ny ? bl[k][2] : bl[k][3]) - o.y) * rcp.y;
+                const float mz = ((nz ? bl[k][5] : bl[k][4]) - o.z) * rcp.z;
+                const float Mz = ((nz ? bl[k][4] : bl[k][5]) - o.z) * rcp.z;
+                t0 = mx > t0 ? mx : t0; t1 = Mx < t1 ? Mx : t1;
+                t0 = my > t0 ? my : t0; t1 = My < t1 ? My : t1;
+                t0 = mz > t0 ? mz : t0; t1 = Mz < t1 ? Mz : t1;
+                // The watertight t of a triangle in this box is a convex combination of its
+                // vertices' distances along the dominant axis, so its rounding error scales
+                // with the box's parametric span on that axis, not with t itself: widen the
+                // test against the current closest hit by 2^-19 of that span.
+                const float sm = kdom == 0 ? mx : (kdom == 1 ? my : mz);
+                const float sM = kdom == 0 ? Mx : (kdom == 1 ? My : Mz);
+                const float slack = fmaxf(fabsf(sm), fabsf(sM)) * 0x1p-19f;
+                const bool hit = (t0 <= t1 * mult + slack) && (t0 <= tb + slack) && cc[k] != kEmptyChild;
+                key[k] = hit ? t0 : INFINITY;
+                kid[k] = cc[k];
+            }
+            // sorting network on (key, kid), ascending
+#define PHG_CSWAP(i, j)                                                   \
+    if (key[j] < key[i]) {                                                \
+        const float tk = key[i]; key[i] = key[j]; key[j] = tk;            \
+        const int32_t ti = kid[i]; kid[i] = kid[j]; kid[j] = ti;          \
+    }
+            PHG_CSWAP(0, 1) PHG_CSWAP(2, 3) PHG_CSWAP(0, 2) PHG_CSWAP(1, 3) PHG_CSWAP(1, 2)
+#undef PHG_CSWAP
+            if (key[0] == INFINITY) {
+                if (sp == 0) break;
+                cur = stk[(--sp) * stride];
+                continue;
+            }
+#pragma unroll
+            for (int k = 3; k >= 1; --k) {
+                if (key[k] != INFINITY) {
+                    if (sp < STACK) stk[(sp++) * stride] = kid[k];
+                    else *overflow = 1u;
+                }
+            }
+            cur = kid[0];
+        } else {
+            const uint32_t first = (uint32_t)cur & ((1u << kLeafFirstBits) - 1u);
+            const uint32_t cnt = (((uint32_t)cur >> kLeafFirstBits) & 7u) + 1u;
+            if (COUNT) (*ntri) += cnt;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                V3 a, b, c;
+                load_tri(tris, first + i, &a, &b, &c);
+                float t, ba, bb, bc;
+                if (wt_intersect(a, b, c, o, d, tmin, tb, &t, &ba, &bb, &bc)) {
+                    tb = t;
+                    r.t = t; r.tri = (int32_t)(first + i); r.a = ba; r.b = bb; r.c = bc;
+                    if (ANY) return r;
+                }
+            }
+            if (sp == 0) break;
+            cur = stk[(--sp) * stride];
+        }
+    }
+    return r;
+}
+
+
+}  // namespace phg

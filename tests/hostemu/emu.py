@@ -1,0 +1,37 @@
+"""ctypes binding of the test-only host build of the product's BVH build + traversal."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libhostemu.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        _lib = C.CDLL(LIB)
+        vp = C.c_void_p
+        _lib.emu_intersect.argtypes = [vp, C.c_uint32, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32,
+                                       vp, vp, vp, vp, vp, vp]
+    return _lib
+
+
+def intersect(verts, rays, any_hit=False, max_leaf=4, bins=16):
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    prim = np.zeros(n, np.int32)
+    t, b, c = (np.zeros(n, np.float32) for _ in range(3))
+    nv, nt = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    st = lib().emu_intersect(verts.ctypes.data, verts.shape[0], rays.ctypes.data, n, int(any_hit), max_leaf, bins,
+                             prim.ctypes.data, t.ctypes.data, b.ctypes.data, c.ctypes.data, nv.ctypes.data,
+                             nt.ctypes.data)
+    if st < 0:
+        raise RuntimeError(f"emu_intersect failed ({st})")
+    return prim, t, b, c, nv, nt

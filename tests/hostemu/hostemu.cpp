@@ -1,0 +1,40 @@
+// TEST INFRASTRUCTURE ONLY: runs the product's wide-BVH build and its traversal code
+// (photon-v2_amd/csrc/traverse.h, compiled for the host) on the CPU, so BVH conservativeness
+// and traversal semantics can be checked against the oracle without a GPU.  Never loaded by
+// the product.
+#include <cstring>
+#include <vector>
+
+#include "../../photon-v2_amd/csrc/traverse.h"
+
+using namespace phg;
+
+extern "C" int emu_intersect(const float* verts, uint32_t ntri, const float* rays, uint32_t n, int any,
+                             uint32_t max_leaf, uint32_t bins, int32_t* out_prim, float* out_t, float* out_b,
+                             float* out_c, uint32_t* out_nvis, uint32_t* out_ntri) {
+    BvhBuildParams p;
+    p.max_leaf_tris = max_leaf ? max_leaf : 4;
+    p.sah_bins = bins ? bins : 16;
+    BvhBuildResult bvh = build_wide_bvh(verts, ntri, p);
+    if (bvh.stack_need > 64) return -1;
+    std::vector<TriRec> tr(ntri);
+    for (uint32_t k = 0; k < ntri; ++k) {
+        const uint32_t t = bvh.tri_order[k];
+        std::memcpy(tr[k].a, verts + 9 * (size_t)t, 9 * sizeof(float));
+        tr[k].prim = (int32_t)t; tr[k].mat = 0; tr[k].emitter = -1;
+    }
+    int32_t stack[64];
+    uint32_t overflow = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * (size_t)i;
+        uint32_t nv = 0, nt = 0;
+        TraceResult res = any ? traverse<64, true, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]),
+                                                        r[3], r[7], stack, 1, &nv, &nt, &overflow)
+                              : traverse<64, false, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]),
+                                                         r[3], r[7], stack, 1, &nv, &nt, &overflow);
+        out_prim[i] = res.tri >= 0 ? tr[res.tri].prim : -1;
+        out_t[i] = res.t; out_b[i] = res.b; out_c[i] = res.c;
+        out_nvis[i] = nv; out_ntri[i] = nt;
+    }
+    return overflow ? -2 : (int)bvh.stack_need;
+}

@@ -1,0 +1,66 @@
+"""The product's wide-BVH builder and traversal code (photon-v2_amd/csrc/traverse.h),
+compiled for the host by tests/hostemu, against the oracle's brute-force closest hit.
+Same bar as the GPU test: identical primitive ids and bit-identical t except exact ties."""
+import numpy as np
+import pytest
+
+import phg
+from oracle import oracle as orc
+from helpers import FLT_MAX, random_rays, random_soup, rays_towards
+from hostemu import emu
+
+
+def check(verts, rays, **kw):
+    prim, t, b, c, nv, nt = emu.intersect(verts, rays, **kw)
+    tri, tt, bary, flags = orc.intersect_brute(verts, rays)
+    ok = (flags & 1) == 0
+    assert np.array_equal(prim[ok], tri[ok])
+    h = ok & (tri >= 0)
+    assert np.array_equal(t[h].view(np.uint32), tt[h].view(np.uint32))
+    assert np.array_equal(b[h].view(np.uint32), bary[h, 1].view(np.uint32))
+    assert np.array_equal(c[h].view(np.uint32), bary[h, 2].view(np.uint32))
+    anyp = emu.intersect(verts, rays, any_hit=True, **kw)[0]
+    assert np.array_equal(anyp >= 0, tri >= 0)
+    return nv, nt
+
+
+def test_cornell_coplanar_faces():
+    # the box bottoms are coplanar with the floor: exercises the widened box test
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    check(sc.verts, np.concatenate([rays_towards(sc.verts, 50_000, 1), random_rays(sc.verts, 50_000, 2)]))
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8])
+def test_soup_leaf_sizes(leaf):
+    v = random_soup(1500, 7)
+    check(v, np.concatenate([rays_towards(v, 30_000, 8, jitter=0.01), random_rays(v, 20_000, 9)]), max_leaf=leaf)
+
+
+def test_segments_and_backward_rays():
+    v = random_soup(400, 11)
+    rays = rays_towards(v, 20_000, 12)
+    rng = np.random.default_rng(13)
+    rays[:, 3] = rng.uniform(0, 5, len(rays)).astype(np.float32)
+    rays[:, 7] = (rays[:, 3] + rng.uniform(0, 30, len(rays))).astype(np.float32)
+    away = rays.copy()
+    away[:, 4:7] *= -1
+    check(v, np.concatenate([rays, away]))
+
+
+def test_axis_aligned_directions():
+    # zero direction components: infinite reciprocals in the slab test
+    v = random_soup(300, 14)
+    rng = np.random.default_rng(15)
+    n = 6000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform(-12, 12, (n, 3))
+    axis = rng.integers(0, 3, n)
+    rays[np.arange(n), 4 + axis] = rng.choice([-1.0, 1.0], n)
+    rays[:, 7] = FLT_MAX
+    check(v, rays)
+
+
+def test_procedural_scene_sample():
+    sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=3, target_tris=60_000, num_instances=8, num_lights=4, subdiv=3)
+    nv, nt = check(sc.verts, np.concatenate([rays_towards(sc.verts, 3000, 21), random_rays(sc.verts, 3000, 22)]))
+    assert nv.mean() < 80
