@@ -129,10 +129,18 @@ typedef struct phg_render_opts {
     uint32_t spp_per_pass;     /* samples per pixel per wavefront pass; 0 = automatic */
     uint64_t seed;
     uint32_t flags;            /* PHG_RENDER_* */
+    float guide_prob;          /* PHG_RENDER_GUIDED: probability of sampling the learned lobes (one-sample MIS) */
+    float guide_lr;            /* PHG_RENDER_TRAIN: Adam learning rate */
+    uint32_t guide_batch;      /* PHG_RENDER_TRAIN: records per training step (0 = 2^18) */
+    uint32_t guide_steps;      /* PHG_RENDER_TRAIN: Adam steps after each pass */
     uint32_t pad;
 } phg_render_opts;
 
-enum { PHG_RENDER_COUNT_TRAVERSAL = 1 };  /* count node visits / triangle tests (slower) */
+enum {
+    PHG_RENDER_COUNT_TRAVERSAL = 1,  /* count node visits / triangle tests (slower) */
+    PHG_RENDER_GUIDED = 2,           /* sample directions from the guiding network too (one-sample MIS with the BSDF) */
+    PHG_RENDER_TRAIN = 4             /* train the guiding network online on this render's radiance samples */
+};
 
 typedef struct phg_stats {
     uint64_t paths;             /* completed camera paths (samples) */
@@ -156,6 +164,12 @@ typedef struct phg_stats {
     uint64_t shadow_tri_tests;
     uint32_t trace_launches;      /* closest-hit kernel launches in the last render */
     uint32_t shadow_launches;
+    double guide_infer_ms;        /* GPU time in guiding-network inference (last render) */
+    double guide_train_ms;        /* GPU time in guiding-network training (last render) */
+    uint64_t guide_queries;       /* network evaluations (last render) */
+    uint64_t guide_records;       /* training records used (last render) */
+    uint32_t guide_steps;         /* Adam steps taken so far on this context */
+    uint32_t pad2;
 } phg_stats;
 
 /* Context lifetime.  Replaces nothing in the reference: the CPU engine needs no device. */
@@ -201,6 +215,24 @@ phg_status phg_render(void* ctx, const phg_render_opts* opts);
 /* Statistics of the last render/intersect (Renderer::asyncQueryRenderStats,
  * Core/Renderer/Sampling/EqualSamplingRenderer.cpp:258-280, reports paths/second). */
 phg_status phg_get_stats(void* ctx, phg_stats* out);
+
+/* ---- Path-guiding network (no counterpart in the reference, which has no guiding).
+ * A 64-64-64-64-64-40 ReLU MLP over a frequency encoding of (position, normal, outgoing
+ * direction) producing 8 von Mises-Fisher lobes over the incident direction; BF16 on the
+ * matrix cores with FP32 accumulation.  Parameters: one FP32 blob of PHG_GUIDE_PARAMS values,
+ * W0 b0 W1 b1 W2 b2 W3 b3 W4 b4 with each W stored [in][out].
+ * Training record: 13 floats = 9 features in [0,1] (position in the scene box, normal and
+ * outgoing direction mapped from [-1,1]), incident direction xyz, weight (radiance / pdf). */
+enum { PHG_GUIDE_PARAMS = 19240, PHG_GUIDE_OUTPUTS = 40, PHG_GUIDE_RECORD = 13 };
+
+phg_status phg_guide_set_params(void* ctx, const float* params, uint32_t n);
+phg_status phg_guide_get_params(void* ctx, float* params, uint32_t n);
+/* raw network outputs (40 per query) for 9 features per query */
+phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out);
+/* gradient of the mean radiance-weighted negative log-likelihood over the records */
+phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, float* grads);
+/* `steps` Adam steps on the same records (learning rate lr) */
+phg_status phg_guide_train(void* ctx, const float* records, uint64_t n, float lr, uint32_t steps);
 
 /* ---- Procedural scene generation (host).  Not a reference interface: produces the
  * benchmark workloads as flattened scenes.  The returned arrays are owned by the library

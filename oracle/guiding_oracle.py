@@ -1,0 +1,184 @@
+"""CPU oracle of the path-guiding network — TEST INFRASTRUCTURE ONLY.
+
+Restates, in numpy FP64, the guiding model the GPU trains and queries: frequency encoding of
+(position, normal, outgoing direction), a 64-64-64-64-64-40 ReLU MLP, and a mixture of 8
+von Mises-Fisher lobes over the incident direction, with the radiance-weighted negative
+log-likelihood used for online training and its analytic gradients.  The reference renderer
+has no path guiding; this is the specification the GPU kernels are checked against.  The
+analytic gradients are themselves pinned by central finite differences in the tests.
+
+Optionally rounds the layer inputs and weights to bfloat16 (round to nearest even) at the
+points where the GPU feeds the matrix cores, so that forward values can be compared tightly.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+N_IN = 64
+N_HID = 64
+N_LAYERS = 5            # 4 hidden layers of 64, then the output layer
+N_LOBES = 8
+N_OUT = 5 * N_LOBES     # per lobe: mean direction (3, unnormalised), log kappa, weight logit
+KAPPA_LOG_MIN, KAPPA_LOG_MAX = -8.0, 7.0
+LAYER_SHAPES = [(N_IN, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_OUT)]
+
+
+def bf16_round(x):
+    """Round float values to bfloat16 (nearest, ties to even), returned as float64."""
+    a = np.asarray(x, np.float32).copy()
+    u = a.view(np.uint32).astype(np.uint64)
+    rounding = ((u >> 16) & 1) + 0x7FFF
+    u = ((u + rounding) >> 16) << 16
+    out = u.astype(np.uint32).view(np.float32)
+    nan = np.isnan(a)
+    out[nan] = np.nan
+    return out.astype(np.float64)
+
+
+def features(P, N, V, bbox_min, bbox_max):
+    """9 scalars in [0, 1]: position in the scene box, normal and outgoing direction."""
+    P, N, V = (np.asarray(a, np.float64).reshape(-1, 3) for a in (P, N, V))
+    lo, hi = np.asarray(bbox_min, np.float64), np.asarray(bbox_max, np.float64)
+    p = (P - lo) / np.maximum(hi - lo, 1e-30)
+    return np.concatenate([p, 0.5 * (N + 1.0), 0.5 * (V + 1.0)], axis=1)
+
+
+def encode(x9):
+    """63 values (9 raw + sin/cos at 3 frequency bands) padded with a constant 1 -> 64."""
+    x = np.asarray(x9, np.float64).reshape(-1, 9)
+    n = x.shape[0]
+    e = np.zeros((n, N_IN))
+    e[:, 0:9] = x
+    for k in range(3):
+        a = (2.0 ** k) * np.pi * x
+        e[:, 9 + 18 * k + 0:9 + 18 * k + 18:2] = np.sin(a)
+        e[:, 9 + 18 * k + 1:9 + 18 * k + 18:2] = np.cos(a)
+    e[:, 63] = 1.0
+    return e
+
+
+def init_params(seed):
+    """He-uniform weights (stored as [in, out]) and zero biases from a numpy stream; tests
+    hand these arrays to the GPU so that both sides start from the same parameters."""
+    rng = np.random.default_rng(seed)
+    params = []
+    for fan_in, fan_out in LAYER_SHAPES:
+        lim = np.sqrt(6.0 / fan_in)
+        params.append((rng.uniform(-lim, lim, (fan_in, fan_out)), np.zeros(fan_out)))
+    return params
+
+
+def forward(params, e, bf16=False):
+    """Returns the list of layer inputs (e, h1..h4) and the raw outputs."""
+    acts = []
+    h = np.asarray(e, np.float64)
+    for li, (W, b) in enumerate(params):
+        x = bf16_round(h) if bf16 else h
+        Wq = bf16_round(W) if bf16 else W
+        acts.append(x)
+        z = x @ Wq + b
+        h = np.maximum(z, 0.0) if li < N_LAYERS - 1 else z
+    return acts, h
+
+
+def decode(o):
+    """Mixture parameters from raw outputs: mu (n,8,3), kappa (n,8), weights (n,8)."""
+    o = np.asarray(o, np.float64).reshape(-1, N_LOBES, 5)
+    r = o[:, :, 0:3]
+    rl = np.linalg.norm(r, axis=2, keepdims=True)
+    mu = np.where(rl > 0, r / np.where(rl > 0, rl, 1), np.array([0.0, 0.0, 1.0]))
+    kappa = np.exp(np.clip(o[:, :, 3], KAPPA_LOG_MIN, KAPPA_LOG_MAX))
+    logits = o[:, :, 4]
+    w = np.exp(logits - logits.max(axis=1, keepdims=True))
+    w /= w.sum(axis=1, keepdims=True)
+    return mu, kappa, w
+
+
+def vmf_norm(kappa):
+    """kappa / (2 pi (1 - exp(-2 kappa))), with the kappa -> 0 limit 1 / (4 pi)."""
+    k = np.asarray(kappa, np.float64)
+    small = k < 1e-6
+    ks = np.where(small, 1.0, k)
+    return np.where(small, 1.0 / (4 * np.pi), ks / (2 * np.pi * (-np.expm1(-2 * ks))))
+
+
+def mixture_pdf(o, omega):
+    mu, kappa, w = decode(o)
+    om = np.asarray(omega, np.float64).reshape(-1, 1, 3)
+    c = (mu * om).sum(axis=2)
+    p = vmf_norm(kappa) * np.exp(kappa * (c - 1.0))
+    return (w * p).sum(axis=1)
+
+
+def loss(params, e, omega, weight, bf16=False):
+    _, o = forward(params, e, bf16)
+    q = mixture_pdf(o, omega)
+    return -(np.asarray(weight, np.float64) * np.log(q)).mean()
+
+
+def dlogp_dkappa(k):
+    """d/dkappa of log(kappa / (2 pi (1 - exp(-2 kappa)))) + kappa (mu.w - 1), minus the last term."""
+    k = np.asarray(k, np.float64)
+    small = k < 1e-2
+    series = 1.0 - k / 3.0 + k ** 3 / 45.0 - 1.0  # 1/k - 2/expm1(2k) - 1, series near 0, minus 1
+    ks = np.where(small, 1.0, k)
+    with np.errstate(over="ignore"):  # expm1 overflows to inf for large kappa: 2 / inf = 0
+        exact = 1.0 / ks - 2.0 / np.expm1(2.0 * ks) - 1.0
+    return np.where(small, series, exact)
+
+
+def output_grad(o, omega, weight):
+    """d(-weight * log q) / d(raw outputs) per sample (not yet divided by the batch size)."""
+    o = np.asarray(o, np.float64).reshape(-1, N_LOBES, 5)
+    n = o.shape[0]
+    r = o[:, :, 0:3]
+    rl = np.linalg.norm(r, axis=2, keepdims=True)
+    mu, kappa, w = decode(o.reshape(n, -1))
+    om = np.asarray(omega, np.float64).reshape(-1, 1, 3)
+    c = (mu * om).sum(axis=2)
+    p = vmf_norm(kappa) * np.exp(kappa * (c - 1.0))
+    q = (w * p).sum(axis=1, keepdims=True)
+    scale = -np.asarray(weight, np.float64).reshape(-1, 1) / q          # d loss / d q
+    g = np.zeros_like(o)
+    wp = w * p
+    # mean direction through the normalisation
+    g[:, :, 0:3] = (scale * wp * kappa)[:, :, None] * (om - c[:, :, None] * mu) / np.where(rl > 0, rl, 1)
+    # log kappa (zero gradient where clamped)
+    inside = (o[:, :, 3] > KAPPA_LOG_MIN) & (o[:, :, 3] < KAPPA_LOG_MAX)
+    dk = wp * (dlogp_dkappa(kappa) + c) * kappa   # dlogp/dkappa = 1/k - 2/expm1(2k) + (c - 1)
+    g[:, :, 3] = scale * np.where(inside, dk, 0.0)
+    # softmax logits
+    g[:, :, 4] = scale * (wp - w * q)
+    return g.reshape(n, N_OUT)
+
+
+def backward(params, e, omega, weight, bf16=False):
+    """Gradients of the mean loss w.r.t. every (W, b), in the parameter layout."""
+    acts, o = forward(params, e, bf16)
+    n = acts[0].shape[0]
+    g = output_grad(o, omega, weight) / n
+    grads = [None] * N_LAYERS
+    for li in range(N_LAYERS - 1, -1, -1):
+        W, b = params[li]
+        Wq = bf16_round(W) if bf16 else W
+        x = acts[li]
+        grads[li] = (x.T @ g, g.sum(axis=0))
+        if li > 0:
+            gx = g @ Wq.T
+            g = gx * (acts[li] > 0)
+    return grads
+
+
+def adam_step(params, grads, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015) in place; t is the 1-based step count."""
+    out = []
+    for i, ((W, b), (gW, gb)) in enumerate(zip(params, grads)):
+        new = []
+        for j, (x, g) in enumerate(((W, gW), (b, gb))):
+            m[i][j] = b1 * m[i][j] + (1 - b1) * g
+            v[i][j] = b2 * v[i][j] + (1 - b2) * g * g
+            mh = m[i][j] / (1 - b1 ** t)
+            vh = v[i][j] / (1 - b2 ** t)
+            new.append(x - lr * mh / (np.sqrt(vh) + eps))
+        out.append(tuple(new))
+    return out

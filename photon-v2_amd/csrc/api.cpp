@@ -13,6 +13,7 @@
 
 #include "../../include/phg.h"
 #include "bvh_build.h"
+#include "guide.h"
 #include "kernels.h"
 
 namespace phg {
@@ -64,9 +65,21 @@ struct Ctx {
     DevBuf<phg_ray> q_rays;
     DevBuf<phg_hit> q_hits;
     DevBuf<uint32_t> q_nvis, q_ntri;
+    // guiding network
+    DevBuf<float> g_params, g_m, g_v, g_grads, g_bias;
+    DevBuf<uint16_t> g_wt, g_wn;
+    DevBuf<float> g_lobes;       // per queued extension ray
+    DevBuf<float4> g_rec;        // per path, per recorded bounce
+    DevBuf<float> g_train;       // collected training records
+    DevBuf<float> g_q;           // staging for the guide entry points
+    DevBuf<uint32_t> g_counters;
+    bool g_init = false, g_ready = false;
+    uint32_t g_steps = 0, g_pass_salt = 0;
+    float g_keep = 1.0f;
+    GuideScene g_scene{};
     // stats
     phg_stats stats{};
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t cu_count = 256;
 };
 
@@ -90,6 +103,76 @@ static inline float hluminance(const float* c) {
 }
 
 static phg_status set_err(Ctx* c, phg_status s, const std::string& msg) { c->err = msg; return s; }
+
+static GuideNetDev guide_net(Ctx* c) {
+    GuideNetDev n{};
+    n.params = c->g_params.p; n.wt = c->g_wt.p; n.wn = c->g_wn.p; n.bias = c->g_bias.p;
+    n.params_mut = c->g_params.p; n.wt_mut = c->g_wt.p; n.wn_mut = c->g_wn.p; n.bias_mut = c->g_bias.p;
+    return n;
+}
+
+// He-uniform initialisation (fan-in), zero biases, from a splitmix64 stream.
+static std::vector<float> guide_default_params(uint64_t seed) {
+    std::vector<float> p(kGuideParamCount, 0.0f);
+    const int offs[11] = {0, 4096, 4160, 8256, 8320, 12416, 12480, 16576, 16640, 19200, 19240};
+    uint64_t st = seed * 0x9E3779B97F4A7C15ULL + 0xD1B54A32D192ED03ULL;
+    auto next = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    for (int l = 0; l < 5; ++l) {
+        const float lim = std::sqrt(6.0f / 64.0f);
+        for (int i = offs[2 * l]; i < offs[2 * l + 1]; ++i)
+            p[i] = (float)(((double)(next() >> 11) * 0x1p-53) * 2.0 - 1.0) * lim;
+    }
+    return p;
+}
+
+static phg_status guide_alloc(Ctx* c) {
+    if (c->g_init) return PHG_OK;
+    PHG_HIP(c, c->g_params.ensure(kGuideParamCount));
+    PHG_HIP(c, c->g_m.ensure(kGuideParamCount));
+    PHG_HIP(c, c->g_v.ensure(kGuideParamCount));
+    PHG_HIP(c, c->g_grads.ensure(kGuideParamCount));
+    PHG_HIP(c, c->g_bias.ensure(5 * 64));
+    PHG_HIP(c, c->g_wt.ensure(5 * 4096));
+    PHG_HIP(c, c->g_wn.ensure(5 * 4096));
+    PHG_HIP(c, c->g_counters.ensure(4));
+    c->g_init = true;
+    return PHG_OK;
+}
+
+static phg_status guide_load(Ctx* c, const float* host_params) {
+    phg_status st = guide_alloc(c);
+    if (st != PHG_OK) return st;
+    PHG_HIP(c, hipMemcpyAsync(c->g_params.p, host_params, kGuideParamCount * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    PHG_HIP(c, guide_fill(c->g_m.p, kGuideParamCount, 0.0f, c->stream));
+    PHG_HIP(c, guide_fill(c->g_v.p, kGuideParamCount, 0.0f, c->stream));
+    PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
+    PHG_HIP(c, guide_prepare(guide_net(c), c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    c->g_steps = 0;
+    return PHG_OK;
+}
+
+// `steps` Adam steps over device records, minibatches taken round robin
+static phg_status guide_train_device(Ctx* c, const float* d_rec, uint32_t nrec, uint32_t batch, float lr, uint32_t steps) {
+    if (nrec == 0 || steps == 0) return PHG_OK;
+    const uint32_t bs = std::min(batch ? batch : nrec, nrec);
+    const uint32_t grid = std::min<uint32_t>(c->cu_count, (bs + 31) / 32);
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < steps; ++k) {
+        const uint32_t n = std::min(bs, nrec - off);
+        PHG_HIP(c, guide_gradients(guide_net(c), d_rec + (size_t)off * kGuideRecordFloats, n, c->g_grads.p, grid, c->stream));
+        PHG_HIP(c, guide_adam(guide_net(c), c->g_grads.p, c->g_m.p, c->g_v.p, lr, (int)(++c->g_steps), c->stream));
+        PHG_HIP(c, guide_prepare(guide_net(c), c->stream));
+        off += n;
+        if (off >= nrec) off = 0;
+    }
+    return PHG_OK;
+}
 
 }  // namespace
 }  // namespace phg
@@ -130,6 +213,8 @@ phg_status phg_destroy(void* ctx) {
     c->shadow_c.free_(); c->ext0.free_(); c->ext1.free_(); c->shadow.free_(); c->ext_pid0.free_(); c->ext_pid1.free_();
     c->shadow_pid.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
     c->q_nvis.free_(); c->q_ntri.free_();
+    c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
+    c->g_wn.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -247,6 +332,20 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_op
     c->sc.emit_tris = c->emit_tris.p;
     c->sc.num_emitters = ne;
     c->sc.cam = s->camera;
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t i = 0; i < 3ull * s->num_tris; ++i) {
+            const float* v = s->verts + 3 * i;
+            for (int k = 0; k < 3; ++k) {
+                if (std::isfinite(v[k])) { lo[k] = std::min(lo[k], v[k]); hi[k] = std::max(hi[k], v[k]); }
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            const float e = hi[k] - lo[k];
+            c->g_scene.lo[k] = std::isfinite(lo[k]) ? lo[k] : 0.0f;
+            c->g_scene.inv_ext[k] = (e > 0.0f && std::isfinite(e)) ? 1.0f / e : 1.0f;
+        }
+    }
     c->stack_need = bvh.stack_need;
     c->bvh_depth = bvh.depth;
     c->bvh_nodes = (uint32_t)bvh.nodes.size();
@@ -380,6 +479,29 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     pb.hits = c->hits.p; pb.shadow = c->shadow.p; pb.shadow_pid = c->shadow_pid.p; pb.shadow_c = c->shadow_c.p;
     pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = c->film.p;
     const uint32_t count_trav = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) ? 1u : 0u;
+    const bool guided = (o->flags & PHG_RENDER_GUIDED) != 0;
+    const bool train = (o->flags & PHG_RENDER_TRAIN) != 0;
+    uint32_t rec_bounces = 0;
+    uint32_t train_cap = 0;
+    const uint32_t batch = o->guide_batch ? o->guide_batch : (1u << 18);
+    if (guided || train) {
+        if (!c->g_init) {
+            const std::vector<float> p0 = guide_default_params(o->seed + 1);
+            phg_status st = guide_load(c, p0.data());
+            if (st != PHG_OK) return st;
+        }
+        PHG_HIP(c, c->g_lobes.ensure(cap * (size_t)kGuideOutputs));
+    }
+    if (train) {
+        rec_bounces = std::min<uint32_t>(kGuideRecBounces, o->max_bounces + 1);
+        PHG_HIP(c, c->g_rec.ensure((size_t)rec_bounces * cap * kRecFloat4));
+        train_cap = 4 * batch;
+        PHG_HIP(c, c->g_train.ensure((size_t)train_cap * kGuideRecordFloats));
+    }
+    pb.lobes = c->g_lobes.p;
+    pb.rec = c->g_rec.p;
+    double t_ginfer = 0, t_gtrain = 0;
+    uint64_t g_queries = 0, g_records = 0;
     const uint32_t grid_trace = c->cu_count * 8;   // persistent-style grid-stride launches
     const uint32_t grid_shade = c->cu_count * 4;
     double t_trace = 0, t_shadow = 0, t_shade = 0;
@@ -392,6 +514,9 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
         pp.width = o->width; pp.height = o->height; pp.npix = (uint32_t)npix;
         pp.paths = (uint32_t)(npix * kk); pp.sample_begin = o->spp_begin + s0; pp.max_bounces = o->max_bounces;
         pp.seed = o->seed;
+        pp.guide_prob = (guided && c->g_ready) ? o->guide_prob : 0.0f;
+        pp.record = train ? 1u : 0u;
+        pp.rec_bounces = rec_bounces;
         rays_primary += pp.paths;
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
         PHG_HIP(c, launch_generate(c->sc, pp, pb, c->stream));
@@ -399,29 +524,62 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         float ms = 0; PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
         for (uint32_t b = 0;; ++b) {
+            const bool use_guide = guided && pp.guide_prob > 0.0f && b < o->max_bounces;
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
             PHG_HIP(c, launch_extend(c->sc, pb, b, grid_trace, c->stack_need, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
-            PHG_HIP(c, launch_shade(c->sc, pp, pb, b, grid_shade, c->stream));
+            if (use_guide) {
+                PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], c->sc.tris, c->g_scene,
+                                            c->counts.p + b % kCountSlots, c->g_lobes.p, c->cu_count * 2, c->stream));
+            }
+            PHG_HIP(c, hipEventRecord(c->ev[4], c->stream));
+            PHG_HIP(c, launch_shade(c->sc, pp, pb, b, grid_shade, guided || train, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            PHG_HIP(c, launch_shadow(c->sc, pb, b, grid_trace, c->stack_need, count_trav, c->stream));
+            PHG_HIP(c, launch_shadow(c->sc, pb, b, grid_trace, c->stack_need, count_trav, rec_bounces, pp.paths, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[3], c->stream));
-            uint32_t next = 0;
-            PHG_HIP(c, hipMemcpyAsync(&next, c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            uint32_t next[2] = {0, 0};
+            PHG_HIP(c, hipMemcpyAsync(&next[0], c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            PHG_HIP(c, hipMemcpyAsync(&next[1], c->counts.p + b % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));
-            float a = 0, bb = 0, cc = 0;
+            float a = 0, bb = 0, cc = 0, gi = 0;
             PHG_HIP(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-            PHG_HIP(c, hipEventElapsedTime(&bb, c->ev[1], c->ev[2]));
+            PHG_HIP(c, hipEventElapsedTime(&gi, c->ev[1], c->ev[4]));
+            PHG_HIP(c, hipEventElapsedTime(&bb, c->ev[4], c->ev[2]));
             PHG_HIP(c, hipEventElapsedTime(&cc, c->ev[2], c->ev[3]));
-            t_trace += a; t_shade += bb; t_shadow += cc;
+            t_trace += a; t_shade += bb; t_shadow += cc; t_ginfer += gi;
+            if (use_guide) g_queries += next[1];
+            const uint32_t next0 = next[0];
             ++n_trace; ++n_shadow;
-            if (next == 0 || b >= o->max_bounces) break;
+            if (next0 == 0 || b >= o->max_bounces) break;
         }
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
         PHG_HIP(c, launch_film(pp, pb, kk, c->stream));
         PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
+        if (train) {
+            // online training on this pass's radiance samples
+            PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+            PHG_HIP(c, hipMemsetAsync(c->g_counters.p, 0, 4 * sizeof(uint32_t), c->stream));
+            PHG_HIP(c, guide_collect(pb, pp, c->g_scene, c->g_train.p, train_cap, c->g_counters.p, c->g_keep,
+                                     ++c->g_pass_salt, c->stream));
+            uint32_t cnt[2] = {0, 0};
+            PHG_HIP(c, hipMemcpyAsync(cnt, c->g_counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            PHG_HIP(c, hipStreamSynchronize(c->stream));
+            const uint32_t nrec = std::min(cnt[0], train_cap);
+            const uint32_t steps = o->guide_steps ? o->guide_steps : 4;
+            phg_status gs = guide_train_device(c, c->g_train.p, nrec, batch, o->guide_lr > 0 ? o->guide_lr : 1e-3f, steps);
+            if (gs != PHG_OK) return gs;
+            PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+            PHG_HIP(c, hipStreamSynchronize(c->stream));
+            PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_gtrain += ms;
+            g_records += nrec;
+            // keep about as many records as the next pass's steps consume
+            const double want = (double)steps * batch;
+            c->g_keep = cnt[1] > 0 ? (float)std::min(1.0, 1.25 * want / (double)cnt[1]) : 1.0f;
+            if (nrec > 0) c->g_ready = true;
+            pp.guide_prob = 0.0f;  // (next pass re-reads g_ready)
+        }
     }
     uint32_t flags = 0;
     unsigned long long tc[8];
@@ -441,10 +599,82 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     c->stats.shadow_tri_tests = tc[3];
     c->stats.trace_launches = n_trace;
     c->stats.shadow_launches = n_shadow;
+    c->stats.guide_infer_ms = t_ginfer;
+    c->stats.guide_train_ms = t_gtrain;
+    c->stats.guide_queries = g_queries;
+    c->stats.guide_records = g_records;
+    c->stats.guide_steps = c->g_steps;
     c->stats.trace_ms = t_trace;
     c->stats.shadow_ms = t_shadow;
     c->stats.shade_ms = t_shade;
     if (flags) return set_err(c, PHG_ERR_UNSUPPORTED, "traversal stack overflow");
+    return PHG_OK;
+}
+
+phg_status phg_guide_set_params(void* ctx, const float* params, uint32_t n) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !params) return PHG_ERR_INVALID_ARG;
+    if (n != kGuideParamCount) return set_err(c, PHG_ERR_INVALID_ARG, "expected PHG_GUIDE_PARAMS values");
+    PHG_HIP(c, hipSetDevice(c->device));
+    phg_status st = guide_load(c, params);
+    if (st == PHG_OK) c->g_ready = true;
+    return st;
+}
+
+phg_status phg_guide_get_params(void* ctx, float* params, uint32_t n) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !params) return PHG_ERR_INVALID_ARG;
+    if (n != kGuideParamCount) return set_err(c, PHG_ERR_INVALID_ARG, "expected PHG_GUIDE_PARAMS values");
+    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, hipMemcpyAsync(params, c->g_params.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    return PHG_OK;
+}
+
+phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !features || !out || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
+    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+    if (n == 0) return PHG_OK;
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, c->g_q.ensure(n * (9 + kGuideOutputs)));
+    float* d_in = c->g_q.p;
+    float* d_out = c->g_q.p + n * 9;
+    PHG_HIP(c, hipMemcpyAsync(d_in, features, n * 9 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    PHG_HIP(c, guide_infer_features(guide_net(c), d_in, (uint32_t)n, d_out, 1, c->cu_count * 2, c->stream));
+    PHG_HIP(c, hipMemcpyAsync(out, d_out, n * kGuideOutputs * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    return PHG_OK;
+}
+
+phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, float* grads) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !records || !grads || n == 0 || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
+    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, c->g_q.ensure(n * kGuideRecordFloats));
+    PHG_HIP(c, hipMemcpyAsync(c->g_q.p, records, n * kGuideRecordFloats * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
+    const uint32_t grid = std::min<uint32_t>(c->cu_count, (uint32_t)((n + 31) / 32));
+    PHG_HIP(c, guide_gradients(guide_net(c), c->g_q.p, (uint32_t)n, c->g_grads.p, grid, c->stream));
+    PHG_HIP(c, hipMemcpyAsync(grads, c->g_grads.p, kGuideParamCount * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    return PHG_OK;
+}
+
+phg_status phg_guide_train(void* ctx, const float* records, uint64_t n, float lr, uint32_t steps) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !records || n == 0 || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
+    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, c->g_q.ensure(n * kGuideRecordFloats));
+    PHG_HIP(c, hipMemcpyAsync(c->g_q.p, records, n * kGuideRecordFloats * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    phg_status st = guide_train_device(c, c->g_q.p, (uint32_t)n, (uint32_t)n, lr, steps);
+    if (st != PHG_OK) return st;
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    c->g_ready = true;
     return PHG_OK;
 }
 

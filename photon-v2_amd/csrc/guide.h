@@ -1,0 +1,45 @@
+// Host-visible declarations of the guiding-MLP kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace phg {
+
+struct GuideScene {
+    float lo[3];
+    float inv_ext[3];
+};
+
+// bf16 arrays are carried as uint16_t bit patterns on the host side
+struct GuideNetDev {
+    const float* params;     // fp32 master parameters (19240)
+    const uint16_t* wt;      // 5 x [64 out][64 in] bf16
+    const uint16_t* wn;      // 5 x [64 in][64 out] bf16
+    const float* bias;       // 5 x 64 fp32
+    float* params_mut;
+    uint16_t* wt_mut;
+    uint16_t* wn_mut;
+    float* bias_mut;
+};
+
+constexpr uint32_t kGuideParamCount = 19240;
+constexpr uint32_t kGuideRecordFloats = 13;
+constexpr uint32_t kGuideOutputs = 40;
+
+hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st);
+hipError_t guide_infer_features(const GuideNetDev& net, const float* features, uint32_t n, float* out, int raw,
+                                uint32_t grid, hipStream_t st);
+hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const phg_ray* rays, const TriRec* tris,
+                            const GuideScene& gs, const uint32_t* count_ptr, float* out, uint32_t grid, hipStream_t st);
+hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nrec, float* grads, uint32_t grid,
+                           hipStream_t st);
+hipError_t guide_adam(const GuideNetDev& net, float* grads, float* m, float* v, float lr, int step, hipStream_t st);
+hipError_t guide_fill(float* p, uint32_t n, float v, hipStream_t st);
+// training records of a finished pass: incident radiance along each recorded sampled direction
+hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
+                         uint32_t capacity, uint32_t* counters, float keep_prob, uint32_t salt, hipStream_t st);
+
+}  // namespace phg

@@ -1,0 +1,358 @@
+// Guiding MLP on the gfx950 matrix cores: BF16 operands, FP32 accumulation, with
+// v_mfma_f32_16x16x32_bf16.  Operand layout of that instruction (lane l of a wave64):
+//   A (16 x 32): row l % 16, k = 8 * (l / 16) .. + 7
+//   B (32 x 16): column l % 16, k = 8 * (l / 16) .. + 7
+//   D (16 x 16): column l % 16, rows 4 * (l / 16) .. + 3
+// Activations live in LDS as bf16 [row][64] so that an A operand is one 16-byte read;
+// weights are kept twice, transposed ([out][in], B operand of the forward product) and
+// plain ([in][out], B operand of the backward product dX = dZ W^T), both 16-byte reads.
+#include <hip/hip_runtime.h>
+
+#include "guide.h"
+#include "guide_device.h"
+
+namespace phg {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int layer_out(int l) { return l == kGuideLayers - 1 ? kGuideOut : kGuideHid; }
+
+// fp32 master parameters -> bf16 transposed / plain copies (rows padded to 64) and fp32 biases
+__global__ void k_guide_prepare(const float* __restrict__ params, uint16_t* __restrict__ wt_bits,
+                                uint16_t* __restrict__ wn_bits, float* __restrict__ bias) {
+    __bf16* wt = reinterpret_cast<__bf16*>(wt_bits);
+    __bf16* wn = reinterpret_cast<__bf16*>(wn_bits);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kGuideLayers * 64 * 64) {
+        const int l = i / 4096, rem = i % 4096, a = rem / 64, b = rem % 64;  // wt: a = out, b = in
+        const int nout = layer_out(l);
+        const float* W = params + kGuideParamOffsets[2 * l];
+        const float v = a < nout ? W[b * nout + a] : 0.0f;
+        wt[i] = (__bf16)v;
+        // wn: [in][out]: a = in, b = out
+        const float v2 = b < nout ? W[a * nout + b] : 0.0f;
+        wn[i] = (__bf16)v2;
+    }
+    if (i < kGuideLayers * 64) {
+        const int l = i / 64, o = i % 64;
+        bias[i] = o < layer_out(l) ? params[kGuideParamOffsets[2 * l + 1] + o] : 0.0f;
+    }
+}
+
+// One dense layer for a 16-row tile held by one wave: act_out = act(act_in . W + b).
+// act_in/act_out: bf16 [16][64] in LDS; wt_l: bf16 [64 out][64 in]; out_f32 (last layer): [16][48].
+__device__ __forceinline__ void layer_forward(const __bf16* act_in, const __bf16* wt_l, const float* bias_l,
+                                              __bf16* act_out, float* out_f32, bool last) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(act_in + r * 64 + 8 * g);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(act_in + r * 64 + 32 + 8 * g);
+    const int ntiles = last ? kGuideOutPad / 16 : 4;
+    for (int nt = 0; nt < ntiles; ++nt) {
+        const int col = nt * 16 + r;
+        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(wt_l + col * 64 + 8 * g);
+        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(wt_l + col * 64 + 32 + 8 * g);
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = mfma16(a0, b0, c);
+        c = mfma16(a1, b1, c);
+        const float bv = bias_l[col];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * g + i;
+            const float v = c[i] + bv;
+            if (last) out_f32[row * kGuideOutPad + col] = v;
+            else act_out[row * 64 + col] = (__bf16)fmaxf(v, 0.0f);
+        }
+    }
+}
+
+// Features of one query: from a feature array, or from a queued hit (position, shading
+// normal and outgoing direction of the surface point, as the shade stage computes them).
+struct InferSource {
+    const float* features;   // 9 per query, or null
+    const float4* hits;      // tri (bits), bary a, b, c
+    const phg_ray* rays;     // queued extension rays (direction)
+    const TriRec* tris;
+    GuideScene gs;
+};
+
+__device__ __forceinline__ void query_features(const InferSource& src, uint32_t q, float* x) {
+    if (src.features) {
+        for (int k = 0; k < 9; ++k) x[k] = src.features[9 * (size_t)q + k];
+        return;
+    }
+    const float4 h = src.hits[q];
+    const int32_t tri = __float_as_int(h.x);
+    if (tri < 0) { for (int k = 0; k < 9; ++k) x[k] = 0.0f; return; }
+    V3 a, b, c;
+    load_tri(src.tris, (uint32_t)tri, &a, &b, &c);
+    const SurfaceHit X = make_hit(a, b, c, (uint32_t)tri, h.y, h.z, h.w);
+    const float4 rd = reinterpret_cast<const float4*>(src.rays + q)[1];
+    guide_features(src.gs, X.P, X.Ns, mk(-rd.x, -rd.y, -rd.z), x);
+}
+
+// Inference: decoded mixture (8 x (mu, kappa, weight)) per query, or raw outputs.
+// Block = 4 waves, each wave a 16-query tile per iteration; weights staged in LDS.
+__global__ __launch_bounds__(256) void k_guide_infer(GuideNetDev net, InferSource src, const uint32_t* count_ptr,
+                                                     uint32_t count, float* __restrict__ out, int raw) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_wt[kGuideLayers * 64 * 64];
+    __shared__ float s_bias[kGuideLayers * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 s_act[4][2][16 * 64];
+    __shared__ float s_out[4][16 * kGuideOutPad];
+    const uint32_t n = count_ptr ? *count_ptr : count;
+    for (int i = threadIdx.x; i < kGuideLayers * 64 * 64 / 8; i += 256)
+        reinterpret_cast<bf16x8*>(s_wt)[i] = reinterpret_cast<const bf16x8*>(net.wt)[i];  // bf16 bit patterns
+    for (int i = threadIdx.x; i < kGuideLayers * 64; i += 256) s_bias[i] = net.bias[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    for (uint32_t tile = blockIdx.x * 4 + wave; tile * 16 < n; tile += gridDim.x * 4) {
+        const uint32_t q = tile * 16 + r;
+        float x[9];
+        if (q < n) query_features(src, q, x);
+        else for (int k = 0; k < 9; ++k) x[k] = 0.0f;
+        __bf16* a0 = s_act[wave][0];
+        for (int j = 16 * g; j < 16 * g + 16; ++j) a0[r * 64 + j] = (__bf16)guide_encode(x, j);
+        wave_sync();
+        for (int l = 0; l < kGuideLayers; ++l) {
+            const bool last = l == kGuideLayers - 1;
+            layer_forward(s_act[wave][l & 1], s_wt + l * 4096, s_bias + l * 64, s_act[wave][(l + 1) & 1], s_out[wave], last);
+            wave_sync();
+        }
+        if (g == 0 && q < n) {
+            const float* o = s_out[wave] + r * kGuideOutPad;
+            float* dst = out + (size_t)q * kGuideOut;
+            if (raw) { for (int k = 0; k < kGuideOut; ++k) dst[k] = o[k]; }
+            else {
+                float lobes[kGuideOut];
+                guide_decode(o, lobes);
+                for (int k = 0; k < kGuideOut; ++k) dst[k] = lobes[k];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Training: gradients of the mean radiance-weighted NLL over a batch of records
+// (9 features, direction xyz, weight).  One wave per block, 32 records per iteration;
+// per-block gradient accumulator in LDS, added to the global gradient at the end.
+__global__ __launch_bounds__(64) void k_guide_train(GuideNetDev net, const float* __restrict__ rec, uint32_t nrec,
+                                                    float inv_batch, float* __restrict__ grads) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_act[kGuideLayers][32 * 64];  // inputs of each layer
+    __shared__ __attribute__((aligned(16))) __bf16 s_dz[2][32 * 64];
+    __shared__ float s_out[32 * kGuideOutPad];
+    __shared__ float s_g[kGuideParams];
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    for (int i = lane; i < kGuideParams; i += 64) s_g[i] = 0.0f;
+    wave_sync();
+    for (uint32_t base = blockIdx.x * 32; base < nrec; base += gridDim.x * 32) {
+        // encode 32 records: lane handles row lane % 32, encoding half lane / 32
+        {
+            const int row = lane & 31, half = lane >> 5;
+            const uint32_t s = base + row;
+            float x[9];
+            for (int k = 0; k < 9; ++k) x[k] = s < nrec ? rec[13 * (size_t)s + k] : 0.0f;
+            for (int j = 32 * half; j < 32 * half + 32; ++j) s_act[0][row * 64 + j] = (__bf16)guide_encode(x, j);
+        }
+        wave_sync();
+        // forward, two 16-row tiles per layer
+        for (int l = 0; l < kGuideLayers; ++l) {
+            const bool last = l == kGuideLayers - 1;
+            for (int t = 0; t < 2; ++t)
+                layer_forward(s_act[l] + t * 16 * 64, reinterpret_cast<const __bf16*>(net.wt) + l * 4096, net.bias + l * 64,
+                              last ? nullptr : s_act[l + 1] + t * 16 * 64, s_out + t * 16 * kGuideOutPad, last);
+            wave_sync();
+        }
+        // output gradient per record
+        if (lane < 32) {
+            const uint32_t s = base + lane;
+            float gr[kGuideOut];
+            if (s < nrec) {
+                const float* R = rec + 13 * (size_t)s;
+                guide_output_grad(s_out + lane * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], gr);
+            } else {
+                for (int k = 0; k < kGuideOut; ++k) gr[k] = 0.0f;
+            }
+            for (int k = 0; k < 64; ++k) s_dz[0][lane * 64 + k] = (__bf16)(k < kGuideOut ? gr[k] * inv_batch : 0.0f);
+            for (int k = 0; k < kGuideOut; ++k) atomicAdd(&s_g[kGuideParamOffsets[9] + k], gr[k] * inv_batch);
+        }
+        wave_sync();
+        int cur = 0;
+        for (int l = kGuideLayers - 1; l >= 0; --l) {
+            const int nout = layer_out(l);
+            const __bf16* dz = s_dz[cur];
+            const __bf16* X = s_act[l];
+            // dW_l = X^T dz over the 32 records: M = in (4 tiles), N = out, K = 32
+            for (int mt = 0; mt < 4; ++mt) {
+                bf16x8 a;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a[j] = X[(8 * g + j) * 64 + mt * 16 + r];
+                for (int nt = 0; nt * 16 < nout; ++nt) {
+                    bf16x8 b;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) b[j] = dz[(8 * g + j) * 64 + nt * 16 + r];
+                    f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                    c = mfma16(a, b, c);
+                    const int col = nt * 16 + r;
+                    if (col < nout) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int in = mt * 16 + 4 * g + i;
+                            atomicAdd(&s_g[kGuideParamOffsets[2 * l] + in * nout + col], c[i]);
+                        }
+                    }
+                }
+            }
+            // db_l for hidden layers (the output layer's bias gradient was added above)
+            if (l < kGuideLayers - 1) {
+                float acc = 0.0f;
+                for (int s = 0; s < 32; ++s) acc += (float)dz[s * 64 + lane];
+                atomicAdd(&s_g[kGuideParamOffsets[2 * l + 1] + lane], acc);
+            }
+            if (l > 0) {
+                // dX = dz W^T (M = 32 records, N = 64 in, K = 64 out), then the ReLU mask of X
+                __bf16* dz_next = s_dz[cur ^ 1];
+                const __bf16* wn_l = reinterpret_cast<const __bf16*>(net.wn) + l * 4096;
+                for (int t = 0; t < 2; ++t) {
+                    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(dz + (t * 16 + r) * 64 + 8 * g);
+                    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(dz + (t * 16 + r) * 64 + 32 + 8 * g);
+                    for (int nt = 0; nt < 4; ++nt) {
+                        const int col = nt * 16 + r;  // input index of layer l
+                        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(wn_l + col * 64 + 8 * g);
+                        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(wn_l + col * 64 + 32 + 8 * g);
+                        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                        c = mfma16(a0, b0, c);
+                        c = mfma16(a1, b1, c);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int row = t * 16 + 4 * g + i;
+                            const bool active = (float)X[row * 64 + col] > 0.0f;
+                            dz_next[row * 64 + col] = (__bf16)(active ? c[i] : 0.0f);
+                        }
+                    }
+                }
+                cur ^= 1;
+            }
+            wave_sync();
+        }
+    }
+    wave_sync();
+    for (int i = lane; i < kGuideParams; i += 64) {
+        const float v = s_g[i];
+        if (v != 0.0f) atomicAdd(&grads[i], v);
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the fp32 master parameters; clears the gradient.
+__global__ void k_guide_adam(float* __restrict__ params, float* __restrict__ grads, float* __restrict__ m,
+                             float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kGuideParams) return;
+    const float gr = grads[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gr;
+    const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+    m[i] = mi; v[i] = vi;
+    params[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    grads[i] = 0.0f;
+}
+
+__global__ void k_fill_float(float* p, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Training records of a finished pass.  For each recorded vertex, the radiance that arrived
+// along its sampled direction is (final path radiance - radiance before the direction was
+// followed) / (throughput x sample weight), channel by channel; its luminance over the
+// sampling density is the record's weight.  Records are kept with probability keep_prob.
+__global__ __launch_bounds__(256) void k_guide_collect(PathBuffers pb, PassParams pp, GuideScene gs, float* __restrict__ out,
+                                                       uint32_t capacity, uint32_t* __restrict__ counters, float keep_prob,
+                                                       uint32_t salt) {
+    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
+    if (pid >= pp.paths) return;
+    const float4 L = pb.L[pid];
+    uint32_t valid = 0;
+    for (uint32_t b = 0; b < pp.rec_bounces; ++b) {
+        const float4* rc = pb.rec + ((size_t)b * pp.paths + pid) * kRecFloat4;
+        const float4 r0 = rc[0];
+        if (!(r0.w > 0.0f)) break;
+        const float4 wem = rc[4], lb = rc[5];
+        const float lr = wem.x > 0.0f ? (L.x - lb.x) / wem.x : 0.0f;
+        const float lg = wem.y > 0.0f ? (L.y - lb.y) / wem.y : 0.0f;
+        const float lbb = wem.z > 0.0f ? (L.z - lb.z) / wem.z : 0.0f;
+        const float lum = luminance(rgb3(fmaxf(lr, 0.0f), fmaxf(lg, 0.0f), fmaxf(lbb, 0.0f)));
+        const float w = lum / r0.w;
+        if (!(w > 0.0f) || !__builtin_isfinite(w)) continue;
+        ++valid;
+        uint32_t h = pid * 0x9E3779B1u ^ (b + 1u) * 0x85EBCA77u ^ salt * 0xC2B2AE3Du;
+        h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12; h *= 0x297A2D39u; h ^= h >> 15;
+        if ((float)(h >> 8) * 0x1p-24f >= keep_prob) continue;
+        const uint32_t slot = atomicAdd(&counters[0], 1u);
+        if (slot >= capacity) continue;
+        const float4 n4 = rc[1], v4 = rc[2], d4 = rc[3];
+        float x[9];
+        guide_features(gs, mk(r0.x, r0.y, r0.z), mk(n4.x, n4.y, n4.z), mk(v4.x, v4.y, v4.z), x);
+        float* o = out + (size_t)slot * kGuideRecordFloats;
+        for (int k = 0; k < 9; ++k) o[k] = x[k];
+        o[9] = d4.x; o[10] = d4.y; o[11] = d4.z; o[12] = w;
+    }
+    if (valid) atomicAdd(&counters[1], valid);
+}
+
+hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
+                         uint32_t capacity, uint32_t* counters, float keep_prob, uint32_t salt, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_collect, dim3((pp.paths + 255) / 256), dim3(256), 0, st, pb, pp, gs, out_rec, capacity,
+                       counters, keep_prob, salt);
+    return hipGetLastError();
+}
+
+hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_prepare, dim3((kGuideLayers * 4096 + 255) / 256), dim3(256), 0, st,
+                       net.params, net.wt_mut, net.wn_mut, net.bias_mut);
+    return hipGetLastError();
+}
+
+hipError_t guide_infer_features(const GuideNetDev& net, const float* features, uint32_t n, float* out, int raw,
+                                uint32_t grid, hipStream_t st) {
+    InferSource src{};
+    src.features = features;
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(256), 0, st, net, src, (const uint32_t*)nullptr, n, out, raw);
+    return hipGetLastError();
+}
+
+hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const phg_ray* rays, const TriRec* tris,
+                            const GuideScene& gs, const uint32_t* count_ptr, float* out, uint32_t grid, hipStream_t st) {
+    InferSource src{};
+    src.hits = hits; src.rays = rays; src.tris = tris; src.gs = gs;
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(256), 0, st, net, src, count_ptr, 0u, out, 0);
+    return hipGetLastError();
+}
+
+hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nrec, float* grads, uint32_t grid,
+                           hipStream_t st) {
+    if (nrec == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(64), 0, st, net, rec, nrec, 1.0f / (float)nrec, grads);
+    return hipGetLastError();
+}
+
+hipError_t guide_adam(const GuideNetDev& net, float* grads, float* m, float* v, float lr, int step, hipStream_t st) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kGuideParams + 255) / 256), dim3(256), 0, st, net.params_mut, grads, m, v, lr,
+                       b1, b2, 1e-8f, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t guide_fill(float* p, uint32_t n, float v, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_float, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+}  // namespace phg

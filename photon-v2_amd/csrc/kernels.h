@@ -53,7 +53,14 @@ struct PassParams {
     uint32_t sample_begin;   // sample index of path 0
     uint32_t max_bounces;
     uint64_t seed;
+    float guide_prob;        // guided shading: probability of sampling the learned lobes
+    uint32_t record;         // guided shading: write training records
+    uint32_t rec_bounces;    // bounces with a record slot
+    uint32_t pad;
 };
+
+constexpr uint32_t kGuideRecBounces = 8;   // training records are kept for the first bounces
+constexpr uint32_t kRecFloat4 = 6;         // P+pdf, N, V, omega, Wem, L before
 
 // Structure-of-arrays path state and queues; all device pointers.
 struct PathBuffers {
@@ -72,6 +79,9 @@ struct PathBuffers {
     unsigned long long* traversal_counters;  // node visits / tri tests: ext, ext, shadow, shadow
     uint32_t* error_flags;
     double* film;            // 8 doubles per pixel
+    // guided shading
+    const float* lobes;      // per queued extension ray: 8 x (mu xyz, kappa, weight)
+    float4* rec;             // training records [bounce][path][kRecFloat4]
 };
 
 hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
@@ -80,9 +90,9 @@ hipError_t launch_generate(const SceneDev& sc, const PassParams& pp, const PathB
 hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
                          uint32_t count, hipStream_t st);
 hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
-                         uint32_t count, hipStream_t st);
+                         uint32_t count, uint32_t rec_bounces, uint32_t npaths, hipStream_t st);
 hipError_t launch_shade(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
-                        hipStream_t st);
+                        bool guided, hipStream_t st);
 hipError_t launch_film(const PassParams& pp, const PathBuffers& pb, uint32_t spp_in_pass, hipStream_t st);
 hipError_t launch_clear_counts(uint32_t* counts, uint32_t n, hipStream_t st);
 hipError_t launch_fill_zero(double* p, uint64_t n, hipStream_t st);

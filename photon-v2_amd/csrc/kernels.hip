@@ -11,6 +11,7 @@
 #include "kernels.h"
 #include "pt_device.h"
 #include "traverse.h"
+#include "guide_device.h"
 
 namespace phg {
 
@@ -106,6 +107,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, PassParam
     pb.Wem[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pb.prevP[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
     pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (pp.record)
+        for (uint32_t b = 0; b < pp.rec_bounces; ++b)
+            pb.rec[((size_t)b * pp.paths + i) * kRecFloat4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 template <int STACK>
@@ -141,7 +145,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
 }
 
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal,
+                                                        uint32_t rec_bounces, uint32_t npaths) {
     __shared__ int32_t stack_mem[STACK * kTraceBlock];
     const uint32_t n = pb.counts[kCountSlots + bounce % kCountSlots];
     const uint32_t stride = gridDim.x * kTraceBlock;
@@ -163,6 +168,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers
             float4 L = pb.L[pid];
             L.x += c.x; L.y += c.y; L.z += c.z;
             pb.L[pid] = L;
+            if (bounce < rec_bounces) {
+                // NEE at this vertex is not radiance arriving along the sampled direction
+                float4* lb = pb.rec + ((size_t)bounce * npaths + pid) * kRecFloat4 + 5;
+                float4 v = *lb;
+                v.x += c.x; v.y += c.y; v.z += c.z;
+                *lb = v;
+            }
         }
     }
     if (count_traversal) {
@@ -178,6 +190,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers
 // BSDF sample (MIS against NEE), then NEE at this vertex (shadow ray queued with its
 // contribution), then the BSDF sample that extends the path, throughput update and
 // Russian roulette.  Whole waves iterate together because the queue pushes use ballots.
+// GUIDED: the extension direction comes from one-sample MIS between the BSDF sampler and
+// the guiding network's lobes (probability guide_prob), and the combined density replaces
+// the BSDF pdf in both MIS weights against NEE, which keeps the estimator unbiased.
+template <bool GUIDED>
 __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams pp, PathBuffers pb, uint32_t bounce) {
     const uint32_t n = pb.counts[bounce % kCountSlots];
     const uint32_t qin = bounce & 1u, qout = qin ^ 1u;
@@ -256,9 +272,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                 path_ids(pp, pid, &pixel, &sample);
                 const uint64_t key = path_key(pp.seed, pixel, sample, pp.npix);
                 Stream s = stream_resume(key, pb.rng[pid]);
-                float u[8];
+                float u[GUIDED ? 12 : 8];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) u[k] = s.u();
+                for (int k = 0; k < (GUIDED ? 12 : 8); ++k) u[k] = s.u();
+                const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideOut : nullptr;
+                const float alpha = GUIDED ? pp.guide_prob : 0.0f;
                 const phg_material m = sc.mats[tr.mat];
                 if (m.type == PHG_MAT_ABSORBER) stop = true;
                 if (!stop) {
@@ -292,6 +310,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                                 if (be.ok) {
                                     float bp = bsdf_pdf(m, X, Ld, V);
                                     if (!pdf_ok(bp)) bp = 0.0f;
+                                    if (GUIDED && alpha > 0.0f) bp = alpha * guide_pdf(lob, Ld) + (1.0f - alpha) * bp;
                                     const float w = mis_power(pdf, bp);
                                     RGB weight = muls(be.f, absdot(X.Ns, Ld));
                                     weight = mulc(weight, T);
@@ -304,14 +323,48 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                         }
                     }
                     // ---- BSDF sample + extension (BNEEPTEstimator.cpp:151-255)
-                    const BsdfSample bs = bsdf_sample(m, X, V, u[4], u[5], u[6]);
-                    if (bs.ok) {
+                    bool ok = false;
+                    V3 Ls = mk(0.0f, 0.0f, 0.0f);
+                    RGB ws = rgb1(0.0f);
+                    float relIor = 1.0f, pdfc = 0.0f;
+                    if (GUIDED && u[8] < alpha) {
+                        // direction from the learned lobes, weighted by the combined density
+                        Ls = guide_sample(lob, u[9], u[10], u[11]);
+                        const BsdfEval be = bsdf_eval(m, X, Ls, V);
+                        float bp = bsdf_pdf(m, X, Ls, V);
+                        if (!pdf_ok(bp)) bp = 0.0f;
+                        pdfc = alpha * guide_pdf(lob, Ls) + (1.0f - alpha) * bp;
+                        if (be.ok && pdf_ok(pdfc)) {
+                            const float cosl = absdot(X.Ns, Ls);
+                            ws = muls(be.f, cosl / pdfc);
+                            if (m.type == PHG_MAT_DIELECTRIC && opposite_hemi(X, V, Ls)) {
+                                float etaI = m.ior_outer, etaT = m.ior_inner;
+                                if (dot(X.Ns, Ls) < 0.0f) { const float t = etaI; etaI = etaT; etaT = t; }
+                                relIor = etaT / etaI;
+                            }
+                            ok = measurable(ws, cosl) && side_agreed(X, Ls);
+                        }
+                    } else {
+                        const BsdfSample bs = bsdf_sample(m, X, V, u[4], u[5], u[6]);
+                        if (bs.ok) {
+                            Ls = bs.L; ws = bs.w; relIor = bs.relIor;
+                            const float bp = bsdf_pdf(m, X, bs.L, V);
+                            if (GUIDED && alpha > 0.0f) {
+                                pdfc = alpha * guide_pdf(lob, Ls) + (1.0f - alpha) * bp;
+                                ok = pdf_ok(bp) && pdf_ok(pdfc);
+                                if (ok) ws = muls(ws, bp / pdfc);
+                            } else {
+                                pdfc = bp;
+                                ok = true;
+                            }
+                        }
+                    }
+                    if (ok) {
                         V3 off, ud;
-                        iterative_offset(X, tv, bs.L, &off, &ud);
-                        const float bsdfPdf = bsdf_pdf(m, X, bs.L, V);
-                        const RGB wem = mulc(bs.w, T);
-                        RGB Tn = mulc(T, bs.w);
-                        rrScale /= bs.relIor * bs.relIor;
+                        iterative_offset(X, tv, Ls, &off, &ud);
+                        const RGB wem = mulc(ws, T);
+                        RGB Tn = mulc(T, ws);
+                        rrScale /= relIor * relIor;
                         bool die = false;
                         if (bounce >= 3) {  // RussianRoulette.h:30-55
                             const float surv = safe_clamp(luminance(muls(Tn, rrScale)), 0.0f, 0.95f);
@@ -320,12 +373,21 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                         }
                         if (!die && rgb_zero(Tn)) die = true;
                         pb.T[pid] = make_float4(Tn.r, Tn.g, Tn.b, rrScale);
-                        pb.Wem[pid] = make_float4(wem.r, wem.g, wem.b, bsdfPdf);
+                        pb.Wem[pid] = make_float4(wem.r, wem.g, wem.b, pdfc);
                         pb.prevP[pid] = make_float4(X.P.x, X.P.y, X.P.z, __uint_as_float(die ? kFlagDieAfterEmission : 0u));
                         pb.rng[pid] = s.state;
                         push_ext = true;
                         ext_o = X.P + off;
                         ext_d = ud;
+                        if (GUIDED && pp.record && bounce < pp.rec_bounces) {
+                            float4* rc = pb.rec + ((size_t)bounce * pp.paths + pid) * kRecFloat4;
+                            rc[0] = make_float4(X.P.x, X.P.y, X.P.z, pdfc);
+                            rc[1] = make_float4(X.Ns.x, X.Ns.y, X.Ns.z, 0.0f);
+                            rc[2] = make_float4(V.x, V.y, V.z, 0.0f);
+                            rc[3] = make_float4(ud.x, ud.y, ud.z, 0.0f);
+                            rc[4] = make_float4(wem.r, wem.g, wem.b, 0.0f);
+                            rc[5] = make_float4(Lacc.x, Lacc.y, Lacc.z, 0.0f);
+                        }
                     }
                 }
             }
@@ -415,16 +477,17 @@ hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bou
 }
 
 hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
-                         uint32_t count, hipStream_t st) {
-    if (stack <= 32) hipLaunchKernelGGL(k_shadow<32>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
-    else if (stack <= 64) hipLaunchKernelGGL(k_shadow<64>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
+                         uint32_t count, uint32_t rec_bounces, uint32_t npaths, hipStream_t st) {
+    if (stack <= 32) hipLaunchKernelGGL(k_shadow<32>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count, rec_bounces, npaths);
+    else if (stack <= 64) hipLaunchKernelGGL(k_shadow<64>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count, rec_bounces, npaths);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 hipError_t launch_shade(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
+                        bool guided, hipStream_t st) {
+    if (guided) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
     return hipGetLastError();
 }
 

@@ -1,0 +1,85 @@
+// GpuBneeptRenderer: host-side mirror of the reference's renderer seam for the GPU BNEEPT
+// path.  Same method names and meaning as ph::Renderer (Core/Renderer/Renderer.h:38-93);
+// where the reference takes cooked engine objects (CoreCookedUnit, VisualWorld) this class
+// takes the flattened scene they cook into (phg_scene_desc).  All device work goes through
+// the C ABI of include/phg.h.
+//
+// Rendering proceeds in progressive chunks of samples so that the async queries observe
+// progress the way EqualSamplingRenderer's work reports do
+// (Core/Renderer/Sampling/EqualSamplingRenderer.cpp:205-322).
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/phg.h"
+
+namespace phg {
+
+// RenderStats of the reference: integer stat 0 = finished paths per pixel,
+// real stat 0 = paths per second (EqualSamplingRenderer.cpp:258-280).
+struct RenderStats {
+    uint64_t paths_per_pixel = 0;
+    double paths_per_second = 0.0;
+};
+
+struct RenderProgress {
+    uint64_t total_work = 0;   // samples per pixel requested
+    uint64_t work_done = 0;    // samples per pixel finished
+    uint64_t elapsed_ms = 0;
+};
+
+// Linear RGB frame, row 0 = bottom row (HdrRgbFrame as developed by HdrRgbFilm::developRegion).
+struct HdrRgbFrame {
+    uint32_t width = 0, height = 0;
+    std::vector<float> rgb;  // 3 floats per pixel
+};
+
+struct GpuBneeptSettings {
+    uint32_t width = 0, height = 0;
+    uint32_t spp = 16;
+    uint32_t max_bounces = 10000;   // reference semantics: MAX_RAY_BOUNCES
+    uint64_t seed = 0;
+    uint32_t chunk_spp = 4;         // samples per pixel per progressive chunk
+    int32_t device = 0;
+};
+
+class GpuBneeptRenderer {
+public:
+    explicit GpuBneeptRenderer(const GpuBneeptSettings& settings);
+    ~GpuBneeptRenderer();
+    GpuBneeptRenderer(const GpuBneeptRenderer&) = delete;
+    GpuBneeptRenderer& operator=(const GpuBneeptRenderer&) = delete;
+
+    // Renderer::doUpdate: upload the cooked (flattened) scene, build the BVH, clear the film.
+    void doUpdate(const phg_scene_desc& scene);
+    // Renderer::doRender: render all samples (blocking), chunk by chunk.
+    void doRender();
+    // Renderer::retrieveFrame: developed frame (sum / weight per pixel).  Layer 0 only.
+    void retrieveFrame(std::size_t layerIndex, HdrRgbFrame& out_frame);
+    // Renderer::asyncQueryRenderStats / asyncQueryRenderProgress: safe during doRender.
+    RenderStats asyncQueryRenderStats() const;
+    RenderProgress asyncQueryRenderProgress() const;
+    // Renderer::asyncPeekFrame: the frame as of the last finished chunk (region = whole image).
+    void asyncPeekFrame(std::size_t layerIndex, HdrRgbFrame& out_frame);
+    // Renderer::getObservationInfo: names of the stats above.
+    std::vector<std::string> getObservationInfo() const;
+
+private:
+    void check(phg_status st, const char* what);
+    void develop_locked(HdrRgbFrame& out);
+
+    GpuBneeptSettings m_settings;
+    void* m_ctx = nullptr;
+    std::vector<double> m_film;          // host copy of the last finished chunk's film
+    mutable std::mutex m_filmMutex;
+    std::atomic<uint64_t> m_sppDone{0};
+    std::atomic<uint64_t> m_elapsedMs{0};
+    std::atomic<double> m_pathsPerSecond{0.0};
+};
+
+}  // namespace phg

@@ -28,7 +28,8 @@ MAT_ABSORBER, MAT_LAMBERT, MAT_CONDUCTOR, MAT_DIELECTRIC = 0, 1, 2, 3
 GEN_CORNELL, GEN_PROCEDURAL, GEN_OCCLUDED_ROOM = 1, 2, 3
 QUERY_CLOSEST, QUERY_ANY = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
-RENDER_COUNT_TRAVERSAL = 1
+RENDER_COUNT_TRAVERSAL, RENDER_GUIDED, RENDER_TRAIN = 1, 2, 4
+GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 13
 FLT_MAX = float(np.finfo(np.float32).max)
 
 
@@ -67,7 +68,8 @@ class phg_bvh_opts(C.Structure):
 class phg_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("spp_begin", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("spp_per_pass", C.c_uint32), ("seed", C.c_uint64),
-                ("flags", C.c_uint32), ("pad", C.c_uint32)]
+                ("flags", C.c_uint32), ("guide_prob", C.c_float), ("guide_lr", C.c_float),
+                ("guide_batch", C.c_uint32), ("guide_steps", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class phg_stats(C.Structure):
@@ -78,7 +80,9 @@ class phg_stats(C.Structure):
                 ("bvh_stack_need", C.c_uint32), ("pad", C.c_uint32), ("bvh_build_ms", C.c_double),
                 ("trace_node_visits", C.c_uint64), ("trace_tri_tests", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
-                ("trace_launches", C.c_uint32), ("shadow_launches", C.c_uint32)]
+                ("trace_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
+                ("guide_infer_ms", C.c_double), ("guide_train_ms", C.c_double), ("guide_queries", C.c_uint64),
+                ("guide_records", C.c_uint64), ("guide_steps", C.c_uint32), ("pad2", C.c_uint32)]
 
 
 class phg_gen_opts(C.Structure):
@@ -94,7 +98,8 @@ class phg_scene_host(C.Structure):
 # every entry point declared in include/phg.h
 EXPORTED = ["phg_create", "phg_destroy", "phg_last_error", "phg_upload_scene", "phg_intersect",
             "phg_intersect_counted", "phg_film_clear", "phg_film_read", "phg_render", "phg_get_stats",
-            "phg_scene_generate", "phg_scene_free"]
+            "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_gradients",
+            "phg_guide_train", "phg_scene_generate", "phg_scene_free"]
 
 _lib = None
 
@@ -119,6 +124,11 @@ def lib() -> C.CDLL:
     L.phg_film_read.argtypes = [vp, vp, C.c_uint64]
     L.phg_render.argtypes = [vp, C.POINTER(phg_render_opts)]
     L.phg_get_stats.argtypes = [vp, C.POINTER(phg_stats)]
+    L.phg_guide_set_params.argtypes = [vp, vp, C.c_uint32]
+    L.phg_guide_get_params.argtypes = [vp, vp, C.c_uint32]
+    L.phg_guide_forward.argtypes = [vp, vp, C.c_uint64, vp]
+    L.phg_guide_gradients.argtypes = [vp, vp, C.c_uint64, vp]
+    L.phg_guide_train.argtypes = [vp, vp, C.c_uint64, C.c_float, C.c_uint32]
     L.phg_scene_generate.argtypes = [C.POINTER(phg_gen_opts), C.POINTER(C.POINTER(phg_scene_host))]
     L.phg_scene_free.argtypes = [C.POINTER(phg_scene_host)]
     L.phg_scene_free.restype = None
@@ -289,10 +299,40 @@ class Context:
         self.width, self.height = width, height
 
     def render(self, spp: int, max_bounces: int = 10000, seed: int = 0, spp_begin: int = 0,
-               spp_per_pass: int = 0, count_traversal: bool = False):
-        o = phg_render_opts(self.width, self.height, spp, spp_begin, max_bounces, spp_per_pass, seed,
-                            RENDER_COUNT_TRAVERSAL if count_traversal else 0, 0)
+               spp_per_pass: int = 0, count_traversal: bool = False, guided: bool = False, train: bool = False,
+               guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4):
+        flags = (RENDER_COUNT_TRAVERSAL if count_traversal else 0) | (RENDER_GUIDED if guided else 0) | \
+            (RENDER_TRAIN if train else 0)
+        o = phg_render_opts(self.width, self.height, spp, spp_begin, max_bounces, spp_per_pass, seed, flags,
+                            guide_prob, guide_lr, guide_batch, guide_steps, 0)
         self._check(self._L.phg_render(self._h, C.byref(o)), "phg_render")
+
+    # ---- guiding network
+    def guide_set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self._L.phg_guide_set_params(self._h, p.ctypes.data, p.size), "phg_guide_set_params")
+
+    def guide_get_params(self) -> np.ndarray:
+        p = np.zeros(GUIDE_PARAMS, np.float32)
+        self._check(self._L.phg_guide_get_params(self._h, p.ctypes.data, p.size), "phg_guide_get_params")
+        return p
+
+    def guide_forward(self, features: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 9)
+        out = np.zeros((f.shape[0], GUIDE_OUTPUTS), np.float32)
+        self._check(self._L.phg_guide_forward(self._h, f.ctypes.data, f.shape[0], out.ctypes.data), "phg_guide_forward")
+        return out
+
+    def guide_gradients(self, records: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, GUIDE_RECORD)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        self._check(self._L.phg_guide_gradients(self._h, r.ctypes.data, r.shape[0], g.ctypes.data),
+                    "phg_guide_gradients")
+        return g
+
+    def guide_train(self, records: np.ndarray, lr: float = 1e-3, steps: int = 1):
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, GUIDE_RECORD)
+        self._check(self._L.phg_guide_train(self._h, r.ctypes.data, r.shape[0], lr, steps), "phg_guide_train")
 
     def film(self) -> np.ndarray:
         """(H, W, 8) float64: sum r, g, b, weight, sum r^2, g^2, b^2, 0; row 0 = bottom row."""

@@ -19,6 +19,10 @@ def lib():
         vp = C.c_void_p
         _lib.emu_intersect.argtypes = [vp, C.c_uint32, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32,
                                        vp, vp, vp, vp, vp, vp]
+        _lib.emu_guide_output_grad.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        _lib.emu_guide_pdf.argtypes = [vp, C.c_uint32, vp, vp]
+        _lib.emu_guide_sample.argtypes = [vp, C.c_uint32, vp, vp]
+        _lib.emu_guide_encode.argtypes = [vp, C.c_uint32, vp]
     return _lib
 
 
@@ -35,3 +39,37 @@ def intersect(verts, rays, any_hit=False, max_leaf=4, bins=16):
     if st < 0:
         raise RuntimeError(f"emu_intersect failed ({st})")
     return prim, t, b, c, nv, nt
+
+
+def _f(a, shape):
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))
+
+
+def guide_output_grad(o, w3, weight):
+    o = _f(o, (-1, 40)); n = o.shape[0]
+    w3 = _f(w3, (n, 3)); weight = _f(weight, (n,))
+    out = np.zeros((n, 40), np.float32)
+    lib().emu_guide_output_grad(o.ctypes.data, n, w3.ctypes.data, weight.ctypes.data, out.ctypes.data)
+    return out
+
+
+def guide_pdf(o, w3):
+    o = _f(o, (-1, 40)); n = o.shape[0]
+    w3 = _f(w3, (n, 3))
+    out = np.zeros(n, np.float32)
+    lib().emu_guide_pdf(o.ctypes.data, n, w3.ctypes.data, out.ctypes.data)
+    return out
+
+
+def guide_sample(o40, u3):
+    o40 = _f(o40, (40,)); u3 = _f(u3, (-1, 3)); n = u3.shape[0]
+    out = np.zeros((n, 3), np.float32)
+    lib().emu_guide_sample(o40.ctypes.data, n, u3.ctypes.data, out.ctypes.data)
+    return out
+
+
+def guide_encode(x9):
+    x9 = _f(x9, (-1, 9)); n = x9.shape[0]
+    out = np.zeros((n, 64), np.float32)
+    lib().emu_guide_encode(x9.ctypes.data, n, out.ctypes.data)
+    return out

@@ -38,3 +38,33 @@ extern "C" int emu_intersect(const float* verts, uint32_t ntri, const float* ray
     }
     return overflow ? -2 : (int)bvh.stack_need;
 }
+
+// ---- guiding-model device math, compiled for the host
+#include "../../photon-v2_amd/csrc/guide_device.h"
+
+extern "C" void emu_guide_output_grad(const float* o, uint32_t n, const float* w3, const float* weight, float* out) {
+    for (uint32_t i = 0; i < n; ++i)
+        guide_output_grad(o + 40 * (size_t)i, mk(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]), weight[i], out + 40 * (size_t)i);
+}
+
+extern "C" void emu_guide_pdf(const float* o, uint32_t n, const float* w3, float* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        float lobes[40];
+        guide_decode(o + 40 * (size_t)i, lobes);
+        out[i] = guide_pdf(lobes, mk(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]));
+    }
+}
+
+extern "C" void emu_guide_sample(const float* o40, uint32_t n, const float* u3, float* out3) {
+    float lobes[40];
+    guide_decode(o40, lobes);
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3 d = guide_sample(lobes, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2]);
+        out3[3 * i] = d.x; out3[3 * i + 1] = d.y; out3[3 * i + 2] = d.z;
+    }
+}
+
+extern "C" void emu_guide_encode(const float* x9, uint32_t n, float* out64) {
+    for (uint32_t i = 0; i < n; ++i)
+        for (int j = 0; j < 64; ++j) out64[64 * (size_t)i + j] = guide_encode(x9 + 9 * (size_t)i, j);
+}

@@ -1,0 +1,179 @@
+"""Path-guiding network on the GPU against its CPU oracle (oracle/guiding_oracle.py), and
+the guided estimator against closed forms and plain BNEEPT.
+
+Tolerances: the GPU feeds BF16 operands to the matrix cores and accumulates in FP32; the
+oracle rounds the same operands to BF16 and accumulates in FP64, so outputs agree to a few
+BF16 ulps (2% of the output scale) and gradients to 5% in norm (cosine > 0.995)."""
+import numpy as np
+import pytest
+
+import phg
+from oracle import guiding_oracle as go
+from helpers import furnace_scene, pixel_z
+
+pytestmark = pytest.mark.gpu
+
+
+def flat(params):
+    return np.concatenate([np.concatenate([W.ravel(), b.ravel()]) for W, b in params]).astype(np.float32)
+
+
+def unflat(v):
+    out, off = [], 0
+    for fi, fo in go.LAYER_SHAPES:
+        W = v[off:off + fi * fo].reshape(fi, fo).astype(np.float64)
+        off += fi * fo
+        b = v[off:off + fo].astype(np.float64)
+        off += fo
+        out.append((W, b))
+    return out
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = phg.Context(0)
+    c.upload(phg.Scene.generate(phg.GEN_CORNELL))
+    yield c
+    c.close()
+
+
+def random_records(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, 9))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = rng.exponential(1.0, n)
+    return np.concatenate([x, d, w[:, None]], axis=1).astype(np.float32)
+
+
+def test_forward_matches_oracle(ctx):
+    params = go.init_params(3)
+    ctx.guide_set_params(flat(params))
+    x = np.random.default_rng(4).random((1000, 9)).astype(np.float32)
+    gpu = ctx.guide_forward(x)
+    _, ref = go.forward(unflat(flat(params)), go.encode(x.astype(np.float64)), bf16=True)
+    scale = np.abs(ref).max()
+    err = np.abs(gpu - ref).max()
+    print(f"[guide] forward max_abs_err={err:.3e} scale={scale:.3e}", flush=True)
+    assert err <= 0.02 * max(1.0, scale)
+
+
+def test_gradients_match_oracle(ctx):
+    params = go.init_params(5)
+    ctx.guide_set_params(flat(params))
+    rec = random_records(4096, 6)
+    gpu = ctx.guide_gradients(rec).astype(np.float64)
+    e = go.encode(rec[:, :9].astype(np.float64))
+    grads = go.backward(unflat(flat(params)), e, rec[:, 9:12], rec[:, 12], bf16=True)
+    ref = np.concatenate([np.concatenate([gW.ravel(), gb.ravel()]) for gW, gb in grads])
+    rel = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
+    cos = gpu @ ref / (np.linalg.norm(gpu) * np.linalg.norm(ref))
+    print(f"[guide] gradient rel_err={rel:.3e} cosine={cos:.6f}", flush=True)
+    assert rel < 0.05 and cos > 0.995
+    # per layer, too
+    off = 0
+    for li, (gW, gb) in enumerate(grads):
+        n = gW.size + gb.size
+        g_l, r_l = gpu[off:off + n], ref[off:off + n]
+        off += n
+        assert np.linalg.norm(g_l - r_l) <= 0.08 * np.linalg.norm(r_l) + 1e-9, li
+
+
+def test_adam_steps_follow_oracle(ctx):
+    params = go.init_params(7)
+    ctx.guide_set_params(flat(params))
+    rec = random_records(2048, 8)
+    ctx.guide_train(rec, lr=1e-3, steps=3)
+    gpu = ctx.guide_get_params().astype(np.float64)
+    p = unflat(flat(params))
+    m = [[np.zeros_like(W), np.zeros_like(b)] for W, b in p]
+    v = [[np.zeros_like(W), np.zeros_like(b)] for W, b in p]
+    e = go.encode(rec[:, :9].astype(np.float64))
+    for t in range(1, 4):
+        g = go.backward(p, e, rec[:, 9:12], rec[:, 12], bf16=True)
+        p = go.adam_step(p, g, m, v, t)
+    ref = flat(p).astype(np.float64)
+    moved_ref = ref - flat(params)
+    moved_gpu = gpu - flat(params)
+    # each Adam step moves a parameter by about lr; compare the displacement
+    cos = moved_gpu @ moved_ref / (np.linalg.norm(moved_gpu) * np.linalg.norm(moved_ref))
+    print(f"[guide] adam displacement cosine={cos:.4f}", flush=True)
+    assert cos > 0.97
+
+
+def test_training_fits_a_known_distribution(ctx):
+    """Directions drawn from a vMF whose mean depends on the position feature: training
+    must raise the mean log-likelihood of held-out samples well above the uniform 1/(4 pi)."""
+    rng = np.random.default_rng(9)
+    n = 1 << 16
+
+    def draw(n):
+        x = rng.random((n, 9))
+        mu = np.stack([np.cos(2 * np.pi * x[:, 0]), np.sin(2 * np.pi * x[:, 0]), 0.3 * np.ones(n)], 1)
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        k = 20.0
+        u = rng.random(n)
+        w = 1 + np.log(u + (1 - u) * np.exp(-2 * k)) / k
+        phi = 2 * np.pi * rng.random(n)
+        s = np.sqrt(np.maximum(0, 1 - w * w))
+        t1 = np.cross(mu, np.array([0, 0, 1.0]))
+        t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
+        t2 = np.cross(mu, t1)
+        d = mu * w[:, None] + t1 * (s * np.cos(phi))[:, None] + t2 * (s * np.sin(phi))[:, None]
+        return np.concatenate([x, d, np.ones((n, 1))], 1).astype(np.float32)
+
+    train = draw(n)
+    test = draw(4096)
+    ctx.guide_set_params(flat(go.init_params(10)))
+    for _ in range(20):
+        ctx.guide_train(train, lr=3e-3, steps=10)
+    o = ctx.guide_forward(test[:, :9])
+    ll = np.log(go.mixture_pdf(o.astype(np.float64), test[:, 9:12].astype(np.float64))).mean()
+    print(f"[guide] held-out mean log-likelihood={ll:.3f} (uniform={np.log(1 / (4 * np.pi)):.3f})", flush=True)
+    assert ll > np.log(1 / (4 * np.pi)) + 1.5
+
+
+@pytest.mark.parametrize("albedo,radiance", [(0.5, 0.5)])
+def test_guided_furnace_stays_unbiased(albedo, radiance):
+    """Guided BNEEPT with online training: the furnace's closed form Le / (1 - rho) holds."""
+    sc = furnace_scene(albedo=albedo, radiance=radiance)
+    with phg.Context(0) as c:
+        c.upload(sc)
+        c.film_clear(32, 32)
+        c.render(8, max_bounces=10000, seed=1, spp_per_pass=1, guided=True, train=True, guide_batch=4096,
+                 guide_steps=4)
+        c.film_clear(32, 32)
+        c.render(64, max_bounces=10000, seed=2, spp_per_pass=1, guided=True, train=False)
+        film = c.film()
+        st = c.stats()
+    expect = radiance / (1 - albedo)
+    n = film[..., 3].sum()
+    mean = film[..., 0].sum() / n
+    se = np.sqrt(max(film[..., 4].sum() / n - mean * mean, 0) / n)
+    print(f"[guide] furnace guided mean={mean:.6f} expect={expect:.6f} se={se:.2e} queries={st['guide_queries']}",
+          flush=True)
+    assert st["guide_queries"] > 0
+    assert abs(mean - expect) < 4 * se + 1e-4
+
+
+def test_guided_matches_plain_in_expectation():
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    W = H = 32
+    with phg.Context(0) as c:
+        c.upload(sc)
+        c.film_clear(W, H)
+        c.render(16, max_bounces=10000, seed=3, spp_per_pass=1, guided=True, train=True, guide_batch=8192)
+        c.film_clear(W, H)
+        c.render(256, max_bounces=10000, seed=4, spp_per_pass=4, guided=True)
+        guided = c.film()
+        c.film_clear(W, H)
+        c.render(256, max_bounces=10000, seed=5, spp_per_pass=4)
+        plain = c.film()
+    z = pixel_z(guided, plain)
+    frac = np.mean(np.abs(z) > 2.5758)
+    mg = guided[..., :3].sum() / guided[..., 3].sum()
+    mp = plain[..., :3].sum() / plain[..., 3].sum()
+    print(f"[guide] cornell guided vs plain: outside_99CI={frac:.4f} mean_guided={mg:.5f} mean_plain={mp:.5f}",
+          flush=True)
+    assert frac < 0.02
+    assert abs(mg - mp) < 0.01 * mp
