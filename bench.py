@@ -26,6 +26,21 @@ for p in (os.path.join(ROOT, "photon-v2_amd"), ROOT):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bneept_procedural1M_traffic.json")
+
+
+def measured_traffic(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed FETCH_SIZE/WRITE_SIZE profile of
+    this workload (rocprofv3 counters, copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for name, v in prof.get("kernels", {}).items():
+        if name.startswith(kernel_prefix):
+            return round(v["traffic_bytes"])
+    return None
 METRIC = "Msamples/s at 1 GPU; time to relMSE 0.01, guided vs plain GPU BNEEPT"
 
 
@@ -145,7 +160,8 @@ def main():
                                  "extension": round(rays[1] / total_s / 1e6, 3),
                                  "shadow": round(rays[2] / total_s / 1e6, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic("phg::k_extend"),
+                     "traffic_source": "profiles/bneept_procedural1M_traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch)",
                      "kernel": "k_extend (closest-hit traversal)",
                      "launches": trace_launch, "avg_launch_ms": round(trace_ms / max(trace_launch, 1), 4),
                      "bytes_per_launch": round(bytes_closest / max(trace_launch, 1)),

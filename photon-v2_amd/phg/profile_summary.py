@@ -1,0 +1,71 @@
+"""Summaries of rocprofv3 result databases (rocpd SQLite) for the profiles/ folder.
+
+    python -m phg.profile_summary stats  <kernel_trace.db>
+    python -m phg.profile_summary pmc    <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> <copy_bytes>
+
+`stats` prints per-kernel call counts and durations; `pmc` prints per-kernel FETCH_SIZE and
+WRITE_SIZE per dispatch, scaled by what the same counters read for a plain device copy of
+`copy_bytes` bytes, and emits a JSON line with the HBM traffic per launch of each kernel.
+"""
+import json
+import re
+import sqlite3
+import sys
+
+
+def _rows(path, sql, args=()):
+    con = sqlite3.connect(path)
+    try:
+        return con.execute(sql, args).fetchall()
+    finally:
+        con.close()
+
+
+def short(name):
+    return re.sub(r"\(.*", "", name)
+
+
+def kernel_stats(path):
+    rows = _rows(path, "select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                       "from kernels group by name order by sum(duration) desc")
+    total = sum(r[2] for r in rows) or 1
+    return [{"kernel": short(n), "calls": c, "total_ms": s / 1e6, "avg_ms": a / 1e6, "min_ms": lo / 1e6,
+             "max_ms": hi / 1e6, "percent": 100.0 * s / total} for n, c, s, a, lo, hi in rows]
+
+
+def counter(path, name):
+    rows = _rows(path, "select name, count(*), avg(counter_value) from pmc_events where counter_name=? group by name",
+                 (name,))
+    return {short(n): (c, a) for n, c, a in rows}
+
+
+def main(argv):
+    if len(argv) >= 2 and argv[0] == "stats":
+        st = kernel_stats(argv[1])
+        print(f"{'kernel':60s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'min ms':>9s} {'max ms':>9s} {'%':>6s}")
+        for r in st:
+            print(f"{r['kernel'][:60]:60s} {r['calls']:6d} {r['total_ms']:10.3f} {r['avg_ms']:9.4f} "
+                  f"{r['min_ms']:9.4f} {r['max_ms']:9.4f} {r['percent']:6.2f}")
+        return 0
+    if len(argv) == 6 and argv[0] == "pmc":
+        f, w, cf, cw = (counter(p, c) for p, c in zip(argv[1:5], ["FETCH_SIZE", "WRITE_SIZE", "FETCH_SIZE", "WRITE_SIZE"]))
+        copy_kib = float(argv[5]) / 1024.0
+        kf = copy_kib / max(v[1] for v in cf.values())
+        kw = copy_kib / max(v[1] for v in cw.values())
+        print(f"copy calibration: FETCH_SIZE x {kf:.4f}, WRITE_SIZE x {kw:.4f} (copy of {copy_kib:.0f} KiB)")
+        out = {"fetch_scale": kf, "write_scale": kw, "kernels": {}}
+        print(f"{'kernel':40s} {'dispatches':>10s} {'fetch MB':>10s} {'write MB':>10s} {'HBM MB/launch':>14s}")
+        for k in sorted(f, key=lambda k: -f[k][1]):
+            n, fa = f[k]
+            wa = w.get(k, (0, 0.0))[1]
+            fb, wb = fa * kf * 1024, wa * kw * 1024
+            out["kernels"][k] = {"dispatches": n, "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb}
+            print(f"{k[:40]:40s} {n:10d} {fb / 1e6:10.2f} {wb / 1e6:10.2f} {(fb + wb) / 1e6:14.2f}")
+        print(json.dumps(out))
+        return 0
+    print(__doc__)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
