@@ -41,6 +41,8 @@ def measured_traffic(kernel_prefix):
         if name.startswith(kernel_prefix):
             return round(v["traffic_bytes"])
     return None
+
+
 METRIC = "Msamples/s at 1 GPU; time to relMSE 0.01, guided vs plain GPU BNEEPT"
 
 

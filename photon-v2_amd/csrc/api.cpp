@@ -59,6 +59,7 @@ struct Ctx {
     DevBuf<phg_ray> ext0, ext1, shadow;
     DevBuf<uint32_t> ext_pid0, ext_pid1, shadow_pid;
     DevBuf<uint32_t> counts;
+    DevBuf<int32_t> spill;       // traversal-stack spill of the persistent traversal grid
     DevBuf<unsigned long long> tcount;
     DevBuf<uint32_t> errflags;
     // intersect staging
@@ -81,6 +82,8 @@ struct Ctx {
     phg_stats stats{};
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t cu_count = 256;
+    uint32_t trace_blocks_per_cu = 8;   // resident blocks per compute unit, from the occupancy query
+    uint32_t shade_blocks_per_cu = 4;
 };
 
 #define PHG_HIP(ctx, expr)                                                                        \
@@ -198,6 +201,11 @@ phg_status phg_create(void** out_ctx, const phg_device_opts* opts) {
         delete c;
         return PHG_ERR_HIP;
     }
+    if (resident_blocks(&c->trace_blocks_per_cu, &c->shade_blocks_per_cu) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return PHG_ERR_HIP;
+    }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
     *out_ctx = c;
     return PHG_OK;
@@ -211,7 +219,7 @@ phg_status phg_destroy(void* ctx) {
     c->nodes.free_(); c->tris.free_(); c->mats.free_(); c->emitters.free_(); c->emit_cdf.free_(); c->emit_tris.free_();
     c->film.free_(); c->rng.free_(); c->T.free_(); c->Wem.free_(); c->prevP.free_(); c->L.free_(); c->hits.free_();
     c->shadow_c.free_(); c->ext0.free_(); c->ext1.free_(); c->shadow.free_(); c->ext_pid0.free_(); c->ext_pid1.free_();
-    c->shadow_pid.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
+    c->shadow_pid.free_(); c->spill.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
     c->q_nvis.free_(); c->q_ntri.free_();
     c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
     c->g_wn.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
@@ -261,8 +269,8 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_op
         return set_err(c, PHG_ERR_INVALID_ARG, ex.what());
     }
     c->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (bvh.stack_need > 64)
-        return set_err(c, PHG_ERR_UNSUPPORTED, "BVH needs a traversal stack deeper than 64 entries");
+    if (bvh.stack_need > 4096)
+        return set_err(c, PHG_ERR_UNSUPPORTED, "BVH needs a traversal stack deeper than 4096 entries");
     // triangles in leaf order
     std::vector<TriRec> tr(s->num_tris);
     std::vector<uint32_t> pos_of(s->num_tris);
@@ -357,6 +365,16 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_op
     return PHG_OK;
 }
 
+static uint32_t trace_grid(const Ctx* c) { return c->cu_count * c->trace_blocks_per_cu; }
+static uint32_t spill_cap(const Ctx* c) {
+    return c->stack_need > (uint32_t)kTraceRing ? c->stack_need - (uint32_t)kTraceRing : 0u;
+}
+static phg_status ensure_spill(Ctx* c) {
+    const size_t n = (size_t)trace_grid(c) * kTraceBlock * std::max<uint32_t>(spill_cap(c), 1u);
+    PHG_HIP(c, c->spill.ensure(n));
+    return PHG_OK;
+}
+
 static phg_status do_intersect(Ctx* c, const phg_ray* rays, uint64_t n, phg_hit* hits, int query, int ptr_kind,
                                uint32_t* nvis, uint32_t* ntri) {
     if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
@@ -386,8 +404,12 @@ static phg_status do_intersect(Ctx* c, const phg_ray* rays, uint64_t n, phg_hit*
         d_nvis = nvis; d_ntri = ntri;
     }
     PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    {
+        const phg_status st = ensure_spill(c);
+        if (st != PHG_OK) return st;
+    }
     PHG_HIP(c, launch_intersect(c->sc, d_rays, n, d_hits, query == PHG_QUERY_ANY, d_nvis, d_ntri, c->errflags.p,
-                                c->stack_need, c->stream));
+                                c->spill.p, spill_cap(c), trace_grid(c), c->stream));
     PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
     if (ptr_kind == PHG_PTR_HOST) {
         PHG_HIP(c, hipMemcpyAsync(hits, d_hits, n * sizeof(phg_hit), hipMemcpyDeviceToHost, c->stream));
@@ -478,6 +500,12 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     pb.ext[0] = c->ext0.p; pb.ext[1] = c->ext1.p; pb.ext_pid[0] = c->ext_pid0.p; pb.ext_pid[1] = c->ext_pid1.p;
     pb.hits = c->hits.p; pb.shadow = c->shadow.p; pb.shadow_pid = c->shadow_pid.p; pb.shadow_c = c->shadow_c.p;
     pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = c->film.p;
+    {
+        const phg_status st = ensure_spill(c);
+        if (st != PHG_OK) return st;
+    }
+    pb.spill = c->spill.p;
+    pb.spill_cap = spill_cap(c);
     const uint32_t count_trav = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) ? 1u : 0u;
     const bool guided = (o->flags & PHG_RENDER_GUIDED) != 0;
     const bool train = (o->flags & PHG_RENDER_TRAIN) != 0;
@@ -502,8 +530,8 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     pb.rec = c->g_rec.p;
     double t_ginfer = 0, t_gtrain = 0;
     uint64_t g_queries = 0, g_records = 0;
-    const uint32_t grid_trace = c->cu_count * 8;   // persistent-style grid-stride launches
-    const uint32_t grid_shade = c->cu_count * 4;
+    const uint32_t grid_trace = trace_grid(c);     // persistent-style grid-stride launches
+    const uint32_t grid_shade = c->cu_count * c->shade_blocks_per_cu;
     double t_trace = 0, t_shadow = 0, t_shade = 0;
     uint64_t rays_primary = 0;
     uint32_t n_trace = 0, n_shadow = 0;
@@ -526,7 +554,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
         for (uint32_t b = 0;; ++b) {
             const bool use_guide = guided && pp.guide_prob > 0.0f && b < o->max_bounces;
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-            PHG_HIP(c, launch_extend(c->sc, pb, b, grid_trace, c->stack_need, count_trav, c->stream));
+            PHG_HIP(c, launch_extend(c->sc, pb, b, grid_trace, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
             if (use_guide) {
                 PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], c->sc.tris, c->g_scene,
@@ -535,7 +563,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
             PHG_HIP(c, hipEventRecord(c->ev[4], c->stream));
             PHG_HIP(c, launch_shade(c->sc, pp, pb, b, grid_shade, guided || train, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            PHG_HIP(c, launch_shadow(c->sc, pb, b, grid_trace, c->stack_need, count_trav, rec_bounces, pp.paths, c->stream));
+            PHG_HIP(c, launch_shadow(c->sc, pb, b, grid_trace, count_trav, rec_bounces, pp.paths, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[3], c->stream));
             uint32_t next[2] = {0, 0};
             PHG_HIP(c, hipMemcpyAsync(&next[0], c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));

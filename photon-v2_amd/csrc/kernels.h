@@ -10,7 +10,8 @@
 
 namespace phg {
 
-constexpr uint32_t kTraceBlock = 128;   // traversal kernels: 2 waves, LDS stack [depth][128]
+constexpr uint32_t kTraceBlock = 128;   // traversal kernels: 2 waves, LDS stack ring [slot][128]
+constexpr int kTraceRing = 16;          // LDS ring entries per lane; older entries spill to global memory
 constexpr uint32_t kShadeBlock = 256;
 constexpr uint32_t kCountSlots = 64;    // queue counters, used modulo this many bounces
 
@@ -82,15 +83,21 @@ struct PathBuffers {
     // guided shading
     const float* lobes;      // per queued extension ray: 8 x (mu xyz, kappa, weight)
     float4* rec;             // training records [bounce][path][kRecFloat4]
+    int32_t* spill;          // traversal-stack spill, spill_cap entries per traversal thread
+    uint32_t spill_cap;
 };
 
 hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
-                            uint32_t* nvis, uint32_t* ntri, uint32_t* flags, uint32_t stack, hipStream_t st);
+                            uint32_t* nvis, uint32_t* ntri, uint32_t* flags, int32_t* spill, uint32_t spill_cap,
+                            uint32_t grid, hipStream_t st);
 hipError_t launch_generate(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, hipStream_t st);
-hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
-                         uint32_t count, hipStream_t st);
-hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
-                         uint32_t count, uint32_t rec_bounces, uint32_t npaths, hipStream_t st);
+// Blocks of the traversal kernels (kTraceBlock threads) and of the shading kernel
+// (kShadeBlock threads) that one compute unit holds at once; sizes the persistent grids.
+hipError_t resident_blocks(uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu);
+hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
+                         hipStream_t st);
+hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
+                         uint32_t rec_bounces, uint32_t npaths, hipStream_t st);
 hipError_t launch_shade(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
                         bool guided, hipStream_t st);
 hipError_t launch_film(const PassParams& pp, const PathBuffers& pb, uint32_t spp_in_pass, hipStream_t st);

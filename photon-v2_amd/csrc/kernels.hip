@@ -8,6 +8,8 @@
 // atomic per wave.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "pt_device.h"
 #include "traverse.h"
@@ -43,29 +45,34 @@ __device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
 // ---------------------------------------------------------------------------------------
 // Batched ray queries (phg_intersect): rays as phg_ray, hits as phg_hit with input ids.
 // ---------------------------------------------------------------------------------------
-template <int STACK, bool ANY, bool COUNT>
+template <int RING, bool ANY, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_intersect(SceneDev sc, const phg_ray* __restrict__ rays, uint64_t n,
                                                           phg_hit* __restrict__ hits, uint32_t* __restrict__ nvis_out,
-                                                          uint32_t* __restrict__ ntri_out, uint32_t* __restrict__ flags) {
-    __shared__ int32_t stack_mem[STACK * kTraceBlock];
-    const uint64_t i = (uint64_t)xcd_block_id() * kTraceBlock + threadIdx.x;
-    if (i >= n) return;
-    const float4* rp = reinterpret_cast<const float4*>(rays + i);
-    const float4 r0 = rp[0], r1 = rp[1];
-    uint32_t nvis = 0, ntri = 0, ovf = 0;
-    const TraceResult tr = traverse<STACK, ANY, COUNT>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
-                                                       stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
-    phg_hit h;
-    if (tr.tri >= 0) {
-        h.t = ANY ? 0.0f : tr.t;
-        h.prim = ANY ? 1 : sc.tris[tr.tri].prim;
-        h.b = ANY ? 0.0f : tr.b;
-        h.c = ANY ? 0.0f : tr.c;
-    } else {
-        h.t = 0.0f; h.prim = -1; h.b = 0.0f; h.c = 0.0f;
+                                                          uint32_t* __restrict__ ntri_out, uint32_t* __restrict__ flags,
+                                                          int32_t* __restrict__ spill, uint32_t spill_cap) {
+    __shared__ int32_t stack_mem[RING * kTraceBlock];
+    const uint32_t gtid = xcd_block_id() * kTraceBlock + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
+    uint32_t ovf = 0;
+    for (uint64_t i = gtid; i < n; i += stride) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        uint32_t nvis = 0, ntri = 0;
+        const TraceResult tr = traverse<RING, ANY, COUNT>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z),
+                                                          r0.w, r1.w, stack_mem + threadIdx.x, kTraceBlock,
+                                                          spill + (size_t)gtid * spill_cap, spill_cap, &nvis, &ntri, &ovf);
+        phg_hit h;
+        if (tr.tri >= 0) {
+            h.t = ANY ? 0.0f : tr.t;
+            h.prim = ANY ? 1 : sc.tris[tr.tri].prim;
+            h.b = ANY ? 0.0f : tr.b;
+            h.c = ANY ? 0.0f : tr.c;
+        } else {
+            h.t = 0.0f; h.prim = -1; h.b = 0.0f; h.c = 0.0f;
+        }
+        hits[i] = h;
+        if (COUNT) { nvis_out[i] = nvis; ntri_out[i] = ntri; }
     }
-    hits[i] = h;
-    if (COUNT) { nvis_out[i] = nvis; ntri_out[i] = ntri; }
     if (ovf) atomicOr(flags, 1u);
 }
 
@@ -112,9 +119,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, PassParam
             pb.rec[((size_t)b * pp.paths + i) * kRecFloat4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-template <int STACK>
+template <int RING>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
-    __shared__ int32_t stack_mem[STACK * kTraceBlock];
+    __shared__ int32_t stack_mem[RING * kTraceBlock];
+    int32_t* spill = pb.spill + (size_t)(xcd_block_id() * kTraceBlock + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[bounce % kCountSlots];
     // the counters the shade stage of this bounce pushes to start from zero
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -130,11 +138,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
         const float4 r0 = rp[0], r1 = rp[1];
         TraceResult tr;
         if (count_traversal)
-            tr = traverse<STACK, false, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
-                                              stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+            tr = traverse<RING, false, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                              stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         else
-            tr = traverse<STACK, false, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
-                                               stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+            tr = traverse<RING, false, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                               stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
     }
     if (count_traversal) {
@@ -144,10 +152,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
     if (ovf) atomicOr(pb.error_flags, 1u);
 }
 
-template <int STACK>
+template <int RING>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal,
                                                         uint32_t rec_bounces, uint32_t npaths) {
-    __shared__ int32_t stack_mem[STACK * kTraceBlock];
+    __shared__ int32_t stack_mem[RING * kTraceBlock];
+    int32_t* spill = pb.spill + (size_t)(xcd_block_id() * kTraceBlock + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[kCountSlots + bounce % kCountSlots];
     const uint32_t stride = gridDim.x * kTraceBlock;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&pb.traversal_counters[5], (unsigned long long)n);
@@ -157,11 +166,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers
         const float4 r0 = rp[0], r1 = rp[1];
         TraceResult tr;
         if (count_traversal)
-            tr = traverse<STACK, true, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
-                                             stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+            tr = traverse<RING, true, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                             stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         else
-            tr = traverse<STACK, true, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
-                                              stack_mem + threadIdx.x, kTraceBlock, &nvis, &ntri, &ovf);
+            tr = traverse<RING, true, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+                                              stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         if (tr.tri < 0) {
             const uint32_t pid = pb.shadow_pid[i];
             const float4 c = pb.shadow_c[i];
@@ -441,26 +450,26 @@ __global__ void k_fill_double(double* p, uint64_t n) {
 // ---------------------------------------------------------------------------------------
 static inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
-template <int STACK>
+template <int RING>
 static hipError_t launch_intersect_t(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
-                                     uint32_t* nvis, uint32_t* ntri, uint32_t* flags, hipStream_t st) {
-    const uint32_t nb = blocks_for(n, kTraceBlock);
+                                     uint32_t* nvis, uint32_t* ntri, uint32_t* flags, int32_t* spill, uint32_t spill_cap,
+                                     uint32_t grid, hipStream_t st) {
+    const uint32_t nb = (uint32_t)std::min<uint64_t>(grid, blocks_for(n, kTraceBlock));
     if (nvis) {
-        hipLaunchKernelGGL((k_intersect<STACK, false, true>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags);
+        hipLaunchKernelGGL((k_intersect<RING, false, true>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags, spill, spill_cap);
     } else if (any) {
-        hipLaunchKernelGGL((k_intersect<STACK, true, false>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags);
+        hipLaunchKernelGGL((k_intersect<RING, true, false>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags, spill, spill_cap);
     } else {
-        hipLaunchKernelGGL((k_intersect<STACK, false, false>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags);
+        hipLaunchKernelGGL((k_intersect<RING, false, false>), dim3(nb), dim3(kTraceBlock), 0, st, sc, rays, n, hits, nvis, ntri, flags, spill, spill_cap);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
-                            uint32_t* nvis, uint32_t* ntri, uint32_t* flags, uint32_t stack, hipStream_t st) {
+                            uint32_t* nvis, uint32_t* ntri, uint32_t* flags, int32_t* spill, uint32_t spill_cap,
+                            uint32_t grid, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    if (stack <= 32) return launch_intersect_t<32>(sc, rays, n, hits, any, nvis, ntri, flags, st);
-    if (stack <= 64) return launch_intersect_t<64>(sc, rays, n, hits, any, nvis, ntri, flags, st);
-    return hipErrorInvalidValue;
+    return launch_intersect_t<kTraceRing>(sc, rays, n, hits, any, nvis, ntri, flags, spill, spill_cap, grid, st);
 }
 
 hipError_t launch_generate(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, hipStream_t st) {
@@ -468,19 +477,15 @@ hipError_t launch_generate(const SceneDev& sc, const PassParams& pp, const PathB
     return hipGetLastError();
 }
 
-hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
-                         uint32_t count, hipStream_t st) {
-    if (stack <= 32) hipLaunchKernelGGL(k_extend<32>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
-    else if (stack <= 64) hipLaunchKernelGGL(k_extend<64>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
-    else return hipErrorInvalidValue;
+hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_extend<kTraceRing>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
     return hipGetLastError();
 }
 
-hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t stack,
-                         uint32_t count, uint32_t rec_bounces, uint32_t npaths, hipStream_t st) {
-    if (stack <= 32) hipLaunchKernelGGL(k_shadow<32>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count, rec_bounces, npaths);
-    else if (stack <= 64) hipLaunchKernelGGL(k_shadow<64>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count, rec_bounces, npaths);
-    else return hipErrorInvalidValue;
+hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
+                         uint32_t rec_bounces, uint32_t npaths, hipStream_t st) {
+    hipLaunchKernelGGL(k_shadow<kTraceRing>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count, rec_bounces, npaths);
     return hipGetLastError();
 }
 
@@ -489,6 +494,20 @@ hipError_t launch_shade(const SceneDev& sc, const PassParams& pp, const PathBuff
     if (guided) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
     else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
     return hipGetLastError();
+}
+
+hipError_t resident_blocks(uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu) {
+    int a = 0, b = 0, c = 0, d = 0, e = 0;
+    hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<kTraceRing>, kTraceBlock, 0);
+    if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<kTraceRing>, kTraceBlock, 0);
+    if (err == hipSuccess)
+        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_intersect<kTraceRing, false, false>, kTraceBlock, 0);
+    if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_shade<false>, kShadeBlock, 0);
+    if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_shade<true>, kShadeBlock, 0);
+    if (err != hipSuccess) return err;
+    *trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, std::min(b, c)));
+    *shade_blocks_per_cu = (uint32_t)std::max(1, std::min(d, e));
+    return hipSuccess;
 }
 
 hipError_t launch_film(const PassParams& pp, const PathBuffers& pb, uint32_t spp_in_pass, hipStream_t st) {

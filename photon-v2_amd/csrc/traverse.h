@@ -1,9 +1,11 @@
 // 4-wide BVH traversal of one ray (host and device): 128-byte node fetches, child boxes
 // tested against the shrinking segment with the reference's robust slab test
 // (Math/Geometry/TAABB3D.ipp:390-420) widened by the watertight test's error bound,
-// near-first order with a per-ray stack addressed as
-// stk[depth * stride] (in LDS on the device: [depth][lane], bank-conflict free), leaves
-// tested with the watertight triangle test.  Closest hit keeps shrinking the segment to the
+// near-first order, leaves tested with the watertight triangle test.
+// Traversal stack: the newest RING entries live in a ring addressed as stk[slot * stride]
+// (LDS on the device, laid out [slot][lane]: bank-conflict free); older entries spill to a
+// per-thread global area of spill_cap entries, so any tree depth stays exact while the LDS
+// footprint per lane stays small (occupancy).  Closest hit keeps shrinking the segment to the
 // accepted hit as TLinearDepthFirstBinaryBvh::nearestTraversal does
 // (Math/Algorithm/BVH/TLinearDepthFirstBinaryBvh.ipp:43-139); any hit returns at the first.
 #pragma once
@@ -14,10 +16,39 @@ namespace phg {
 
 struct TraceResult { float t; int32_t tri; float a, b, c; };
 
-template <int STACK, bool ANY, bool COUNT>
+template <int RING>
+struct TraversalStack {
+    int32_t* ring;        // RING slots, slot k at ring[k * stride]
+    uint32_t stride;
+    int32_t* spill;       // spill_cap entries
+    uint32_t spill_cap;
+    int sp;
+    PHG_HD void push(int32_t v, uint32_t* overflow) {
+        if (sp >= RING) {
+            const int j = sp - RING;  // oldest ring entry moves to the spill area
+            if ((uint32_t)j < spill_cap) spill[j] = ring[(j & (RING - 1)) * stride];
+            else *overflow = 1u;
+        }
+        ring[(sp & (RING - 1)) * stride] = v;
+        ++sp;
+    }
+    PHG_HD int32_t pop() {
+        --sp;
+        const int32_t v = ring[(sp & (RING - 1)) * stride];
+        if (sp >= RING) {
+            const int j = sp - RING;  // bring the newest spilled entry back into the freed slot
+            if ((uint32_t)j < spill_cap) ring[(sp & (RING - 1)) * stride] = spill[j];
+        }
+        return v;
+    }
+};
+
+template <int RING, bool ANY, bool COUNT>
 PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris,
-                                                V3 o, V3 d, float tmin, float tmax, int32_t* __restrict__ stk,
-                                                uint32_t stride, uint32_t* nvis, uint32_t* ntri, uint32_t* overflow) {
+                            V3 o, V3 d, float tmin, float tmax, int32_t* __restrict__ stk, uint32_t stride,
+                            int32_t* __restrict__ spill, uint32_t spill_cap,
+                            uint32_t* nvis, uint32_t* ntri, uint32_t* overflow) {
+    static_assert((RING & (RING - 1)) == 0, "ring size must be a power of two");
     TraceResult r; r.t = 0.0f; r.tri = -1; r.a = 0.0f; r.b = 0.0f; r.c = 0.0f;
     const V3 rcp = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     const bool nx = d.x < 0.0f, ny = d.y < 0.0f, nz = d.z < 0.0f;
@@ -26,7 +57,8 @@ PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __
     const float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
     const int kdom = adx > ady ? (adx > adz ? 0 : 2) : (ady > adz ? 1 : 2);
     float tb = tmax;
-    int sp = 0;
+    TraversalStack<RING> st;
+    st.ring = stk; st.stride = stride; st.spill = spill; st.spill_cap = spill_cap; st.sp = 0;
     int32_t cur = 0;
     while (true) {
         if (cur >= 0) {
@@ -69,19 +101,21 @@ PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __
         const float tk = key[i]; key[i] = key[j]; key[j] = tk;            \
         const int32_t ti = kid[i]; kid[i] = kid[j]; kid[j] = ti;          \
     }
-            PHG_CSWAP(0, 1) PHG_CSWAP(2, 3) PHG_CSWAP(0, 2) PHG_CSWAP(1, 3) PHG_CSWAP(1, 2)
+            if (ANY) {
+                // any hit: order does not matter; move the hits to the front
+                PHG_CSWAP(0, 1) PHG_CSWAP(2, 3) PHG_CSWAP(0, 2) PHG_CSWAP(1, 3)
+            } else {
+                PHG_CSWAP(0, 1) PHG_CSWAP(2, 3) PHG_CSWAP(0, 2) PHG_CSWAP(1, 3) PHG_CSWAP(1, 2)
+            }
 #undef PHG_CSWAP
             if (key[0] == INFINITY) {
-                if (sp == 0) break;
-                cur = stk[(--sp) * stride];
+                if (st.sp == 0) break;
+                cur = st.pop();
                 continue;
             }
 #pragma unroll
             for (int k = 3; k >= 1; --k) {
-                if (key[k] != INFINITY) {
-                    if (sp < STACK) stk[(sp++) * stride] = kid[k];
-                    else *overflow = 1u;
-                }
+                if (key[k] != INFINITY) st.push(kid[k], overflow);
             }
             cur = kid[0];
         } else {
@@ -98,8 +132,8 @@ PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __
                     if (ANY) return r;
                 }
             }
-            if (sp == 0) break;
-            cur = stk[(--sp) * stride];
+            if (st.sp == 0) break;
+            cur = st.pop();
         }
     }
     return r;

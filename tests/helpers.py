@@ -12,6 +12,17 @@ def random_soup(n, seed, spread=10.0, size=1.0):
     return v.reshape(n, 9).astype(np.float32)
 
 
+def nested_soup(n, seed, ratio=0.97):
+    """Triangles at geometrically shrinking sizes and distances from the origin: SAH peels
+    one off at a time, so the BVH is deep and traversal needs a long stack."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(n)
+    s = ratio ** k
+    c = np.stack([s, 0.2 * s * rng.standard_normal(n), 0.2 * s * rng.standard_normal(n)], 1)[:, None, :]
+    v = c + 0.3 * s[:, None, None] * rng.uniform(-1, 1, (n, 3, 3))
+    return v.reshape(n, 9).astype(np.float32)
+
+
 def rays_towards(verts, n, seed, jitter=0.0):
     """Rays from random origins aimed at random points of random triangles (mostly hits)."""
     rng = np.random.default_rng(seed)

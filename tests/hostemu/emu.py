@@ -38,6 +38,7 @@ def intersect(verts, rays, any_hit=False, max_leaf=4, bins=16):
                              nt.ctypes.data)
     if st < 0:
         raise RuntimeError(f"emu_intersect failed ({st})")
+    intersect.last_stack_need = st
     return prim, t, b, c, nv, nt
 
 

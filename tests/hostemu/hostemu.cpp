@@ -16,22 +16,25 @@ extern "C" int emu_intersect(const float* verts, uint32_t ntri, const float* ray
     p.max_leaf_tris = max_leaf ? max_leaf : 4;
     p.sah_bins = bins ? bins : 16;
     BvhBuildResult bvh = build_wide_bvh(verts, ntri, p);
-    if (bvh.stack_need > 64) return -1;
+    const uint32_t spill_cap = bvh.stack_need > (uint32_t)kTraceRing ? bvh.stack_need - (uint32_t)kTraceRing : 0u;
+    std::vector<int32_t> spill(spill_cap + 1);
     std::vector<TriRec> tr(ntri);
     for (uint32_t k = 0; k < ntri; ++k) {
         const uint32_t t = bvh.tri_order[k];
         std::memcpy(tr[k].a, verts + 9 * (size_t)t, 9 * sizeof(float));
         tr[k].prim = (int32_t)t; tr[k].mat = 0; tr[k].emitter = -1;
     }
-    int32_t stack[64];
+    int32_t stack[kTraceRing];
     uint32_t overflow = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const float* r = rays + 8 * (size_t)i;
         uint32_t nv = 0, nt = 0;
-        TraceResult res = any ? traverse<64, true, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]),
-                                                        r[3], r[7], stack, 1, &nv, &nt, &overflow)
-                              : traverse<64, false, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]), mk(r[4], r[5], r[6]),
-                                                         r[3], r[7], stack, 1, &nv, &nt, &overflow);
+        TraceResult res = any ? traverse<kTraceRing, true, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]),
+                                                                mk(r[4], r[5], r[6]), r[3], r[7], stack, 1, spill.data(),
+                                                                spill_cap, &nv, &nt, &overflow)
+                              : traverse<kTraceRing, false, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]),
+                                                                 mk(r[4], r[5], r[6]), r[3], r[7], stack, 1, spill.data(),
+                                                                 spill_cap, &nv, &nt, &overflow);
         out_prim[i] = res.tri >= 0 ? tr[res.tri].prim : -1;
         out_t[i] = res.t; out_b[i] = res.b; out_c[i] = res.c;
         out_nvis[i] = nv; out_ntri[i] = nt;

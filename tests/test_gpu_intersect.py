@@ -9,7 +9,7 @@ import pytest
 
 import phg
 from oracle import oracle as orc
-from helpers import FLT_MAX, random_rays, random_soup, rays_towards, scene_from_tris
+from helpers import FLT_MAX, nested_soup, random_rays, random_soup, rays_towards, scene_from_tris
 
 pytestmark = pytest.mark.gpu
 
@@ -104,6 +104,17 @@ def test_shared_edge_is_watertight(ctx):
     assert np.all(hits["prim"] >= 0), "a ray through the shared edge missed both triangles"
     tri, t, bary, flags = orc.intersect_brute(v, rays)
     assert np.array_equal(hits["prim"], tri)
+
+
+def test_deep_bvh_spills_the_stack_ring(ctx):
+    # traversal stack deeper than the on-chip ring: entries spill to the overflow area
+    v = nested_soup(1000, 21, ratio=0.98)
+    rays = np.concatenate([rays_towards(v, 20_000, 22), random_rays(v, 10_000, 23)])
+    rays[:, 0:3] *= 0.05
+    compare_closest(ctx, v, rays, "deep_bvh")
+    assert ctx.stats()["bvh_stack_need"] > 32
+    tri = orc.intersect_brute(v, rays)[0]
+    assert np.array_equal(ctx.intersect(rays, any_hit=True)["prim"] >= 0, tri >= 0)
 
 
 def test_procedural_against_reference_bvh(ctx):

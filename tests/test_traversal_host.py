@@ -6,7 +6,7 @@ import pytest
 
 import phg
 from oracle import oracle as orc
-from helpers import FLT_MAX, random_rays, random_soup, rays_towards
+from helpers import FLT_MAX, nested_soup, random_rays, random_soup, rays_towards
 from hostemu import emu
 
 
@@ -64,3 +64,13 @@ def test_procedural_scene_sample():
     sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=3, target_tris=60_000, num_instances=8, num_lights=4, subdiv=3)
     nv, nt = check(sc.verts, np.concatenate([rays_towards(sc.verts, 3000, 21), random_rays(sc.verts, 3000, 22)]))
     assert nv.mean() < 80
+
+
+def test_deep_bvh_spills_the_stack_ring():
+    # nested triangles make a BVH whose traversal stack outgrows the 16-entry on-chip ring:
+    # the oldest entries spill to the per-lane overflow area and must come back intact
+    v = nested_soup(1000, 21, ratio=0.98)
+    rays = np.concatenate([rays_towards(v, 20_000, 22), random_rays(v, 10_000, 23)])
+    rays[:, 0:3] *= 0.05  # origins among the small triangles
+    check(v, rays)
+    assert emu.intersect.last_stack_need > 32
