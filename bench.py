@@ -38,7 +38,7 @@ def measured_traffic(kernel_prefix):
     except (OSError, ValueError):
         return None
     for name, v in prof.get("kernels", {}).items():
-        if name.startswith(kernel_prefix):
+        if name.removeprefix("void ").startswith(kernel_prefix):
             return round(v["traffic_bytes"])
     return None
 
