@@ -12,6 +12,10 @@ Reported beside the throughput:
                 launch durations, against the 8 TB/s HBM peak.
   cpu_baseline  the CPU oracle (scalar C++ restatement of the reference path) on this
                 host's cores, over a bounded sample of the same image (every k-th row).
+  convergence   time to relMSE 0.01 on the occluded-light room (~199k triangles, light behind
+                a wall with a 0.1-wide slit, GGX conductors), guided vs plain BNEEPT: both
+                render progressively against a plain reference of the same scene; the guided
+                run trains its network online (included in its time).
 """
 import argparse
 import json
@@ -57,6 +61,46 @@ def cpu_model():
     return platform.processor()
 
 
+def convergence_leg(args):
+    """Time to relMSE 0.01, plain vs guided, on the occluded-light room."""
+    import numpy as np
+    import phg
+    from phg import convergence as cv
+
+    res = args.conv_res
+    scene = phg.Scene.generate(phg.GEN_OCCLUDED_ROOM)
+    with phg.Context(0) as ctx:
+        ctx.upload(scene)
+        ctx.film_clear(res, res)
+        t0 = time.perf_counter()
+        chunk = 1024
+        for s0 in range(0, args.conv_ref_spp, chunk):
+            ctx.render(min(chunk, args.conv_ref_spp - s0), seed=7, spp_begin=s0)
+        ref_s = time.perf_counter() - t0
+        R, vR = cv.mean_and_var_of_mean(ctx.film())
+        print(f"[bench] convergence reference: {args.conv_ref_spp} spp at {res}x{res} in {ref_s:.1f}s "
+              f"(its own relMSE {float(np.mean(vR / (R * R + cv.EPS))):.2e})", file=sys.stderr, flush=True)
+        plain = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, target=0.005)
+    with phg.Context(0) as ctx:
+        ctx.upload(scene)
+        guided = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, guided=True,
+                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=16, target=0.005)
+        gst = ctx.stats()
+    tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
+    for name, cur in (("plain", plain), ("guided", guided)):
+        print(f"[bench] {name}: " + " ".join(f"{s}spp/{t:.2f}s/{e:.2e}" for s, t, e in cur), file=sys.stderr,
+              flush=True)
+    return {"scene": f"occluded-light room, {scene.num_tris} triangles, GGX conductors, light behind a slit",
+            "width": res, "height": res, "reference_spp": args.conv_ref_spp, "max_bounces": 10000,
+            "relmse_target": 0.01, "plain_s": None if tp is None else round(tp, 4),
+            "guided_s": None if tg is None else round(tg, 4),
+            "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
+            "guided": {"guide_prob": 0.5, "train_spp": args.conv_train_spp, "steps_per_pass": 16, "lr": 1e-3,
+                       "batch": 1 << 18},
+            "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
+            "guided_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in guided]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +112,11 @@ def main():
     ap.add_argument("--bounces", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-conv", action="store_true", help="skip the guided-vs-plain convergence leg")
+    ap.add_argument("--conv-res", type=int, default=256)
+    ap.add_argument("--conv-ref-spp", type=int, default=16384)
+    ap.add_argument("--conv-max-spp", type=int, default=8192)
+    ap.add_argument("--conv-train-spp", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)
@@ -141,6 +190,8 @@ def main():
                "sample": f"every {step}th row of the same {W}x{H} image at {spp} spp ({cst['paths']} paths, "
                          f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}"}
 
+    conv = None if args.no_conv else convergence_leg(args)
+
     out = {
         "metric": METRIC,
         "value": round(msps, 3),
@@ -169,6 +220,7 @@ def main():
                      "bytes_per_launch": round(bytes_closest / max(trace_launch, 1)),
                      "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
         "cpu_baseline": cpu,
+        "convergence": conv,
         "image_mean": [round(float(x), 6) for x in img.mean(axis=(0, 1))],
     }
     print(json.dumps(out))

@@ -221,15 +221,19 @@ phg_status phg_get_stats(void* ctx, phg_stats* out);
  * direction) producing 8 von Mises-Fisher lobes over the incident direction; BF16 on the
  * matrix cores with FP32 accumulation.  Parameters: one FP32 blob of PHG_GUIDE_PARAMS values,
  * W0 b0 W1 b1 W2 b2 W3 b3 W4 b4 with each W stored [in][out].
- * Training record: 13 floats = 9 features in [0,1] (position in the scene box, normal and
- * outgoing direction mapped from [-1,1]), incident direction xyz, weight (radiance / pdf). */
-enum { PHG_GUIDE_PARAMS = 19240, PHG_GUIDE_OUTPUTS = 40, PHG_GUIDE_RECORD = 13 };
+ * Training record: 16 floats = 9 features in [0,1] (position in the scene box, normal and
+ * outgoing direction mapped from [-1,1]), incident direction xyz, weight (target / sampling
+ * density), BSDF density b of the direction, selection probability a, and one unused float.
+ * The loss is -weight * log(a q + (1 - a) b) with q the lobes' density: the lobes are fitted
+ * as one component of the one-sample MIS mixture that samples the direction; with b = 0 (or
+ * a = 0) it is the plain weighted negative log-likelihood of q. */
+enum { PHG_GUIDE_PARAMS = 19240, PHG_GUIDE_OUTPUTS = 40, PHG_GUIDE_RECORD = 16 };
 
 phg_status phg_guide_set_params(void* ctx, const float* params, uint32_t n);
 phg_status phg_guide_get_params(void* ctx, float* params, uint32_t n);
 /* raw network outputs (40 per query) for 9 features per query */
 phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out);
-/* gradient of the mean radiance-weighted negative log-likelihood over the records */
+/* gradient of the mean of the records' losses (above) */
 phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, float* grads);
 /* `steps` Adam steps on the same records (learning rate lr) */
 phg_status phg_guide_train(void* ctx, const float* records, uint64_t n, float lr, uint32_t steps);

@@ -2,8 +2,10 @@
 
 Restates, in numpy FP64, the guiding model the GPU trains and queries: frequency encoding of
 (position, normal, outgoing direction), a 64-64-64-64-64-40 ReLU MLP, and a mixture of 8
-von Mises-Fisher lobes over the incident direction, with the radiance-weighted negative
-log-likelihood used for online training and its analytic gradients.  The reference renderer
+von Mises-Fisher lobes over the incident direction, with the weighted negative
+log-likelihood used for online training, -w log(a q + (1 - a) b) (q = the lobes' density, b =
+the BSDF density, a = the lobes' selection probability in the one-sample MIS mixture), and
+its analytic gradients.  The reference renderer
 has no path guiding; this is the specification the GPU kernels are checked against.  The
 analytic gradients are themselves pinned by central finite differences in the tests.
 
@@ -110,10 +112,32 @@ def mixture_pdf(o, omega):
     return (w * p).sum(axis=1)
 
 
-def loss(params, e, omega, weight, bf16=False):
+def _sel_share(q, bsdf_pdf, sel):
+    """a q / (a q + (1 - a) b): the lobes' share of the one-sample MIS density (1 where b or a is 0)."""
+    if bsdf_pdf is None or sel is None:
+        return np.ones_like(q)
+    b = np.broadcast_to(np.asarray(bsdf_pdf, np.float64), q.shape)
+    a = np.broadcast_to(np.asarray(sel, np.float64), q.shape)
+    use = (a > 0) & (b > 0)
+    den = np.where(use, a * q + (1 - a) * b, 1.0)
+    return np.where(use, a * q / den, 1.0)
+
+
+def record_density(q, bsdf_pdf, sel):
+    """Density whose log the record loss takes: a q + (1 - a) b, or q where b or a is 0."""
+    if bsdf_pdf is None or sel is None:
+        return q
+    b = np.broadcast_to(np.asarray(bsdf_pdf, np.float64), q.shape)
+    a = np.broadcast_to(np.asarray(sel, np.float64), q.shape)
+    use = (a > 0) & (b > 0)
+    return np.where(use, a * q + (1 - a) * b, q)
+
+
+def loss(params, e, omega, weight, bf16=False, bsdf_pdf=None, sel=None):
+    """Mean of -weight * log(a q + (1 - a) b) (a q's constant factor dropped where b = 0)."""
     _, o = forward(params, e, bf16)
     q = mixture_pdf(o, omega)
-    return -(np.asarray(weight, np.float64) * np.log(q)).mean()
+    return -(np.asarray(weight, np.float64) * np.log(record_density(q, bsdf_pdf, sel))).mean()
 
 
 def dlogp_dkappa(k):
@@ -127,8 +151,9 @@ def dlogp_dkappa(k):
     return np.where(small, series, exact)
 
 
-def output_grad(o, omega, weight):
-    """d(-weight * log q) / d(raw outputs) per sample (not yet divided by the batch size)."""
+def output_grad(o, omega, weight, bsdf_pdf=None, sel=None):
+    """d(-weight * log(a q + (1 - a) b)) / d(raw outputs) per sample (not yet divided by the
+    batch size); without b and a (or where either is 0) it is d(-weight * log q)."""
     o = np.asarray(o, np.float64).reshape(-1, N_LOBES, 5)
     n = o.shape[0]
     r = o[:, :, 0:3]
@@ -138,7 +163,8 @@ def output_grad(o, omega, weight):
     c = (mu * om).sum(axis=2)
     p = vmf_norm(kappa) * np.exp(kappa * (c - 1.0))
     q = (w * p).sum(axis=1, keepdims=True)
-    scale = -np.asarray(weight, np.float64).reshape(-1, 1) / q          # d loss / d q
+    share = _sel_share(q[:, 0], bsdf_pdf, sel).reshape(-1, 1)
+    scale = -np.asarray(weight, np.float64).reshape(-1, 1) * share / q  # d loss / d q
     g = np.zeros_like(o)
     wp = w * p
     # mean direction through the normalisation
@@ -152,11 +178,11 @@ def output_grad(o, omega, weight):
     return g.reshape(n, N_OUT)
 
 
-def backward(params, e, omega, weight, bf16=False):
+def backward(params, e, omega, weight, bf16=False, bsdf_pdf=None, sel=None):
     """Gradients of the mean loss w.r.t. every (W, b), in the parameter layout."""
     acts, o = forward(params, e, bf16)
     n = acts[0].shape[0]
-    g = output_grad(o, omega, weight) / n
+    g = output_grad(o, omega, weight, bsdf_pdf, sel) / n
     grads = [None] * N_LAYERS
     for li in range(N_LAYERS - 1, -1, -1):
         W, b = params[li]

@@ -523,7 +523,8 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     if (train) {
         rec_bounces = std::min<uint32_t>(kGuideRecBounces, o->max_bounces + 1);
         PHG_HIP(c, c->g_rec.ensure((size_t)rec_bounces * cap * kRecFloat4));
-        train_cap = 4 * batch;
+        // fresh records for every step of a pass (no record is reused within the pass)
+        train_cap = std::max<uint32_t>(4u, o->guide_steps ? o->guide_steps : 4u) * batch;
         PHG_HIP(c, c->g_train.ensure((size_t)train_cap * kGuideRecordFloats));
     }
     pb.lobes = c->g_lobes.p;
@@ -590,7 +591,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
             PHG_HIP(c, hipMemsetAsync(c->g_counters.p, 0, 4 * sizeof(uint32_t), c->stream));
             PHG_HIP(c, guide_collect(pb, pp, c->g_scene, c->g_train.p, train_cap, c->g_counters.p, c->g_keep,
-                                     ++c->g_pass_salt, c->stream));
+                                     0.0f, ++c->g_pass_salt, c->stream));
             uint32_t cnt[2] = {0, 0};
             PHG_HIP(c, hipMemcpyAsync(cnt, c->g_counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));

@@ -26,7 +26,7 @@ struct GuideNetDev {
 };
 
 constexpr uint32_t kGuideParamCount = 19240;
-constexpr uint32_t kGuideRecordFloats = 13;
+constexpr uint32_t kGuideRecordFloats = 16;
 constexpr uint32_t kGuideOutputs = 40;
 
 hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st);
@@ -40,6 +40,7 @@ hipError_t guide_adam(const GuideNetDev& net, float* grads, float* m, float* v, 
 hipError_t guide_fill(float* p, uint32_t n, float v, hipStream_t st);
 // training records of a finished pass: incident radiance along each recorded sampled direction
 hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
-                         uint32_t capacity, uint32_t* counters, float keep_prob, uint32_t salt, hipStream_t st);
+                         uint32_t capacity, uint32_t* counters, float keep_prob, float sel, uint32_t salt,
+                         hipStream_t st);
 
 }  // namespace phg

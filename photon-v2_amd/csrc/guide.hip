@@ -140,8 +140,8 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNetDev net, InferSourc
     }
 }
 
-// Training: gradients of the mean radiance-weighted NLL over a batch of records
-// (9 features, direction xyz, weight).  One wave per block, 32 records per iteration;
+// Training: gradients of the mean record loss -w log(a q + (1 - a) b) over a batch of records
+// (9 features, direction xyz, weight w, BSDF density b, selection probability a, pad).  One wave per block, 32 records per iteration;
 // per-block gradient accumulator in LDS, added to the global gradient at the end.
 __global__ __launch_bounds__(64) void k_guide_train(GuideNetDev net, const float* __restrict__ rec, uint32_t nrec,
                                                     float inv_batch, float* __restrict__ grads) {
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNetDev net, const float
             const int row = lane & 31, half = lane >> 5;
             const uint32_t s = base + row;
             float x[9];
-            for (int k = 0; k < 9; ++k) x[k] = s < nrec ? rec[13 * (size_t)s + k] : 0.0f;
+            for (int k = 0; k < 9; ++k) x[k] = s < nrec ? rec[kGuideRecordFloats * (size_t)s + k] : 0.0f;
             for (int j = 32 * half; j < 32 * half + 32; ++j) s_act[0][row * 64 + j] = (__bf16)guide_encode(x, j);
         }
         wave_sync();
@@ -175,8 +175,8 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNetDev net, const float
             const uint32_t s = base + lane;
             float gr[kGuideOut];
             if (s < nrec) {
-                const float* R = rec + 13 * (size_t)s;
-                guide_output_grad(s_out + lane * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], gr);
+                const float* R = rec + kGuideRecordFloats * (size_t)s;
+                guide_output_grad(s_out + lane * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], R[13], R[14], gr);
             } else {
                 for (int k = 0; k < kGuideOut; ++k) gr[k] = 0.0f;
             }
@@ -255,7 +255,7 @@ __global__ void k_guide_adam(float* __restrict__ params, float* __restrict__ gra
                              float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kGuideParams) return;
-    const float gr = grads[i];
+    const float gr = __builtin_isfinite(grads[i]) ? grads[i] : 0.0f;  // a non-finite gradient skips the update
     const float mi = b1 * m[i] + (1.0f - b1) * gr;
     const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
     m[i] = mi; v[i] = vi;
@@ -268,13 +268,15 @@ __global__ void k_fill_float(float* p, uint32_t n, float v) {
     if (i < n) p[i] = v;
 }
 
-// Training records of a finished pass.  For each recorded vertex, the radiance that arrived
-// along its sampled direction is (final path radiance - radiance before the direction was
-// followed) / (throughput x sample weight), channel by channel; its luminance over the
-// sampling density is the record's weight.  Records are kept with probability keep_prob.
+// Training records of a finished pass.  For each recorded vertex, (final path radiance -
+// radiance before the direction was followed) / (throughput before the vertex), channel by
+// channel, is f cos L_i / pdf of the sampled direction: a one-sample estimate of the target
+// density (BSDF x cosine x incident radiance, which depends on the outgoing direction the
+// network sees) over the sampling density.  Its luminance is the record's weight in the
+// KL-divergence gradient.  Records are kept with probability keep_prob.
 __global__ __launch_bounds__(256) void k_guide_collect(PathBuffers pb, PassParams pp, GuideScene gs, float* __restrict__ out,
                                                        uint32_t capacity, uint32_t* __restrict__ counters, float keep_prob,
-                                                       uint32_t salt) {
+                                                       float sel, uint32_t salt) {
     const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
     if (pid >= pp.paths) return;
     const float4 L = pb.L[pid];
@@ -283,12 +285,13 @@ __global__ __launch_bounds__(256) void k_guide_collect(PathBuffers pb, PassParam
         const float4* rc = pb.rec + ((size_t)b * pp.paths + pid) * kRecFloat4;
         const float4 r0 = rc[0];
         if (!(r0.w > 0.0f)) break;
-        const float4 wem = rc[4], lb = rc[5];
-        const float lr = wem.x > 0.0f ? (L.x - lb.x) / wem.x : 0.0f;
-        const float lg = wem.y > 0.0f ? (L.y - lb.y) / wem.y : 0.0f;
-        const float lbb = wem.z > 0.0f ? (L.z - lb.z) / wem.z : 0.0f;
-        const float lum = luminance(rgb3(fmaxf(lr, 0.0f), fmaxf(lg, 0.0f), fmaxf(lbb, 0.0f)));
-        const float w = lum / r0.w;
+        // (L - Lb) / T = f cos L_i / pdf along the sampled direction: the sample's weight for
+        // fitting the lobes to the product of BSDF, cosine and incident radiance
+        const float4 T = rc[4], lb = rc[5];
+        const float lr = T.x > 0.0f ? (L.x - lb.x) / T.x : 0.0f;
+        const float lg = T.y > 0.0f ? (L.y - lb.y) / T.y : 0.0f;
+        const float lbb = T.z > 0.0f ? (L.z - lb.z) / T.z : 0.0f;
+        const float w = luminance(rgb3(fmaxf(lr, 0.0f), fmaxf(lg, 0.0f), fmaxf(lbb, 0.0f)));
         if (!(w > 0.0f) || !__builtin_isfinite(w)) continue;
         ++valid;
         uint32_t h = pid * 0x9E3779B1u ^ (b + 1u) * 0x85EBCA77u ^ salt * 0xC2B2AE3Du;
@@ -302,14 +305,16 @@ __global__ __launch_bounds__(256) void k_guide_collect(PathBuffers pb, PassParam
         float* o = out + (size_t)slot * kGuideRecordFloats;
         for (int k = 0; k < 9; ++k) o[k] = x[k];
         o[9] = d4.x; o[10] = d4.y; o[11] = d4.z; o[12] = w;
+        o[13] = n4.w; o[14] = sel; o[15] = 0.0f;
     }
     if (valid) atomicAdd(&counters[1], valid);
 }
 
 hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
-                         uint32_t capacity, uint32_t* counters, float keep_prob, uint32_t salt, hipStream_t st) {
+                         uint32_t capacity, uint32_t* counters, float keep_prob, float sel, uint32_t salt,
+                         hipStream_t st) {
     hipLaunchKernelGGL(k_guide_collect, dim3((pp.paths + 255) / 256), dim3(256), 0, st, pb, pp, gs, out_rec, capacity,
-                       counters, keep_prob, salt);
+                       counters, keep_prob, sel, salt);
     return hipGetLastError();
 }
 

@@ -64,6 +64,14 @@ PHG_HD void guide_decode(const float* o, float* lobes) {
     }
 }
 
+// false if any decoded lobe value is not finite (a diverged network): the caller then
+// samples the BSDF alone at this vertex, which keeps the estimator unbiased
+PHG_HD bool guide_lobes_ok(const float* lobes) {
+    float acc = 0.0f;
+    for (int k = 0; k < kGuideOut; ++k) acc += lobes[k];
+    return __builtin_isfinite(acc);
+}
+
 PHG_HD float guide_pdf(const float* lobes, V3 w) {
     float q = 0.0f;
     for (int j = 0; j < kGuideLobes; ++j) {
@@ -96,8 +104,10 @@ PHG_HD V3 guide_sample(const float* lobes, float u0, float u1, float u2) {
     return normalize(bx * (s * cosf(phi)) + mu * w + bz * (s * sinf(phi)));
 }
 
-// d(-weight * log q)/d(raw outputs) for one sample (matches guiding_oracle.output_grad)
-PHG_HD void guide_output_grad(const float* o, V3 w, float weight, float* g) {
+// d(-weight * log(a q + (1 - a) b))/d(raw outputs) for one sample, q = the lobes' density at
+// w, b = the BSDF density there, a = the selection probability of the lobes; b = 0 or a = 0
+// gives d(-weight * log q) (matches guiding_oracle.output_grad)
+PHG_HD void guide_output_grad(const float* o, V3 w, float weight, float bsdf_pdf, float sel, float* g) {
     float lobes[kGuideOut];
     guide_decode(o, lobes);
     float p[kGuideLobes], c[kGuideLobes];
@@ -108,7 +118,9 @@ PHG_HD void guide_output_grad(const float* o, V3 w, float weight, float* g) {
         p[j] = vmf_norm(L[3]) * expf(L[3] * (c[j] - 1.0f));
         q += L[4] * p[j];
     }
-    const float scale = q > 0.0f ? -weight / q : 0.0f;
+    // d log(a q + (1 - a) b) = (a q / (a q + (1 - a) b)) d log q
+    const float share = (sel > 0.0f && bsdf_pdf > 0.0f) ? sel * q / (sel * q + (1.0f - sel) * bsdf_pdf) : 1.0f;
+    const float scale = q > 0.0f ? -weight * share / q : 0.0f;
     for (int j = 0; j < kGuideLobes; ++j) {
         const float* L = lobes + 5 * j;
         const float k = L[3];

@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
 #pragma unroll
                 for (int k = 0; k < (GUIDED ? 12 : 8); ++k) u[k] = s.u();
                 const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideOut : nullptr;
-                const float alpha = GUIDED ? pp.guide_prob : 0.0f;
+                const float alpha = (GUIDED && pp.guide_prob > 0.0f && guide_lobes_ok(lob)) ? pp.guide_prob : 0.0f;
                 const phg_material m = sc.mats[tr.mat];
                 if (m.type == PHG_MAT_ABSORBER) stop = true;
                 if (!stop) {
@@ -336,12 +336,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                     V3 Ls = mk(0.0f, 0.0f, 0.0f);
                     RGB ws = rgb1(0.0f);
                     float relIor = 1.0f, pdfc = 0.0f;
+                    float bsdf_pdf_ls = 0.0f;  // BSDF density of the chosen direction (training records)
                     if (GUIDED && u[8] < alpha) {
                         // direction from the learned lobes, weighted by the combined density
                         Ls = guide_sample(lob, u[9], u[10], u[11]);
                         const BsdfEval be = bsdf_eval(m, X, Ls, V);
                         float bp = bsdf_pdf(m, X, Ls, V);
                         if (!pdf_ok(bp)) bp = 0.0f;
+                        bsdf_pdf_ls = bp;
                         pdfc = alpha * guide_pdf(lob, Ls) + (1.0f - alpha) * bp;
                         if (be.ok && pdf_ok(pdfc)) {
                             const float cosl = absdot(X.Ns, Ls);
@@ -358,6 +360,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                         if (bs.ok) {
                             Ls = bs.L; ws = bs.w; relIor = bs.relIor;
                             const float bp = bsdf_pdf(m, X, bs.L, V);
+                            bsdf_pdf_ls = pdf_ok(bp) ? bp : 0.0f;
                             if (GUIDED && alpha > 0.0f) {
                                 pdfc = alpha * guide_pdf(lob, Ls) + (1.0f - alpha) * bp;
                                 ok = pdf_ok(bp) && pdf_ok(pdfc);
@@ -391,10 +394,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                         if (GUIDED && pp.record && bounce < pp.rec_bounces) {
                             float4* rc = pb.rec + ((size_t)bounce * pp.paths + pid) * kRecFloat4;
                             rc[0] = make_float4(X.P.x, X.P.y, X.P.z, pdfc);
-                            rc[1] = make_float4(X.Ns.x, X.Ns.y, X.Ns.z, 0.0f);
+                            rc[1] = make_float4(X.Ns.x, X.Ns.y, X.Ns.z, bsdf_pdf_ls);
                             rc[2] = make_float4(V.x, V.y, V.z, 0.0f);
                             rc[3] = make_float4(ud.x, ud.y, ud.z, 0.0f);
-                            rc[4] = make_float4(wem.r, wem.g, wem.b, 0.0f);
+                            rc[4] = make_float4(T.r, T.g, T.b, 0.0f);  // throughput before this vertex
                             rc[5] = make_float4(Lacc.x, Lacc.y, Lacc.z, 0.0f);
                         }
                     }

@@ -29,7 +29,7 @@ GEN_CORNELL, GEN_PROCEDURAL, GEN_OCCLUDED_ROOM = 1, 2, 3
 QUERY_CLOSEST, QUERY_ANY = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
 RENDER_COUNT_TRAVERSAL, RENDER_GUIDED, RENDER_TRAIN = 1, 2, 4
-GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 13
+GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 
 

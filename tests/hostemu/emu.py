@@ -19,7 +19,7 @@ def lib():
         vp = C.c_void_p
         _lib.emu_intersect.argtypes = [vp, C.c_uint32, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32,
                                        vp, vp, vp, vp, vp, vp]
-        _lib.emu_guide_output_grad.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        _lib.emu_guide_output_grad.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp]
         _lib.emu_guide_pdf.argtypes = [vp, C.c_uint32, vp, vp]
         _lib.emu_guide_sample.argtypes = [vp, C.c_uint32, vp, vp]
         _lib.emu_guide_encode.argtypes = [vp, C.c_uint32, vp]
@@ -46,11 +46,14 @@ def _f(a, shape):
     return np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))
 
 
-def guide_output_grad(o, w3, weight):
+def guide_output_grad(o, w3, weight, bsdf_pdf=None, sel=None):
     o = _f(o, (-1, 40)); n = o.shape[0]
     w3 = _f(w3, (n, 3)); weight = _f(weight, (n,))
+    b = _f(np.zeros(n) if bsdf_pdf is None else bsdf_pdf, (n,))
+    a = _f(np.zeros(n) if sel is None else sel, (n,))
     out = np.zeros((n, 40), np.float32)
-    lib().emu_guide_output_grad(o.ctypes.data, n, w3.ctypes.data, weight.ctypes.data, out.ctypes.data)
+    lib().emu_guide_output_grad(o.ctypes.data, n, w3.ctypes.data, weight.ctypes.data, b.ctypes.data, a.ctypes.data,
+                                out.ctypes.data)
     return out
 
 

@@ -45,9 +45,11 @@ extern "C" int emu_intersect(const float* verts, uint32_t ntri, const float* ray
 // ---- guiding-model device math, compiled for the host
 #include "../../photon-v2_amd/csrc/guide_device.h"
 
-extern "C" void emu_guide_output_grad(const float* o, uint32_t n, const float* w3, const float* weight, float* out) {
+extern "C" void emu_guide_output_grad(const float* o, uint32_t n, const float* w3, const float* weight,
+                                      const float* bsdf_pdf, const float* sel, float* out) {
     for (uint32_t i = 0; i < n; ++i)
-        guide_output_grad(o + 40 * (size_t)i, mk(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]), weight[i], out + 40 * (size_t)i);
+        guide_output_grad(o + 40 * (size_t)i, mk(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]), weight[i], bsdf_pdf[i], sel[i],
+                          out + 40 * (size_t)i);
 }
 
 extern "C" void emu_guide_pdf(const float* o, uint32_t n, const float* w3, float* out) {

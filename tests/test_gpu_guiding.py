@@ -37,13 +37,21 @@ def ctx():
     c.close()
 
 
-def random_records(n, seed):
+def random_records(n, seed, mixture=True):
+    """Records of 16 floats: features, direction, weight, BSDF density, selection probability, pad."""
     rng = np.random.default_rng(seed)
     x = rng.random((n, 9))
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     w = rng.exponential(1.0, n)
-    return np.concatenate([x, d, w[:, None]], axis=1).astype(np.float32)
+    b = rng.exponential(0.5, n) * (rng.random(n) < 0.8) if mixture else np.zeros(n)
+    a = rng.choice([0.3, 0.5, 0.7], n) if mixture else np.zeros(n)
+    return np.concatenate([x, d, w[:, None], b[:, None], a[:, None], np.zeros((n, 1))], axis=1).astype(np.float32)
+
+
+def oracle_grads(params, rec):
+    e = go.encode(rec[:, :9].astype(np.float64))
+    return go.backward(params, e, rec[:, 9:12], rec[:, 12], bf16=True, bsdf_pdf=rec[:, 13], sel=rec[:, 14])
 
 
 def test_forward_matches_oracle(ctx):
@@ -63,8 +71,7 @@ def test_gradients_match_oracle(ctx):
     ctx.guide_set_params(flat(params))
     rec = random_records(4096, 6)
     gpu = ctx.guide_gradients(rec).astype(np.float64)
-    e = go.encode(rec[:, :9].astype(np.float64))
-    grads = go.backward(unflat(flat(params)), e, rec[:, 9:12], rec[:, 12], bf16=True)
+    grads = oracle_grads(unflat(flat(params)), rec)
     ref = np.concatenate([np.concatenate([gW.ravel(), gb.ravel()]) for gW, gb in grads])
     rel = np.linalg.norm(gpu - ref) / np.linalg.norm(ref)
     cos = gpu @ ref / (np.linalg.norm(gpu) * np.linalg.norm(ref))
@@ -88,9 +95,8 @@ def test_adam_steps_follow_oracle(ctx):
     p = unflat(flat(params))
     m = [[np.zeros_like(W), np.zeros_like(b)] for W, b in p]
     v = [[np.zeros_like(W), np.zeros_like(b)] for W, b in p]
-    e = go.encode(rec[:, :9].astype(np.float64))
     for t in range(1, 4):
-        g = go.backward(p, e, rec[:, 9:12], rec[:, 12], bf16=True)
+        g = oracle_grads(p, rec)
         p = go.adam_step(p, g, m, v, t)
     ref = flat(p).astype(np.float64)
     moved_ref = ref - flat(params)
@@ -120,7 +126,7 @@ def test_training_fits_a_known_distribution(ctx):
         t1 /= np.linalg.norm(t1, axis=1, keepdims=True)
         t2 = np.cross(mu, t1)
         d = mu * w[:, None] + t1 * (s * np.cos(phi))[:, None] + t2 * (s * np.sin(phi))[:, None]
-        return np.concatenate([x, d, np.ones((n, 1))], 1).astype(np.float32)
+        return np.concatenate([x, d, np.ones((n, 1)), np.zeros((n, 3))], 1).astype(np.float32)
 
     train = draw(n)
     test = draw(4096)
@@ -131,6 +137,55 @@ def test_training_fits_a_known_distribution(ctx):
     ll = np.log(go.mixture_pdf(o.astype(np.float64), test[:, 9:12].astype(np.float64))).mean()
     print(f"[guide] held-out mean log-likelihood={ll:.3f} (uniform={np.log(1 / (4 * np.pi)):.3f})", flush=True)
     assert ll > np.log(1 / (4 * np.pi)) + 1.5
+
+
+def slit_records(n, rng):
+    """Floor points of a room (y = 0, x < 2.9) lit only through a slit at x = 3, z in
+    [1.95, 2.05]: cosine-sampled directions, record weight 5 through the slit and 0.01
+    elsewhere (the ratio f cos L_i / pdf for a white Lambertian floor)."""
+    P = np.stack([rng.uniform(0.1, 2.9, n), np.zeros(n), rng.uniform(0.1, 3.9, n)], 1)
+    N = np.tile([0.0, 1.0, 0.0], (n, 1))
+    V = rng.normal(size=(n, 3))
+    V[:, 1] = np.abs(V[:, 1])
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    u1, u2 = rng.random(n), rng.random(n)
+    r, ph = np.sqrt(u1), 2 * np.pi * u2
+    D = np.stack([r * np.cos(ph), np.sqrt(1 - u1), r * np.sin(ph)], 1)
+    ok = D[:, 0] > 1e-6
+    t = np.where(ok, (3.0 - P[:, 0]) / np.where(ok, D[:, 0], 1.0), np.inf)
+    through = ok & (np.abs(P[:, 2] + t * D[:, 2] - 2.0) < 0.05) & (t * D[:, 1] < 3.0)
+    w = np.where(through, 5.0, 0.01)
+    x = go.features(P, N, V, [0, 0, 0], [4, 3, 4])
+    rec = np.concatenate([x, D, w[:, None], np.zeros((n, 3))], 1).astype(np.float32)
+    return rec, through
+
+
+def test_training_gain_on_a_slit_lit_floor(ctx):
+    """What the trained network gains over BSDF (cosine) sampling alone: after 512 Adam steps
+    (lr 1e-3, 2^16-record batches) on slit-lit floor records, the lobes put about 20x the
+    cosine density's mass through the slit and the one-sample MIS mixture (a = 0.5) estimates
+    the floor's irradiance integral with less variance (1.5x measured; the lobes do not yet
+    cover the whole slit, and directions they miss keep only half the cosine density).
+    Second moments under the mixture are computed by importance reweighting cosine samples:
+    E_mix[(g / p)^2] = E_cos[w^2 c / p] with g = w c."""
+    rng = np.random.default_rng(11)
+    ctx.guide_set_params(flat(go.init_params(12)))
+    test, through = slit_records(200_000, rng)
+    for _ in range(8):
+        ctx.guide_train(slit_records(1 << 16, rng)[0], lr=1e-3, steps=64)
+    o = ctx.guide_forward(test[:, :9]).astype(np.float64)
+    q = go.mixture_pdf(o, test[:, 9:12].astype(np.float64))
+    c = test[:, 10].astype(np.float64) / np.pi
+    w = test[:, 12].astype(np.float64)
+    mean = w.mean()
+    var_bsdf = (w * w).mean() - mean ** 2
+    a = 0.5
+    var_mix = (w * w * c / (a * q + (1 - a) * c)).mean() - mean ** 2
+    mass_q = np.mean(np.where(through, q / c, 0.0))
+    print(f"[guide] slit mass cosine={through.mean():.4f} trained={mass_q:.4f} "
+          f"variance cosine={var_bsdf:.4f} mixture={var_mix:.4f} gain={var_bsdf / var_mix:.2f}x", flush=True)
+    assert mass_q > 5 * through.mean()
+    assert var_bsdf / var_mix > 1.25
 
 
 @pytest.mark.parametrize("albedo,radiance", [(0.5, 0.5)])

@@ -26,6 +26,14 @@ def test_mixture_pdf_and_gradient_match_oracle():
     scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-12
     assert np.abs(g - g_ref).max() / 1.0 < 1e-3 * scale.max()
     assert np.all(np.abs(g - g_ref) <= 2e-3 * scale + 1e-6)
+    # as a component of the one-sample MIS mixture with the BSDF density (some b = 0)
+    b = rng.exponential(0.5, 2000) * (rng.random(2000) < 0.8)
+    a = rng.choice([0.0, 0.3, 0.5, 1.0], 2000)
+    b32, a32 = b.astype(np.float32).astype(np.float64), a.astype(np.float32).astype(np.float64)
+    g_ref = go.output_grad(o32, w32, weight.astype(np.float32), b32, a32)
+    g = emu.guide_output_grad(o, w, weight, b, a)
+    scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-12
+    assert np.all(np.abs(g - g_ref) <= 2e-3 * scale + 1e-6)
 
 
 def test_lobe_sampling_follows_mixture_pdf():

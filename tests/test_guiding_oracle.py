@@ -54,6 +54,32 @@ def test_output_gradient_small_and_large_kappa():
         assert np.allclose(g[:, k], fd, rtol=1e-4, atol=1e-6), k
 
 
+def test_mixture_loss_gradient_matches_finite_differences():
+    # -w log(a q + (1 - a) b): the lobes as one component of the sampling mixture
+    rng = np.random.default_rng(5)
+    o = rng.normal(size=(8, 40))
+    o[:, 3::5] = rng.choice([-7.5, -3.0, 0.0, 2.0, 5.5], size=(8, 8))
+    om = rng.normal(size=(8, 3))
+    om /= np.linalg.norm(om, axis=1, keepdims=True)
+    w = rng.random(8) + 0.5
+    b = np.array([0.0, 0.1, 0.5, 2.0, 10.0, 0.3, 0.0, 1.0])
+    a = np.array([0.5, 0.5, 0.3, 0.7, 0.5, 1.0, 0.0, 0.9])
+    g = go.output_grad(o, om, w, b, a)
+
+    def f(oo):
+        return -w * np.log(go.record_density(go.mixture_pdf(oo, om), b, a))
+    for k in range(40):
+        d = np.zeros_like(o)
+        d[:, k] = 1e-6
+        fd = (f(o + d) - f(o - d)) / 2e-6
+        assert np.allclose(g[:, k], fd, rtol=1e-4, atol=1e-6), k
+    # b = 0 or a = 0 leaves the plain weighted log-likelihood gradient
+    plain = go.output_grad(o, om, w)
+    assert np.allclose(g[[0, 6]], plain[[0, 6]])
+    # a = 1 also reduces to it (the BSDF component has no weight)
+    assert np.allclose(g[5], plain[5])
+
+
 def test_mixture_integrates_to_one():
     rng = np.random.default_rng(4)
     o = rng.normal(size=(1, 40))
