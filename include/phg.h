@@ -169,7 +169,7 @@ typedef struct phg_stats {
     uint64_t guide_queries;       /* network evaluations (last render) */
     uint64_t guide_records;       /* training records used (last render) */
     uint32_t guide_steps;         /* Adam steps taken so far on this context */
-    uint32_t pad2;
+    uint32_t batch_passes;        /* wavefront passes of the last phg_render_batch */
 } phg_stats;
 
 /* Context lifetime.  Replaces nothing in the reference: the CPU engine needs no device. */
@@ -211,6 +211,46 @@ phg_status phg_film_read(void* ctx, double* out, uint64_t out_len);
  * -> BNEEPTEstimator::estimate (Core/Estimator/BNEEPTEstimator.cpp:38-262)
  * -> HdrRgbFilm::addSample (HdrRgbFilm.cpp:62-122). */
 phg_status phg_render(void* ctx, const phg_render_opts* opts);
+
+/* ---- Batched rendering (no counterpart in the reference, which renders one scene per
+ * engine): several scenes uploaded together, then many render jobs over them whose paths
+ * share every wavefront pass, so small jobs fill the device alongside large ones.
+ *
+ * phg_upload_scenes: like phg_upload_scene for n scenes at once (BVHs built in parallel);
+ * scene i is addressed by index i in the jobs.  phg_render (and phg_intersect) use scene 0. */
+phg_status phg_upload_scenes(void* ctx, const phg_scene_desc* scenes, uint32_t n, const phg_bvh_opts* bvh);
+
+typedef struct phg_batch_job {
+    uint32_t scene;            /* index into the scenes of phg_upload_scenes */
+    uint32_t width, height;
+    uint32_t spp;
+    uint32_t max_bounces;
+    uint32_t seed;             /* < 2^24; sample streams as in phg_render */
+    uint32_t pad[2];
+} phg_batch_job;
+
+enum { PHG_BATCH_ORDER_GIVEN = 0, PHG_BATCH_ORDER_SHORTEST_FIRST = 1 };
+enum { PHG_BATCH_COUNT_TRAVERSAL = 1 };
+
+typedef struct phg_batch_opts {
+    uint32_t paths_per_pass;   /* paths in flight per pass (0 = 4M); passes carry whole samples */
+    uint32_t order;            /* PHG_BATCH_ORDER_*: which jobs the passes take first */
+    uint32_t flags;            /* PHG_BATCH_COUNT_TRAVERSAL */
+    uint32_t pad;
+} phg_batch_opts;
+
+typedef struct phg_batch_result {
+    double done_ms;            /* from the start of the batch to the end of the job's last pass */
+    uint64_t paths;            /* camera paths of the job */
+} phg_batch_result;
+
+/* Render all jobs (each into its own FP64 film, cleared first).  Every job's film equals
+ * what phg_render of that job alone produces: the sample streams depend only on (seed,
+ * pixel, sample) and each pixel accumulates its samples in order. */
+phg_status phg_render_batch(void* ctx, const phg_batch_job* jobs, uint32_t n, const phg_batch_opts* opts,
+                            phg_batch_result* results);
+/* The film of job `job` of the last batch: 8 doubles per pixel as phg_film_read. */
+phg_status phg_batch_film_read(void* ctx, uint32_t job, double* out, uint64_t out_len);
 
 /* Statistics of the last render/intersect (Renderer::asyncQueryRenderStats,
  * Core/Renderer/Sampling/EqualSamplingRenderer.cpp:258-280, reports paths/second). */

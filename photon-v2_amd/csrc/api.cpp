@@ -3,12 +3,14 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/phg.h"
@@ -35,6 +37,15 @@ struct DevBuf {
     }
 };
 
+// Where an uploaded scene lives in the shared device arrays.
+struct SceneInfo {
+    uint32_t root = 0;            // first node of its BVH
+    uint32_t emitter_begin = 0, num_emitters = 0, cdf_begin = 0;
+    uint32_t num_tris = 0, num_nodes = 0;
+    phg_camera cam{};
+    GuideScene box{};             // bounds, for the guiding network's position features
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -48,6 +59,7 @@ struct Ctx {
     DevBuf<float> emit_cdf;
     DevBuf<EmitTri> emit_tris;
     SceneDev sc{};
+    std::vector<SceneInfo> scenes;
     uint32_t stack_need = 0, bvh_depth = 0, bvh_nodes = 0;
     double bvh_build_ms = 0;
     // film
@@ -59,6 +71,14 @@ struct Ctx {
     DevBuf<phg_ray> ext0, ext1, shadow;
     DevBuf<uint32_t> ext_pid0, ext_pid1, shadow_pid;
     DevBuf<uint32_t> counts;
+    DevBuf<uint32_t> path_seg;   // per path: its pass segment
+    DevBuf<JobDev> jobs;         // job table of the current render
+    DevBuf<PassParams> d_pass;   // pass parameters and segments, rewritten before every pass
+    DevBuf<PassSeg> d_segs;
+    PassParams* h_pass = nullptr;  // pinned staging of the above
+    PassSeg* h_segs = nullptr;
+    DevBuf<double> bfilm;        // films of the last batch, one after another
+    std::vector<JobDev> bfilm_jobs;
     DevBuf<int32_t> spill;       // traversal-stack spill of the persistent traversal grid
     DevBuf<unsigned long long> tcount;
     DevBuf<uint32_t> errflags;
@@ -219,7 +239,9 @@ phg_status phg_destroy(void* ctx) {
     c->nodes.free_(); c->tris.free_(); c->mats.free_(); c->emitters.free_(); c->emit_cdf.free_(); c->emit_tris.free_();
     c->film.free_(); c->rng.free_(); c->T.free_(); c->Wem.free_(); c->prevP.free_(); c->L.free_(); c->hits.free_();
     c->shadow_c.free_(); c->ext0.free_(); c->ext1.free_(); c->shadow.free_(); c->ext_pid0.free_(); c->ext_pid1.free_();
-    c->shadow_pid.free_(); c->spill.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
+    c->shadow_pid.free_(); c->path_seg.free_(); c->jobs.free_(); c->d_pass.free_(); c->d_segs.free_(); c->bfilm.free_();
+    if (c->h_pass) (void)hipHostFree(c->h_pass);
+    c->spill.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
     c->q_nvis.free_(); c->q_ntri.free_();
     c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
     c->g_wn.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
@@ -234,50 +256,68 @@ const char* phg_last_error(void* ctx) {
     return c ? c->err.c_str() : "null context";
 }
 
-phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_opts* bo) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !s) return PHG_ERR_INVALID_ARG;
+}  // extern "C"
+
+namespace phg {
+namespace {
+
+// One scene prepared on the host: BVH, triangles in leaf order and emitter tables, all with
+// scene-local indices (rebased when the scenes are packed together).
+struct HostScene {
+    std::vector<WideNode> nodes;
+    std::vector<TriRec> tris;
+    std::vector<phg_material> mats;
+    std::vector<EmitterRec> emitters;
+    std::vector<float> cdf;         // num_emitters + 1
+    std::vector<EmitTri> etris;
+    phg_camera cam{};
+    GuideScene box{};
+    uint32_t stack_need = 0, depth = 0;
+    double build_ms = 0.0;
+    std::string err;
+    phg_status status = PHG_OK;
+};
+
+static phg_status scene_fail(HostScene& h, phg_status st, const char* msg) { h.status = st; h.err = msg; return st; }
+
+static phg_status prepare_scene(const phg_scene_desc* s, const BvhBuildParams& bp, HostScene& h) {
     if (s->num_tris == 0 || !s->verts || !s->tri_material || !s->tri_emitter || !s->materials || s->num_materials == 0)
-        return set_err(c, PHG_ERR_INVALID_ARG, "scene needs triangles, per-triangle material/emitter ids and materials");
+        return scene_fail(h, PHG_ERR_INVALID_ARG, "scene needs triangles, per-triangle material/emitter ids and materials");
     if (s->num_emitters > 0 && (!s->emitters || !s->emitter_tris))
-        return set_err(c, PHG_ERR_INVALID_ARG, "emitters declared without emitter arrays");
+        return scene_fail(h, PHG_ERR_INVALID_ARG, "emitters declared without emitter arrays");
     for (uint32_t t = 0; t < s->num_tris; ++t) {
-        if (s->tri_material[t] >= s->num_materials) return set_err(c, PHG_ERR_INVALID_ARG, "triangle material id out of range");
-        if (s->tri_emitter[t] >= (int32_t)s->num_emitters) return set_err(c, PHG_ERR_INVALID_ARG, "triangle emitter id out of range");
+        if (s->tri_material[t] >= s->num_materials) return scene_fail(h, PHG_ERR_INVALID_ARG, "triangle material id out of range");
+        if (s->tri_emitter[t] >= (int32_t)s->num_emitters) return scene_fail(h, PHG_ERR_INVALID_ARG, "triangle emitter id out of range");
     }
     for (uint32_t m = 0; m < s->num_materials; ++m)
-        if (s->materials[m].type > PHG_MAT_DIELECTRIC) return set_err(c, PHG_ERR_INVALID_ARG, "unknown material type");
+        if (s->materials[m].type > PHG_MAT_DIELECTRIC) return scene_fail(h, PHG_ERR_INVALID_ARG, "unknown material type");
     for (uint32_t e = 0; e < s->num_emitters; ++e) {
         const phg_emitter& em = s->emitters[e];
         if (em.tri_count == 0 || (uint64_t)em.tri_begin + em.tri_count > s->num_emitter_tris)
-            return set_err(c, PHG_ERR_INVALID_ARG, "emitter triangle range out of bounds");
+            return scene_fail(h, PHG_ERR_INVALID_ARG, "emitter triangle range out of bounds");
         for (uint32_t k = 0; k < em.tri_count; ++k)
-            if (s->emitter_tris[em.tri_begin + k] >= s->num_tris) return set_err(c, PHG_ERR_INVALID_ARG, "emitter triangle id out of range");
-    }
-    PHG_HIP(c, hipSetDevice(c->device));
-    BvhBuildParams bp;
-    if (bo) {
-        if (bo->max_leaf_tris) bp.max_leaf_tris = bo->max_leaf_tris;
-        if (bo->sah_bins) bp.sah_bins = bo->sah_bins;
-        bp.nthreads = bo->nthreads;
+            if (s->emitter_tris[em.tri_begin + k] >= s->num_tris)
+                return scene_fail(h, PHG_ERR_INVALID_ARG, "emitter triangle id out of range");
     }
     BvhBuildResult bvh;
     const auto t0 = std::chrono::steady_clock::now();
     try {
         bvh = build_wide_bvh(s->verts, s->num_tris, bp);
     } catch (const std::exception& ex) {
-        return set_err(c, PHG_ERR_INVALID_ARG, ex.what());
+        h.err = ex.what();
+        h.status = PHG_ERR_INVALID_ARG;
+        return h.status;
     }
-    c->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    h.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (bvh.stack_need > 4096)
-        return set_err(c, PHG_ERR_UNSUPPORTED, "BVH needs a traversal stack deeper than 4096 entries");
+        return scene_fail(h, PHG_ERR_UNSUPPORTED, "BVH needs a traversal stack deeper than 4096 entries");
     // triangles in leaf order
-    std::vector<TriRec> tr(s->num_tris);
+    h.tris.resize(s->num_tris);
     std::vector<uint32_t> pos_of(s->num_tris);
     for (uint32_t k = 0; k < s->num_tris; ++k) {
         const uint32_t t = bvh.tri_order[k];
         pos_of[t] = k;
-        TriRec& r = tr[k];
+        TriRec& r = h.tris[k];
         std::memcpy(r.a, s->verts + 9 * (size_t)t, 9 * sizeof(float));
         r.prim = (int32_t)t;
         r.mat = s->tri_material[t];
@@ -285,14 +325,14 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_op
     }
     // emitters: power-favouring CDF (ESPowerFavoring::update, TPwcDistribution1D ctor)
     const uint32_t ne = s->num_emitters;
-    std::vector<EmitterRec> er(std::max(ne, 1u));
-    std::vector<float> cdf(ne + 1, 0.0f);
-    std::vector<EmitTri> etris(std::max(s->num_emitter_tris, 1u));
+    h.emitters.assign(ne, EmitterRec{});
+    h.cdf.assign(ne + 1, 0.0f);
+    h.etris.resize(s->num_emitter_tris);
     for (uint32_t i = 0; i < s->num_emitter_tris; ++i) {
         const uint32_t t = s->emitter_tris[i];
         const float* v = s->verts + 9 * (size_t)t;
-        etris[i].tri = pos_of[t];
-        etris[i].area = htri_area(H3{v[0], v[1], v[2]}, H3{v[3], v[4], v[5]}, H3{v[6], v[7], v[8]});
+        h.etris[i].tri = pos_of[t];
+        h.etris[i].area = htri_area(H3{v[0], v[1], v[2]}, H3{v[3], v[4], v[5]}, H3{v[6], v[7], v[8]});
     }
     if (ne > 0) {
         std::vector<float> w(ne, 0.0f);
@@ -300,11 +340,12 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_op
             const phg_emitter& em = s->emitters[e];
             float total = 0.0f;
             for (uint32_t k = 0; k < em.tri_count; ++k) {
-                const float flux = hluminance(em.radiance) * etris[em.tri_begin + k].area * 3.14159265358979323846f;
+                const float flux = hluminance(em.radiance) * h.etris[em.tri_begin + k].area * 3.14159265358979323846f;
                 total += flux;
             }
             w[e] = total;
         }
+        std::vector<float>& cdf = h.cdf;
         const float delta = 1.0f / (float)ne;
         for (uint32_t i = 1; i <= ne; ++i) cdf[i] = cdf[i - 1] + w[i - 1] * delta;
         const float rcpSum = 1.0f / cdf[ne];
@@ -313,23 +354,101 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_op
         else { for (uint32_t i = 1; i < ne; ++i) cdf[i] = (float)i / (float)ne; }
         for (uint32_t e = 0; e < ne; ++e) {
             const phg_emitter& em = s->emitters[e];
-            EmitterRec& r = er[e];
+            EmitterRec& r = h.emitters[e];
             std::memset(&r, 0, sizeof(r));
             r.radiance[0] = em.radiance[0]; r.radiance[1] = em.radiance[1]; r.radiance[2] = em.radiance[2];
             r.pick_pdf = cdf[e + 1] - cdf[e];
             r.tri_begin = em.tri_begin; r.tri_count = em.tri_count;
         }
     }
-    PHG_HIP(c, c->nodes.ensure(bvh.nodes.size()));
-    PHG_HIP(c, c->tris.ensure(tr.size()));
-    PHG_HIP(c, c->mats.ensure(s->num_materials));
-    PHG_HIP(c, c->emitters.ensure(er.size()));
+    h.mats.assign(s->materials, s->materials + s->num_materials);
+    h.cam = s->camera;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < 3ull * s->num_tris; ++i) {
+        const float* v = s->verts + 3 * i;
+        for (int k = 0; k < 3; ++k) {
+            if (std::isfinite(v[k])) { lo[k] = std::min(lo[k], v[k]); hi[k] = std::max(hi[k], v[k]); }
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        const float e = hi[k] - lo[k];
+        h.box.lo[k] = std::isfinite(lo[k]) ? lo[k] : 0.0f;
+        h.box.inv_ext[k] = (e > 0.0f && std::isfinite(e)) ? 1.0f / e : 1.0f;
+    }
+    h.nodes = std::move(bvh.nodes);
+    h.stack_need = bvh.stack_need;
+    h.depth = bvh.depth;
+    return PHG_OK;
+}
+
+// Pack prepared scenes into the shared device arrays, rebasing node, triangle, material,
+// emitter and CDF indices by each scene's offsets.
+static phg_status upload_arena(Ctx* c, std::vector<HostScene>& hs) {
+    size_t nn = 0, nt = 0, nm = 0, ne = 0, nc = 0, net = 0;
+    for (const HostScene& h : hs) {
+        nn += h.nodes.size(); nt += h.tris.size(); nm += h.mats.size(); ne += h.emitters.size();
+        nc += h.cdf.size(); net += h.etris.size();
+    }
+    if (nt >= (1ull << kLeafFirstBits) || nn >= 0x7fffffffull)
+        return set_err(c, PHG_ERR_UNSUPPORTED, "scenes exceed 2^28 triangles in total");
+    std::vector<WideNode> nodes; nodes.reserve(nn);
+    std::vector<TriRec> tris; tris.reserve(nt);
+    std::vector<phg_material> mats; mats.reserve(nm);
+    std::vector<EmitterRec> emitters; emitters.reserve(std::max<size_t>(ne, 1));
+    std::vector<float> cdf; cdf.reserve(nc);
+    std::vector<EmitTri> etris; etris.reserve(std::max<size_t>(net, 1));
+    std::vector<SceneInfo> info(hs.size());
+    uint32_t stack_need = 0, depth = 0;
+    for (size_t si = 0; si < hs.size(); ++si) {
+        HostScene& h = hs[si];
+        const uint32_t n0 = (uint32_t)nodes.size(), t0 = (uint32_t)tris.size(), m0 = (uint32_t)mats.size();
+        const uint32_t e0 = (uint32_t)emitters.size(), c0 = (uint32_t)cdf.size(), et0 = (uint32_t)etris.size();
+        for (WideNode w : h.nodes) {
+            for (int k = 0; k < 4; ++k) {
+                const int32_t ch = w.child[k];
+                if (ch == kEmptyChild) continue;
+                if (ch < 0) {
+                    const uint32_t u = (uint32_t)ch;
+                    const uint32_t first = u & ((1u << kLeafFirstBits) - 1u);
+                    w.child[k] = (int32_t)((u & ~((1u << kLeafFirstBits) - 1u)) | (first + t0));
+                } else {
+                    w.child[k] = ch + (int32_t)n0;
+                }
+            }
+            nodes.push_back(w);
+        }
+        for (TriRec r : h.tris) {
+            r.mat += m0;
+            if (r.emitter >= 0) r.emitter += (int32_t)e0;
+            tris.push_back(r);
+        }
+        mats.insert(mats.end(), h.mats.begin(), h.mats.end());
+        for (EmitterRec r : h.emitters) { r.tri_begin += et0; emitters.push_back(r); }
+        cdf.insert(cdf.end(), h.cdf.begin(), h.cdf.end());
+        for (EmitTri e : h.etris) { e.tri += t0; etris.push_back(e); }
+        SceneInfo& in = info[si];
+        in.root = n0; in.emitter_begin = e0; in.num_emitters = (uint32_t)h.emitters.size(); in.cdf_begin = c0;
+        in.cam = h.cam; in.box = h.box; in.num_tris = (uint32_t)h.tris.size(); in.num_nodes = (uint32_t)h.nodes.size();
+        stack_need = std::max(stack_need, h.stack_need);
+        depth = std::max(depth, h.depth);
+        // host copies are no longer needed
+        std::vector<WideNode>().swap(h.nodes);
+        std::vector<TriRec>().swap(h.tris);
+    }
+    if (emitters.empty()) emitters.push_back(EmitterRec{});
+    if (etris.empty()) etris.push_back(EmitTri{});
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    PHG_HIP(c, c->nodes.ensure(nodes.size()));
+    PHG_HIP(c, c->tris.ensure(tris.size()));
+    PHG_HIP(c, c->mats.ensure(mats.size()));
+    PHG_HIP(c, c->emitters.ensure(emitters.size()));
     PHG_HIP(c, c->emit_cdf.ensure(cdf.size()));
     PHG_HIP(c, c->emit_tris.ensure(etris.size()));
-    PHG_HIP(c, hipMemcpy(c->nodes.p, bvh.nodes.data(), bvh.nodes.size() * sizeof(WideNode), hipMemcpyHostToDevice));
-    PHG_HIP(c, hipMemcpy(c->tris.p, tr.data(), tr.size() * sizeof(TriRec), hipMemcpyHostToDevice));
-    PHG_HIP(c, hipMemcpy(c->mats.p, s->materials, s->num_materials * sizeof(phg_material), hipMemcpyHostToDevice));
-    PHG_HIP(c, hipMemcpy(c->emitters.p, er.data(), er.size() * sizeof(EmitterRec), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->nodes.p, nodes.data(), nodes.size() * sizeof(WideNode), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->tris.p, tris.data(), tris.size() * sizeof(TriRec), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->mats.p, mats.data(), mats.size() * sizeof(phg_material), hipMemcpyHostToDevice));
+    PHG_HIP(c, hipMemcpy(c->emitters.p, emitters.data(), emitters.size() * sizeof(EmitterRec), hipMemcpyHostToDevice));
     PHG_HIP(c, hipMemcpy(c->emit_cdf.p, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice));
     PHG_HIP(c, hipMemcpy(c->emit_tris.p, etris.data(), etris.size() * sizeof(EmitTri), hipMemcpyHostToDevice));
     c->sc.nodes = c->nodes.p;
@@ -338,31 +457,65 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_op
     c->sc.emitters = c->emitters.p;
     c->sc.emit_cdf = c->emit_cdf.p;
     c->sc.emit_tris = c->emit_tris.p;
-    c->sc.num_emitters = ne;
-    c->sc.cam = s->camera;
-    {
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (size_t i = 0; i < 3ull * s->num_tris; ++i) {
-            const float* v = s->verts + 3 * i;
-            for (int k = 0; k < 3; ++k) {
-                if (std::isfinite(v[k])) { lo[k] = std::min(lo[k], v[k]); hi[k] = std::max(hi[k], v[k]); }
-            }
-        }
-        for (int k = 0; k < 3; ++k) {
-            const float e = hi[k] - lo[k];
-            c->g_scene.lo[k] = std::isfinite(lo[k]) ? lo[k] : 0.0f;
-            c->g_scene.inv_ext[k] = (e > 0.0f && std::isfinite(e)) ? 1.0f / e : 1.0f;
-        }
-    }
-    c->stack_need = bvh.stack_need;
-    c->bvh_depth = bvh.depth;
-    c->bvh_nodes = (uint32_t)bvh.nodes.size();
+    c->scenes = std::move(info);
+    c->g_scene = c->scenes[0].box;
+    c->stack_need = stack_need;
+    c->bvh_depth = depth;
+    c->bvh_nodes = (uint32_t)nodes.size();
     c->have_scene = true;
     c->stats.bvh_nodes = c->bvh_nodes;
     c->stats.bvh_depth = c->bvh_depth;
     c->stats.bvh_stack_need = c->stack_need;
     c->stats.bvh_build_ms = c->bvh_build_ms;
     return PHG_OK;
+}
+
+static phg_status upload_scenes(Ctx* c, const phg_scene_desc* scenes, uint32_t n, const phg_bvh_opts* bo) {
+    BvhBuildParams bp;
+    if (bo) {
+        if (bo->max_leaf_tris) bp.max_leaf_tris = bo->max_leaf_tris;
+        if (bo->sah_bins) bp.sah_bins = bo->sah_bins;
+        bp.nthreads = bo->nthreads;
+    }
+    std::vector<HostScene> hs(n);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (n == 1) {
+        prepare_scene(&scenes[0], bp, hs[0]);
+    } else {
+        // one scene per worker, each build single-threaded
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned workers = std::min<unsigned>(n, bo && bo->nthreads ? bo->nthreads : hw);
+        BvhBuildParams bp1 = bp;
+        bp1.nthreads = 1;
+        std::atomic<uint32_t> next{0};
+        std::vector<std::thread> pool;
+        for (unsigned w = 0; w < workers; ++w)
+            pool.emplace_back([&] {
+                for (uint32_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) prepare_scene(&scenes[i], bp1, hs[i]);
+            });
+        for (auto& t : pool) t.join();
+    }
+    c->bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (uint32_t i = 0; i < n; ++i)
+        if (hs[i].status != PHG_OK) return set_err(c, hs[i].status, "scene " + std::to_string(i) + ": " + hs[i].err);
+    return upload_arena(c, hs);
+}
+
+}  // namespace
+}  // namespace phg
+
+extern "C" {
+
+phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_opts* bo) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !s) return PHG_ERR_INVALID_ARG;
+    return upload_scenes(c, s, 1, bo);
+}
+
+phg_status phg_upload_scenes(void* ctx, const phg_scene_desc* scenes, uint32_t n, const phg_bvh_opts* bo) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !scenes || n == 0) return PHG_ERR_INVALID_ARG;
+    return upload_scenes(c, scenes, n, bo);
 }
 
 static uint32_t trace_grid(const Ctx* c) { return c->cu_count * c->trace_blocks_per_cu; }
@@ -466,65 +619,89 @@ phg_status phg_film_read(void* ctx, double* out, uint64_t out_len) {
     return PHG_OK;
 }
 
-phg_status phg_render(void* ctx, const phg_render_opts* o) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !o) return PHG_ERR_INVALID_ARG;
-    if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
-    if (o->width != c->film_w || o->height != c->film_h)
-        return set_err(c, PHG_ERR_INVALID_ARG, "film size differs from render size; call phg_film_clear first");
-    const uint64_t npix = (uint64_t)o->width * o->height;
-    if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "image size out of range");
-    if (o->seed >= (1ull << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "seed must be below 2^24");
-    if ((uint64_t)o->spp_begin + o->spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "sample index range too large");
-    if (o->spp == 0) return PHG_OK;
+}  // extern "C"
+
+namespace phg {
+namespace {
+
+struct RenderMode {
+    bool count_traversal = false, guided = false, train = false;
+    float guide_prob = 0.5f, guide_lr = 1e-3f;
+    uint32_t guide_batch = 1u << 18, guide_steps = 4;
+};
+
+struct PassPlan {
+    std::vector<PassSeg> segs;
+    uint32_t paths = 0, film_threads = 0, max_bounces = 0;
+};
+
+// Render the planned passes of the given jobs into c->film / c->bfilm (per job film_base).
+// job_done_ms (optional, one per job) receives the time from the start to the end of the
+// pass that completes each job.
+static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const std::vector<PassPlan>& passes,
+                                double* film, const RenderMode& m, std::vector<double>* job_done_ms) {
+    uint32_t cap = 0;
+    for (const PassPlan& pl : passes) {
+        cap = std::max(cap, pl.paths);
+        if (pl.segs.empty() || pl.segs.size() > kMaxSegs) return set_err(c, PHG_ERR_INVALID_ARG, "bad pass plan");
+    }
+    if (cap == 0) return PHG_OK;
     PHG_HIP(c, hipSetDevice(c->device));
-    // samples per pass: fill ~4M paths in flight
-    uint32_t k = o->spp_per_pass;
-    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, (4ull << 20) / npix);
-    k = std::min(k, o->spp);
-    uint64_t cap = npix * k;
-    if (cap > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
     PHG_HIP(c, c->rng.ensure(cap));
     PHG_HIP(c, c->T.ensure(cap)); PHG_HIP(c, c->Wem.ensure(cap)); PHG_HIP(c, c->prevP.ensure(cap)); PHG_HIP(c, c->L.ensure(cap));
     PHG_HIP(c, c->hits.ensure(cap)); PHG_HIP(c, c->shadow_c.ensure(cap));
     PHG_HIP(c, c->ext0.ensure(cap)); PHG_HIP(c, c->ext1.ensure(cap)); PHG_HIP(c, c->shadow.ensure(cap));
     PHG_HIP(c, c->ext_pid0.ensure(cap)); PHG_HIP(c, c->ext_pid1.ensure(cap)); PHG_HIP(c, c->shadow_pid.ensure(cap));
+    PHG_HIP(c, c->path_seg.ensure(cap));
     PHG_HIP(c, c->counts.ensure(2 * kCountSlots));
     PHG_HIP(c, c->tcount.ensure(8));
     PHG_HIP(c, c->errflags.ensure(1));
+    PHG_HIP(c, c->jobs.ensure(jobs.size()));
+    PHG_HIP(c, c->d_pass.ensure(1));
+    PHG_HIP(c, c->d_segs.ensure(kMaxSegs));
+    if (!c->h_pass) {
+        void* hp = nullptr;
+        PHG_HIP(c, hipHostMalloc(&hp, sizeof(PassParams) + kMaxSegs * sizeof(PassSeg), hipHostMallocDefault));
+        c->h_pass = static_cast<PassParams*>(hp);
+        c->h_segs = reinterpret_cast<PassSeg*>(static_cast<char*>(hp) + sizeof(PassParams));
+    }
+    PHG_HIP(c, hipMemcpyAsync(c->jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, c->stream));
     PHG_HIP(c, hipMemsetAsync(c->tcount.p, 0, 8 * sizeof(unsigned long long), c->stream));
     PHG_HIP(c, hipMemsetAsync(c->errflags.p, 0, sizeof(uint32_t), c->stream));
     PHG_HIP(c, launch_clear_counts(c->counts.p, 2 * kCountSlots, c->stream));
+    SceneDev sc = c->sc;
+    sc.jobs = c->jobs.p;
     PathBuffers pb{};
     pb.rng = c->rng.p; pb.T = c->T.p; pb.Wem = c->Wem.p; pb.prevP = c->prevP.p; pb.L = c->L.p;
     pb.ext[0] = c->ext0.p; pb.ext[1] = c->ext1.p; pb.ext_pid[0] = c->ext_pid0.p; pb.ext_pid[1] = c->ext_pid1.p;
     pb.hits = c->hits.p; pb.shadow = c->shadow.p; pb.shadow_pid = c->shadow_pid.p; pb.shadow_c = c->shadow_c.p;
-    pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = c->film.p;
+    pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = film;
+    pb.path_seg = c->path_seg.p;
     {
         const phg_status st = ensure_spill(c);
         if (st != PHG_OK) return st;
     }
     pb.spill = c->spill.p;
     pb.spill_cap = spill_cap(c);
-    const uint32_t count_trav = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) ? 1u : 0u;
-    const bool guided = (o->flags & PHG_RENDER_GUIDED) != 0;
-    const bool train = (o->flags & PHG_RENDER_TRAIN) != 0;
+    const bool guided = m.guided, train = m.train;
     uint32_t rec_bounces = 0;
     uint32_t train_cap = 0;
-    const uint32_t batch = o->guide_batch ? o->guide_batch : (1u << 18);
+    const uint32_t batch = m.guide_batch ? m.guide_batch : (1u << 18);
     if (guided || train) {
         if (!c->g_init) {
-            const std::vector<float> p0 = guide_default_params(o->seed + 1);
+            const std::vector<float> p0 = guide_default_params(jobs[0].seed + 1);
             phg_status st = guide_load(c, p0.data());
             if (st != PHG_OK) return st;
         }
-        PHG_HIP(c, c->g_lobes.ensure(cap * (size_t)kGuideOutputs));
+        PHG_HIP(c, c->g_lobes.ensure((size_t)cap * kGuideOutputs));
     }
+    uint32_t max_b = 0;
+    for (const PassPlan& pl : passes) max_b = std::max(max_b, pl.max_bounces);
     if (train) {
-        rec_bounces = std::min<uint32_t>(kGuideRecBounces, o->max_bounces + 1);
+        rec_bounces = std::min<uint32_t>(kGuideRecBounces, max_b + 1);
         PHG_HIP(c, c->g_rec.ensure((size_t)rec_bounces * cap * kRecFloat4));
         // fresh records for every step of a pass (no record is reused within the pass)
-        train_cap = std::max<uint32_t>(4u, o->guide_steps ? o->guide_steps : 4u) * batch;
+        train_cap = std::max<uint32_t>(4u, m.guide_steps ? m.guide_steps : 4u) * batch;
         PHG_HIP(c, c->g_train.ensure((size_t)train_cap * kGuideRecordFloats));
     }
     pb.lobes = c->g_lobes.p;
@@ -534,37 +711,45 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     const uint32_t grid_trace = trace_grid(c);     // persistent-style grid-stride launches
     const uint32_t grid_shade = c->cu_count * c->shade_blocks_per_cu;
     double t_trace = 0, t_shadow = 0, t_shade = 0;
-    uint64_t rays_primary = 0;
+    uint64_t rays_primary = 0, paths_total = 0;
     uint32_t n_trace = 0, n_shadow = 0;
+    const uint32_t count_trav = m.count_traversal ? 1u : 0u;
+    const PassParams* ppd = c->d_pass.p;
     const auto wall0 = std::chrono::steady_clock::now();
-    for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
-        const uint32_t kk = std::min(k, o->spp - s0);
+    for (const PassPlan& pl : passes) {
         PassParams pp{};
-        pp.width = o->width; pp.height = o->height; pp.npix = (uint32_t)npix;
-        pp.paths = (uint32_t)(npix * kk); pp.sample_begin = o->spp_begin + s0; pp.max_bounces = o->max_bounces;
-        pp.seed = o->seed;
-        pp.guide_prob = (guided && c->g_ready) ? o->guide_prob : 0.0f;
+        pp.paths = pl.paths;
+        pp.num_segs = (uint32_t)pl.segs.size();
+        pp.film_threads = pl.film_threads;
+        pp.max_bounces = pl.max_bounces;
+        pp.guide_prob = (guided && c->g_ready) ? m.guide_prob : 0.0f;
         pp.record = train ? 1u : 0u;
         pp.rec_bounces = rec_bounces;
+        pp.segs = c->d_segs.p;
+        *c->h_pass = pp;
+        std::memcpy(c->h_segs, pl.segs.data(), pl.segs.size() * sizeof(PassSeg));
+        PHG_HIP(c, hipMemcpyAsync(c->d_pass.p, c->h_pass, sizeof(PassParams), hipMemcpyHostToDevice, c->stream));
+        PHG_HIP(c, hipMemcpyAsync(c->d_segs.p, c->h_segs, pl.segs.size() * sizeof(PassSeg), hipMemcpyHostToDevice, c->stream));
         rays_primary += pp.paths;
+        paths_total += pp.paths;
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-        PHG_HIP(c, launch_generate(c->sc, pp, pb, c->stream));
+        PHG_HIP(c, launch_generate(sc, ppd, pb, grid_shade, c->stream));
         PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         float ms = 0; PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
         for (uint32_t b = 0;; ++b) {
-            const bool use_guide = guided && pp.guide_prob > 0.0f && b < o->max_bounces;
+            const bool use_guide = guided && pp.guide_prob > 0.0f && b < pl.max_bounces;
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-            PHG_HIP(c, launch_extend(c->sc, pb, b, grid_trace, count_trav, c->stream));
+            PHG_HIP(c, launch_extend(sc, pb, b, grid_trace, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
             if (use_guide) {
-                PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], c->sc.tris, c->g_scene,
+                PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], sc.tris, c->g_scene,
                                             c->counts.p + b % kCountSlots, c->g_lobes.p, c->cu_count * 2, c->stream));
             }
             PHG_HIP(c, hipEventRecord(c->ev[4], c->stream));
-            PHG_HIP(c, launch_shade(c->sc, pp, pb, b, grid_shade, guided || train, c->stream));
+            PHG_HIP(c, launch_shade(sc, ppd, pb, b, grid_shade, guided || train, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            PHG_HIP(c, launch_shadow(c->sc, pb, b, grid_trace, count_trav, rec_bounces, pp.paths, c->stream));
+            PHG_HIP(c, launch_shadow(sc, ppd, pb, b, grid_trace, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[3], c->stream));
             uint32_t next[2] = {0, 0};
             PHG_HIP(c, hipMemcpyAsync(&next[0], c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -577,15 +762,18 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
             PHG_HIP(c, hipEventElapsedTime(&cc, c->ev[2], c->ev[3]));
             t_trace += a; t_shade += bb; t_shadow += cc; t_ginfer += gi;
             if (use_guide) g_queries += next[1];
-            const uint32_t next0 = next[0];
             ++n_trace; ++n_shadow;
-            if (next0 == 0 || b >= o->max_bounces) break;
+            if (next[0] == 0 || b >= pl.max_bounces) break;
         }
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-        PHG_HIP(c, launch_film(pp, pb, kk, c->stream));
+        PHG_HIP(c, launch_film(sc, ppd, pb, grid_shade, c->stream));
         PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
+        if (job_done_ms) {
+            const double now = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+            for (const PassSeg& sg : pl.segs) (*job_done_ms)[sg.job] = now;  // the last pass of a job wins
+        }
         if (train) {
             // online training on this pass's radiance samples
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
@@ -596,8 +784,8 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
             PHG_HIP(c, hipMemcpyAsync(cnt, c->g_counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));
             const uint32_t nrec = std::min(cnt[0], train_cap);
-            const uint32_t steps = o->guide_steps ? o->guide_steps : 4;
-            phg_status gs = guide_train_device(c, c->g_train.p, nrec, batch, o->guide_lr > 0 ? o->guide_lr : 1e-3f, steps);
+            const uint32_t steps = m.guide_steps ? m.guide_steps : 4;
+            phg_status gs = guide_train_device(c, c->g_train.p, nrec, batch, m.guide_lr > 0 ? m.guide_lr : 1e-3f, steps);
             if (gs != PHG_OK) return gs;
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));
@@ -607,7 +795,6 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
             const double want = (double)steps * batch;
             c->g_keep = cnt[1] > 0 ? (float)std::min(1.0, 1.25 * want / (double)cnt[1]) : 1.0f;
             if (nrec > 0) c->g_ready = true;
-            pp.guide_prob = 0.0f;  // (next pass re-reads g_ready)
         }
     }
     uint32_t flags = 0;
@@ -616,7 +803,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     PHG_HIP(c, hipMemcpyAsync(tc, c->tcount.p, sizeof(tc), hipMemcpyDeviceToHost, c->stream));
     PHG_HIP(c, hipStreamSynchronize(c->stream));
     c->stats.render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
-    c->stats.paths = npix * (uint64_t)o->spp;
+    c->stats.paths = paths_total;
     c->stats.rays_primary = rays_primary;
     c->stats.rays_extension = tc[4];
     c->stats.rays_shadow = tc[5];
@@ -637,6 +824,148 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     c->stats.shadow_ms = t_shadow;
     c->stats.shade_ms = t_shade;
     if (flags) return set_err(c, PHG_ERR_UNSUPPORTED, "traversal stack overflow");
+    return PHG_OK;
+}
+
+static JobDev make_job(const SceneInfo& si, uint32_t width, uint32_t height, uint32_t max_bounces, uint64_t seed,
+                       uint64_t film_base) {
+    JobDev j{};
+    j.cam = si.cam;
+    j.root = si.root;
+    j.width = width; j.height = height; j.npix = width * height;
+    j.emitter_begin = si.emitter_begin; j.num_emitters = si.num_emitters; j.cdf_begin = si.cdf_begin;
+    j.max_bounces = max_bounces;
+    j.seed = seed;
+    j.film_base = film_base;
+    return j;
+}
+
+}  // namespace
+}  // namespace phg
+
+extern "C" {
+
+phg_status phg_render(void* ctx, const phg_render_opts* o) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !o) return PHG_ERR_INVALID_ARG;
+    if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
+    if (o->width != c->film_w || o->height != c->film_h)
+        return set_err(c, PHG_ERR_INVALID_ARG, "film size differs from render size; call phg_film_clear first");
+    const uint64_t npix = (uint64_t)o->width * o->height;
+    if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "image size out of range");
+    if (o->seed >= (1ull << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "seed must be below 2^24");
+    if ((uint64_t)o->spp_begin + o->spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "sample index range too large");
+    if (o->spp == 0) return PHG_OK;
+    // samples per pass: fill ~4M paths in flight
+    uint32_t k = o->spp_per_pass;
+    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, (4ull << 20) / npix);
+    k = std::min(k, o->spp);
+    if (npix * k > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
+    std::vector<JobDev> jobs{make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0)};
+    std::vector<PassPlan> passes;
+    for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
+        const uint32_t kk = std::min(k, o->spp - s0);
+        PassPlan pl;
+        PassSeg sg{};
+        sg.job = 0; sg.sample_begin = o->spp_begin + s0; sg.samples = kk; sg.path_offset = 0;
+        sg.pix_offset = 0; sg.npix = (uint32_t)npix;
+        pl.segs.push_back(sg);
+        pl.paths = (uint32_t)(npix * kk);
+        pl.film_threads = (uint32_t)npix;
+        pl.max_bounces = o->max_bounces;
+        passes.push_back(std::move(pl));
+    }
+    RenderMode m;
+    m.count_traversal = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) != 0;
+    m.guided = (o->flags & PHG_RENDER_GUIDED) != 0;
+    m.train = (o->flags & PHG_RENDER_TRAIN) != 0;
+    m.guide_prob = o->guide_prob; m.guide_lr = o->guide_lr; m.guide_batch = o->guide_batch; m.guide_steps = o->guide_steps;
+    c->g_scene = c->scenes[0].box;
+    return render_passes(c, jobs, passes, c->film.p, m, nullptr);
+}
+
+phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, const phg_batch_opts* bo,
+                            phg_batch_result* results) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !in || n == 0) return PHG_ERR_INVALID_ARG;
+    if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
+    const uint64_t cap = bo && bo->paths_per_pass ? bo->paths_per_pass : (4ull << 20);
+    if (cap > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass too large");
+    std::vector<JobDev> jobs(n);
+    uint64_t film_doubles = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const phg_batch_job& b = in[j];
+        const uint64_t npix = (uint64_t)b.width * b.height;
+        if (b.scene >= c->scenes.size()) return set_err(c, PHG_ERR_INVALID_ARG, "job scene index out of range");
+        if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "job image size out of range");
+        if (b.seed >= (1u << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "job seed must be below 2^24");
+        if (b.spp == 0 || (uint64_t)b.spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "job spp out of range");
+        jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles);
+        film_doubles += 8 * npix;
+    }
+    // pass plan: jobs in the requested order, each pass filled first-fit with whole samples
+    std::vector<uint32_t> order(n);
+    for (uint32_t j = 0; j < n; ++j) order[j] = j;
+    if (bo && bo->order == PHG_BATCH_ORDER_SHORTEST_FIRST)
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return (uint64_t)jobs[a].npix * in[a].spp < (uint64_t)jobs[b].npix * in[b].spp;
+        });
+    std::vector<uint32_t> left(n), next(n, 0);
+    for (uint32_t j = 0; j < n; ++j) left[j] = in[j].spp;
+    std::vector<PassPlan> passes;
+    size_t first = 0;
+    while (first < n) {
+        PassPlan pl;
+        uint64_t used = 0;
+        for (size_t oi = first; oi < n && pl.segs.size() < kMaxSegs; ++oi) {
+            const uint32_t j = order[oi];
+            if (left[j] == 0) continue;
+            const uint64_t npix = jobs[j].npix;
+            uint64_t fit = (cap - std::min(cap, used)) / npix;
+            if (fit == 0 && used == 0) fit = 1;  // a job larger than a pass: one sample per pass
+            if (fit == 0) continue;
+            const uint32_t take = (uint32_t)std::min<uint64_t>(fit, left[j]);
+            PassSeg sg{};
+            sg.job = j; sg.sample_begin = next[j]; sg.samples = take; sg.path_offset = (uint32_t)used;
+            sg.pix_offset = pl.film_threads; sg.npix = (uint32_t)npix;
+            pl.segs.push_back(sg);
+            used += (uint64_t)take * npix;
+            pl.film_threads += (uint32_t)npix;
+            pl.max_bounces = std::max(pl.max_bounces, in[j].max_bounces);
+            next[j] += take; left[j] -= take;
+        }
+        pl.paths = (uint32_t)used;
+        passes.push_back(std::move(pl));
+        while (first < n && left[order[first]] == 0) ++first;
+    }
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, c->bfilm.ensure(film_doubles));
+    PHG_HIP(c, launch_fill_zero(c->bfilm.p, film_doubles, c->stream));
+    c->bfilm_jobs.assign(jobs.begin(), jobs.end());
+    RenderMode m;
+    m.count_traversal = bo && (bo->flags & PHG_BATCH_COUNT_TRAVERSAL);
+    std::vector<double> done(n, 0.0);
+    const phg_status st = render_passes(c, jobs, passes, c->bfilm.p, m, &done);
+    if (st != PHG_OK) return st;
+    c->stats.batch_passes = (uint32_t)passes.size();
+    if (results)
+        for (uint32_t j = 0; j < n; ++j) {
+            results[j].done_ms = done[j];
+            results[j].paths = (uint64_t)jobs[j].npix * in[j].spp;
+        }
+    return PHG_OK;
+}
+
+phg_status phg_batch_film_read(void* ctx, uint32_t job, double* out, uint64_t out_len) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !out) return PHG_ERR_INVALID_ARG;
+    if (job >= c->bfilm_jobs.size()) return set_err(c, PHG_ERR_INVALID_ARG, "no such job in the last batch");
+    const JobDev& j = c->bfilm_jobs[job];
+    const uint64_t n = 8ull * j.npix;
+    if (out_len < n) return set_err(c, PHG_ERR_INVALID_ARG, "output buffer too small");
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, hipMemcpyAsync(out, c->bfilm.p + j.film_base, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
     return PHG_OK;
 }
 

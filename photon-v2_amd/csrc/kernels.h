@@ -35,29 +35,52 @@ struct alignas(8) EmitTri {       // one emitter triangle: its BVH-order index a
     float area;
 };
 
+// One render job: a camera over one of the uploaded scenes, its film and its sample stream.
+// The scenes share the node / triangle / material / emitter arrays (indices rebased at
+// upload); a job addresses its own part through these offsets.
+struct alignas(16) JobDev {
+    phg_camera cam;
+    uint32_t root;                // first BVH node of the job's scene
+    uint32_t width, height, npix;
+    uint32_t emitter_begin;       // the scene's emitters: [emitter_begin, emitter_begin + num_emitters)
+    uint32_t num_emitters;
+    uint32_t cdf_begin;           // the scene's emitter-pick CDF: num_emitters + 1 floats
+    uint32_t max_bounces;
+    uint64_t seed;                // sample-stream seed (< 2^24)
+    uint64_t film_base;           // first double of the job's film (8 per pixel)
+};
+static_assert(sizeof(JobDev) == 112, "job record is 112 bytes");
+
+// A pass renders whole samples of one or more jobs: segment s holds `samples` consecutive
+// samples of job `job`, starting at sample index sample_begin, as pass paths
+// [path_offset, path_offset + samples * npix) in sample-major order; its pixels are film
+// threads [pix_offset, pix_offset + npix).
+struct alignas(16) PassSeg {
+    uint32_t job, sample_begin, samples, path_offset;
+    uint32_t pix_offset, npix, pad[2];
+};
+constexpr uint32_t kMaxSegs = 4096;
+
 struct SceneDev {
     const WideNode* nodes;
     const TriRec* tris;
     const phg_material* mats;
     const EmitterRec* emitters;
-    const float* emit_cdf;        // num_emitters + 1
+    const float* emit_cdf;
     const EmitTri* emit_tris;
-    uint32_t num_emitters;
-    uint32_t pad;
-    phg_camera cam;
+    const JobDev* jobs;
 };
 
 struct PassParams {
-    uint32_t width, height;
-    uint32_t npix;
-    uint32_t paths;          // paths in this pass = npix * samples in the pass
-    uint32_t sample_begin;   // sample index of path 0
-    uint32_t max_bounces;
-    uint64_t seed;
+    uint32_t paths;          // paths in this pass (sum over the segments)
+    uint32_t num_segs;
+    uint32_t film_threads;   // sum of the segments' pixels
+    uint32_t max_bounces;    // largest over the pass's jobs (bounce-loop length)
     float guide_prob;        // guided shading: probability of sampling the learned lobes
     uint32_t record;         // guided shading: write training records
     uint32_t rec_bounces;    // bounces with a record slot
     uint32_t pad;
+    const PassSeg* segs;     // num_segs entries, ascending path_offset and pix_offset
 };
 
 constexpr uint32_t kGuideRecBounces = 8;   // training records are kept for the first bounces
@@ -85,22 +108,25 @@ struct PathBuffers {
     float4* rec;             // training records [bounce][path][kRecFloat4]
     int32_t* spill;          // traversal-stack spill, spill_cap entries per traversal thread
     uint32_t spill_cap;
+    uint32_t* path_seg;      // per path: its pass segment
 };
 
 hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
                             uint32_t* nvis, uint32_t* ntri, uint32_t* flags, int32_t* spill, uint32_t spill_cap,
                             uint32_t grid, hipStream_t st);
-hipError_t launch_generate(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, hipStream_t st);
+// Wavefront stages.  Pass parameters are read from device memory (ppd), so that a captured
+// bounce loop can be replayed for every pass after the host rewrites them.
+hipError_t launch_generate(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t grid, hipStream_t st);
 // Blocks of the traversal kernels (kTraceBlock threads) and of the shading kernel
 // (kShadeBlock threads) that one compute unit holds at once; sizes the persistent grids.
 hipError_t resident_blocks(uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu);
 hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
                          hipStream_t st);
-hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
-                         uint32_t rec_bounces, uint32_t npaths, hipStream_t st);
-hipError_t launch_shade(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+                         uint32_t count, hipStream_t st);
+hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
                         bool guided, hipStream_t st);
-hipError_t launch_film(const PassParams& pp, const PathBuffers& pb, uint32_t spp_in_pass, hipStream_t st);
+hipError_t launch_film(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t grid, hipStream_t st);
 hipError_t launch_clear_counts(uint32_t* counts, uint32_t n, hipStream_t st);
 hipError_t launch_fill_zero(double* p, uint64_t n, hipStream_t st);
 

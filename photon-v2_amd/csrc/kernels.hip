@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(SceneDev sc, const ph
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
         const float4 r0 = rp[0], r1 = rp[1];
         uint32_t nvis = 0, ntri = 0;
-        const TraceResult tr = traverse<RING, ANY, COUNT>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z),
+        const TraceResult tr = traverse<RING, ANY, COUNT>(sc.nodes, sc.tris, 0, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z),
                                                           r0.w, r1.w, stack_mem + threadIdx.x, kTraceBlock,
                                                           spill + (size_t)gtid * spill_cap, spill_cap, &nvis, &ntri, &ovf);
         phg_hit h;
@@ -83,40 +83,64 @@ enum : uint32_t { kFlagDieAfterEmission = 1u };
 
 
 
-__device__ __forceinline__ void path_ids(const PassParams& pp, uint32_t pid, uint32_t* pixel, uint32_t* sample) {
-    *pixel = pid % pp.npix;
-    *sample = pp.sample_begin + pid / pp.npix;
+// Pass segment holding pass path p: the last segment whose path_offset is <= p.
+__device__ __forceinline__ uint32_t find_seg(const PassSeg* __restrict__ segs, uint32_t nseg, uint32_t p) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segs[mid].path_offset <= p) lo = mid; else hi = mid;
+    }
+    return lo;
 }
 
-__global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, PassParams pp, PathBuffers pb) {
-    const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
-    if (i == 0) pb.counts[0] = pp.paths;
-    if (i >= pp.paths) return;
-    uint32_t pixel, sample;
-    path_ids(pp, i, &pixel, &sample);
-    Stream s = stream_init(path_key(pp.seed, pixel, sample, pp.npix));
-    const float rx = (float)(pixel % pp.width) + s.u();
-    const float ry = (float)(pixel / pp.width) + s.u();
-    // PinholeCamera::genReceiveRayDir with the horizontal-FOV sensor of SingleLensObserver
-    const phg_camera& c = sc.cam;
-    const float cx = (2.0f * (rx / (float)pp.width) - 1.0f) * c.tan_half_fov;
-    const float cy = (2.0f * (ry / (float)pp.height) - 1.0f) * c.tan_half_fov * ((float)pp.height / (float)pp.width);
-    const V3 right = mk(c.right[0], c.right[1], c.right[2]);
-    const V3 up = mk(c.up[0], c.up[1], c.up[2]);
-    const V3 fwd = mk(c.forward[0], c.forward[1], c.forward[2]);
-    const V3 d = normalize(right * cx + up * cy + fwd);
-    float4* rq = reinterpret_cast<float4*>(pb.ext[0] + i);
-    rq[0] = make_float4(c.pos[0], c.pos[1], c.pos[2], 0.0f);
-    rq[1] = make_float4(d.x, d.y, d.z, kFltMax);
-    pb.ext_pid[0][i] = i;
-    pb.rng[i] = s.state;
-    pb.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    pb.Wem[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    pb.prevP[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-    pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (pp.record)
-        for (uint32_t b = 0; b < pp.rec_bounces; ++b)
-            pb.rec[((size_t)b * pp.paths + i) * kRecFloat4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// pixel and sample index of pass path p within its segment's job
+__device__ __forceinline__ void path_ids(const PassSeg& sg, uint32_t p, uint32_t* pixel, uint32_t* sample) {
+    const uint32_t local = p - sg.path_offset;
+    *pixel = local % sg.npix;
+    *sample = sg.sample_begin + local / sg.npix;
+}
+
+// Extension and shadow rays in the path queues: (origin, root node of the job's BVH as int
+// bits), (direction, tmax); the segment starts at t = 0.
+__device__ __forceinline__ void queue_ray(phg_ray* q, uint32_t slot, V3 o, uint32_t root, V3 d, float tmax) {
+    float4* rq = reinterpret_cast<float4*>(q + slot);
+    rq[0] = make_float4(o.x, o.y, o.z, __uint_as_float(root));
+    rq[1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb) {
+    const PassParams pp = *ppd;
+    const uint32_t stride = gridDim.x * kShadeBlock;
+    if (blockIdx.x == 0 && threadIdx.x == 0) pb.counts[0] = pp.paths;
+    for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < pp.paths; i += stride) {
+        const uint32_t si = find_seg(pp.segs, pp.num_segs, i);
+        const PassSeg sg = pp.segs[si];
+        const JobDev& job = sc.jobs[sg.job];
+        uint32_t pixel, sample;
+        path_ids(sg, i, &pixel, &sample);
+        Stream s = stream_init(path_key(job.seed, pixel, sample, job.npix));
+        const float rx = (float)(pixel % job.width) + s.u();
+        const float ry = (float)(pixel / job.width) + s.u();
+        // PinholeCamera::genReceiveRayDir with the horizontal-FOV sensor of SingleLensObserver
+        const phg_camera& c = job.cam;
+        const float cx = (2.0f * (rx / (float)job.width) - 1.0f) * c.tan_half_fov;
+        const float cy = (2.0f * (ry / (float)job.height) - 1.0f) * c.tan_half_fov * ((float)job.height / (float)job.width);
+        const V3 right = mk(c.right[0], c.right[1], c.right[2]);
+        const V3 up = mk(c.up[0], c.up[1], c.up[2]);
+        const V3 fwd = mk(c.forward[0], c.forward[1], c.forward[2]);
+        const V3 d = normalize(right * cx + up * cy + fwd);
+        queue_ray(pb.ext[0], i, mk(c.pos[0], c.pos[1], c.pos[2]), job.root, d, kFltMax);
+        pb.ext_pid[0][i] = i;
+        pb.path_seg[i] = si;
+        pb.rng[i] = s.state;
+        pb.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        pb.Wem[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pb.prevP[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+        pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (pp.record)
+            for (uint32_t b = 0; b < pp.rec_bounces; ++b)
+                pb.rec[((size_t)b * pp.paths + i) * kRecFloat4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
 }
 
 template <int RING>
@@ -138,10 +162,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
         const float4 r0 = rp[0], r1 = rp[1];
         TraceResult tr;
         if (count_traversal)
-            tr = traverse<RING, false, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+            tr = traverse<RING, false, true>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
                                               stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         else
-            tr = traverse<RING, false, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+            tr = traverse<RING, false, false>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
                                                stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
     }
@@ -153,8 +177,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
 }
 
 template <int RING>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal,
-                                                        uint32_t rec_bounces, uint32_t npaths) {
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
+                                                        uint32_t bounce, uint32_t count_traversal) {
+    const uint32_t rec_bounces = ppd->record ? ppd->rec_bounces : 0u, npaths = ppd->paths;
     __shared__ int32_t stack_mem[RING * kTraceBlock];
     int32_t* spill = pb.spill + (size_t)(xcd_block_id() * kTraceBlock + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[kCountSlots + bounce % kCountSlots];
@@ -166,10 +191,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers
         const float4 r0 = rp[0], r1 = rp[1];
         TraceResult tr;
         if (count_traversal)
-            tr = traverse<RING, true, true>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+            tr = traverse<RING, true, true>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
                                              stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         else
-            tr = traverse<RING, true, false>(sc.nodes, sc.tris, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w,
+            tr = traverse<RING, true, false>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
                                               stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
         if (tr.tri < 0) {
             const uint32_t pid = pb.shadow_pid[i];
@@ -203,7 +228,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, PathBuffers
 // the guiding network's lobes (probability guide_prob), and the combined density replaces
 // the BSDF pdf in both MIS weights against NEE, which keeps the estimator unbiased.
 template <bool GUIDED>
-__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams pp, PathBuffers pb, uint32_t bounce) {
+__global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
+                                                       uint32_t bounce) {
+    const PassParams pp = *ppd;
     const uint32_t n = pb.counts[bounce % kCountSlots];
     const uint32_t qin = bounce & 1u, qout = qin ^ 1u;
     const uint32_t stride = gridDim.x * kShadeBlock;
@@ -214,6 +241,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
         V3 ext_o = mk(0.0f, 0.0f, 0.0f), ext_d = mk(0.0f, 0.0f, 0.0f);
         V3 sh_o = mk(0.0f, 0.0f, 0.0f), sh_d = mk(0.0f, 0.0f, 0.0f);
         float sh_t = 0.0f;
+        uint32_t root = 0;
         RGB sh_c = rgb1(0.0f);
         if (i < n) {
             pid = pb.ext_pid[qin][i];
@@ -275,11 +303,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                 const uint32_t flags = __float_as_uint(pb.prevP[pid].w);
                 if (flags & kFlagDieAfterEmission) stop = true;
             }
-            if (!stop && bounce >= pp.max_bounces) stop = true;
+            const PassSeg sg = pp.segs[pb.path_seg[pid]];
+            const JobDev& job = sc.jobs[sg.job];
+            root = job.root;
+            if (!stop && bounce >= job.max_bounces) stop = true;
             if (!stop) {
                 uint32_t pixel, sample;
-                path_ids(pp, pid, &pixel, &sample);
-                const uint64_t key = path_key(pp.seed, pixel, sample, pp.npix);
+                path_ids(sg, pid, &pixel, &sample);
+                const uint64_t key = path_key(job.seed, pixel, sample, job.npix);
                 Stream s = stream_resume(key, pb.rng[pid]);
                 float u[GUIDED ? 12 : 8];
 #pragma unroll
@@ -294,9 +325,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
                     float rrScale = T4.w;
                     const V3 V = dir * -1.0f;
                     // ---- NEE (TDirectLightEstimator.ipp:70-102, BNEEPTEstimator.cpp:103-149)
-                    if (sc.num_emitters > 0) {
-                        const uint32_t ei = cdf_sample(sc.emit_cdf, sc.num_emitters, u[0]);
-                        const EmitterRec em = sc.emitters[ei];
+                    if (job.num_emitters > 0) {
+                        const uint32_t ei = cdf_sample(sc.emit_cdf + job.cdf_begin, job.num_emitters, u[0]);
+                        const EmitterRec em = sc.emitters[job.emitter_begin + ei];
                         uint32_t k = (uint32_t)(u[1] * (float)em.tri_count);
                         if (k >= em.tri_count) k = em.tri_count - 1u;
                         const EmitTri et = sc.emit_tris[em.tri_begin + k];
@@ -407,16 +438,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
         }
         const uint32_t slot_e = wave_push(&pb.counts[(bounce + 1) % kCountSlots], push_ext);
         if (push_ext) {
-            float4* rq = reinterpret_cast<float4*>(pb.ext[qout] + slot_e);
-            rq[0] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0f);
-            rq[1] = make_float4(ext_d.x, ext_d.y, ext_d.z, kFltMax);
+            queue_ray(pb.ext[qout], slot_e, ext_o, root, ext_d, kFltMax);
             pb.ext_pid[qout][slot_e] = pid;
         }
         const uint32_t slot_s = wave_push(&pb.counts[kCountSlots + bounce % kCountSlots], push_shadow);
         if (push_shadow) {
-            float4* rq = reinterpret_cast<float4*>(pb.shadow + slot_s);
-            rq[0] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
-            rq[1] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_t);
+            queue_ray(pb.shadow, slot_s, sh_o, root, sh_d, sh_t);
             pb.shadow_pid[slot_s] = pid;
             pb.shadow_c[slot_s] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.0f);
         }
@@ -425,17 +452,26 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, PassParams p
 
 // FP64 film accumulation, box filter, owner computes: each pixel sums its own samples of
 // this pass in sample order (HdrRgbFilm::addRgbSample, HdrRgbFilm.cpp:82-122).
-__global__ __launch_bounds__(kShadeBlock) void k_film(PassParams pp, PathBuffers pb, uint32_t spp_in_pass) {
-    const uint32_t p = blockIdx.x * kShadeBlock + threadIdx.x;
-    if (p >= pp.npix) return;
-    double2* f = reinterpret_cast<double2*>(pb.film + 8 * (size_t)p);
-    double2 f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
-    for (uint32_t s = 0; s < spp_in_pass; ++s) {
-        const float4 L = pb.L[(size_t)s * pp.npix + p];
-        f0.x += L.x; f0.y += L.y; f1.x += L.z; f1.y += 1.0;
-        f2.x += (double)L.x * L.x; f2.y += (double)L.y * L.y; f3.x += (double)L.z * L.z;
+__global__ __launch_bounds__(kShadeBlock) void k_film(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb) {
+    const uint32_t nseg = ppd->num_segs, total = ppd->film_threads;
+    const PassSeg* __restrict__ segs = ppd->segs;
+    for (uint32_t t = blockIdx.x * kShadeBlock + threadIdx.x; t < total; t += gridDim.x * kShadeBlock) {
+        uint32_t lo = 0, hi = nseg;  // segment of film thread t: last pix_offset <= t
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (segs[mid].pix_offset <= t) lo = mid; else hi = mid;
+        }
+        const PassSeg sg = segs[lo];
+        const uint32_t x = t - sg.pix_offset;
+        double2* f = reinterpret_cast<double2*>(pb.film + sc.jobs[sg.job].film_base + 8 * (size_t)x);
+        double2 f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
+        for (uint32_t s = 0; s < sg.samples; ++s) {
+            const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + x];
+            f0.x += L.x; f0.y += L.y; f1.x += L.z; f1.y += 1.0;
+            f2.x += (double)L.x * L.x; f2.y += (double)L.y * L.y; f3.x += (double)L.z * L.z;
+        }
+        f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
     }
-    f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
 }
 
 __global__ void k_clear_counts(uint32_t* counts, uint32_t n) {
@@ -475,8 +511,8 @@ hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n,
     return launch_intersect_t<kTraceRing>(sc, rays, n, hits, any, nvis, ntri, flags, spill, spill_cap, grid, st);
 }
 
-hipError_t launch_generate(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, hipStream_t st) {
-    hipLaunchKernelGGL(k_generate, dim3(blocks_for(pp.paths, kShadeBlock)), dim3(kShadeBlock), 0, st, sc, pp, pb);
+hipError_t launch_generate(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb);
     return hipGetLastError();
 }
 
@@ -486,16 +522,16 @@ hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bou
     return hipGetLastError();
 }
 
-hipError_t launch_shadow(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
-                         uint32_t rec_bounces, uint32_t npaths, hipStream_t st) {
-    hipLaunchKernelGGL(k_shadow<kTraceRing>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count, rec_bounces, npaths);
+hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+                         uint32_t count, hipStream_t st) {
+    hipLaunchKernelGGL(k_shadow<kTraceRing>, dim3(grid), dim3(kTraceBlock), 0, st, sc, ppd, pb, bounce, count);
     return hipGetLastError();
 }
 
-hipError_t launch_shade(const SceneDev& sc, const PassParams& pp, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
                         bool guided, hipStream_t st) {
-    if (guided) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
-    else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kShadeBlock), 0, st, sc, pp, pb, bounce);
+    if (guided) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb, bounce);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb, bounce);
     return hipGetLastError();
 }
 
@@ -513,8 +549,8 @@ hipError_t resident_blocks(uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks
     return hipSuccess;
 }
 
-hipError_t launch_film(const PassParams& pp, const PathBuffers& pb, uint32_t spp_in_pass, hipStream_t st) {
-    hipLaunchKernelGGL(k_film, dim3(blocks_for(pp.npix, kShadeBlock)), dim3(kShadeBlock), 0, st, pp, pb, spp_in_pass);
+hipError_t launch_film(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_film, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb);
     return hipGetLastError();
 }
 

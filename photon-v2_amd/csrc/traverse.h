@@ -44,7 +44,7 @@ struct TraversalStack {
 };
 
 template <int RING, bool ANY, bool COUNT>
-PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris,
+PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris, int32_t root,
                             V3 o, V3 d, float tmin, float tmax, int32_t* __restrict__ stk, uint32_t stride,
                             int32_t* __restrict__ spill, uint32_t spill_cap,
                             uint32_t* nvis, uint32_t* ntri, uint32_t* overflow) {
@@ -59,7 +59,7 @@ PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __
     float tb = tmax;
     TraversalStack<RING> st;
     st.ring = stk; st.stride = stride; st.spill = spill; st.spill_cap = spill_cap; st.sp = 0;
-    int32_t cur = 0;
+    int32_t cur = root;
     while (true) {
         if (cur >= 0) {
             const float4* p = reinterpret_cast<const float4*>(nodes + cur);

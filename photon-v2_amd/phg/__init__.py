@@ -29,6 +29,8 @@ GEN_CORNELL, GEN_PROCEDURAL, GEN_OCCLUDED_ROOM = 1, 2, 3
 QUERY_CLOSEST, QUERY_ANY = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
 RENDER_COUNT_TRAVERSAL, RENDER_GUIDED, RENDER_TRAIN = 1, 2, 4
+BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST_FIRST = 0, 1
+BATCH_COUNT_TRAVERSAL = 1
 GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -65,6 +67,19 @@ class phg_bvh_opts(C.Structure):
                 ("pad", C.c_uint32)]
 
 
+class phg_batch_job(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("pad", C.c_uint32 * 2)]
+
+
+class phg_batch_opts(C.Structure):
+    _fields_ = [("paths_per_pass", C.c_uint32), ("order", C.c_uint32), ("flags", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class phg_batch_result(C.Structure):
+    _fields_ = [("done_ms", C.c_double), ("paths", C.c_uint64)]
+
+
 class phg_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("spp_begin", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("spp_per_pass", C.c_uint32), ("seed", C.c_uint64),
@@ -82,7 +97,7 @@ class phg_stats(C.Structure):
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
                 ("trace_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
                 ("guide_infer_ms", C.c_double), ("guide_train_ms", C.c_double), ("guide_queries", C.c_uint64),
-                ("guide_records", C.c_uint64), ("guide_steps", C.c_uint32), ("pad2", C.c_uint32)]
+                ("guide_records", C.c_uint64), ("guide_steps", C.c_uint32), ("batch_passes", C.c_uint32)]
 
 
 class phg_gen_opts(C.Structure):
@@ -99,7 +114,8 @@ class phg_scene_host(C.Structure):
 EXPORTED = ["phg_create", "phg_destroy", "phg_last_error", "phg_upload_scene", "phg_intersect",
             "phg_intersect_counted", "phg_film_clear", "phg_film_read", "phg_render", "phg_get_stats",
             "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_gradients",
-            "phg_guide_train", "phg_scene_generate", "phg_scene_free"]
+            "phg_guide_train", "phg_scene_generate", "phg_scene_free", "phg_upload_scenes", "phg_render_batch",
+            "phg_batch_film_read"]
 
 _lib = None
 
@@ -132,6 +148,10 @@ def lib() -> C.CDLL:
     L.phg_scene_generate.argtypes = [C.POINTER(phg_gen_opts), C.POINTER(C.POINTER(phg_scene_host))]
     L.phg_scene_free.argtypes = [C.POINTER(phg_scene_host)]
     L.phg_scene_free.restype = None
+    L.phg_upload_scenes.argtypes = [vp, C.POINTER(phg_scene_desc), C.c_uint32, C.POINTER(phg_bvh_opts)]
+    L.phg_render_batch.argtypes = [vp, C.POINTER(phg_batch_job), C.c_uint32, C.POINTER(phg_batch_opts),
+                                   C.POINTER(phg_batch_result)]
+    L.phg_batch_film_read.argtypes = [vp, C.c_uint32, vp, C.c_uint64]
     for name in EXPORTED:
         if name not in ("phg_last_error", "phg_scene_free"):
             getattr(L, name).restype = C.c_int
@@ -269,6 +289,35 @@ class Context:
         self._check(self._L.phg_upload_scene(self._h, C.byref(d), C.byref(phg_bvh_opts(max_leaf_tris, sah_bins, nthreads, 0))),
                     "phg_upload_scene")
         self._scene_ref = scene
+
+    def upload_many(self, scenes, max_leaf_tris: int = 4, sah_bins: int = 16, nthreads: int = 0):
+        """Upload several scenes at once (phg_upload_scenes); jobs address them by index."""
+        descs = (phg_scene_desc * len(scenes))(*[s.desc() for s in scenes])
+        self._check(self._L.phg_upload_scenes(self._h, descs, len(scenes),
+                                              C.byref(phg_bvh_opts(max_leaf_tris, sah_bins, nthreads, 0))),
+                    "phg_upload_scenes")
+        self._scene_ref = list(scenes)
+        self._batch_npix = []
+
+    def render_batch(self, jobs, paths_per_pass: int = 0, order: int = BATCH_ORDER_GIVEN,
+                     count_traversal: bool = False):
+        """jobs: sequence of dicts with scene, width, height, spp, max_bounces, seed.
+        Returns per job (done_ms, paths)."""
+        arr = (phg_batch_job * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
+                                   j.get("seed", 0), (C.c_uint32 * 2)(0, 0))
+        res = (phg_batch_result * len(jobs))()
+        opts = phg_batch_opts(paths_per_pass, order, BATCH_COUNT_TRAVERSAL if count_traversal else 0, 0)
+        self._check(self._L.phg_render_batch(self._h, arr, len(jobs), C.byref(opts), res), "phg_render_batch")
+        self._batch_shapes = [(j["height"], j["width"]) for j in jobs]
+        return [(r.done_ms, r.paths) for r in res]
+
+    def batch_film(self, job: int) -> np.ndarray:
+        h, w = self._batch_shapes[job]
+        out = np.zeros((h, w, 8), np.float64)
+        self._check(self._L.phg_batch_film_read(self._h, job, out.ctypes.data, out.size), "phg_batch_film_read")
+        return out
 
     def intersect(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
         """rays: (N, 8) float32 -> hits (N, 4) viewed as t, prim, b, c (structured)."""

@@ -29,10 +29,10 @@ extern "C" int emu_intersect(const float* verts, uint32_t ntri, const float* ray
     for (uint32_t i = 0; i < n; ++i) {
         const float* r = rays + 8 * (size_t)i;
         uint32_t nv = 0, nt = 0;
-        TraceResult res = any ? traverse<kTraceRing, true, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]),
+        TraceResult res = any ? traverse<kTraceRing, true, true>(bvh.nodes.data(), tr.data(), 0, mk(r[0], r[1], r[2]),
                                                                 mk(r[4], r[5], r[6]), r[3], r[7], stack, 1, spill.data(),
                                                                 spill_cap, &nv, &nt, &overflow)
-                              : traverse<kTraceRing, false, true>(bvh.nodes.data(), tr.data(), mk(r[0], r[1], r[2]),
+                              : traverse<kTraceRing, false, true>(bvh.nodes.data(), tr.data(), 0, mk(r[0], r[1], r[2]),
                                                                  mk(r[4], r[5], r[6]), r[3], r[7], stack, 1, spill.data(),
                                                                  spill_cap, &nv, &nt, &overflow);
         out_prim[i] = res.tri >= 0 ? tr[res.tri].prim : -1;

@@ -1,0 +1,66 @@
+"""Batched rendering (phg_render_batch): several scenes and many jobs share wavefront passes.
+
+Bar: every job's FP64 film is bit-identical to rendering that job alone with phg_render —
+the sample streams depend only on (seed, pixel, sample) and each pixel accumulates its
+samples in order, so how jobs are packed into passes must not change a single bit."""
+import numpy as np
+import pytest
+
+import phg
+
+pytestmark = pytest.mark.gpu
+
+
+def scenes():
+    return [phg.Scene.generate(phg.GEN_CORNELL),
+            phg.Scene.generate(phg.GEN_PROCEDURAL, seed=5, target_tris=30_000, num_instances=4, num_lights=3),
+            phg.Scene.generate(phg.GEN_OCCLUDED_ROOM, seed=3, target_tris=20_000)]
+
+
+JOBS = [dict(scene=0, width=64, height=48, spp=9, max_bounces=5, seed=11),
+        dict(scene=1, width=96, height=64, spp=5, max_bounces=8, seed=12),
+        dict(scene=2, width=40, height=40, spp=17, max_bounces=10000, seed=13),
+        dict(scene=0, width=33, height=17, spp=3, max_bounces=2, seed=14),
+        dict(scene=1, width=200, height=150, spp=2, max_bounces=4, seed=15)]
+
+
+def single(scene, job):
+    with phg.Context(0) as c:
+        c.upload(scene)
+        c.film_clear(job["width"], job["height"])
+        c.render(job["spp"], max_bounces=job["max_bounces"], seed=job["seed"])
+        return c.film()
+
+
+@pytest.mark.parametrize("paths_per_pass,order", [(0, phg.BATCH_ORDER_GIVEN), (20_000, phg.BATCH_ORDER_GIVEN),
+                                                  (20_000, phg.BATCH_ORDER_SHORTEST_FIRST)])
+def test_batch_films_equal_single_renders(paths_per_pass, order):
+    sc = scenes()
+    print(f"[batch] case paths_per_pass={paths_per_pass} order={order} jobs={len(JOBS)}", flush=True)
+    with phg.Context(0) as c:
+        c.upload_many(sc)
+        res = c.render_batch(JOBS, paths_per_pass=paths_per_pass, order=order)
+        films = [c.batch_film(i) for i in range(len(JOBS))]
+        st = c.stats()
+    print(f"[batch] passes={st['batch_passes']} done_ms={[round(r[0], 2) for r in res]}", flush=True)
+    for i, job in enumerate(JOBS):
+        assert res[i][1] == job["width"] * job["height"] * job["spp"]
+        ref = single(sc[job["scene"]], job)
+        assert np.array_equal(films[i], ref), f"job {i} differs from its single render"
+    if paths_per_pass:
+        # the 200x150 job exceeds a pass: one sample per pass
+        assert st["batch_passes"] >= 2
+
+
+def test_batch_reports_completion_order():
+    sc = scenes()[:2]
+    jobs = [dict(scene=1, width=256, height=256, spp=8, max_bounces=8, seed=1),
+            dict(scene=0, width=32, height=32, spp=4, max_bounces=5, seed=2)]
+    with phg.Context(0) as c:
+        c.upload_many(sc)
+        given = c.render_batch(jobs, paths_per_pass=1 << 17)
+        sjf = c.render_batch(jobs, paths_per_pass=1 << 17, order=phg.BATCH_ORDER_SHORTEST_FIRST)
+    print(f"[batch] given={given} shortest_first={sjf}", flush=True)
+    # shortest first finishes the small job in the first pass
+    assert sjf[1][0] <= sjf[0][0]
+    assert all(r[0] > 0 for r in given + sjf)
