@@ -12,6 +12,9 @@ Reported beside the throughput:
                 launch durations, against the 8 TB/s HBM peak.
   cpu_baseline  the CPU oracle (scalar C++ restatement of the reference path) on this
                 host's cores, over a bounded sample of the same image (every k-th row).
+  batch         48 independent jobs (procedural scenes of 1e4..2e6 triangles, 128^2..1024^2,
+                4..256 spp, log-uniform, depth 8) rendered by phg_render_batch in shared passes:
+                aggregate Msamples/s and per-job completion p50 / p99.
   convergence   time to relMSE 0.01 on the occluded-light room (~199k triangles, light behind
                 a wall with a 0.1-wide slit, GGX conductors), guided vs plain BNEEPT: both
                 render progressively against a plain reference of the same scene; the guided
@@ -59,6 +62,39 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor()
+
+
+def batch_leg(args):
+    """Many independent render jobs in shared wavefront passes."""
+    import phg
+    from phg import workloads as wl
+
+    specs, jobs = wl.batch_workload(args.batch_jobs, seed=0)
+    t0 = time.time()
+    scenes = wl.generate_scenes(specs)
+    gen_s = time.time() - t0
+    with phg.Context(0) as ctx:
+        t0 = time.time()
+        ctx.upload_many(scenes)
+        up_s = time.time() - t0
+        ntris = sum(s.num_tris for s in scenes)
+        del scenes
+        print(f"[bench] batch: {len(jobs)} jobs, {ntris} triangles, scenes {gen_s:.1f}s, upload+BVH {up_s:.1f}s",
+              file=sys.stderr, flush=True)
+        ctx.render_batch(jobs)                      # warm-up (same work)
+        t0 = time.perf_counter()
+        res = ctx.render_batch(jobs)
+        wall = time.perf_counter() - t0
+        st = ctx.stats()
+    paths = sum(p for _, p in res)
+    done = [d for d, _ in res]
+    print(f"[bench] batch: {paths / 1e6:.0f}M paths in {wall:.2f}s, {st['batch_passes']} passes", file=sys.stderr,
+          flush=True)
+    return {"jobs": len(jobs), "triangles": ntris, "paths": paths, "seconds": round(wall, 4),
+            "msamples_per_s": round(paths / wall / 1e6, 3), "mrays_per_s": round(
+                (st["rays_primary"] + st["rays_extension"] + st["rays_shadow"]) / wall / 1e6, 3),
+            "p50_done_ms": round(wl.percentile(done, 50), 2), "p99_done_ms": round(wl.percentile(done, 99), 2),
+            "passes": st["batch_passes"], "paths_per_pass": 4 << 20, "order": "given", "max_bounces": 8}
 
 
 def convergence_leg(args):
@@ -113,6 +149,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-conv", action="store_true", help="skip the guided-vs-plain convergence leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
+    ap.add_argument("--batch-jobs", type=int, default=48)
     ap.add_argument("--conv-res", type=int, default=256)
     ap.add_argument("--conv-ref-spp", type=int, default=16384)
     ap.add_argument("--conv-max-spp", type=int, default=8192)
@@ -190,6 +228,7 @@ def main():
                "sample": f"every {step}th row of the same {W}x{H} image at {spp} spp ({cst['paths']} paths, "
                          f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}"}
 
+    batch = None if args.no_batch else batch_leg(args)
     conv = None if args.no_conv else convergence_leg(args)
 
     out = {
@@ -220,6 +259,7 @@ def main():
                      "bytes_per_launch": round(bytes_closest / max(trace_launch, 1)),
                      "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
         "cpu_baseline": cpu,
+        "batch": batch,
         "convergence": conv,
         "image_mean": [round(float(x), 6) for x in img.mean(axis=(0, 1))],
     }
