@@ -139,7 +139,9 @@ typedef struct phg_render_opts {
 enum {
     PHG_RENDER_COUNT_TRAVERSAL = 1,  /* count node visits / triangle tests (slower) */
     PHG_RENDER_GUIDED = 2,           /* sample directions from the guiding network too (one-sample MIS with the BSDF) */
-    PHG_RENDER_TRAIN = 4             /* train the guiding network online on this render's radiance samples */
+    PHG_RENDER_TRAIN = 4,            /* train the guiding network online on this render's radiance samples */
+    PHG_RENDER_GRAPH = 8             /* unguided, max_bounces < 62: each pass replays a captured HIP graph of the
+                                        whole bounce loop (no host round trip per bounce); same results */
 };
 
 typedef struct phg_stats {
@@ -170,6 +172,8 @@ typedef struct phg_stats {
     uint64_t guide_records;       /* training records used (last render) */
     uint32_t guide_steps;         /* Adam steps taken so far on this context */
     uint32_t batch_passes;        /* wavefront passes of the last phg_render_batch */
+    double graph_ms;              /* GPU time of the replayed pass graphs (PHG_RENDER_GRAPH / PHG_BATCH_GRAPH);
+                                     trace/shadow/shade times then cover only eager work */
 } phg_stats;
 
 /* Context lifetime.  Replaces nothing in the reference: the CPU engine needs no device. */
@@ -230,12 +234,12 @@ typedef struct phg_batch_job {
 } phg_batch_job;
 
 enum { PHG_BATCH_ORDER_GIVEN = 0, PHG_BATCH_ORDER_SHORTEST_FIRST = 1 };
-enum { PHG_BATCH_COUNT_TRAVERSAL = 1 };
+enum { PHG_BATCH_COUNT_TRAVERSAL = 1, PHG_BATCH_GRAPH = 2 /* as PHG_RENDER_GRAPH */ };
 
 typedef struct phg_batch_opts {
     uint32_t paths_per_pass;   /* paths in flight per pass (0 = 4M); passes carry whole samples */
     uint32_t order;            /* PHG_BATCH_ORDER_*: which jobs the passes take first */
-    uint32_t flags;            /* PHG_BATCH_COUNT_TRAVERSAL */
+    uint32_t flags;            /* PHG_BATCH_COUNT_TRAVERSAL | PHG_BATCH_GRAPH */
     uint32_t pad;
 } phg_batch_opts;
 

@@ -110,6 +110,7 @@ struct Ctx {
     do {                                                                                          \
         const hipError_t e_ = (expr);                                                             \
         if (e_ != hipSuccess) {                                                                   \
+            (void)hipGetLastError(); /* do not leave the error to the next launch check */        \
             (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                       \
             return e_ == hipErrorOutOfMemory ? PHG_ERR_OUT_OF_MEMORY : PHG_ERR_HIP;               \
         }                                                                                         \
@@ -626,6 +627,7 @@ namespace {
 
 struct RenderMode {
     bool count_traversal = false, guided = false, train = false;
+    bool graph = false;           // replay a captured bounce loop per pass (bounded depth, unguided)
     float guide_prob = 0.5f, guide_lr = 1e-3f;
     uint32_t guide_batch = 1u << 18, guide_steps = 4;
 };
@@ -715,6 +717,41 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     uint32_t n_trace = 0, n_shadow = 0;
     const uint32_t count_trav = m.count_traversal ? 1u : 0u;
     const PassParams* ppd = c->d_pass.p;
+    // Captured pass: generate, max_b + 1 bounce iterations (extend, shade, shadow; a bounce
+    // whose queue is empty costs three near-empty launches), film.  Replayed once per pass
+    // after the pass parameters are copied; timed as a whole (no per-kernel times).
+    hipGraphExec_t gexec = nullptr;
+    const bool use_graph = m.graph && !guided && !train && max_b + 2 < kCountSlots;
+    if (use_graph) {
+        hipGraph_t graph = nullptr;
+        PHG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = launch_generate(sc, ppd, pb, grid_shade, c->stream);
+        for (uint32_t b = 0; b <= max_b && ce == hipSuccess; ++b) {
+            ce = launch_extend(sc, pb, b, grid_trace, count_trav, c->stream);
+            if (ce == hipSuccess) ce = launch_shade(sc, ppd, pb, b, grid_shade, false, c->stream);
+            if (ce == hipSuccess) ce = launch_shadow(sc, ppd, pb, b, grid_trace, count_trav, c->stream);
+        }
+        if (ce == hipSuccess) ce = launch_film(sc, ppd, pb, grid_shade, c->stream);
+        const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            c->err = std::string("bounce-loop capture: ") + hipGetErrorString(ce != hipSuccess ? ce : ee);
+            return PHG_ERR_HIP;
+        }
+        const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            (void)hipGetLastError();
+            c->err = std::string("bounce-loop graph: ") + hipGetErrorString(ie);
+            return PHG_ERR_HIP;
+        }
+    }
+    double t_graph = 0;
+    struct GraphGuard {
+        hipGraphExec_t g;
+        ~GraphGuard() { if (g) (void)hipGraphExecDestroy(g); }
+    } graph_guard{gexec};
     const auto wall0 = std::chrono::steady_clock::now();
     for (const PassPlan& pl : passes) {
         PassParams pp{};
@@ -732,6 +769,21 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         PHG_HIP(c, hipMemcpyAsync(c->d_segs.p, c->h_segs, pl.segs.size() * sizeof(PassSeg), hipMemcpyHostToDevice, c->stream));
         rays_primary += pp.paths;
         paths_total += pp.paths;
+        if (gexec) {
+            PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+            PHG_HIP(c, hipGraphLaunch(gexec, c->stream));
+            PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+            PHG_HIP(c, hipStreamSynchronize(c->stream));
+            float ms = 0;
+            PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            t_graph += ms;
+            n_trace += max_b + 1; n_shadow += max_b + 1;
+            if (job_done_ms) {
+                const double now = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+                for (const PassSeg& sg : pl.segs) (*job_done_ms)[sg.job] = now;
+            }
+            continue;
+        }
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
         PHG_HIP(c, launch_generate(sc, ppd, pb, grid_shade, c->stream));
         PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
@@ -823,6 +875,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     c->stats.trace_ms = t_trace;
     c->stats.shadow_ms = t_shadow;
     c->stats.shade_ms = t_shade;
+    c->stats.graph_ms = t_graph;
     if (flags) return set_err(c, PHG_ERR_UNSUPPORTED, "traversal stack overflow");
     return PHG_OK;
 }
@@ -879,6 +932,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     m.count_traversal = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) != 0;
     m.guided = (o->flags & PHG_RENDER_GUIDED) != 0;
     m.train = (o->flags & PHG_RENDER_TRAIN) != 0;
+    m.graph = (o->flags & PHG_RENDER_GRAPH) != 0;
     m.guide_prob = o->guide_prob; m.guide_lr = o->guide_lr; m.guide_batch = o->guide_batch; m.guide_steps = o->guide_steps;
     c->g_scene = c->scenes[0].box;
     return render_passes(c, jobs, passes, c->film.p, m, nullptr);
@@ -944,6 +998,7 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
     c->bfilm_jobs.assign(jobs.begin(), jobs.end());
     RenderMode m;
     m.count_traversal = bo && (bo->flags & PHG_BATCH_COUNT_TRAVERSAL);
+    m.graph = bo && (bo->flags & PHG_BATCH_GRAPH);
     std::vector<double> done(n, 0.0);
     const phg_status st = render_passes(c, jobs, passes, c->bfilm.p, m, &done);
     if (st != PHG_OK) return st;

@@ -28,9 +28,9 @@ MAT_ABSORBER, MAT_LAMBERT, MAT_CONDUCTOR, MAT_DIELECTRIC = 0, 1, 2, 3
 GEN_CORNELL, GEN_PROCEDURAL, GEN_OCCLUDED_ROOM = 1, 2, 3
 QUERY_CLOSEST, QUERY_ANY = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
-RENDER_COUNT_TRAVERSAL, RENDER_GUIDED, RENDER_TRAIN = 1, 2, 4
+RENDER_COUNT_TRAVERSAL, RENDER_GUIDED, RENDER_TRAIN, RENDER_GRAPH = 1, 2, 4, 8
 BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST_FIRST = 0, 1
-BATCH_COUNT_TRAVERSAL = 1
+BATCH_COUNT_TRAVERSAL, BATCH_GRAPH = 1, 2
 GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -97,7 +97,8 @@ class phg_stats(C.Structure):
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
                 ("trace_launches", C.c_uint32), ("shadow_launches", C.c_uint32),
                 ("guide_infer_ms", C.c_double), ("guide_train_ms", C.c_double), ("guide_queries", C.c_uint64),
-                ("guide_records", C.c_uint64), ("guide_steps", C.c_uint32), ("batch_passes", C.c_uint32)]
+                ("guide_records", C.c_uint64), ("guide_steps", C.c_uint32), ("batch_passes", C.c_uint32),
+                ("graph_ms", C.c_double)]
 
 
 class phg_gen_opts(C.Structure):
@@ -300,7 +301,7 @@ class Context:
         self._batch_npix = []
 
     def render_batch(self, jobs, paths_per_pass: int = 0, order: int = BATCH_ORDER_GIVEN,
-                     count_traversal: bool = False):
+                     count_traversal: bool = False, graph: bool = False):
         """jobs: sequence of dicts with scene, width, height, spp, max_bounces, seed.
         Returns per job (done_ms, paths)."""
         arr = (phg_batch_job * len(jobs))()
@@ -308,7 +309,8 @@ class Context:
             arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
                                    j.get("seed", 0), (C.c_uint32 * 2)(0, 0))
         res = (phg_batch_result * len(jobs))()
-        opts = phg_batch_opts(paths_per_pass, order, BATCH_COUNT_TRAVERSAL if count_traversal else 0, 0)
+        opts = phg_batch_opts(paths_per_pass, order, (BATCH_COUNT_TRAVERSAL if count_traversal else 0) |
+                              (BATCH_GRAPH if graph else 0), 0)
         self._check(self._L.phg_render_batch(self._h, arr, len(jobs), C.byref(opts), res), "phg_render_batch")
         self._batch_shapes = [(j["height"], j["width"]) for j in jobs]
         return [(r.done_ms, r.paths) for r in res]
@@ -349,9 +351,10 @@ class Context:
 
     def render(self, spp: int, max_bounces: int = 10000, seed: int = 0, spp_begin: int = 0,
                spp_per_pass: int = 0, count_traversal: bool = False, guided: bool = False, train: bool = False,
-               guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4):
+               guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4,
+               graph: bool = False):
         flags = (RENDER_COUNT_TRAVERSAL if count_traversal else 0) | (RENDER_GUIDED if guided else 0) | \
-            (RENDER_TRAIN if train else 0)
+            (RENDER_TRAIN if train else 0) | (RENDER_GRAPH if graph else 0)
         o = phg_render_opts(self.width, self.height, spp, spp_begin, max_bounces, spp_per_pass, seed, flags,
                             guide_prob, guide_lr, guide_batch, guide_steps, 0)
         self._check(self._L.phg_render(self._h, C.byref(o)), "phg_render")

@@ -64,3 +64,47 @@ def test_batch_reports_completion_order():
     # shortest first finishes the small job in the first pass
     assert sjf[1][0] <= sjf[0][0]
     assert all(r[0] > 0 for r in given + sjf)
+
+
+def test_captured_bounce_loop_equals_eager():
+    """PHG_RENDER_GRAPH replays one captured pass (generate, every bounce, film) per pass;
+    its films must equal the eager loop's bit for bit, replay after replay, with eager
+    renders and synchronisations in between."""
+    sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=9, target_tris=50_000, num_instances=6, num_lights=4)
+    W, H = 80, 60
+    with phg.Context(0) as c:
+        c.upload(sc)
+        films = []
+        for graph in (False, True, False, True):
+            c.film_clear(W, H)
+            c.render(24, max_bounces=8, seed=21, spp_per_pass=3, graph=graph)   # 8 passes = 8 replays
+            films.append(c.film())
+            st = c.stats()
+            print(f"[batch] graph={graph} trace_ms={st['trace_ms']:.3f} graph_ms={st['graph_ms']:.3f} "
+                  f"launches={st['trace_launches']}", flush=True)
+            assert (st["graph_ms"] if graph else st["trace_ms"]) > 0 and st["rays_extension"] > 0
+        for f in films[1:]:
+            assert np.array_equal(f, films[0])
+        # counters agree too
+        c.film_clear(W, H)
+        c.render(4, max_bounces=8, seed=21, count_traversal=True)
+        eager = c.stats()
+        c.film_clear(W, H)
+        c.render(4, max_bounces=8, seed=21, count_traversal=True, graph=True)
+        graph = c.stats()
+        for k in ("rays_extension", "rays_shadow", "trace_node_visits", "trace_tri_tests", "shadow_node_visits"):
+            assert eager[k] == graph[k], k
+
+
+def test_batch_graph_equals_eager():
+    sc = scenes()
+    jobs = [dict(j, max_bounces=min(j["max_bounces"], 12)) for j in JOBS]
+    with phg.Context(0) as c:
+        c.upload_many(sc)
+        c.render_batch(jobs, paths_per_pass=20_000)
+        eager = [c.batch_film(i) for i in range(len(jobs))]
+        c.render_batch(jobs, paths_per_pass=20_000, graph=True)
+        graph = [c.batch_film(i) for i in range(len(jobs))]
+        print(f"[batch] graph passes={c.stats()['batch_passes']}", flush=True)
+    for i in range(len(jobs)):
+        assert np.array_equal(eager[i], graph[i]), i
