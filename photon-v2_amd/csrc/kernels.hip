@@ -54,13 +54,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(SceneDev sc, const ph
     const uint32_t gtid = xcd_block_id() * kTraceBlock + threadIdx.x;
     const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
     uint32_t ovf = 0;
-    for (uint64_t i = gtid; i < n; i += stride) {
+    Traversal<RING, ANY, COUNT> tv;
+    tv.attach(stack_mem + threadIdx.x, kTraceBlock, spill + (size_t)gtid * spill_cap, spill_cap);
+    uint64_t i = gtid;
+    uint32_t nvis = 0, ntri = 0;
+    if (i < n) {
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
         const float4 r0 = rp[0], r1 = rp[1];
-        uint32_t nvis = 0, ntri = 0;
-        const TraceResult tr = traverse<RING, ANY, COUNT>(sc.nodes, sc.tris, 0, mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z),
-                                                          r0.w, r1.w, stack_mem + threadIdx.x, kTraceBlock,
-                                                          spill + (size_t)gtid * spill_cap, spill_cap, &nvis, &ntri, &ovf);
+        tv.begin(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w, 0);
+    }
+    while (i < n) {
+        if (!tv.step(sc.nodes, sc.tris, &nvis, &ntri, &ovf)) continue;
+        const TraceResult tr = tv.r;
         phg_hit h;
         if (tr.tri >= 0) {
             h.t = ANY ? 0.0f : tr.t;
@@ -72,6 +77,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect(SceneDev sc, const ph
         }
         hits[i] = h;
         if (COUNT) { nvis_out[i] = nvis; ntri_out[i] = ntri; }
+        // this lane's next ray
+        nvis = 0; ntri = 0;
+        i += stride;
+        if (i < n) {
+            const float4* rp = reinterpret_cast<const float4*>(rays + i);
+            const float4 r0 = rp[0], r1 = rp[1];
+            tv.begin(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), r0.w, r1.w, 0);
+        }
     }
     if (ovf) atomicOr(flags, 1u);
 }
@@ -143,6 +156,31 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
     }
 }
 
+// Every ray of a path queue, grid-stride over the lanes of the persistent grid: a lane starts
+// its next ray as soon as its current one finishes.  Queue rays are (origin, root node bits),
+// (direction, tmax) with tmin = 0; done(i, result) consumes each finished ray.
+template <int RING, bool ANY, bool COUNT, typename Done>
+__device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* __restrict__ queue, uint32_t n,
+                                            int32_t* stk, int32_t* spill, uint32_t spill_cap, uint32_t* nvis,
+                                            uint32_t* ntri, uint32_t* ovf, Done done) {
+    const uint32_t stride = gridDim.x * kTraceBlock;
+    uint32_t i = xcd_block_id() * kTraceBlock + threadIdx.x;
+    Traversal<RING, ANY, COUNT> tv;
+    tv.attach(stk, kTraceBlock, spill, spill_cap);
+    auto start = [&](uint32_t k) {
+        const float4* rp = reinterpret_cast<const float4*>(queue + k);
+        const float4 r0 = rp[0], r1 = rp[1];
+        tv.begin(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w, (int32_t)__float_as_uint(r0.w));
+    };
+    if (i < n) start(i);
+    while (i < n) {
+        if (!tv.step(sc.nodes, sc.tris, nvis, ntri, ovf)) continue;
+        done(i, tv.r);
+        i += stride;
+        if (i < n) start(i);
+    }
+}
+
 template <int RING>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
     __shared__ int32_t stack_mem[RING * kTraceBlock];
@@ -155,20 +193,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
         if (bounce > 0) atomicAdd(&pb.traversal_counters[4], (unsigned long long)n);
     }
     const uint32_t q = bounce & 1u;
-    const uint32_t stride = gridDim.x * kTraceBlock;
     uint32_t nvis = 0, ntri = 0, ovf = 0;
-    for (uint32_t i = xcd_block_id() * kTraceBlock + threadIdx.x; i < n; i += stride) {
-        const float4* rp = reinterpret_cast<const float4*>(pb.ext[q] + i);
-        const float4 r0 = rp[0], r1 = rp[1];
-        TraceResult tr;
-        if (count_traversal)
-            tr = traverse<RING, false, true>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
-                                              stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
-        else
-            tr = traverse<RING, false, false>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
-                                               stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
-        pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
-    }
+    if (count_traversal)
+        trace_queue<RING, false, true>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+                                       [&](uint32_t i, const TraceResult& tr) {
+                                           pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
+                                       });
+    else
+        trace_queue<RING, false, false>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+                                        [&](uint32_t i, const TraceResult& tr) {
+                                            pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
+                                        });
     if (count_traversal) {
         atomicAdd(&pb.traversal_counters[0], (unsigned long long)nvis);
         atomicAdd(&pb.traversal_counters[1], (unsigned long long)ntri);
@@ -183,19 +218,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, const PassP
     __shared__ int32_t stack_mem[RING * kTraceBlock];
     int32_t* spill = pb.spill + (size_t)(xcd_block_id() * kTraceBlock + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[kCountSlots + bounce % kCountSlots];
-    const uint32_t stride = gridDim.x * kTraceBlock;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&pb.traversal_counters[5], (unsigned long long)n);
     uint32_t nvis = 0, ntri = 0, ovf = 0;
-    for (uint32_t i = xcd_block_id() * kTraceBlock + threadIdx.x; i < n; i += stride) {
-        const float4* rp = reinterpret_cast<const float4*>(pb.shadow + i);
-        const float4 r0 = rp[0], r1 = rp[1];
-        TraceResult tr;
-        if (count_traversal)
-            tr = traverse<RING, true, true>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
-                                             stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
-        else
-            tr = traverse<RING, true, false>(sc.nodes, sc.tris, (int32_t)__float_as_uint(r0.w), mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w,
-                                              stack_mem + threadIdx.x, kTraceBlock, spill, pb.spill_cap, &nvis, &ntri, &ovf);
+    auto unoccluded = [&](uint32_t i, const TraceResult& tr) {
         if (tr.tri < 0) {
             const uint32_t pid = pb.shadow_pid[i];
             const float4 c = pb.shadow_c[i];
@@ -210,7 +235,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, const PassP
                 *lb = v;
             }
         }
-    }
+    };
+    if (count_traversal)
+        trace_queue<RING, true, true>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+                                      unoccluded);
+    else
+        trace_queue<RING, true, false>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+                                       unoccluded);
     if (count_traversal) {
         atomicAdd(&pb.traversal_counters[2], (unsigned long long)nvis);
         atomicAdd(&pb.traversal_counters[3], (unsigned long long)ntri);

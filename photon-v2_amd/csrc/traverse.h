@@ -43,24 +43,39 @@ struct TraversalStack {
     }
 };
 
+// Traversal of one ray as a resumable state machine: begin() sets the ray up, each step()
+// visits one inner node or one leaf and returns true once the ray is finished (result in r).
+// Kernels that stream many rays per lane start the lane's next ray as soon as the current one
+// finishes, instead of waiting for the slowest ray of the wave.
 template <int RING, bool ANY, bool COUNT>
-PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris, int32_t root,
-                            V3 o, V3 d, float tmin, float tmax, int32_t* __restrict__ stk, uint32_t stride,
-                            int32_t* __restrict__ spill, uint32_t spill_cap,
-                            uint32_t* nvis, uint32_t* ntri, uint32_t* overflow) {
-    static_assert((RING & (RING - 1)) == 0, "ring size must be a power of two");
-    TraceResult r; r.t = 0.0f; r.tri = -1; r.a = 0.0f; r.b = 0.0f; r.c = 0.0f;
-    const V3 rcp = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    const bool nx = d.x < 0.0f, ny = d.y < 0.0f, nz = d.z < 0.0f;
-    constexpr float mult = 1.1920928955078125e-07f * 2.0f + 1.0f;  // TAABB3D.ipp:417
-    // dominant axis of the direction, chosen as the watertight test permutes it
-    const float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
-    const int kdom = adx > ady ? (adx > adz ? 0 : 2) : (ady > adz ? 1 : 2);
-    float tb = tmax;
+struct Traversal {
+    V3 o, d, rcp;
+    float tmin, tb;
+    bool nx, ny, nz;
+    int kdom;
+    int32_t cur;
+    TraceResult r;
     TraversalStack<RING> st;
-    st.ring = stk; st.stride = stride; st.spill = spill; st.spill_cap = spill_cap; st.sp = 0;
-    int32_t cur = root;
-    while (true) {
+
+    PHG_HD void attach(int32_t* stk, uint32_t stride, int32_t* spill, uint32_t spill_cap) {
+        st.ring = stk; st.stride = stride; st.spill = spill; st.spill_cap = spill_cap; st.sp = 0;
+    }
+
+    PHG_HD void begin(V3 o_, V3 d_, float tmin_, float tmax_, int32_t root) {
+        o = o_; d = d_; tmin = tmin_; tb = tmax_;
+        rcp = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        nx = d.x < 0.0f; ny = d.y < 0.0f; nz = d.z < 0.0f;
+        // dominant axis of the direction, chosen as the watertight test permutes it
+        const float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
+        kdom = adx > ady ? (adx > adz ? 0 : 2) : (ady > adz ? 1 : 2);
+        r.t = 0.0f; r.tri = -1; r.a = 0.0f; r.b = 0.0f; r.c = 0.0f;
+        st.sp = 0;
+        cur = root;
+    }
+
+    PHG_HD bool step(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris, uint32_t* nvis,
+                     uint32_t* ntri, uint32_t* overflow) {
+        constexpr float mult = 1.1920928955078125e-07f * 2.0f + 1.0f;  // TAABB3D.ipp:417
         if (cur >= 0) {
             const float4* p = reinterpret_cast<const float4*>(nodes + cur);
             const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
@@ -109,34 +124,48 @@ PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __
             }
 #undef PHG_CSWAP
             if (key[0] == INFINITY) {
-                if (st.sp == 0) break;
+                if (st.sp == 0) return true;
                 cur = st.pop();
-                continue;
+                return false;
             }
 #pragma unroll
             for (int k = 3; k >= 1; --k) {
                 if (key[k] != INFINITY) st.push(kid[k], overflow);
             }
             cur = kid[0];
-        } else {
-            const uint32_t first = (uint32_t)cur & ((1u << kLeafFirstBits) - 1u);
-            const uint32_t cnt = (((uint32_t)cur >> kLeafFirstBits) & 7u) + 1u;
-            if (COUNT) (*ntri) += cnt;
-            for (uint32_t i = 0; i < cnt; ++i) {
-                V3 a, b, c;
-                load_tri(tris, first + i, &a, &b, &c);
-                float t, ba, bb, bc;
-                if (wt_intersect(a, b, c, o, d, tmin, tb, &t, &ba, &bb, &bc)) {
-                    tb = t;
-                    r.t = t; r.tri = (int32_t)(first + i); r.a = ba; r.b = bb; r.c = bc;
-                    if (ANY) return r;
-                }
-            }
-            if (st.sp == 0) break;
-            cur = st.pop();
+            return false;
         }
+        const uint32_t first = (uint32_t)cur & ((1u << kLeafFirstBits) - 1u);
+        const uint32_t cnt = (((uint32_t)cur >> kLeafFirstBits) & 7u) + 1u;
+        if (COUNT) (*ntri) += cnt;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            V3 a, b, c;
+            load_tri(tris, first + i, &a, &b, &c);
+            float t, ba, bb, bc;
+            if (wt_intersect(a, b, c, o, d, tmin, tb, &t, &ba, &bb, &bc)) {
+                tb = t;
+                r.t = t; r.tri = (int32_t)(first + i); r.a = ba; r.b = bb; r.c = bc;
+                if (ANY) return true;
+            }
+        }
+        if (st.sp == 0) return true;
+        cur = st.pop();
+        return false;
     }
-    return r;
+};
+
+template <int RING, bool ANY, bool COUNT>
+PHG_HD TraceResult traverse(const WideNode* __restrict__ nodes, const TriRec* __restrict__ tris, int32_t root,
+                            V3 o, V3 d, float tmin, float tmax, int32_t* __restrict__ stk, uint32_t stride,
+                            int32_t* __restrict__ spill, uint32_t spill_cap,
+                            uint32_t* nvis, uint32_t* ntri, uint32_t* overflow) {
+    static_assert((RING & (RING - 1)) == 0, "ring size must be a power of two");
+    Traversal<RING, ANY, COUNT> tv;
+    tv.attach(stk, stride, spill, spill_cap);
+    tv.begin(o, d, tmin, tmax, root);
+    while (!tv.step(nodes, tris, nvis, ntri, overflow)) {
+    }
+    return tv.r;
 }
 
 
