@@ -275,7 +275,7 @@ def main():
                           "(the roofline's kernel times come from these steps)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic("phg::k_extend"),
-                     "traffic_source": "profiles/bneept_procedural1M_traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch)",
+                     "traffic_source": "profiles/bneept_procedural1M_traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch: bytes leaving L2, served by Infinity Cache or HBM)",
                      "kernel": "k_extend (closest-hit traversal)",
                      "timed_in": "the eager steps: HIP events on the library stream around each launch",
                      "launches": trace_launch, "avg_launch_ms": round(trace_ms / max(trace_launch, 1), 4),

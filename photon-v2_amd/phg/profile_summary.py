@@ -2,10 +2,12 @@
 
     python -m phg.profile_summary stats  <kernel_trace.db>
     python -m phg.profile_summary pmc    <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> <copy_bytes>
+    python -m phg.profile_summary l2     <tcc.db>
 
 `stats` prints per-kernel call counts and durations; `pmc` prints per-kernel FETCH_SIZE and
 WRITE_SIZE per dispatch, scaled by what the same counters read for a plain device copy of
-`copy_bytes` bytes, and emits a JSON line with the HBM traffic per launch of each kernel.
+`copy_bytes` bytes, and emits a JSON line with the HBM traffic per launch of each kernel;
+`l2` prints each kernel's L2 hit rate from TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum).
 """
 import json
 import re
@@ -62,6 +64,15 @@ def main(argv):
             out["kernels"][k] = {"dispatches": n, "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb}
             print(f"{k[:40]:40s} {n:10d} {fb / 1e6:10.2f} {wb / 1e6:10.2f} {(fb + wb) / 1e6:14.2f}")
         print(json.dumps(out))
+        return 0
+    if len(argv) == 2 and argv[0] == "l2":
+        hit, miss = counter(argv[1], "TCC_HIT_sum"), counter(argv[1], "TCC_MISS_sum")
+        print(f"{'kernel':40s} {'dispatches':>10s} {'hits/launch':>12s} {'misses/launch':>14s} {'L2 hit rate':>12s}")
+        for k in sorted(hit, key=lambda k: -(hit[k][1] + miss.get(k, (0, 0.0))[1])):
+            n, h = hit[k]
+            m = miss.get(k, (0, 0.0))[1]
+            rate = h / (h + m) if h + m > 0 else float("nan")
+            print(f"{k[:40]:40s} {n:10d} {h:12.0f} {m:14.0f} {rate:12.4f}")
         return 0
     print(__doc__)
     return 2
