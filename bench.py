@@ -78,20 +78,27 @@ def batch_leg(args):
     gen_s = time.time() - t0
     with phg.Context(0) as ctx:
         t0 = time.time()
-        ctx.upload_many(scenes)
+        best, trials = ctx.autotune_batch(scenes, jobs)
+        tune_s = time.time() - t0
+        untuned = next(t for t in trials if t["max_leaf_tris"] == 4 and t["width"] == 4 and t["trace_block"] == 128
+                       and t["paths_per_pass"] == 4 << 20)
+        print(f"[bench] batch autotune: {len(trials)} trials in {tune_s:.1f}s, best {best} "
+              f"(default {untuned['msamples_per_s']:.0f} Msamples/s on the sample)", file=sys.stderr, flush=True)
+        t0 = time.time()
+        ctx.upload_many(scenes, max_leaf_tris=best["max_leaf_tris"], width=best["width"])
         up_s = time.time() - t0
         ntris = sum(s.num_tris for s in scenes)
         del scenes
         print(f"[bench] batch: {len(jobs)} jobs, {ntris} triangles, scenes {gen_s:.1f}s, upload+BVH {up_s:.1f}s",
               file=sys.stderr, flush=True)
-        ctx.render_batch(jobs)                      # warm-up (same work)
+        ctx.render_batch(jobs, graph=True)          # warm-up (same work)
         t0 = time.perf_counter()
-        res = ctx.render_batch(jobs)
+        res = ctx.render_batch(jobs, graph=True)
         wall = time.perf_counter() - t0
         st = ctx.stats()
         t0 = time.perf_counter()
-        gres = ctx.render_batch(jobs, graph=True)
-        gwall = time.perf_counter() - t0
+        eres = ctx.render_batch(jobs)
+        ewall = time.perf_counter() - t0
     paths = sum(p for _, p in res)
     done = [d for d, _ in res]
     print(f"[bench] batch: {paths / 1e6:.0f}M paths in {wall:.2f}s, {st['batch_passes']} passes", file=sys.stderr,
@@ -100,10 +107,12 @@ def batch_leg(args):
             "msamples_per_s": round(paths / wall / 1e6, 3), "mrays_per_s": round(
                 (st["rays_primary"] + st["rays_extension"] + st["rays_shadow"]) / wall / 1e6, 3),
             "p50_done_ms": round(wl.percentile(done, 50), 2), "p99_done_ms": round(wl.percentile(done, 99), 2),
-            "passes": st["batch_passes"], "paths_per_pass": 4 << 20, "order": "given", "max_bounces": 8,
-            "graph": {"seconds": round(gwall, 4), "msamples_per_s": round(paths / gwall / 1e6, 3),
-                      "p50_done_ms": round(wl.percentile([d for d, _ in gres], 50), 2),
-                      "p99_done_ms": round(wl.percentile([d for d, _ in gres], 99), 2)}}
+            "passes": st["batch_passes"], "order": "given", "max_bounces": 8,
+            "mode": "captured-graph passes", "tuning": best, "autotune_trials": len(trials),
+            "autotune_seconds": round(tune_s, 2), "untuned_sample_msamples_per_s": round(untuned["msamples_per_s"], 3),
+            "eager": {"seconds": round(ewall, 4), "msamples_per_s": round(paths / ewall / 1e6, 3),
+                      "p50_done_ms": round(wl.percentile([d for d, _ in eres], 50), 2),
+                      "p99_done_ms": round(wl.percentile([d for d, _ in eres], 99), 2)}}
 
 
 def convergence_leg(args):

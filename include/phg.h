@@ -98,7 +98,7 @@ typedef struct phg_bvh_opts {
     uint32_t max_leaf_tris;   /* 1..8, default 4 */
     uint32_t sah_bins;        /* 8..64, default 16 */
     uint32_t nthreads;        /* host build threads, 0 = all cores */
-    uint32_t pad;
+    uint32_t width;           /* children per node, 2..4 (0 = 4); the node format stays 4-wide */
 } phg_bvh_opts;
 
 /* One ray: origin, minT, direction, maxT (TLineSegment, Math/Geometry/TLineSegment.ipp). */
@@ -255,6 +255,42 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* jobs, uint32_t n, co
                             phg_batch_result* results);
 /* The film of job `job` of the last batch: 8 doubles per pixel as phg_film_read. */
 phg_status phg_batch_film_read(void* ctx, uint32_t job, double* out, uint64_t out_len);
+
+/* Launch tuning of the path kernels (no counterpart in the reference).  trace_block: threads
+ * per block of the traversal kernels (64, 128 or 256; 0 = 128); paths_per_pass: paths in
+ * flight per wavefront pass (2^16..2^26; 0 = 4M), the default for phg_render passes and
+ * batches that do not set their own.  Results do not depend on either.  phg_get_tuning also
+ * reports the resident traversal blocks per compute unit that size the persistent grid. */
+typedef struct phg_tuning {
+    uint32_t trace_block;
+    uint32_t paths_per_pass;
+    uint32_t trace_blocks_per_cu;   /* out only */
+    uint32_t pad;
+} phg_tuning;
+phg_status phg_set_tuning(void* ctx, const phg_tuning* t);
+phg_status phg_get_tuning(void* ctx, phg_tuning* t);
+
+/* Autotuner for batched rendering (no counterpart in the reference).  Renders a sample of
+ * the jobs (up to sample_jobs of them, spread over their sizes, at most sample_spp samples
+ * each) over their scenes and times it: first every traversal block size x paths-per-pass
+ * candidate with the default BVH, then every BVH leaf size x width candidate with the best
+ * launch parameters.  On return the context holds the sample scenes (upload the real ones
+ * again with best.max_leaf_tris / best.width) and the best launch tuning.  trials (optional,
+ * up to max_trials entries) receives every measurement. */
+typedef struct phg_tune_trial {
+    uint32_t max_leaf_tris, width;      /* BVH */
+    uint32_t trace_block, paths_per_pass;
+    double msamples_per_s;              /* sample batch, captured-graph passes */
+} phg_tune_trial;
+typedef struct phg_autotune_opts {
+    uint32_t sample_jobs;      /* 0 = 8 */
+    uint32_t sample_spp;       /* 0 = 4 */
+    uint32_t repeats;          /* timed renders per candidate, best kept; 0 = 2 */
+    uint32_t pad;
+} phg_autotune_opts;
+phg_status phg_autotune_batch(void* ctx, const phg_scene_desc* scenes, uint32_t nscenes, const phg_batch_job* jobs,
+                              uint32_t njobs, const phg_autotune_opts* opts, phg_tune_trial* best,
+                              phg_tune_trial* trials, uint32_t max_trials, uint32_t* ntrials);
 
 /* Statistics of the last render/intersect (Renderer::asyncQueryRenderStats,
  * Core/Renderer/Sampling/EqualSamplingRenderer.cpp:258-280, reports paths/second). */

@@ -102,8 +102,10 @@ struct Ctx {
     phg_stats stats{};
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     uint32_t cu_count = 256;
+    uint32_t trace_block = 128;         // threads per block of the path traversal kernels
     uint32_t trace_blocks_per_cu = 8;   // resident blocks per compute unit, from the occupancy query
     uint32_t shade_blocks_per_cu = 4;
+    uint32_t paths_per_pass = 4u << 20; // paths in flight per wavefront pass
 };
 
 #define PHG_HIP(ctx, expr)                                                                        \
@@ -222,7 +224,7 @@ phg_status phg_create(void** out_ctx, const phg_device_opts* opts) {
         delete c;
         return PHG_ERR_HIP;
     }
-    if (resident_blocks(&c->trace_blocks_per_cu, &c->shade_blocks_per_cu) != hipSuccess) {
+    if (resident_blocks(c->trace_block, &c->trace_blocks_per_cu, &c->shade_blocks_per_cu) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return PHG_ERR_HIP;
@@ -476,6 +478,7 @@ static phg_status upload_scenes(Ctx* c, const phg_scene_desc* scenes, uint32_t n
     if (bo) {
         if (bo->max_leaf_tris) bp.max_leaf_tris = bo->max_leaf_tris;
         if (bo->sah_bins) bp.sah_bins = bo->sah_bins;
+        if (bo->width) bp.width = bo->width;
         bp.nthreads = bo->nthreads;
     }
     std::vector<HostScene> hs(n);
@@ -520,11 +523,12 @@ phg_status phg_upload_scenes(void* ctx, const phg_scene_desc* scenes, uint32_t n
 }
 
 static uint32_t trace_grid(const Ctx* c) { return c->cu_count * c->trace_blocks_per_cu; }
+static uint32_t trace_threads(const Ctx* c) { return trace_grid(c) * c->trace_block; }
 static uint32_t spill_cap(const Ctx* c) {
     return c->stack_need > (uint32_t)kTraceRing ? c->stack_need - (uint32_t)kTraceRing : 0u;
 }
 static phg_status ensure_spill(Ctx* c) {
-    const size_t n = (size_t)trace_grid(c) * kTraceBlock * std::max<uint32_t>(spill_cap(c), 1u);
+    const size_t n = (size_t)trace_threads(c) * std::max<uint32_t>(spill_cap(c), 1u);
     PHG_HIP(c, c->spill.ensure(n));
     return PHG_OK;
 }
@@ -563,7 +567,7 @@ static phg_status do_intersect(Ctx* c, const phg_ray* rays, uint64_t n, phg_hit*
         if (st != PHG_OK) return st;
     }
     PHG_HIP(c, launch_intersect(c->sc, d_rays, n, d_hits, query == PHG_QUERY_ANY, d_nvis, d_ntri, c->errflags.p,
-                                c->spill.p, spill_cap(c), trace_grid(c), c->stream));
+                                c->spill.p, spill_cap(c), trace_threads(c) / kTraceBlock, c->stream));
     PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
     if (ptr_kind == PHG_PTR_HOST) {
         PHG_HIP(c, hipMemcpyAsync(hits, d_hits, n * sizeof(phg_hit), hipMemcpyDeviceToHost, c->stream));
@@ -727,9 +731,9 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         PHG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t ce = launch_generate(sc, ppd, pb, grid_shade, c->stream);
         for (uint32_t b = 0; b <= max_b && ce == hipSuccess; ++b) {
-            ce = launch_extend(sc, pb, b, grid_trace, count_trav, c->stream);
+            ce = launch_extend(sc, pb, b, grid_trace, c->trace_block, count_trav, c->stream);
             if (ce == hipSuccess) ce = launch_shade(sc, ppd, pb, b, grid_shade, false, c->stream);
-            if (ce == hipSuccess) ce = launch_shadow(sc, ppd, pb, b, grid_trace, count_trav, c->stream);
+            if (ce == hipSuccess) ce = launch_shadow(sc, ppd, pb, b, grid_trace, c->trace_block, count_trav, c->stream);
         }
         if (ce == hipSuccess) ce = launch_film(sc, ppd, pb, grid_shade, c->stream);
         const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
@@ -792,7 +796,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         for (uint32_t b = 0;; ++b) {
             const bool use_guide = guided && pp.guide_prob > 0.0f && b < pl.max_bounces;
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-            PHG_HIP(c, launch_extend(sc, pb, b, grid_trace, count_trav, c->stream));
+            PHG_HIP(c, launch_extend(sc, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
             if (use_guide) {
                 PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], sc.tris, c->g_scene,
@@ -801,7 +805,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             PHG_HIP(c, hipEventRecord(c->ev[4], c->stream));
             PHG_HIP(c, launch_shade(sc, ppd, pb, b, grid_shade, guided || train, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
-            PHG_HIP(c, launch_shadow(sc, ppd, pb, b, grid_trace, count_trav, c->stream));
+            PHG_HIP(c, launch_shadow(sc, ppd, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[3], c->stream));
             uint32_t next[2] = {0, 0};
             PHG_HIP(c, hipMemcpyAsync(&next[0], c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -911,7 +915,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     if (o->spp == 0) return PHG_OK;
     // samples per pass: fill ~4M paths in flight
     uint32_t k = o->spp_per_pass;
-    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, (4ull << 20) / npix);
+    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, (uint64_t)c->paths_per_pass / npix);
     k = std::min(k, o->spp);
     if (npix * k > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
     std::vector<JobDev> jobs{make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0)};
@@ -943,7 +947,7 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
     Ctx* c = static_cast<Ctx*>(ctx);
     if (!c || !in || n == 0) return PHG_ERR_INVALID_ARG;
     if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
-    const uint64_t cap = bo && bo->paths_per_pass ? bo->paths_per_pass : (4ull << 20);
+    const uint64_t cap = bo && bo->paths_per_pass ? bo->paths_per_pass : (uint64_t)c->paths_per_pass;
     if (cap > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass too large");
     std::vector<JobDev> jobs(n);
     uint64_t film_doubles = 0;
@@ -1088,6 +1092,131 @@ phg_status phg_guide_train(void* ctx, const float* records, uint64_t n, float lr
     if (st != PHG_OK) return st;
     PHG_HIP(c, hipStreamSynchronize(c->stream));
     c->g_ready = true;
+    return PHG_OK;
+}
+
+phg_status phg_set_tuning(void* ctx, const phg_tuning* t) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !t) return PHG_ERR_INVALID_ARG;
+    const uint32_t block = t->trace_block ? t->trace_block : 128;
+    if (block != 64 && block != 128 && block != 256) return set_err(c, PHG_ERR_INVALID_ARG, "trace_block must be 64, 128 or 256");
+    if (t->paths_per_pass != 0 && (t->paths_per_pass < (1u << 16) || t->paths_per_pass > (1u << 26)))
+        return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass must be in [2^16, 2^26]");
+    PHG_HIP(c, hipSetDevice(c->device));
+    PHG_HIP(c, hipStreamSynchronize(c->stream));
+    uint32_t tb = 0, sb = 0;
+    PHG_HIP(c, resident_blocks(block, &tb, &sb));
+    c->trace_block = block;
+    c->trace_blocks_per_cu = tb;
+    c->shade_blocks_per_cu = sb;
+    c->paths_per_pass = t->paths_per_pass ? t->paths_per_pass : (4u << 20);
+    return PHG_OK;
+}
+
+phg_status phg_get_tuning(void* ctx, phg_tuning* t) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !t) return PHG_ERR_INVALID_ARG;
+    std::memset(t, 0, sizeof(*t));
+    t->trace_block = c->trace_block;
+    t->paths_per_pass = c->paths_per_pass;
+    t->trace_blocks_per_cu = c->trace_blocks_per_cu;
+    return PHG_OK;
+}
+
+phg_status phg_autotune_batch(void* ctx, const phg_scene_desc* scenes, uint32_t nscenes, const phg_batch_job* jobs,
+                              uint32_t njobs, const phg_autotune_opts* ao, phg_tune_trial* best, phg_tune_trial* trials,
+                              uint32_t max_trials, uint32_t* ntrials) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    if (!c || !scenes || nscenes == 0 || !jobs || njobs == 0 || !best) return PHG_ERR_INVALID_ARG;
+    for (uint32_t j = 0; j < njobs; ++j)
+        if (jobs[j].scene >= nscenes) return set_err(c, PHG_ERR_INVALID_ARG, "job scene index out of range");
+    const uint32_t want = ao && ao->sample_jobs ? ao->sample_jobs : 8;
+    const uint32_t sspp = ao && ao->sample_spp ? ao->sample_spp : 4;
+    const uint32_t reps = ao && ao->repeats ? ao->repeats : 2;
+    // sample: jobs spread evenly over the work-size order, with their scenes renumbered
+    std::vector<uint32_t> order(njobs);
+    for (uint32_t j = 0; j < njobs; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return (uint64_t)jobs[a].width * jobs[a].height * jobs[a].spp < (uint64_t)jobs[b].width * jobs[b].height * jobs[b].spp;
+    });
+    const uint32_t ns = std::min(want, njobs);
+    std::vector<phg_batch_job> sjobs;
+    std::vector<phg_scene_desc> sscenes;
+    std::vector<int64_t> remap(nscenes, -1);
+    for (uint32_t k = 0; k < ns; ++k) {
+        const uint32_t j = order[ns > 1 ? (uint64_t)k * (njobs - 1) / (ns - 1) : 0];
+        phg_batch_job b = jobs[j];
+        if (remap[b.scene] < 0) { remap[b.scene] = (int64_t)sscenes.size(); sscenes.push_back(scenes[b.scene]); }
+        b.scene = (uint32_t)remap[b.scene];
+        b.spp = std::min(b.spp, sspp);
+        sjobs.push_back(b);
+    }
+    uint64_t paths = 0;
+    for (const phg_batch_job& b : sjobs) paths += (uint64_t)b.width * b.height * b.spp;
+    std::vector<phg_tune_trial> all;
+    auto measure = [&](phg_tune_trial& t) -> phg_status {
+        phg_tuning tu{};
+        tu.trace_block = t.trace_block;
+        tu.paths_per_pass = t.paths_per_pass;
+        phg_status st = phg_set_tuning(c, &tu);
+        if (st != PHG_OK) return st;
+        phg_batch_opts bo{};
+        bo.flags = PHG_BATCH_GRAPH;
+        double best_ms = INFINITY;
+        for (uint32_t r = 0; r <= reps; ++r) {  // the first render warms up
+            const auto t0 = std::chrono::steady_clock::now();
+            st = phg_render_batch(c, sjobs.data(), (uint32_t)sjobs.size(), &bo, nullptr);
+            if (st != PHG_OK) return st;
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (r > 0) best_ms = std::min(best_ms, ms);
+        }
+        t.msamples_per_s = (double)paths / (best_ms * 1e3);
+        all.push_back(t);
+        return PHG_OK;
+    };
+    auto upload = [&](uint32_t leaf, uint32_t width) -> phg_status {
+        phg_bvh_opts bo{};
+        bo.max_leaf_tris = leaf;
+        bo.width = width;
+        return phg_upload_scenes(c, sscenes.data(), (uint32_t)sscenes.size(), &bo);
+    };
+    phg_tune_trial cur{4, 4, 128, 4u << 20, 0.0};
+    phg_status st = upload(cur.max_leaf_tris, cur.width);
+    if (st != PHG_OK) return st;
+    // stage 1: launch parameters on the default BVH
+    phg_tune_trial top{};
+    top.msamples_per_s = -1.0;
+    for (uint32_t block : {64u, 128u, 256u})
+        for (uint32_t ppp : {1u << 20, 2u << 20, 4u << 20, 8u << 20}) {
+            phg_tune_trial t = cur;
+            t.trace_block = block;
+            t.paths_per_pass = ppp;
+            st = measure(t);
+            if (st != PHG_OK) return st;
+            if (t.msamples_per_s > top.msamples_per_s) top = t;
+        }
+    // stage 2: BVH shape with the best launch parameters
+    for (uint32_t leaf : {2u, 4u, 8u})
+        for (uint32_t width : {2u, 4u}) {
+            if (leaf == 4 && width == 4) continue;  // measured in stage 1
+            st = upload(leaf, width);
+            if (st != PHG_OK) return st;
+            phg_tune_trial t = top;
+            t.max_leaf_tris = leaf;
+            t.width = width;
+            st = measure(t);
+            if (st != PHG_OK) return st;
+            if (t.msamples_per_s > top.msamples_per_s) top = t;
+        }
+    *best = top;
+    phg_tuning tu{};
+    tu.trace_block = top.trace_block;
+    tu.paths_per_pass = top.paths_per_pass;
+    st = phg_set_tuning(c, &tu);
+    if (st != PHG_OK) return st;
+    if (trials)
+        for (uint32_t i = 0; i < std::min<uint32_t>(max_trials, (uint32_t)all.size()); ++i) trials[i] = all[i];
+    if (ntrials) *ntrials = (uint32_t)all.size();
     return PHG_OK;
 }
 

@@ -148,6 +148,7 @@ inline float up_f(double v) {
 struct Collapser {
     const std::vector<BNode>& bn;
     std::vector<WideNode>& out;
+    uint32_t width = 4;
     uint32_t depth = 0;
 
     // Emit the wide node for binary inner node bi at index wi; returns its stack need.
@@ -155,7 +156,7 @@ struct Collapser {
         depth = std::max(depth, level);
         uint32_t kids[4] = {bn[bi].left, bn[bi].right, 0, 0};
         uint32_t nk = 2;
-        while (nk < 4) {
+        while (nk < width) {
             int pick = -1; double best = -1.0;
             for (uint32_t i = 0; i < nk; ++i) {
                 const BNode& c = bn[kids[i]];
@@ -203,6 +204,7 @@ BvhBuildResult build_wide_bvh(const float* verts, uint32_t ntri, const BvhBuildP
     if (params.max_leaf_tris < 1 || params.max_leaf_tris > kMaxLeafTris)
         throw std::runtime_error("bvh: max_leaf_tris must be in [1, 8]");
     if (params.sah_bins < 2 || params.sah_bins > 64) throw std::runtime_error("bvh: sah_bins must be in [2, 64]");
+    if (params.width < 2 || params.width > 4) throw std::runtime_error("bvh: width must be in [2, 4]");
     if (ntri >= (1u << kLeafFirstBits)) throw std::runtime_error("bvh: too many triangles");
     BvhBuildResult res;
     if (ntri == 0) return res;
@@ -230,7 +232,7 @@ BvhBuildResult build_wide_bvh(const float* verts, uint32_t ntri, const BvhBuildP
     res.tri_order = b.idx;
     res.nodes.reserve(res.binary_nodes / 2 + 2);
     res.nodes.emplace_back();
-    Collapser c{b.nodes, res.nodes};
+    Collapser c{b.nodes, res.nodes, params.width};
     if (b.nodes[root].count > 0) {
         // single leaf: a root with one leaf child
         WideNode w{};

@@ -36,6 +36,7 @@ struct BvhBuildParams {
     uint32_t max_leaf_tris = 4;
     uint32_t sah_bins = 16;
     uint32_t nthreads = 0;
+    uint32_t width = 4;              // children per wide node (2..4; unused slots stay empty)
 };
 
 struct BvhBuildResult {

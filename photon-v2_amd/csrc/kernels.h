@@ -10,7 +10,7 @@
 
 namespace phg {
 
-constexpr uint32_t kTraceBlock = 128;   // traversal kernels: 2 waves, LDS stack ring [slot][128]
+constexpr uint32_t kTraceBlock = 128;   // ray-query kernel block (2 waves); path kernels take 64, 128 or 256
 constexpr int kTraceRing = 16;          // LDS ring entries per lane; older entries spill to global memory
 constexpr uint32_t kShadeBlock = 256;
 constexpr uint32_t kCountSlots = 64;    // queue counters, used modulo this many bounces
@@ -117,13 +117,14 @@ hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n,
 // Wavefront stages.  Pass parameters are read from device memory (ppd), so that a captured
 // bounce loop can be replayed for every pass after the host rewrites them.
 hipError_t launch_generate(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t grid, hipStream_t st);
-// Blocks of the traversal kernels (kTraceBlock threads) and of the shading kernel
-// (kShadeBlock threads) that one compute unit holds at once; sizes the persistent grids.
-hipError_t resident_blocks(uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu);
-hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
-                         hipStream_t st);
-hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+// Blocks of the path traversal kernels (trace_block threads: 64, 128 or 256) and of the
+// shading kernel (kShadeBlock threads) that one compute unit holds at once; sizes the
+// persistent grids.
+hipError_t resident_blocks(uint32_t trace_block, uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu);
+hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t block,
                          uint32_t count, hipStream_t st);
+hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+                         uint32_t block, uint32_t count, hipStream_t st);
 hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
                         bool guided, hipStream_t st);
 hipError_t launch_film(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t grid, hipStream_t st);

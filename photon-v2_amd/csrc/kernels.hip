@@ -159,14 +159,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
 // Every ray of a path queue, grid-stride over the lanes of the persistent grid: a lane starts
 // its next ray as soon as its current one finishes.  Queue rays are (origin, root node bits),
 // (direction, tmax) with tmin = 0; done(i, result) consumes each finished ray.
-template <int RING, bool ANY, bool COUNT, typename Done>
+template <int RING, uint32_t BLOCK, bool ANY, bool COUNT, typename Done>
 __device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* __restrict__ queue, uint32_t n,
                                             int32_t* stk, int32_t* spill, uint32_t spill_cap, uint32_t* nvis,
                                             uint32_t* ntri, uint32_t* ovf, Done done) {
-    const uint32_t stride = gridDim.x * kTraceBlock;
-    uint32_t i = xcd_block_id() * kTraceBlock + threadIdx.x;
+    const uint32_t stride = gridDim.x * BLOCK;
+    uint32_t i = xcd_block_id() * BLOCK + threadIdx.x;
     Traversal<RING, ANY, COUNT> tv;
-    tv.attach(stk, kTraceBlock, spill, spill_cap);
+    tv.attach(stk, BLOCK, spill, spill_cap);
     auto start = [&](uint32_t k) {
         const float4* rp = reinterpret_cast<const float4*>(queue + k);
         const float4 r0 = rp[0], r1 = rp[1];
@@ -181,10 +181,10 @@ __device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* _
     }
 }
 
-template <int RING>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
-    __shared__ int32_t stack_mem[RING * kTraceBlock];
-    int32_t* spill = pb.spill + (size_t)(xcd_block_id() * kTraceBlock + threadIdx.x) * pb.spill_cap;
+template <int RING, uint32_t BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
+    __shared__ int32_t stack_mem[RING * BLOCK];
+    int32_t* spill = pb.spill + (size_t)(xcd_block_id() * BLOCK + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[bounce % kCountSlots];
     // the counters the shade stage of this bounce pushes to start from zero
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -195,12 +195,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
     const uint32_t q = bounce & 1u;
     uint32_t nvis = 0, ntri = 0, ovf = 0;
     if (count_traversal)
-        trace_queue<RING, false, true>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, true>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        [&](uint32_t i, const TraceResult& tr) {
                                            pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                        });
     else
-        trace_queue<RING, false, false>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, false>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                         [&](uint32_t i, const TraceResult& tr) {
                                             pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                         });
@@ -211,12 +211,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(SceneDev sc, PathBuffers
     if (ovf) atomicOr(pb.error_flags, 1u);
 }
 
-template <int RING>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
-                                                        uint32_t bounce, uint32_t count_traversal) {
+template <int RING, uint32_t BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
+                                                  uint32_t bounce, uint32_t count_traversal) {
     const uint32_t rec_bounces = ppd->record ? ppd->rec_bounces : 0u, npaths = ppd->paths;
-    __shared__ int32_t stack_mem[RING * kTraceBlock];
-    int32_t* spill = pb.spill + (size_t)(xcd_block_id() * kTraceBlock + threadIdx.x) * pb.spill_cap;
+    __shared__ int32_t stack_mem[RING * BLOCK];
+    int32_t* spill = pb.spill + (size_t)(xcd_block_id() * BLOCK + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[kCountSlots + bounce % kCountSlots];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&pb.traversal_counters[5], (unsigned long long)n);
     uint32_t nvis = 0, ntri = 0, ovf = 0;
@@ -237,10 +237,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(SceneDev sc, const PassP
         }
     };
     if (count_traversal)
-        trace_queue<RING, true, true>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, true>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                       unoccluded);
     else
-        trace_queue<RING, true, false>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, false>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        unoccluded);
     if (count_traversal) {
         atomicAdd(&pb.traversal_counters[2], (unsigned long long)nvis);
@@ -547,15 +547,25 @@ hipError_t launch_generate(const SceneDev& sc, const PassParams* ppd, const Path
     return hipGetLastError();
 }
 
-hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t count,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(k_extend<kTraceRing>, dim3(grid), dim3(kTraceBlock), 0, st, sc, pb, bounce, count);
+hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t block,
+                         uint32_t count, hipStream_t st) {
+    switch (block) {
+        case 64: hipLaunchKernelGGL((k_extend<kTraceRing, 64>), dim3(grid), dim3(64), 0, st, sc, pb, bounce, count); break;
+        case 128: hipLaunchKernelGGL((k_extend<kTraceRing, 128>), dim3(grid), dim3(128), 0, st, sc, pb, bounce, count); break;
+        case 256: hipLaunchKernelGGL((k_extend<kTraceRing, 256>), dim3(grid), dim3(256), 0, st, sc, pb, bounce, count); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
-                         uint32_t count, hipStream_t st) {
-    hipLaunchKernelGGL(k_shadow<kTraceRing>, dim3(grid), dim3(kTraceBlock), 0, st, sc, ppd, pb, bounce, count);
+                         uint32_t block, uint32_t count, hipStream_t st) {
+    switch (block) {
+        case 64: hipLaunchKernelGGL((k_shadow<kTraceRing, 64>), dim3(grid), dim3(64), 0, st, sc, ppd, pb, bounce, count); break;
+        case 128: hipLaunchKernelGGL((k_shadow<kTraceRing, 128>), dim3(grid), dim3(128), 0, st, sc, ppd, pb, bounce, count); break;
+        case 256: hipLaunchKernelGGL((k_shadow<kTraceRing, 256>), dim3(grid), dim3(256), 0, st, sc, ppd, pb, bounce, count); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -566,16 +576,26 @@ hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuf
     return hipGetLastError();
 }
 
-hipError_t resident_blocks(uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu) {
+template <uint32_t BLOCK>
+static hipError_t trace_occupancy(int* a, int* b) {
+    hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(a, k_extend<kTraceRing, BLOCK>, BLOCK, 0);
+    if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(b, k_shadow<kTraceRing, BLOCK>, BLOCK, 0);
+    return err;
+}
+
+hipError_t resident_blocks(uint32_t trace_block, uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu) {
     int a = 0, b = 0, c = 0, d = 0, e = 0;
-    hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<kTraceRing>, kTraceBlock, 0);
-    if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<kTraceRing>, kTraceBlock, 0);
+    hipError_t err = trace_block == 64 ? trace_occupancy<64>(&a, &b)
+                   : trace_block == 128 ? trace_occupancy<128>(&a, &b)
+                   : trace_block == 256 ? trace_occupancy<256>(&a, &b) : hipErrorInvalidValue;
     if (err == hipSuccess)
         err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_intersect<kTraceRing, false, false>, kTraceBlock, 0);
     if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_shade<false>, kShadeBlock, 0);
     if (err == hipSuccess) err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_shade<true>, kShadeBlock, 0);
     if (err != hipSuccess) return err;
-    *trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, std::min(b, c)));
+    // the intersect kernel runs 128-thread blocks: express its residency in trace blocks
+    const int c_tb = (int)((uint64_t)c * kTraceBlock / trace_block);
+    *trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, std::min(b, c_tb)));
     *shade_blocks_per_cu = (uint32_t)std::max(1, std::min(d, e));
     return hipSuccess;
 }

@@ -64,7 +64,7 @@ class phg_device_opts(C.Structure):
 
 class phg_bvh_opts(C.Structure):
     _fields_ = [("max_leaf_tris", C.c_uint32), ("sah_bins", C.c_uint32), ("nthreads", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("width", C.c_uint32)]
 
 
 class phg_batch_job(C.Structure):
@@ -78,6 +78,20 @@ class phg_batch_opts(C.Structure):
 
 class phg_batch_result(C.Structure):
     _fields_ = [("done_ms", C.c_double), ("paths", C.c_uint64)]
+
+
+class phg_tuning(C.Structure):
+    _fields_ = [("trace_block", C.c_uint32), ("paths_per_pass", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class phg_tune_trial(C.Structure):
+    _fields_ = [("max_leaf_tris", C.c_uint32), ("width", C.c_uint32), ("trace_block", C.c_uint32),
+                ("paths_per_pass", C.c_uint32), ("msamples_per_s", C.c_double)]
+
+
+class phg_autotune_opts(C.Structure):
+    _fields_ = [("sample_jobs", C.c_uint32), ("sample_spp", C.c_uint32), ("repeats", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class phg_render_opts(C.Structure):
@@ -116,7 +130,7 @@ EXPORTED = ["phg_create", "phg_destroy", "phg_last_error", "phg_upload_scene", "
             "phg_intersect_counted", "phg_film_clear", "phg_film_read", "phg_render", "phg_get_stats",
             "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_gradients",
             "phg_guide_train", "phg_scene_generate", "phg_scene_free", "phg_upload_scenes", "phg_render_batch",
-            "phg_batch_film_read"]
+            "phg_batch_film_read", "phg_set_tuning", "phg_get_tuning", "phg_autotune_batch"]
 
 _lib = None
 
@@ -153,6 +167,11 @@ def lib() -> C.CDLL:
     L.phg_render_batch.argtypes = [vp, C.POINTER(phg_batch_job), C.c_uint32, C.POINTER(phg_batch_opts),
                                    C.POINTER(phg_batch_result)]
     L.phg_batch_film_read.argtypes = [vp, C.c_uint32, vp, C.c_uint64]
+    L.phg_set_tuning.argtypes = [vp, C.POINTER(phg_tuning)]
+    L.phg_get_tuning.argtypes = [vp, C.POINTER(phg_tuning)]
+    L.phg_autotune_batch.argtypes = [vp, C.POINTER(phg_scene_desc), C.c_uint32, C.POINTER(phg_batch_job), C.c_uint32,
+                                     C.POINTER(phg_autotune_opts), C.POINTER(phg_tune_trial), C.POINTER(phg_tune_trial),
+                                     C.c_uint32, C.POINTER(C.c_uint32)]
     for name in EXPORTED:
         if name not in ("phg_last_error", "phg_scene_free"):
             getattr(L, name).restype = C.c_int
@@ -285,17 +304,18 @@ class Context:
             msg = self._L.phg_last_error(self._h)
             raise PhgError(f"{what}: {STATUS_NAMES.get(st, st)}: {msg.decode() if msg else ''}")
 
-    def upload(self, scene: Scene, max_leaf_tris: int = 4, sah_bins: int = 16, nthreads: int = 0):
+    def upload(self, scene: Scene, max_leaf_tris: int = 4, sah_bins: int = 16, nthreads: int = 0, width: int = 4):
         d = scene.desc()
-        self._check(self._L.phg_upload_scene(self._h, C.byref(d), C.byref(phg_bvh_opts(max_leaf_tris, sah_bins, nthreads, 0))),
+        self._check(self._L.phg_upload_scene(self._h, C.byref(d),
+                                             C.byref(phg_bvh_opts(max_leaf_tris, sah_bins, nthreads, width))),
                     "phg_upload_scene")
         self._scene_ref = scene
 
-    def upload_many(self, scenes, max_leaf_tris: int = 4, sah_bins: int = 16, nthreads: int = 0):
+    def upload_many(self, scenes, max_leaf_tris: int = 4, sah_bins: int = 16, nthreads: int = 0, width: int = 4):
         """Upload several scenes at once (phg_upload_scenes); jobs address them by index."""
         descs = (phg_scene_desc * len(scenes))(*[s.desc() for s in scenes])
         self._check(self._L.phg_upload_scenes(self._h, descs, len(scenes),
-                                              C.byref(phg_bvh_opts(max_leaf_tris, sah_bins, nthreads, 0))),
+                                              C.byref(phg_bvh_opts(max_leaf_tris, sah_bins, nthreads, width))),
                     "phg_upload_scenes")
         self._scene_ref = list(scenes)
         self._batch_npix = []
@@ -314,6 +334,37 @@ class Context:
         self._check(self._L.phg_render_batch(self._h, arr, len(jobs), C.byref(opts), res), "phg_render_batch")
         self._batch_shapes = [(j["height"], j["width"]) for j in jobs]
         return [(r.done_ms, r.paths) for r in res]
+
+    def set_tuning(self, trace_block: int = 128, paths_per_pass: int = 0):
+        self._check(self._L.phg_set_tuning(self._h, C.byref(phg_tuning(trace_block, paths_per_pass, 0, 0))),
+                    "phg_set_tuning")
+
+    def tuning(self) -> dict:
+        t = phg_tuning()
+        self._check(self._L.phg_get_tuning(self._h, C.byref(t)), "phg_get_tuning")
+        return {"trace_block": t.trace_block, "paths_per_pass": t.paths_per_pass,
+                "trace_blocks_per_cu": t.trace_blocks_per_cu}
+
+    def autotune_batch(self, scenes, jobs, sample_jobs: int = 8, sample_spp: int = 4, repeats: int = 2):
+        """Run the batch autotuner; returns (best, trials) as dicts.  Leaves the sample scenes
+        uploaded and the best launch tuning set."""
+        descs = (phg_scene_desc * len(scenes))(*[s.desc() for s in scenes])
+        arr = (phg_batch_job * len(jobs))()
+        for i, j in enumerate(jobs):
+            arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
+                                   j.get("seed", 0), (C.c_uint32 * 2)(0, 0))
+        best = phg_tune_trial()
+        trials = (phg_tune_trial * 64)()
+        n = C.c_uint32()
+        self._check(self._L.phg_autotune_batch(self._h, descs, len(scenes), arr, len(jobs),
+                                               C.byref(phg_autotune_opts(sample_jobs, sample_spp, repeats, 0)),
+                                               C.byref(best), trials, 64, C.byref(n)), "phg_autotune_batch")
+        self._scene_ref = list(scenes)
+
+        def d(t):
+            return {"max_leaf_tris": t.max_leaf_tris, "width": t.width, "trace_block": t.trace_block,
+                    "paths_per_pass": t.paths_per_pass, "msamples_per_s": t.msamples_per_s}
+        return d(best), [d(trials[i]) for i in range(min(n.value, 64))]
 
     def batch_film(self, job: int) -> np.ndarray:
         h, w = self._batch_shapes[job]

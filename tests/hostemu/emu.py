@@ -17,7 +17,7 @@ def lib():
             subprocess.run(["make", "-s", "-C", HERE], check=True)
         _lib = C.CDLL(LIB)
         vp = C.c_void_p
-        _lib.emu_intersect.argtypes = [vp, C.c_uint32, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32,
+        _lib.emu_intersect.argtypes = [vp, C.c_uint32, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                        vp, vp, vp, vp, vp, vp]
         _lib.emu_guide_output_grad.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp]
         _lib.emu_guide_pdf.argtypes = [vp, C.c_uint32, vp, vp]
@@ -26,14 +26,14 @@ def lib():
     return _lib
 
 
-def intersect(verts, rays, any_hit=False, max_leaf=4, bins=16):
+def intersect(verts, rays, any_hit=False, max_leaf=4, bins=16, width=4):
     verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
     n = rays.shape[0]
     prim = np.zeros(n, np.int32)
     t, b, c = (np.zeros(n, np.float32) for _ in range(3))
     nv, nt = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
-    st = lib().emu_intersect(verts.ctypes.data, verts.shape[0], rays.ctypes.data, n, int(any_hit), max_leaf, bins,
+    st = lib().emu_intersect(verts.ctypes.data, verts.shape[0], rays.ctypes.data, n, int(any_hit), max_leaf, bins, width,
                              prim.ctypes.data, t.ctypes.data, b.ctypes.data, c.ctypes.data, nv.ctypes.data,
                              nt.ctypes.data)
     if st < 0:

@@ -10,10 +10,11 @@
 using namespace phg;
 
 extern "C" int emu_intersect(const float* verts, uint32_t ntri, const float* rays, uint32_t n, int any,
-                             uint32_t max_leaf, uint32_t bins, int32_t* out_prim, float* out_t, float* out_b,
+                             uint32_t max_leaf, uint32_t bins, uint32_t width, int32_t* out_prim, float* out_t, float* out_b,
                              float* out_c, uint32_t* out_nvis, uint32_t* out_ntri) {
     BvhBuildParams p;
     p.max_leaf_tris = max_leaf ? max_leaf : 4;
+    p.width = width ? width : 4;
     p.sah_bins = bins ? bins : 16;
     BvhBuildResult bvh = build_wide_bvh(verts, ntri, p);
     const uint32_t spill_cap = bvh.stack_need > (uint32_t)kTraceRing ? bvh.stack_need - (uint32_t)kTraceRing : 0u;

@@ -108,3 +108,56 @@ def test_batch_graph_equals_eager():
         print(f"[batch] graph passes={c.stats()['batch_passes']}", flush=True)
     for i in range(len(jobs)):
         assert np.array_equal(eager[i], graph[i]), i
+
+
+def test_launch_tuning_does_not_change_films():
+    sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=4, target_tris=40_000, num_instances=5, num_lights=3)
+    W, H = 72, 40
+    films = []
+    with phg.Context(0) as c:
+        c.upload(sc)
+        for block, ppp in ((128, 0), (64, 1 << 16), (256, 3 << 16)):
+            c.set_tuning(trace_block=block, paths_per_pass=ppp)
+            t = c.tuning()
+            print(f"[batch] tuning {t}", flush=True)
+            assert t["trace_block"] == block and t["trace_blocks_per_cu"] >= 1
+            c.film_clear(W, H)
+            c.render(12, max_bounces=8, seed=5, graph=(block == 256))
+            films.append(c.film())
+    for f in films[1:]:
+        assert np.array_equal(f, films[0])
+
+
+@pytest.mark.parametrize("leaf,width", [(1, 2), (8, 3), (2, 4)])
+def test_bvh_shapes_keep_intersection_parity(leaf, width):
+    from oracle import oracle as orc
+    from helpers import random_rays, random_soup, rays_towards, scene_from_tris
+    v = random_soup(3000, 31)
+    rays = np.concatenate([rays_towards(v, 30_000, 32), random_rays(v, 20_000, 33)])
+    with phg.Context(0) as c:
+        c.upload(scene_from_tris(v), max_leaf_tris=leaf, width=width)
+        hits = c.intersect(rays)
+    tri, t, bary, flags = orc.intersect_brute(v, rays)
+    ok = (flags & 1) == 0
+    assert np.array_equal(hits["prim"][ok], tri[ok])
+    h = ok & (tri >= 0)
+    assert np.array_equal(hits["t"][h].view(np.uint32), t[h].view(np.uint32))
+
+
+def test_autotuner_picks_a_measured_configuration():
+    sc = scenes()
+    with phg.Context(0) as c:
+        best, trials = c.autotune_batch(sc, JOBS, sample_jobs=4, sample_spp=2, repeats=1)
+        print(f"[batch] autotune best={best} trials={len(trials)}", flush=True)
+        assert len(trials) == 17
+        assert best["msamples_per_s"] == max(t["msamples_per_s"] for t in trials)
+        assert c.tuning()["trace_block"] == best["trace_block"]
+        # the tuned context renders the real jobs, and the films are still the single-job films
+        c.upload_many(sc, max_leaf_tris=best["max_leaf_tris"], width=best["width"])
+        c.render_batch(JOBS[:2], graph=True)
+        film0 = c.batch_film(0)
+    with phg.Context(0) as c:
+        c.upload(sc[JOBS[0]["scene"]], max_leaf_tris=best["max_leaf_tris"], width=best["width"])
+        c.film_clear(JOBS[0]["width"], JOBS[0]["height"])
+        c.render(JOBS[0]["spp"], max_bounces=JOBS[0]["max_bounces"], seed=JOBS[0]["seed"])
+        assert np.array_equal(c.film(), film0)

@@ -36,6 +36,12 @@ def test_soup_leaf_sizes(leaf):
     check(v, np.concatenate([rays_towards(v, 30_000, 8, jitter=0.01), random_rays(v, 20_000, 9)]), max_leaf=leaf)
 
 
+@pytest.mark.parametrize("width", [2, 3])
+def test_narrower_bvh_widths(width):
+    v = random_soup(1500, 17)
+    check(v, np.concatenate([rays_towards(v, 20_000, 18, jitter=0.01), random_rays(v, 10_000, 19)]), width=width)
+
+
 def test_segments_and_backward_rays():
     v = random_soup(400, 11)
     rays = rays_towards(v, 20_000, 12)
