@@ -169,6 +169,9 @@ def main():
     ap.add_argument("--no-conv", action="store_true", help="skip the guided-vs-plain convergence leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--batch-jobs", type=int, default=48)
+    ap.add_argument("--no-tune", action="store_true", help="default launch and BVH parameters (no autotuner)")
+    ap.add_argument("--tuning", default=None, metavar="LEAF,WIDTH,BLOCK,PATHS",
+                    help="fixed BVH leaf size, BVH width, traversal block and paths per pass instead of the autotuner")
     ap.add_argument("--conv-res", type=int, default=256)
     ap.add_argument("--conv-ref-spp", type=int, default=16384)
     ap.add_argument("--conv-max-spp", type=int, default=8192)
@@ -188,8 +191,22 @@ def main():
     print(f"[bench] scene: {scene.num_tris} triangles, {len(scene.emitters)} lights ({gen_s:.1f}s)",
           file=sys.stderr, flush=True)
     ctx = phg.Context(0)
+    tuned = None
+    if args.tuning:
+        leaf, width, block, ppp = (int(x) for x in args.tuning.split(","))
+        ctx.set_tuning(trace_block=block, paths_per_pass=ppp)
+        tuned = dict(max_leaf_tris=leaf, width=width, trace_block=block, paths_per_pass=ppp, trials=0, seconds=0.0)
+    elif not args.no_tune:
+        t0 = time.time()
+        job = dict(scene=0, width=W, height=H, spp=spp, max_bounces=B, seed=1)
+        best, trials = ctx.autotune_batch([scene], [job], sample_jobs=1, sample_spp=8)
+        tuned = dict(best, trials=len(trials), seconds=round(time.time() - t0, 2))
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     t0 = time.time()
-    ctx.upload(scene)
+    if tuned:
+        ctx.upload(scene, max_leaf_tris=tuned["max_leaf_tris"], width=tuned["width"])
+    else:
+        ctx.upload(scene)
     upload_s = time.time() - t0
     st0 = ctx.stats()
     print(f"[bench] BVH: {st0['bvh_nodes']} nodes, depth {st0['bvh_depth']}, build {st0['bvh_build_ms']:.0f} ms, "
@@ -274,7 +291,8 @@ def main():
         "config": {"workload": f"plain BNEEPT, procedural {scene.num_tris}-triangle scene, {W}x{H}, {spp} spp, "
                                f"{B} bounces", "width": W, "height": H, "spp": spp, "max_bounces": B,
                    "triangles": scene.num_tris, "lights": int(len(scene.emitters))},
-        "mode": "each 4M-path pass replays one captured HIP graph of generate, 9 bounces and film",
+        "mode": "each pass replays one captured HIP graph of generate, 9 bounces and film",
+        "tuning": tuned if tuned else ctx.tuning(),
         "mrays_per_s": round(sum(rays) / graph_s / 1e6, 3),
         "mrays_per_s_by_class": {"primary": round(rays[0] / graph_s / 1e6, 3),
                                  "extension": round(rays[1] / graph_s / 1e6, 3),
