@@ -230,16 +230,28 @@ typedef struct phg_batch_job {
     uint32_t spp;
     uint32_t max_bounces;
     uint32_t seed;             /* < 2^24; sample streams as in phg_render */
-    uint32_t pad[2];
+    uint32_t flags;            /* PHG_JOB_CAMERA: render through `camera` instead of the scene's */
+    uint32_t pad;
+    phg_camera camera;
 } phg_batch_job;
 
+enum { PHG_JOB_CAMERA = 1 };
 enum { PHG_BATCH_ORDER_GIVEN = 0, PHG_BATCH_ORDER_SHORTEST_FIRST = 1 };
-enum { PHG_BATCH_COUNT_TRAVERSAL = 1, PHG_BATCH_GRAPH = 2 /* as PHG_RENDER_GRAPH */ };
+enum {
+    PHG_BATCH_COUNT_TRAVERSAL = 1,
+    PHG_BATCH_GRAPH = 2,       /* as PHG_RENDER_GRAPH */
+    PHG_BATCH_GUIDED = 4,      /* as PHG_RENDER_GUIDED; all jobs must render the same scene (one network) */
+    PHG_BATCH_TRAIN = 8        /* as PHG_RENDER_TRAIN, on the records of every job in the pass */
+};
 
 typedef struct phg_batch_opts {
-    uint32_t paths_per_pass;   /* paths in flight per pass (0 = 4M); passes carry whole samples */
+    uint32_t paths_per_pass;   /* paths in flight per pass (0 = the context's tuning); passes carry whole samples */
     uint32_t order;            /* PHG_BATCH_ORDER_*: which jobs the passes take first */
-    uint32_t flags;            /* PHG_BATCH_COUNT_TRAVERSAL | PHG_BATCH_GRAPH */
+    uint32_t flags;            /* PHG_BATCH_* */
+    uint32_t guide_steps;      /* PHG_BATCH_TRAIN: Adam steps after each pass (0 = 4) */
+    float guide_prob;          /* PHG_BATCH_GUIDED: probability of sampling the learned lobes */
+    float guide_lr;            /* PHG_BATCH_TRAIN: Adam learning rate (0 = 1e-3) */
+    uint32_t guide_batch;      /* PHG_BATCH_TRAIN: records per step (0 = 2^18) */
     uint32_t pad;
 } phg_batch_opts;
 
@@ -324,7 +336,8 @@ phg_status phg_guide_train(void* ctx, const float* records, uint64_t n, float lr
 enum {
     PHG_GEN_CORNELL = 1,        /* unit-cube Cornell box, 34 + 2 light triangles */
     PHG_GEN_PROCEDURAL = 2,     /* displaced icospheres + ground + area quads */
-    PHG_GEN_OCCLUDED_ROOM = 3   /* room lit through a slit by a hidden quad light */
+    PHG_GEN_OCCLUDED_ROOM = 3,  /* room lit through a slit by a hidden quad light */
+    PHG_GEN_TERRAIN = 4         /* displaced icospheres over an FBM terrain grid (20M triangles), 64 area quads */
 };
 
 typedef struct phg_gen_opts {

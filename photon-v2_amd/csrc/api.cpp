@@ -959,6 +959,7 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
         if (b.seed >= (1u << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "job seed must be below 2^24");
         if (b.spp == 0 || (uint64_t)b.spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "job spp out of range");
         jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles);
+        if (b.flags & PHG_JOB_CAMERA) jobs[j].cam = b.camera;
         film_doubles += 8 * npix;
     }
     // pass plan: jobs in the requested order, each pass filled first-fit with whole samples
@@ -1003,6 +1004,16 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
     RenderMode m;
     m.count_traversal = bo && (bo->flags & PHG_BATCH_COUNT_TRAVERSAL);
     m.graph = bo && (bo->flags & PHG_BATCH_GRAPH);
+    m.guided = bo && (bo->flags & PHG_BATCH_GUIDED);
+    m.train = bo && (bo->flags & PHG_BATCH_TRAIN);
+    if (m.guided || m.train) {
+        for (uint32_t j = 1; j < n; ++j)
+            if (in[j].scene != in[0].scene)
+                return set_err(c, PHG_ERR_UNSUPPORTED, "guided batches render one scene (the network is shared)");
+        m.guide_prob = bo->guide_prob; m.guide_lr = bo->guide_lr; m.guide_batch = bo->guide_batch;
+        m.guide_steps = bo->guide_steps;
+        c->g_scene = c->scenes[in[0].scene].box;
+    }
     std::vector<double> done(n, 0.0);
     const phg_status st = render_passes(c, jobs, passes, c->bfilm.p, m, &done);
     if (st != PHG_OK) return st;

@@ -3,6 +3,8 @@
 // Cornell box: 5 walls + 2 boxes (34 triangles) + a 0.25 x 0.25 ceiling light (2 triangles).
 // Procedural: displaced icosphere instances + a tessellated ground + area-light quads.
 // Occluded room: a room lit only through a 0.1-wide slit in an interior wall.
+// Terrain: displaced icospheres over an FBM height-field grid, 20M triangles by default.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -280,6 +282,58 @@ static void gen_procedural(SceneBuf& s, const phg_gen_opts& o) {
     s.camera(f3(0, 12, -24), f3(0, -6, 0), f3(0, 1, 0), o.fov_deg > 0 ? o.fov_deg : 60.0f);
 }
 
+// Large procedural world: displaced icosphere instances over an FBM terrain grid, mixed GGX
+// conductors and dielectrics, many area lights.
+static void gen_terrain(SceneBuf& s, const phg_gen_opts& o) {
+    Rng rng(o.seed ? o.seed : 4242);
+    const uint32_t ninst = o.num_instances ? o.num_instances : 256;
+    const uint32_t nlights = o.num_lights ? o.num_lights : 64;
+    const uint32_t subdiv = o.subdiv ? o.subdiv : 4;
+    const uint32_t target = o.target_tris ? o.target_tris : 20000000u;
+    const Icosphere ico = make_icosphere(subdiv);
+    const uint32_t absorber = s.add_material(PHG_MAT_ABSORBER, f3(0, 0, 0), 0, 1, 1);
+    const uint32_t ground = s.add_material(PHG_MAT_CONDUCTOR, f3(0.55f, 0.5f, 0.45f), 0.7f, 1, 1);
+    const uint32_t have_est = ninst * (uint32_t)(ico.f.size() / 3) + 2 * nlights;
+    const uint32_t cells = target > have_est ? (target - have_est) / 2 : 1;
+    const uint32_t n = std::max<uint32_t>(1, (uint32_t)std::sqrt((double)cells));
+    const float x0 = -50.0f, size = 100.0f, base = -12.0f, amp = 4.0f;
+    s.verts.reserve(9ull * ((size_t)2 * n * n + have_est));
+    s.tri_mat.reserve((size_t)2 * n * n + have_est);
+    s.tri_emit.reserve((size_t)2 * n * n + have_est);
+    auto height = [&](float x, float z) { return base + amp * fbm(f3(x * 0.04f, 0.0f, z * 0.04f), o.seed * 31u + 7u, 5); };
+    std::vector<float> row0(n + 1), row1(n + 1);
+    for (uint32_t j = 0; j <= n; ++j) row0[j] = height(x0 + size * (float)j / n, x0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float z0 = x0 + size * (float)i / n, z1 = x0 + size * (float)(i + 1) / n;
+        for (uint32_t j = 0; j <= n; ++j) row1[j] = height(x0 + size * (float)j / n, z1);
+        for (uint32_t j = 0; j < n; ++j) {
+            const float xa = x0 + size * (float)j / n, xb = x0 + size * (float)(j + 1) / n;
+            // counter-clockwise seen from +y
+            s.quad(f3(xa, row0[j], z0), f3(xa, row1[j], z1), f3(xb, row1[j + 1], z1), f3(xb, row0[j + 1], z0), ground);
+        }
+        std::swap(row0, row1);
+    }
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const float x = rng.uf(-40, 40), z = rng.uf(-40, 40);
+        const float radius = rng.uf(0.6f, 2.0f);
+        const F3 c = f3(x, height(x, z) + radius + rng.uf(0.0f, 6.0f), z);
+        const float rough = rng.uf(0.05f, 0.6f);
+        uint32_t mat;
+        if (i % 2 == 0) {
+            mat = s.add_material(PHG_MAT_CONDUCTOR, f3(rng.uf(0.5f, 1), rng.uf(0.5f, 1), rng.uf(0.5f, 1)), rough, 1, 1);
+        } else {
+            mat = s.add_material(PHG_MAT_DIELECTRIC, f3(1, 1, 1), rough, 1.0f, rng.uf(1.3f, 1.7f));
+        }
+        add_displaced_sphere(s, ico, c, radius, 0.15f, (uint32_t)(o.seed * 977u + i), mat);
+    }
+    for (uint32_t l = 0; l < nlights; ++l) {
+        const float x = rng.uf(-45, 42), z = rng.uf(-45, 42);
+        const float r = rng.uf(1, 20);
+        s.light_down(x, x + 3.0f, z, z + 3.0f, 20.0f, f3(r, r, r), absorber);
+    }
+    s.camera(f3(0, 10, -60), f3(0, -8, 0), f3(0, 1, 0), o.fov_deg > 0 ? o.fov_deg : 60.0f);
+}
+
 static void gen_room(SceneBuf& s, const phg_gen_opts& o) {
     Rng rng(o.seed ? o.seed : 7);
     const uint32_t target = o.target_tris ? o.target_tris : 200000;
@@ -330,6 +384,7 @@ extern "C" phg_status phg_scene_generate(const phg_gen_opts* opts, phg_scene_hos
             case PHG_GEN_CORNELL: gen_cornell(*s, *opts); break;
             case PHG_GEN_PROCEDURAL: gen_procedural(*s, *opts); break;
             case PHG_GEN_OCCLUDED_ROOM: gen_room(*s, *opts); break;
+            case PHG_GEN_TERRAIN: gen_terrain(*s, *opts); break;
             default: delete s; return PHG_ERR_INVALID_ARG;
         }
     } catch (const std::bad_alloc&) {

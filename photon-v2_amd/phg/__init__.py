@@ -25,12 +25,13 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "OUT_OF_MEMORY", 4: "NO_
                 6: "UNSUPPORTED"}
 
 MAT_ABSORBER, MAT_LAMBERT, MAT_CONDUCTOR, MAT_DIELECTRIC = 0, 1, 2, 3
-GEN_CORNELL, GEN_PROCEDURAL, GEN_OCCLUDED_ROOM = 1, 2, 3
+GEN_CORNELL, GEN_PROCEDURAL, GEN_OCCLUDED_ROOM, GEN_TERRAIN = 1, 2, 3, 4
 QUERY_CLOSEST, QUERY_ANY = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
 RENDER_COUNT_TRAVERSAL, RENDER_GUIDED, RENDER_TRAIN, RENDER_GRAPH = 1, 2, 4, 8
 BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST_FIRST = 0, 1
-BATCH_COUNT_TRAVERSAL, BATCH_GRAPH = 1, 2
+BATCH_COUNT_TRAVERSAL, BATCH_GRAPH, BATCH_GUIDED, BATCH_TRAIN = 1, 2, 4, 8
+JOB_CAMERA = 1
 GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -50,6 +51,12 @@ class phg_camera(C.Structure):
                 ("forward", C.c_float * 3), ("tan_half_fov", C.c_float), ("pad", C.c_float * 3)]
 
 
+class phg_batch_job(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("pad", C.c_uint32),
+                ("camera", phg_camera)]
+
+
 class phg_scene_desc(C.Structure):
     _fields_ = [("num_tris", C.c_uint32), ("num_materials", C.c_uint32), ("num_emitters", C.c_uint32),
                 ("num_emitter_tris", C.c_uint32), ("verts", C.POINTER(C.c_float)),
@@ -67,13 +74,10 @@ class phg_bvh_opts(C.Structure):
                 ("width", C.c_uint32)]
 
 
-class phg_batch_job(C.Structure):
-    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("pad", C.c_uint32 * 2)]
-
-
 class phg_batch_opts(C.Structure):
-    _fields_ = [("paths_per_pass", C.c_uint32), ("order", C.c_uint32), ("flags", C.c_uint32), ("pad", C.c_uint32)]
+    _fields_ = [("paths_per_pass", C.c_uint32), ("order", C.c_uint32), ("flags", C.c_uint32),
+                ("guide_steps", C.c_uint32), ("guide_prob", C.c_float), ("guide_lr", C.c_float),
+                ("guide_batch", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class phg_batch_result(C.Structure):
@@ -181,6 +185,34 @@ def lib() -> C.CDLL:
 
 class PhgError(RuntimeError):
     pass
+
+
+def camera_look_at(pos, target, up=(0.0, 1.0, 0.0), fov_deg: float = 60.0) -> dict:
+    """Pinhole camera dict (Scene.camera layout) at pos looking at target; fov is horizontal."""
+    pos, target, up = (np.asarray(v, np.float64) for v in (pos, target, up))
+    fwd = target - pos
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, up)
+    right /= np.linalg.norm(right)
+    upv = np.cross(right, fwd)
+    return {"pos": list(map(float, pos)), "right": list(map(float, right)), "up": list(map(float, upv)),
+            "forward": list(map(float, fwd)), "tan_half_fov": float(np.tan(np.radians(fov_deg) / 2))}
+
+
+def _camera_struct(cam: dict):
+    return phg_camera((C.c_float * 3)(*cam["pos"]), (C.c_float * 3)(*cam["right"]), (C.c_float * 3)(*cam["up"]),
+                      (C.c_float * 3)(*cam["forward"]), cam["tan_half_fov"], (C.c_float * 3)(0, 0, 0))
+
+
+def _job_array(jobs):
+    """jobs: dicts with scene, width, height, spp, max_bounces, seed and optionally camera."""
+    arr = (phg_batch_job * len(jobs))()
+    for i, j in enumerate(jobs):
+        cam = j.get("camera")
+        arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
+                               j.get("seed", 0), JOB_CAMERA if cam else 0, 0,
+                               _camera_struct(cam) if cam else phg_camera())
+    return arr
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -321,16 +353,15 @@ class Context:
         self._batch_npix = []
 
     def render_batch(self, jobs, paths_per_pass: int = 0, order: int = BATCH_ORDER_GIVEN,
-                     count_traversal: bool = False, graph: bool = False):
+                     count_traversal: bool = False, graph: bool = False, guided: bool = False, train: bool = False,
+                     guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4):
         """jobs: sequence of dicts with scene, width, height, spp, max_bounces, seed.
         Returns per job (done_ms, paths)."""
-        arr = (phg_batch_job * len(jobs))()
-        for i, j in enumerate(jobs):
-            arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
-                                   j.get("seed", 0), (C.c_uint32 * 2)(0, 0))
+        arr = _job_array(jobs)
         res = (phg_batch_result * len(jobs))()
-        opts = phg_batch_opts(paths_per_pass, order, (BATCH_COUNT_TRAVERSAL if count_traversal else 0) |
-                              (BATCH_GRAPH if graph else 0), 0)
+        flags = (BATCH_COUNT_TRAVERSAL if count_traversal else 0) | (BATCH_GRAPH if graph else 0) | \
+            (BATCH_GUIDED if guided else 0) | (BATCH_TRAIN if train else 0)
+        opts = phg_batch_opts(paths_per_pass, order, flags, guide_steps, guide_prob, guide_lr, guide_batch, 0)
         self._check(self._L.phg_render_batch(self._h, arr, len(jobs), C.byref(opts), res), "phg_render_batch")
         self._batch_shapes = [(j["height"], j["width"]) for j in jobs]
         return [(r.done_ms, r.paths) for r in res]
@@ -349,10 +380,7 @@ class Context:
         """Run the batch autotuner; returns (best, trials) as dicts.  Leaves the sample scenes
         uploaded and the best launch tuning set."""
         descs = (phg_scene_desc * len(scenes))(*[s.desc() for s in scenes])
-        arr = (phg_batch_job * len(jobs))()
-        for i, j in enumerate(jobs):
-            arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
-                                   j.get("seed", 0), (C.c_uint32 * 2)(0, 0))
+        arr = _job_array(jobs)
         best = phg_tune_trial()
         trials = (phg_tune_trial * 64)()
         n = C.c_uint32()

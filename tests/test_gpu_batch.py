@@ -161,3 +161,54 @@ def test_autotuner_picks_a_measured_configuration():
         c.film_clear(JOBS[0]["width"], JOBS[0]["height"])
         c.render(JOBS[0]["spp"], max_bounces=JOBS[0]["max_bounces"], seed=JOBS[0]["seed"])
         assert np.array_equal(c.film(), film0)
+
+
+def test_job_cameras_render_their_own_views():
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    cam = phg.camera_look_at([0.2, 0.7, -0.8], [0.55, 0.35, 0.6], fov_deg=55)
+    jobs = [dict(scene=0, width=48, height=40, spp=6, max_bounces=5, seed=3),
+            dict(scene=0, width=48, height=40, spp=6, max_bounces=5, seed=3, camera=sc.camera),
+            dict(scene=0, width=40, height=30, spp=6, max_bounces=5, seed=4, camera=cam)]
+    with phg.Context(0) as c:
+        c.upload_many([sc])
+        c.render_batch(jobs, paths_per_pass=1 << 16)
+        films = [c.batch_film(i) for i in range(3)]
+    assert np.array_equal(films[0], films[1])       # the scene's own camera, given explicitly
+    moved = phg.Scene(sc.verts, sc.tri_material, sc.tri_emitter, sc.materials, sc.emitters, sc.emitter_tris, cam)
+    ref = single(moved, jobs[2])
+    assert np.array_equal(films[2], ref)
+
+
+def test_guided_batch_of_views_matches_plain():
+    """Guided rendering of several views sharing one network, trained online on all of
+    them, agrees with plain rendering per pixel (99% CIs)."""
+    from helpers import pixel_z
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    cams = [sc.camera, phg.camera_look_at([0.15, 0.8, -0.9], [0.5, 0.3, 0.7], fov_deg=50)]
+    jobs = [dict(scene=0, width=32, height=32, spp=256, max_bounces=10000, seed=40 + k, camera=cams[k])
+            for k in range(2)]
+    with phg.Context(0) as c:
+        c.upload_many([sc])
+        train = [dict(j, spp=16, seed=90 + i) for i, j in enumerate(jobs)]
+        c.render_batch(train, paths_per_pass=1 << 14, train=True, guided=True, guide_batch=8192)
+        st = c.stats()
+        assert st["guide_records"] > 0
+        c.render_batch(jobs, paths_per_pass=1 << 16, guided=True)
+        g = [c.batch_film(i) for i in range(2)]
+        st = c.stats()
+        assert st["guide_queries"] > 0
+        c.render_batch([dict(j, seed=j["seed"] + 100) for j in jobs], paths_per_pass=1 << 16)
+        p = [c.batch_film(i) for i in range(2)]
+    for k in range(2):
+        z = pixel_z(g[k], p[k])
+        frac = np.mean(np.abs(z) > 2.5758)
+        print(f"[batch] guided view {k}: outside_99CI={frac:.4f}", flush=True)
+        assert frac < 0.02
+
+
+def test_guided_batch_needs_one_scene():
+    sc = scenes()[:2]
+    with phg.Context(0) as c:
+        c.upload_many(sc)
+        with pytest.raises(phg.PhgError):
+            c.render_batch(JOBS[:2], guided=True)
