@@ -18,6 +18,10 @@ Reported beside the throughput:
   batch         48 independent jobs (procedural scenes of 1e4..2e6 triangles, 128^2..1024^2,
                 4..256 spp, log-uniform, depth 8) rendered by phg_render_batch in shared passes:
                 aggregate Msamples/s and per-job completion p50 / p99.
+  scale         guided + batched at scale: 8 views of uneven resolution (2048^2 down to 640x360)
+                of a 20M-triangle terrain world (256 displaced icospheres, FBM terrain grid, 64
+                area lights), depth 12, one network shared by all views and trained online, in
+                shared passes; plain captured-graph passes of the same views beside it.
   convergence   time to relMSE 0.01 on the occluded-light room (~199k triangles, light behind
                 a wall with a 0.1-wide slit, GGX conductors), guided vs plain BNEEPT: both
                 render progressively against a plain reference of the same scene; the guided
@@ -115,6 +119,67 @@ def batch_leg(args):
                       "p99_done_ms": round(wl.percentile([d for d, _ in eres], 99), 2)}}
 
 
+SCALE_VIEWS = [(2048, 2048), (1920, 1080), (1600, 900), (1280, 720), (1024, 1024), (960, 540), (800, 800), (640, 360)]
+
+
+def scale_leg(args):
+    """20M-triangle terrain world, 8 views of uneven resolution, guided with one shared network."""
+    import numpy as np
+    import phg
+
+    t0 = time.time()
+    scene = phg.Scene.generate(phg.GEN_TERRAIN)
+    gen_s = time.time() - t0
+    jobs = []
+    for k, (w, h) in enumerate(SCALE_VIEWS):
+        a = 2 * np.pi * k / len(SCALE_VIEWS)
+        cam = phg.camera_look_at([55 * np.sin(a), 6 + 2 * k, -55 * np.cos(a)], [0, -8, 0], fov_deg=60)
+        jobs.append(dict(scene=0, width=w, height=h, spp=args.scale_spp, max_bounces=12, seed=100 + k, camera=cam))
+    npix = sum(w * h for w, h in SCALE_VIEWS)
+    with phg.Context(0) as ctx:
+        t0 = time.time()
+        ctx.upload_many([scene])
+        up_s = time.time() - t0
+        st0 = ctx.stats()
+        ntris = scene.num_tris
+        del scene
+        print(f"[bench] scale: {ntris} triangles (generated {gen_s:.1f}s), BVH {st0['bvh_build_ms'] / 1e3:.1f}s, "
+              f"upload {up_s:.1f}s, {len(jobs)} views, {npix} pixels", file=sys.stderr, flush=True)
+        ctx.render_batch([dict(j, spp=1) for j in jobs], count_traversal=True)
+        cs = ctx.stats()
+        rays1 = cs["rays_primary"] + cs["rays_extension"]
+        nodes_per_ray, tris_per_ray = cs["trace_node_visits"] / rays1, cs["trace_tri_tests"] / rays1
+        t0 = time.perf_counter()
+        gres = ctx.render_batch(jobs, guided=True, train=True, guide_steps=8)
+        g_s = time.perf_counter() - t0
+        gs = ctx.stats()
+        paths = sum(p for _, p in gres)
+        print(f"[bench] scale guided+train: {paths / 1e6:.0f}M paths in {g_s:.2f}s, guide infer "
+              f"{gs['guide_infer_ms']:.0f} ms, train {gs['guide_train_ms']:.0f} ms", file=sys.stderr, flush=True)
+        ctx.render_batch(jobs, graph=True)            # warm-up
+        t0 = time.perf_counter()
+        ctx.render_batch(jobs, graph=True)
+        p_s = time.perf_counter() - t0
+        ps = ctx.stats()
+        ctx.render_batch(jobs)
+        es = ctx.stats()
+    rays_c = es["rays_primary"] + es["rays_extension"]
+    bytes_c = rays_c * (48 + 128 * nodes_per_ray + 48 * tris_per_ray)
+    ach = bytes_c / (es["trace_ms"] / 1e3) / 1e9
+    print(f"[bench] scale plain: {p_s:.2f}s, k_extend {ach:.0f} GB/s", file=sys.stderr, flush=True)
+    return {"scene": f"terrain world, {ntris} triangles, 64 area lights", "views": [list(v) for v in SCALE_VIEWS],
+            "spp": args.scale_spp, "max_bounces": 12, "paths": paths,
+            "guided_trained": {"seconds": round(g_s, 4), "msamples_per_s": round(paths / g_s / 1e6, 3),
+                               "guide_infer_ms": round(gs["guide_infer_ms"], 2),
+                               "guide_train_ms": round(gs["guide_train_ms"], 2), "guide_steps": gs["guide_steps"]},
+            "plain_graph": {"seconds": round(p_s, 4), "msamples_per_s": round(paths / p_s / 1e6, 3),
+                            "mrays_per_s": round((ps["rays_primary"] + ps["rays_extension"] + ps["rays_shadow"]) / p_s / 1e6, 3)},
+            "k_extend_roofline": {"achieved_gbps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBPS, 4),
+                                  "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3),
+                                  "timed_in": "eager plain render of the same views (HIP events)"},
+            "bvh_build_s": round(st0["bvh_build_ms"] / 1e3, 2)}
+
+
 def convergence_leg(args):
     """Time to relMSE 0.01, plain vs guided, on the occluded-light room."""
     import numpy as np
@@ -169,6 +234,8 @@ def main():
     ap.add_argument("--no-conv", action="store_true", help="skip the guided-vs-plain convergence leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--batch-jobs", type=int, default=48)
+    ap.add_argument("--no-scale", action="store_true", help="skip the 20M-triangle multi-view leg")
+    ap.add_argument("--scale-spp", type=int, default=16)
     ap.add_argument("--no-tune", action="store_true", help="default launch and BVH parameters (no autotuner)")
     ap.add_argument("--tuning", default=None, metavar="LEAF,WIDTH,BLOCK,PATHS",
                     help="fixed BVH leaf size, BVH width, traversal block and paths per pass instead of the autotuner")
@@ -273,6 +340,7 @@ def main():
                          f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}"}
 
     batch = None if args.no_batch else batch_leg(args)
+    scale = None if args.no_scale else scale_leg(args)
     conv = None if args.no_conv else convergence_leg(args)
 
     out = {
@@ -310,6 +378,7 @@ def main():
                      "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
         "cpu_baseline": cpu,
         "batch": batch,
+        "scale": scale,
         "convergence": conv,
         "image_mean": [round(float(x), 6) for x in img.mean(axis=(0, 1))],
     }
