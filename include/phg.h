@@ -359,6 +359,36 @@ typedef struct phg_scene_host {
 phg_status phg_scene_generate(const phg_gen_opts* opts, phg_scene_host** out);
 void phg_scene_free(phg_scene_host* scene);
 
+/* ---- .p2 scene ingestion (host).
+ * Replaces, for scenes that render on this path, SdlSceneFileReader / SdlCommandParser
+ * (SDL/SdlCommandParser.cpp:100-700) followed by the actor cooking that flattens models,
+ * materials and area lights into the visual world (Actor/AModel.cpp:69-84,
+ * Actor/Light/AAreaLight.cpp:37-87, EngineEnv/Visualizer/PathTracingVisualizer.cpp:18-36).
+ * Understood: option(single-frame-render-session), observer(single-lens),
+ * sample-source(*), visualizer(path-tracing) with estimator bneept | bvpt | bvptdl,
+ * geometry(rectangle | cuboid | sphere | triangle | triangle-mesh), image(constant),
+ * material(matte-opaque | abraded-opaque | abraded-translucent), actor(model),
+ * actor(rectangle-light), and the translate / rotate / scale executors.  Spheres are
+ * tessellated; unsupported commands and fields are skipped and counted in `warnings`.
+ * The returned scene is freed with phg_scene_free.  err (optional) receives a message. */
+enum { PHG_ESTIMATOR_BNEEPT = 0, PHG_ESTIMATOR_BVPT = 1, PHG_ESTIMATOR_BVPTDL = 2 };
+enum { PHG_FILTER_BOX = 0, PHG_FILTER_GAUSSIAN = 1, PHG_FILTER_MN = 2, PHG_FILTER_BH = 3 };
+
+typedef struct phg_p2_settings {
+    uint32_t width, height;    /* frame-size */
+    uint32_t spp;              /* sample-source samples */
+    uint32_t estimator;        /* PHG_ESTIMATOR_* */
+    uint32_t max_bounces;      /* visualizer max-path-length if given, else 10000 (no cap) */
+    uint32_t filter;           /* PHG_FILTER_*; the GPU film accumulates with a box filter */
+    uint32_t warnings;         /* skipped commands / fields / approximations */
+    uint32_t pad;
+} phg_p2_settings;
+
+phg_status phg_scene_parse_p2(const char* text, phg_p2_settings* settings, phg_scene_host** out, char* err,
+                              uint32_t err_len);
+phg_status phg_scene_load_p2(const char* path, phg_p2_settings* settings, phg_scene_host** out, char* err,
+                             uint32_t err_len);
+
 #ifdef __cplusplus
 }
 #endif

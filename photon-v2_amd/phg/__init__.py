@@ -32,6 +32,8 @@ RENDER_COUNT_TRAVERSAL, RENDER_GUIDED, RENDER_TRAIN, RENDER_GRAPH = 1, 2, 4, 8
 BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST_FIRST = 0, 1
 BATCH_COUNT_TRAVERSAL, BATCH_GRAPH, BATCH_GUIDED, BATCH_TRAIN = 1, 2, 4, 8
 JOB_CAMERA = 1
+PHG_ESTIMATOR_BNEEPT, PHG_ESTIMATOR_BVPT, PHG_ESTIMATOR_BVPTDL = 0, 1, 2
+PHG_FILTER_BOX, PHG_FILTER_GAUSSIAN, PHG_FILTER_MN, PHG_FILTER_BH = 0, 1, 2, 3
 GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -98,6 +100,11 @@ class phg_autotune_opts(C.Structure):
     _fields_ = [("sample_jobs", C.c_uint32), ("sample_spp", C.c_uint32), ("repeats", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class phg_p2_settings(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("estimator", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("filter", C.c_uint32), ("warnings", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class phg_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("spp_begin", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("spp_per_pass", C.c_uint32), ("seed", C.c_uint64),
@@ -134,7 +141,8 @@ EXPORTED = ["phg_create", "phg_destroy", "phg_last_error", "phg_upload_scene", "
             "phg_intersect_counted", "phg_film_clear", "phg_film_read", "phg_render", "phg_get_stats",
             "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_gradients",
             "phg_guide_train", "phg_scene_generate", "phg_scene_free", "phg_upload_scenes", "phg_render_batch",
-            "phg_batch_film_read", "phg_set_tuning", "phg_get_tuning", "phg_autotune_batch"]
+            "phg_batch_film_read", "phg_set_tuning", "phg_get_tuning", "phg_autotune_batch", "phg_scene_parse_p2",
+            "phg_scene_load_p2"]
 
 _lib = None
 
@@ -167,6 +175,10 @@ def lib() -> C.CDLL:
     L.phg_scene_generate.argtypes = [C.POINTER(phg_gen_opts), C.POINTER(C.POINTER(phg_scene_host))]
     L.phg_scene_free.argtypes = [C.POINTER(phg_scene_host)]
     L.phg_scene_free.restype = None
+    L.phg_scene_parse_p2.argtypes = [C.c_char_p, C.POINTER(phg_p2_settings), C.POINTER(C.POINTER(phg_scene_host)),
+                                     C.c_char_p, C.c_uint32]
+    L.phg_scene_load_p2.argtypes = [C.c_char_p, C.POINTER(phg_p2_settings), C.POINTER(C.POINTER(phg_scene_host)),
+                                    C.c_char_p, C.c_uint32]
     L.phg_upload_scenes.argtypes = [vp, C.POINTER(phg_scene_desc), C.c_uint32, C.POINTER(phg_bvh_opts)]
     L.phg_render_batch.argtypes = [vp, C.POINTER(phg_batch_job), C.c_uint32, C.POINTER(phg_batch_opts),
                                    C.POINTER(phg_batch_result)]
@@ -263,6 +275,43 @@ class Scene:
         finally:
             L.phg_scene_free(out)
         return Scene(verts, tri_mat, tri_emit, mats, emits, etris, camera)
+
+    @staticmethod
+    def _from_host(out) -> "Scene":
+        L = lib()
+        try:
+            d = out.contents.desc
+            T, M, E, K = d.num_tris, d.num_materials, d.num_emitters, d.num_emitter_tris
+            verts = np.ctypeslib.as_array(d.verts, shape=(T * 9,)).reshape(T, 9).copy()
+            tri_mat = np.ctypeslib.as_array(d.tri_material, shape=(T,)).copy()
+            tri_emit = np.ctypeslib.as_array(d.tri_emitter, shape=(T,)).copy()
+            mats = np.frombuffer(C.string_at(d.materials, M * 32), dtype=Scene.MAT_DTYPE).copy()
+            emits = np.frombuffer(C.string_at(d.emitters, E * 32), dtype=Scene.EMIT_DTYPE).copy() if E else \
+                np.zeros(0, Scene.EMIT_DTYPE)
+            etris = np.ctypeslib.as_array(d.emitter_tris, shape=(K,)).copy() if K else np.zeros(0, np.uint32)
+            cam = d.camera
+            camera = {"pos": list(cam.pos), "right": list(cam.right), "up": list(cam.up),
+                      "forward": list(cam.forward), "tan_half_fov": cam.tan_half_fov}
+        finally:
+            L.phg_scene_free(out)
+        return Scene(verts, tri_mat, tri_emit, mats, emits, etris, camera)
+
+    @staticmethod
+    def parse_p2(text: str = None, path: str = None):
+        """Scene and render settings (dict) from .p2 text or a .p2 file (phg_scene_parse_p2 /
+        phg_scene_load_p2)."""
+        L = lib()
+        st = phg_p2_settings()
+        out = C.POINTER(phg_scene_host)()
+        err = C.create_string_buffer(512)
+        if path is not None:
+            rc = L.phg_scene_load_p2(path.encode(), C.byref(st), C.byref(out), err, 512)
+        else:
+            rc = L.phg_scene_parse_p2(text.encode(), C.byref(st), C.byref(out), err, 512)
+        if rc != PHG_OK:
+            raise PhgError(f"p2: {STATUS_NAMES.get(rc, rc)}: {err.value.decode()}")
+        settings = {k: getattr(st, k) for k, _ in phg_p2_settings._fields_ if k != "pad"}
+        return Scene._from_host(out), settings
 
     def desc(self) -> phg_scene_desc:
         """A phg_scene_desc pointing into this scene's arrays (keep the Scene alive)."""

@@ -113,3 +113,16 @@ def test_stats_report_paths_and_rays(ctx):
     assert s["rays_primary"] == 32 * 32 * 4
     assert s["rays_extension"] > 0 and s["rays_shadow"] > 0
     assert s["node_visits"] > 0 and s["tri_tests"] > 0
+
+
+def test_p2_scene_matches_oracle(ctx):
+    """A .p2 scene file (SDL 1.1) loaded by phg_scene_load_p2 renders with its own settings
+    (frame size, spp, estimator, bounce cap) on the GPU as on the oracle."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes", "glossy_box.p2")
+    sc, st = phg.Scene.parse_p2(path=path)
+    W, H, spp, B = st["width"], st["height"], st["spp"], st["max_bounces"]
+    print(f"[render] case=p2 tris={sc.num_tris} {W}x{H} spp={spp} bounces={B}", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, spp, B, 17)
+    ref, _ = orc.render(sc, W, H, spp, max_bounces=B, seed=17)
+    check_parity(gpu, ref, "p2_glossy_box", 0.5)
