@@ -121,6 +121,11 @@ typedef struct phg_hit {
 enum { PHG_PTR_HOST = 0, PHG_PTR_DEVICE = 1 };
 enum { PHG_QUERY_CLOSEST = 0, PHG_QUERY_ANY = 1 };
 
+/* Estimators (Core/Estimator/): BNEEPT (BNEEPTEstimator.cpp:38-262), BVPT — paths find light
+ * only by hitting it (BVPTEstimator.cpp:25-160), BVPTDL — emission seen directly or after one
+ * BSDF sample (BVPTDLEstimator.cpp:25-98). */
+enum { PHG_ESTIMATOR_BNEEPT = 0, PHG_ESTIMATOR_BVPT = 1, PHG_ESTIMATOR_BVPTDL = 2 };
+
 typedef struct phg_render_opts {
     uint32_t width, height;
     uint32_t spp;              /* samples per pixel rendered by this call */
@@ -133,7 +138,7 @@ typedef struct phg_render_opts {
     float guide_lr;            /* PHG_RENDER_TRAIN: Adam learning rate */
     uint32_t guide_batch;      /* PHG_RENDER_TRAIN: records per training step (0 = 2^18) */
     uint32_t guide_steps;      /* PHG_RENDER_TRAIN: Adam steps after each pass */
-    uint32_t pad;
+    uint32_t estimator;        /* PHG_ESTIMATOR_*; guiding needs BNEEPT */
 } phg_render_opts;
 
 enum {
@@ -231,7 +236,7 @@ typedef struct phg_batch_job {
     uint32_t max_bounces;
     uint32_t seed;             /* < 2^24; sample streams as in phg_render */
     uint32_t flags;            /* PHG_JOB_CAMERA: render through `camera` instead of the scene's */
-    uint32_t pad;
+    uint32_t estimator;        /* PHG_ESTIMATOR_* */
     phg_camera camera;
 } phg_batch_job;
 
@@ -371,7 +376,6 @@ void phg_scene_free(phg_scene_host* scene);
  * actor(rectangle-light), and the translate / rotate / scale executors.  Spheres are
  * tessellated; unsupported commands and fields are skipped and counted in `warnings`.
  * The returned scene is freed with phg_scene_free.  err (optional) receives a message. */
-enum { PHG_ESTIMATOR_BNEEPT = 0, PHG_ESTIMATOR_BVPT = 1, PHG_ESTIMATOR_BVPTDL = 2 };
 enum { PHG_FILTER_BOX = 0, PHG_FILTER_GAUSSIAN = 1, PHG_FILTER_MN = 2, PHG_FILTER_BH = 3 };
 
 typedef struct phg_p2_settings {

@@ -950,6 +950,66 @@ static RGB estimate(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bou
     return energy;
 }
 
+// BVPTEstimator::estimate (Core/Estimator/BVPTEstimator.cpp:25-160): backward path tracing
+// that finds light only by hitting it.  At every vertex: emission times the throughput (the
+// throughput clamped to [0, 1e9] first, as the reference does in place), then a BSDF sample
+// that extends the path, throughput update, Russian roulette from the 4th vertex on.  The
+// bounce cap is the loop bound.  Random numbers per vertex as in estimate(): 8, of which the
+// BSDF sample uses u[4..6] and Russian roulette u[7].
+static RGB estimate_bvpt(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, PathStats* st) {
+    RGB energy = rgb(0), T = rgb(1);
+    float rrScale = 1.0f;
+    Hit h;
+    st->rays_primary++;
+    if (!scene_intersect(S, org, dir, 0.0f, FLT_MAX, &h)) return energy;
+    SurfaceHit X = make_hit(S, h.tri, h.bary, dir);
+    if (!side_agreed(X, dir)) return energy;
+    V3 curDir = dir;
+    for (uint32_t nb = 0;; ++nb) {
+        if (S.s->tri_emitter[X.tri] >= 0) {
+            const RGB Le = eval_emission(S, X);
+            T = RGB{safe_clamp(T.r, 0.0f, 1e9f), safe_clamp(T.g, 0.0f, 1e9f), safe_clamp(T.b, 0.0f, 1e9f)};
+            energy.r += Le.r * T.r; energy.g += Le.g * T.g; energy.b += Le.b * T.b;
+        }
+        if (nb >= max_bounces) break;
+        const orc_material& m = S.s->materials[S.s->tri_material[X.tri]];
+        float u[8];
+        for (int k = 0; k < 8; ++k) u[k] = sf.u();
+        if (m.type == ORC_MAT_ABSORBER) break;
+        const V3 V = curDir * -1.0f;
+        const BsdfSample bs = bsdf_sample(m, X, V, u + 4);
+        if (!bs.ok) break;
+        T = mulc(T, bs.w);
+        rrScale /= bs.relIor * bs.relIor;
+        if (nb >= 3) {
+            const float surv = safe_clamp(luminance(muls(T, rrScale)), 0.0f, 0.95f);
+            if (u[7] < surv) T = muls(T, 1.0f / surv);
+            else break;
+        }
+        if (rgb_zero(T)) break;
+        const EscapeResult er = iterative_offset(S, X, bs.L);
+        const V3 no = X.P + er.offset;
+        st->rays_ext++;
+        Hit nh;
+        if (!scene_intersect(S, no, er.unit_dir, 0.0f, FLT_MAX, &nh)) break;
+        const SurfaceHit nX = make_hit(S, nh.tri, nh.bary, er.unit_dir);
+        if (!side_agreed(nX, er.unit_dir)) break;
+        X = nX;
+        curDir = er.unit_dir;
+    }
+    return energy;
+}
+
+// BVPTDLEstimator::estimate (Core/Estimator/BVPTDLEstimator.cpp:25-98): emission seen directly
+// and emission found by one BSDF sample from the first hit — BVPT with a cap of one bounce
+// (no Russian roulette happens before the 4th vertex).
+static RGB estimate_any(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, uint32_t estimator,
+                        PathStats* st) {
+    if (estimator == 1) return estimate_bvpt(S, org, dir, sf, max_bounces, st);
+    if (estimator == 2) return estimate_bvpt(S, org, dir, sf, std::min(max_bounces, 1u), st);
+    return estimate(S, org, dir, sf, max_bounces, st);
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1234,7 +1294,7 @@ int orc_render(const orc_scene* scene, const orc_render_opts* opt, double* film,
                         const float ry = (float)y + sf.u();
                         V3 o, d;
                         camera_ray(scene->camera, W, H, rx, ry, &o, &d);
-                        const RGB L = estimate(S, o, d, sf, opt->max_bounces, &st);
+                        const RGB L = estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st);
                         f[0] += L.r; f[1] += L.g; f[2] += L.b; f[3] += 1.0;
                         f[4] += (double)L.r * L.r; f[5] += (double)L.g * L.g; f[6] += (double)L.b * L.b;
                         n_paths.fetch_add(1, std::memory_order_relaxed);
@@ -1270,7 +1330,7 @@ void orc_render_paths(const orc_scene* scene, const orc_render_opts* opt, const 
         const float ry = (float)(pix[i] / W) + sf.u();
         V3 o, d;
         camera_ray(scene->camera, W, H, rx, ry, &o, &d);
-        const RGB L = estimate(S, o, d, sf, opt->max_bounces, &st);
+        const RGB L = estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st);
         out_rgb[3 * i] = L.r; out_rgb[3 * i + 1] = L.g; out_rgb[3 * i + 2] = L.b;
     }
 }

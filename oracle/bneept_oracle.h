@@ -59,7 +59,7 @@ typedef struct orc_render_opts {
     uint32_t nthreads;
     uint64_t seed;
     uint32_t row_step;  /* 0 or 1 = all rows; k = only rows y with y % k == 0 */
-    uint32_t pad;
+    uint32_t estimator; /* 0 BNEEPT, 1 BVPT, 2 BVPTDL */
 } orc_render_opts;
 
 typedef struct orc_render_stats {

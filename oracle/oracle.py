@@ -43,7 +43,7 @@ class orc_scene(C.Structure):
 class orc_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp_begin", C.c_uint32), ("spp", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("nthreads", C.c_uint32), ("seed", C.c_uint64),
-                ("row_step", C.c_uint32), ("pad", C.c_uint32)]
+                ("row_step", C.c_uint32), ("estimator", C.c_uint32)]
 
 
 class orc_render_stats(C.Structure):
@@ -145,12 +145,15 @@ class SceneView:
         self.s = s
 
 
-def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_step=1):
-    """FP64 film (H, W, 8) of the oracle BNEEPT render, plus ray counts."""
+def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_step=1,
+           estimator=0):
+    """FP64 film (H, W, 8) of the oracle render (estimator 0 BNEEPT, 1 BVPT, 2 BVPTDL), plus
+    ray counts."""
     L = lib()
     sv = SceneView(scene)
     film = np.zeros((height, width, 8), np.float64)
-    o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, row_step, 0)
+    o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, row_step,
+                        estimator)
     st = orc_render_stats()
     L.orc_render(C.byref(sv.s), C.byref(o), film.ctypes.data, C.byref(st))
     return film, {"rays_primary": st.rays_primary, "rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow,

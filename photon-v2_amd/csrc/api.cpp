@@ -885,8 +885,11 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
 }
 
 static JobDev make_job(const SceneInfo& si, uint32_t width, uint32_t height, uint32_t max_bounces, uint64_t seed,
-                       uint64_t film_base) {
+                       uint64_t film_base, uint32_t estimator) {
     JobDev j{};
+    // BVPTDL is BVPT capped at one bounce
+    j.estimator = estimator == PHG_ESTIMATOR_BNEEPT ? PHG_ESTIMATOR_BNEEPT : PHG_ESTIMATOR_BVPT;
+    if (estimator == PHG_ESTIMATOR_BVPTDL) max_bounces = std::min(max_bounces, 1u);
     j.cam = si.cam;
     j.root = si.root;
     j.width = width; j.height = height; j.npix = width * height;
@@ -918,7 +921,10 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     if (k == 0) k = (uint32_t)std::max<uint64_t>(1, (uint64_t)c->paths_per_pass / npix);
     k = std::min(k, o->spp);
     if (npix * k > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
-    std::vector<JobDev> jobs{make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0)};
+    if (o->estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown estimator");
+    if (o->estimator != PHG_ESTIMATOR_BNEEPT && (o->flags & (PHG_RENDER_GUIDED | PHG_RENDER_TRAIN)))
+        return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
+    std::vector<JobDev> jobs{make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0, o->estimator)};
     std::vector<PassPlan> passes;
     for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
         const uint32_t kk = std::min(k, o->spp - s0);
@@ -958,7 +964,8 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
         if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "job image size out of range");
         if (b.seed >= (1u << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "job seed must be below 2^24");
         if (b.spp == 0 || (uint64_t)b.spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "job spp out of range");
-        jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles);
+        if (b.estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job estimator");
+        jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles, b.estimator);
         if (b.flags & PHG_JOB_CAMERA) jobs[j].cam = b.camera;
         film_doubles += 8 * npix;
     }
@@ -1007,9 +1014,12 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
     m.guided = bo && (bo->flags & PHG_BATCH_GUIDED);
     m.train = bo && (bo->flags & PHG_BATCH_TRAIN);
     if (m.guided || m.train) {
-        for (uint32_t j = 1; j < n; ++j)
+        for (uint32_t j = 0; j < n; ++j) {
             if (in[j].scene != in[0].scene)
                 return set_err(c, PHG_ERR_UNSUPPORTED, "guided batches render one scene (the network is shared)");
+            if (in[j].estimator != PHG_ESTIMATOR_BNEEPT)
+                return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
+        }
         m.guide_prob = bo->guide_prob; m.guide_lr = bo->guide_lr; m.guide_batch = bo->guide_batch;
         m.guide_steps = bo->guide_steps;
         c->g_scene = c->scenes[in[0].scene].box;

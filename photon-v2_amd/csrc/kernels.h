@@ -48,8 +48,10 @@ struct alignas(16) JobDev {
     uint32_t max_bounces;
     uint64_t seed;                // sample-stream seed (< 2^24)
     uint64_t film_base;           // first double of the job's film (8 per pixel)
+    uint32_t estimator;           // PHG_ESTIMATOR_BNEEPT, or BVPT (BVPTDL runs as BVPT capped at one bounce)
+    uint32_t pad[3];
 };
-static_assert(sizeof(JobDev) == 112, "job record is 112 bytes");
+static_assert(sizeof(JobDev) == 128, "job record is 128 bytes");
 
 // A pass renders whole samples of one or more jobs: segment s holds `samples` consecutive
 // samples of job `job`, starting at sample index sample_begin, as pass paths

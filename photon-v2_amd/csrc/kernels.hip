@@ -294,12 +294,23 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, const PassPa
                 X = make_hit(tv.a, tv.b, tv.c, (uint32_t)tri, h.y, h.z, h.w);
                 if (!side_agreed(X, dir)) stop = true;  // SurfaceTracer.h:126-143
             }
+            const PassSeg sg = pp.segs[pb.path_seg[pid]];
+            const JobDev& job = sc.jobs[sg.job];
+            const bool bvpt = job.estimator != PHG_ESTIMATOR_BNEEPT;
+            float4 T4 = pb.T[pid];
             if (!stop && tr.emitter >= 0) {
                 // DiffuseSurfaceEmitterBase::evalEnergyFromSurface: front face only
                 const EmitterRec em = sc.emitters[tr.emitter];
                 const bool front = dot(neg(dir), X.Ns) > 0.0f;
                 const RGB Le = front ? rgb3(em.radiance[0], em.radiance[1], em.radiance[2]) : rgb1(0.0f);
-                if (bounce == 0) {
+                if (bvpt) {
+                    // BVPTEstimator.cpp:68-78: emission times the throughput, the throughput
+                    // clamped to [0, 1e9] in place
+                    T4.x = safe_clamp(T4.x, 0.0f, 1e9f); T4.y = safe_clamp(T4.y, 0.0f, 1e9f);
+                    T4.z = safe_clamp(T4.z, 0.0f, 1e9f);
+                    Lacc.x += Le.r * T4.x; Lacc.y += Le.g * T4.y; Lacc.z += Le.b * T4.z;
+                    l_dirty = true;
+                } else if (bounce == 0) {
                     // zero-bounce emission (SurfaceTracer.h:239-259)
                     Lacc.x += Le.r; Lacc.y += Le.g; Lacc.z += Le.b; l_dirty = true;
                 } else if (!rgb_zero(Le)) {
@@ -334,8 +345,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, const PassPa
                 const uint32_t flags = __float_as_uint(pb.prevP[pid].w);
                 if (flags & kFlagDieAfterEmission) stop = true;
             }
-            const PassSeg sg = pp.segs[pb.path_seg[pid]];
-            const JobDev& job = sc.jobs[sg.job];
             root = job.root;
             if (!stop && bounce >= job.max_bounces) stop = true;
             if (!stop) {
@@ -351,12 +360,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, const PassPa
                 const phg_material m = sc.mats[tr.mat];
                 if (m.type == PHG_MAT_ABSORBER) stop = true;
                 if (!stop) {
-                    const float4 T4 = pb.T[pid];
                     const RGB T = rgb3(T4.x, T4.y, T4.z);
                     float rrScale = T4.w;
                     const V3 V = dir * -1.0f;
                     // ---- NEE (TDirectLightEstimator.ipp:70-102, BNEEPTEstimator.cpp:103-149)
-                    if (job.num_emitters > 0) {
+                    if (!bvpt && job.num_emitters > 0) {
                         const uint32_t ei = cdf_sample(sc.emit_cdf + job.cdf_begin, job.num_emitters, u[0]);
                         const EmitterRec em = sc.emitters[job.emitter_begin + ei];
                         uint32_t k = (uint32_t)(u[1] * (float)em.tri_count);
@@ -446,21 +454,26 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, const PassPa
                             else die = true;
                         }
                         if (!die && rgb_zero(Tn)) die = true;
-                        pb.T[pid] = make_float4(Tn.r, Tn.g, Tn.b, rrScale);
-                        pb.Wem[pid] = make_float4(wem.r, wem.g, wem.b, pdfc);
-                        pb.prevP[pid] = make_float4(X.P.x, X.P.y, X.P.z, __uint_as_float(die ? kFlagDieAfterEmission : 0u));
-                        pb.rng[pid] = s.state;
-                        push_ext = true;
-                        ext_o = X.P + off;
-                        ext_d = ud;
-                        if (GUIDED && pp.record && bounce < pp.rec_bounces) {
-                            float4* rc = pb.rec + ((size_t)bounce * pp.paths + pid) * kRecFloat4;
-                            rc[0] = make_float4(X.P.x, X.P.y, X.P.z, pdfc);
-                            rc[1] = make_float4(X.Ns.x, X.Ns.y, X.Ns.z, bsdf_pdf_ls);
-                            rc[2] = make_float4(V.x, V.y, V.z, 0.0f);
-                            rc[3] = make_float4(ud.x, ud.y, ud.z, 0.0f);
-                            rc[4] = make_float4(T.r, T.g, T.b, 0.0f);  // throughput before this vertex
-                            rc[5] = make_float4(Lacc.x, Lacc.y, Lacc.z, 0.0f);
+                        // BVPT ends a path that Russian roulette (or a zero throughput) stops;
+                        // BNEEPT still follows the sampled direction to weigh the emission it
+                        // finds against NEE, then stops
+                        if (!(bvpt && die)) {
+                            pb.T[pid] = make_float4(Tn.r, Tn.g, Tn.b, rrScale);
+                            pb.Wem[pid] = make_float4(wem.r, wem.g, wem.b, pdfc);
+                            pb.prevP[pid] = make_float4(X.P.x, X.P.y, X.P.z, __uint_as_float(die ? kFlagDieAfterEmission : 0u));
+                            pb.rng[pid] = s.state;
+                            push_ext = true;
+                            ext_o = X.P + off;
+                            ext_d = ud;
+                            if (GUIDED && pp.record && bounce < pp.rec_bounces) {
+                                float4* rc = pb.rec + ((size_t)bounce * pp.paths + pid) * kRecFloat4;
+                                rc[0] = make_float4(X.P.x, X.P.y, X.P.z, pdfc);
+                                rc[1] = make_float4(X.Ns.x, X.Ns.y, X.Ns.z, bsdf_pdf_ls);
+                                rc[2] = make_float4(V.x, V.y, V.z, 0.0f);
+                                rc[3] = make_float4(ud.x, ud.y, ud.z, 0.0f);
+                                rc[4] = make_float4(T.r, T.g, T.b, 0.0f);  // throughput before this vertex
+                                rc[5] = make_float4(Lacc.x, Lacc.y, Lacc.z, 0.0f);
+                            }
                         }
                     }
                 }

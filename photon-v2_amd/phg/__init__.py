@@ -55,7 +55,7 @@ class phg_camera(C.Structure):
 
 class phg_batch_job(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("pad", C.c_uint32),
+                ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("estimator", C.c_uint32),
                 ("camera", phg_camera)]
 
 
@@ -109,7 +109,7 @@ class phg_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("spp_begin", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("spp_per_pass", C.c_uint32), ("seed", C.c_uint64),
                 ("flags", C.c_uint32), ("guide_prob", C.c_float), ("guide_lr", C.c_float),
-                ("guide_batch", C.c_uint32), ("guide_steps", C.c_uint32), ("pad", C.c_uint32)]
+                ("guide_batch", C.c_uint32), ("guide_steps", C.c_uint32), ("estimator", C.c_uint32)]
 
 
 class phg_stats(C.Structure):
@@ -222,7 +222,7 @@ def _job_array(jobs):
     for i, j in enumerate(jobs):
         cam = j.get("camera")
         arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
-                               j.get("seed", 0), JOB_CAMERA if cam else 0, 0,
+                               j.get("seed", 0), JOB_CAMERA if cam else 0, j.get("estimator", 0),
                                _camera_struct(cam) if cam else phg_camera())
     return arr
 
@@ -480,11 +480,11 @@ class Context:
     def render(self, spp: int, max_bounces: int = 10000, seed: int = 0, spp_begin: int = 0,
                spp_per_pass: int = 0, count_traversal: bool = False, guided: bool = False, train: bool = False,
                guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4,
-               graph: bool = False):
+               graph: bool = False, estimator: int = 0):
         flags = (RENDER_COUNT_TRAVERSAL if count_traversal else 0) | (RENDER_GUIDED if guided else 0) | \
             (RENDER_TRAIN if train else 0) | (RENDER_GRAPH if graph else 0)
         o = phg_render_opts(self.width, self.height, spp, spp_begin, max_bounces, spp_per_pass, seed, flags,
-                            guide_prob, guide_lr, guide_batch, guide_steps, 0)
+                            guide_prob, guide_lr, guide_batch, guide_steps, estimator)
         self._check(self._L.phg_render(self._h, C.byref(o)), "phg_render")
 
     # ---- guiding network

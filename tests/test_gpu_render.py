@@ -126,3 +126,32 @@ def test_p2_scene_matches_oracle(ctx):
     gpu = gpu_render(ctx, sc, W, H, spp, B, 17)
     ref, _ = orc.render(sc, W, H, spp, max_bounces=B, seed=17)
     check_parity(gpu, ref, "p2_glossy_box", 0.5)
+
+
+@pytest.mark.parametrize("estimator,bounces", [(phg.PHG_ESTIMATOR_BVPT, 10000), (phg.PHG_ESTIMATOR_BVPTDL, 10000),
+                                               (phg.PHG_ESTIMATOR_BVPT, 6)])
+def test_bvpt_estimators_match_oracle(ctx, estimator, bounces):
+    """BVPT (light found only by hitting it) and BVPTDL (one bounce) on the same kernels as
+    BNEEPT, against the oracle's restatement of BVPTEstimator / BVPTDLEstimator."""
+    from test_oracle import big_light_cornell
+    sc = big_light_cornell()
+    W = H = 48
+    print(f"[render] case=estimator{estimator} bounces={bounces} start", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, 32, bounces, 23, estimator=estimator)
+    st = ctx.stats()
+    assert st["rays_shadow"] == 0
+    ref, _ = orc.render(sc, W, H, 32, max_bounces=bounces, seed=23, estimator=estimator)
+    check_parity(gpu, ref, f"estimator{estimator}_b{bounces}", 0.8)
+
+
+def test_batch_mixes_estimators(ctx):
+    from test_oracle import big_light_cornell
+    sc = big_light_cornell()
+    jobs = [dict(scene=0, width=32, height=32, spp=8, max_bounces=8, seed=5, estimator=e) for e in range(3)]
+    with phg.Context(0) as c:
+        c.upload_many([sc])
+        c.render_batch(jobs, paths_per_pass=1 << 13)
+        films = [c.batch_film(i) for i in range(3)]
+    for e in range(3):
+        ref = gpu_render(ctx, sc, 32, 32, 8, 8, 5, estimator=e)
+        assert np.array_equal(films[e], ref), e

@@ -480,3 +480,47 @@ def test_oracle_deterministic_per_path():
     a, _ = orc.render(sc, 16, 16, 4, max_bounces=5, seed=1, nthreads=1)
     b, _ = orc.render(sc, 16, 16, 4, max_bounces=5, seed=1, nthreads=4)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("estimator", [1, 2])
+def test_oracle_bvpt_furnace(estimator):
+    """BVPT (unbounded) reaches the furnace's Le / (1 - rho); BVPTDL, capped at one bounce,
+    reaches Le (1 + rho), the first two terms of the series."""
+    albedo, radiance = 0.5, 0.5
+    sc = furnace_scene(albedo=albedo, radiance=radiance)
+    film, st = orc.render(sc, 16, 16, 64, max_bounces=10000, seed=7, estimator=estimator)
+    n = film[..., 3].sum()
+    mean = film[..., 0].sum() / n
+    se = np.sqrt(max(film[..., 4].sum() / n - mean * mean, 0) / n)
+    expect = radiance / (1 - albedo) if estimator == 1 else radiance * (1 + albedo)
+    assert st["rays_shadow"] == 0                      # no next-event estimation
+    assert abs(mean - expect) < 4 * se + 1e-4, (mean, expect)
+
+
+def big_light_cornell():
+    """The Cornell box with its ceiling light widened from 0.25 to 0.8: paths that must hit
+    the light to count (BVPT) then have a moderate variance, so per-pixel intervals hold."""
+    import phg
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    v = sc.verts.reshape(-1, 3, 3).copy()
+    for t in sc.emitter_tris:
+        v[t, :, 0] = 0.5 + (v[t, :, 0] - 0.5) * 3.2
+        v[t, :, 2] = 0.5 + (v[t, :, 2] - 0.5) * 3.2
+    sc.verts = v.reshape(-1, 9)
+    return sc
+
+
+@pytest.mark.parametrize("estimator,cap", [(1, 5), (2, 10000)])
+def test_oracle_bvpt_matches_bneept_in_expectation(estimator, cap):
+    """Same integral, different sampling: BVPT at a bounce cap B and BNEEPT at B agree per
+    pixel; BVPTDL agrees with BNEEPT capped at one bounce."""
+    import phg
+    from helpers import pixel_z
+    sc = big_light_cornell()
+    a, _ = orc.render(sc, 24, 24, 512, max_bounces=cap, seed=11, estimator=estimator)
+    b, _ = orc.render(sc, 24, 24, 512, max_bounces=cap if estimator == 1 else 1, seed=12, estimator=0)
+    z = pixel_z(a, b)
+    frac = np.mean(np.abs(z) > 2.5758)
+    ma, mb = a[..., :3].sum() / a[..., 3].sum(), b[..., :3].sum() / b[..., 3].sum()
+    assert frac < 0.03, frac
+    assert abs(ma - mb) < 0.03 * mb, (ma, mb)
