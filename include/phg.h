@@ -125,6 +125,10 @@ enum { PHG_QUERY_CLOSEST = 0, PHG_QUERY_ANY = 1 };
  * only by hitting it (BVPTEstimator.cpp:25-160), BVPTDL — emission seen directly or after one
  * BSDF sample (BVPTDLEstimator.cpp:25-98). */
 enum { PHG_ESTIMATOR_BNEEPT = 0, PHG_ESTIMATOR_BVPT = 1, PHG_ESTIMATOR_BVPTDL = 2 };
+/* Film reconstruction filters (SampleFilter.cpp:8-56): box of 1 pixel; Gaussian (sigma 0.5,
+ * submerged), Mitchell-Netravali (b = c = 1/3) and Blackman-Harris, each 4 x 4 pixels.  With a
+ * filter the film's sums are weighted: sum w r, g, b, sum w, sum w r^2, g^2, b^2. */
+enum { PHG_FILTER_BOX = 0, PHG_FILTER_GAUSSIAN = 1, PHG_FILTER_MN = 2, PHG_FILTER_BH = 3 };
 
 typedef struct phg_render_opts {
     uint32_t width, height;
@@ -139,6 +143,8 @@ typedef struct phg_render_opts {
     uint32_t guide_batch;      /* PHG_RENDER_TRAIN: records per training step (0 = 2^18) */
     uint32_t guide_steps;      /* PHG_RENDER_TRAIN: Adam steps after each pass */
     uint32_t estimator;        /* PHG_ESTIMATOR_*; guiding needs BNEEPT */
+    uint32_t filter;           /* PHG_FILTER_*: film reconstruction filter (SampleFilter.cpp:8-56) */
+    uint32_t pad;
 } phg_render_opts;
 
 enum {
@@ -213,6 +219,12 @@ phg_status phg_intersect_counted(void* ctx, const phg_ray* rays, uint64_t n, phg
  * sum r, g, b, sample weight, sum r^2, g^2, b^2, 0 (the squares feed per-pixel CIs). */
 phg_status phg_film_clear(void* ctx, uint32_t width, uint32_t height);
 phg_status phg_film_read(void* ctx, double* out, uint64_t out_len);
+/* Host film output (no device needed).  phg_film_develop: RGB float per pixel = weighted sum /
+ * weight (HdrRgbFilm::developRegion, HdrRgbFilm.cpp:167-207; 0 where no weight).
+ * phg_film_write_pfm: the developed frame as a .pfm file, as PfmFile::save writes it
+ * (DataIO/PfmFile.cpp:109-145): header "PF\n<w> <h>\n-1\n", rows bottom to top. */
+phg_status phg_film_develop(const double* film, uint32_t width, uint32_t height, float* out_rgb);
+phg_status phg_film_write_pfm(const double* film, uint32_t width, uint32_t height, const char* path);
 
 /* Render spp samples per pixel with BNEEPT into the film.
  * Replaces EqualSamplingRenderer::doRender (Core/Renderer/Sampling/EqualSamplingRenderer.cpp:119-203)
@@ -238,6 +250,8 @@ typedef struct phg_batch_job {
     uint32_t flags;            /* PHG_JOB_CAMERA: render through `camera` instead of the scene's */
     uint32_t estimator;        /* PHG_ESTIMATOR_* */
     phg_camera camera;
+    uint32_t filter;           /* PHG_FILTER_* */
+    uint32_t pad2[3];
 } phg_batch_job;
 
 enum { PHG_JOB_CAMERA = 1 };
@@ -376,7 +390,6 @@ void phg_scene_free(phg_scene_host* scene);
  * actor(rectangle-light), and the translate / rotate / scale executors.  Spheres are
  * tessellated; unsupported commands and fields are skipped and counted in `warnings`.
  * The returned scene is freed with phg_scene_free.  err (optional) receives a message. */
-enum { PHG_FILTER_BOX = 0, PHG_FILTER_GAUSSIAN = 1, PHG_FILTER_MN = 2, PHG_FILTER_BH = 3 };
 
 typedef struct phg_p2_settings {
     uint32_t width, height;    /* frame-size */

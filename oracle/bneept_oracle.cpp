@@ -1347,9 +1347,10 @@ static double filter_eval(int filter, double x, double y) {
         return std::max(g(x, y) - edge, 0.0);
     }
     if (filter == 2) {
+        // TBlackmanHarris2D (Math/Function/TBlackmanHarris2D.h:42-60): no cut at |v| = r;
+        // the splat range keeps |v| <= 2
         auto bh = [](double v) {
             const double r = 2.0;
-            if (std::fabs(v) >= r) return 0.0;
             const double n = (v + r) / (2 * r);
             const double pi = 3.14159265358979323846;
             return 0.35875 - 0.48829 * std::cos(2 * pi * n) + 0.14128 * std::cos(4 * pi * n) - 0.01168 * std::cos(6 * pi * n);
@@ -1357,6 +1358,16 @@ static double filter_eval(int filter, double x, double y) {
         return bh(x) * bh(y);
     }
     if (filter == 3) return x >= 0.0 ? 1.0 : 0.0;
+    if (filter == 4) {
+        // Mitchell & Netravali's cubic with b = c = 1/3 (SampleFilter::makeMitchellNetravali)
+        auto mn = [](double v) {
+            const double b = 1.0 / 3.0, c = 1.0 / 3.0, a = std::fabs(v);
+            if (a < 1.0) return ((12 - 9 * b - 6 * c) * a * a * a + (-18 + 12 * b + 6 * c) * a * a + (6 - 2 * b)) / 6.0;
+            if (a < 2.0) return ((-b - 6 * c) * a * a * a + (6 * b + 30 * c) * a * a + (-12 * b - 48 * c) * a + (8 * b + 24 * c)) / 6.0;
+            return 0.0;
+        };
+        return mn(x) * mn(y);
+    }
     return 0.0;
 }
 static double filter_half(int filter) { return filter == 0 ? 0.5 : filter == 3 ? 0.5 : 2.0; }
@@ -1377,6 +1388,58 @@ void orc_film_splat(int filter, int64_t W, int64_t H, const double* xy, const fl
                 s[0] += rgb3[3 * i] * w; s[1] += rgb3[3 * i + 1] * w; s[2] += rgb3[3 * i + 2] * w; s[3] += w;
             }
     }
+}
+
+// Render with a reconstruction filter: every path's radiance is splatted at its raster
+// position (pixel + the first two numbers of its stream) into the 8-double film layout
+// (sum w r, g, b, sum w, sum w r^2, g^2, b^2, 0), samples in (pixel, sample) order.
+// filter: 0 box, 1 gaussian, 2 Blackman-Harris, 4 Mitchell-Netravali.
+int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int filter, double* film) {
+    orc::Scene S; S.s = scene;
+    orc::bvh_build(S);
+    orc::emitters_prepare(S);
+    const uint32_t W = opt->width, H = opt->height;
+    const uint64_t npix = (uint64_t)W * H, ns = npix * opt->spp;
+    std::vector<double> xy(2 * ns);
+    std::vector<float> rgb(3 * ns);
+    std::atomic<uint64_t> next{0};
+    auto worker = [&]() {
+        orc::PathStats st;
+        for (uint64_t pix = next.fetch_add(1); pix < npix; pix = next.fetch_add(1)) {
+            const uint32_t x = (uint32_t)(pix % W), y = (uint32_t)(pix / W);
+            for (uint32_t k = 0; k < opt->spp; ++k) {
+                const uint64_t i = pix * opt->spp + k;
+                orc::Stream sf(orc::path_key(opt->seed, pix, opt->spp_begin + k, npix));
+                const float rx = (float)x + sf.u();
+                const float ry = (float)y + sf.u();
+                orc::V3 o, d;
+                orc::camera_ray(scene->camera, W, H, rx, ry, &o, &d);
+                const orc::RGB L = orc::estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st);
+                xy[2 * i] = rx; xy[2 * i + 1] = ry;
+                rgb[3 * i] = L.r; rgb[3 * i + 1] = L.g; rgb[3 * i + 2] = L.b;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t i = 0; i < std::max(1u, opt->nthreads); ++i) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    const double hs = filter_half(filter);
+    for (uint64_t i = 0; i < ns; ++i) {
+        const double xp = xy[2 * i], yp = xy[2 * i + 1];
+        const double fminx = std::max(xp - hs, 0.0), fminy = std::max(yp - hs, 0.0);
+        const double fmaxx = std::min(xp + hs, (double)W), fmaxy = std::min(yp + hs, (double)H);
+        const int64_t x0 = (int64_t)std::ceil(fminx - 0.5), y0 = (int64_t)std::ceil(fminy - 0.5);
+        const int64_t x1 = (int64_t)std::floor(fmaxx - 0.5) + 1, y1 = (int64_t)std::floor(fmaxy - 0.5) + 1;
+        for (int64_t y = y0; y < y1; ++y)
+            for (int64_t x = x0; x < x1; ++x) {
+                const double w = filter_eval(filter, x - (xp - 0.5), y - (yp - 0.5));
+                double* f = film + 8 * (y * (int64_t)W + x);
+                const double r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+                f[0] += w * r; f[1] += w * g; f[2] += w * b; f[3] += w;
+                f[4] += w * r * r; f[5] += w * g * g; f[6] += w * b * b;
+            }
+    }
+    return 0;
 }
 
 void orc_film_develop(int64_t W, int64_t H, const double* sensors4, float* out_rgb) {

@@ -98,6 +98,7 @@ def lib() -> C.CDLL:
     L.orc_nee_pdf.restype = f
     L.orc_camera_rays.argtypes = [C.POINTER(orc_camera), u32, u32, vp, u32, vp]
     L.orc_render.argtypes = [C.POINTER(orc_scene), C.POINTER(orc_render_opts), vp, C.POINTER(orc_render_stats)]
+    L.orc_render_filtered.argtypes = [C.POINTER(orc_scene), C.POINTER(orc_render_opts), C.c_int, vp]
     L.orc_render_paths.argtypes = [C.POINTER(orc_scene), C.POINTER(orc_render_opts), vp, vp, u32, vp]
     L.orc_film_splat.argtypes = [C.c_int, C.c_int64, C.c_int64, vp, vp, u32, vp]
     L.orc_film_develop.argtypes = [C.c_int64, C.c_int64, vp, vp]
@@ -143,6 +144,21 @@ class SceneView:
         cam.tan_half_fov = float(scene.camera["tan_half_fov"])
         s.camera = cam
         self.s = s
+
+
+FILTER_IDS = {0: 0, 1: 1, 2: 4, 3: 2}   # PHG_FILTER_* (box, gaussian, mn, bh) -> oracle filter id
+
+
+def render_filtered(scene, width, height, spp, filter, max_bounces=10000, seed=0, spp_begin=0, nthreads=None,
+                    estimator=0):
+    """Weighted FP64 film (H, W, 8) of an oracle render splatted with a PHG_FILTER_* filter."""
+    L = lib()
+    sv = SceneView(scene)
+    film = np.zeros((height, width, 8), np.float64)
+    o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, 1,
+                        estimator)
+    L.orc_render_filtered(C.byref(sv.s), C.byref(o), FILTER_IDS[filter], film.ctypes.data)
+    return film
 
 
 def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_step=1,

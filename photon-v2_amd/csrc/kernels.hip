@@ -494,8 +494,64 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, const PassPa
     }
 }
 
-// FP64 film accumulation, box filter, owner computes: each pixel sums its own samples of
-// this pass in sample order (HdrRgbFilm::addRgbSample, HdrRgbFilm.cpp:82-122).
+// Reconstruction filters of SampleFilter (Core/Filmic/SampleFilter.cpp:8-56), evaluated at the
+// offset from the pixel centre to the sample, in pixels; all three span 4 x 4 pixels.
+//   Gaussian: sigma 0.5, minus its value at the corner (2, 2) ("submerged"), floored at 0
+//   Mitchell-Netravali: cubic with b = c = 1/3 (Mitchell & Netravali 1988)
+//   Blackman-Harris: 4-term window of radius 2 (Math/Function/TBlackmanHarris2D.h:42-60)
+__device__ double film_filter(uint32_t f, double x, double y) {
+    if (f == PHG_FILTER_GAUSSIAN) {
+        const double edge = exp(-(4.0 / 0.5) - (4.0 / 0.5));
+        const double g = exp(-(x * x) / 0.5 - (y * y) / 0.5);
+        return fmax(g - edge, 0.0);
+    }
+    if (f == PHG_FILTER_MN) {
+        auto mn = [](double v) {
+            const double b = 1.0 / 3.0, c = 1.0 / 3.0, a = fabs(v);
+            if (a < 1.0) return ((12.0 - 9.0 * b - 6.0 * c) * a * a * a + (-18.0 + 12.0 * b + 6.0 * c) * a * a + (6.0 - 2.0 * b)) / 6.0;
+            if (a < 2.0) return ((-b - 6.0 * c) * a * a * a + (6.0 * b + 30.0 * c) * a * a + (-12.0 * b - 48.0 * c) * a +
+                                 (8.0 * b + 24.0 * c)) / 6.0;
+            return 0.0;
+        };
+        return mn(x) * mn(y);
+    }
+    auto bh = [](double v) {
+        const double n = (v * 0.5 + 1.0) * 0.5, pi = 3.14159265358979323846;
+        return 0.35875 - 0.48829 * cos(2.0 * pi * n) + 0.14128 * cos(4.0 * pi * n) - 0.01168 * cos(6.0 * pi * n);
+    };
+    return bh(x) * bh(y);
+}
+
+// Filtered film, gathered per pixel: every sample of this pass whose filter footprint covers
+// the pixel centre (the range HdrRgbFilm::addRgbSample splats to, HdrRgbFilm.cpp:82-122, with
+// half size 2) adds its radiance times the filter weight.  A sample's raster position is its
+// pixel plus the first two numbers of its path's stream, recomputed here.
+__device__ void film_gather(const JobDev& job, const PassSeg& sg, const PathBuffers& pb, uint32_t x) {
+    const int W = (int)job.width, H = (int)job.height;
+    const int px = (int)(x % job.width), py = (int)(x / job.width);
+    double2* f = reinterpret_cast<double2*>(pb.film + job.film_base + 8 * (size_t)x);
+    double2 f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
+    for (int qy = max(py - 2, 0); qy <= min(py + 2, H - 1); ++qy)
+        for (int qx = max(px - 2, 0); qx <= min(px + 2, W - 1); ++qx) {
+            const uint32_t q = (uint32_t)(qy * W + qx);
+            for (uint32_t s = 0; s < sg.samples; ++s) {
+                Stream st = stream_init(path_key(job.seed, q, sg.sample_begin + s, job.npix));
+                const double xp = (double)((float)qx + st.u()), yp = (double)((float)qy + st.u());
+                // pixels the sample splats to: ceil(p - 2.5) .. floor(p + 1.5)
+                if (px < (int)ceil(xp - 2.5) || px > (int)floor(xp + 1.5)) continue;
+                if (py < (int)ceil(yp - 2.5) || py > (int)floor(yp + 1.5)) continue;
+                const double w = film_filter(job.filter, (double)px - (xp - 0.5), (double)py - (yp - 0.5));
+                const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + q];
+                f0.x += w * L.x; f0.y += w * L.y; f1.x += w * L.z; f1.y += w;
+                f2.x += w * (double)L.x * L.x; f2.y += w * (double)L.y * L.y; f3.x += w * (double)L.z * L.z;
+            }
+        }
+    f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
+}
+
+// FP64 film accumulation, owner computes: each pixel sums its own samples of this pass in
+// sample order (box filter, HdrRgbFilm::addRgbSample, HdrRgbFilm.cpp:82-122), or gathers the
+// filtered contributions of the samples around it.
 __global__ __launch_bounds__(kShadeBlock) void k_film(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb) {
     const uint32_t nseg = ppd->num_segs, total = ppd->film_threads;
     const PassSeg* __restrict__ segs = ppd->segs;
@@ -507,7 +563,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_film(SceneDev sc, const PassPar
         }
         const PassSeg sg = segs[lo];
         const uint32_t x = t - sg.pix_offset;
-        double2* f = reinterpret_cast<double2*>(pb.film + sc.jobs[sg.job].film_base + 8 * (size_t)x);
+        const JobDev& job = sc.jobs[sg.job];
+        if (job.filter != PHG_FILTER_BOX) {
+            film_gather(job, sg, pb, x);
+            continue;
+        }
+        double2* f = reinterpret_cast<double2*>(pb.film + job.film_base + 8 * (size_t)x);
         double2 f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
         for (uint32_t s = 0; s < sg.samples; ++s) {
             const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + x];

@@ -29,6 +29,7 @@ void GpuBneeptRenderer::doUpdate(const phg_scene_desc& scene) {
     std::lock_guard<std::mutex> lock(m_filmMutex);
     m_film.assign(8ull * m_settings.width * m_settings.height, 0.0);
     m_sppDone = 0;
+    m_sppPolled = 0;
     m_elapsedMs = 0;
     m_pathsPerSecond = 0.0;
 }
@@ -45,6 +46,8 @@ void GpuBneeptRenderer::doRender() {
         o.spp_begin = s;
         o.max_bounces = m_settings.max_bounces;
         o.seed = m_settings.seed;
+        o.estimator = m_settings.estimator;
+        o.filter = m_settings.filter;
         check(phg_render(m_ctx, &o), "phg_render");
         check(phg_film_read(m_ctx, staging.data(), staging.size()), "phg_film_read");
         {
@@ -96,6 +99,34 @@ RenderProgress GpuBneeptRenderer::asyncQueryRenderProgress() const {
 
 std::vector<std::string> GpuBneeptRenderer::getObservationInfo() const {
     return {"paths/pixel", "paths/second"};
+}
+
+std::size_t GpuBneeptRenderer::asyncPollUpdatedRegions(RenderRegionStatus* out_regions, std::size_t capacity) {
+    const uint64_t done = m_sppDone.load();
+    if (capacity == 0 || !out_regions || done == m_sppPolled.load()) return 0;
+    m_sppPolled = done;
+    RenderRegionStatus r;
+    r.x1 = m_settings.width;
+    r.y1 = m_settings.height;
+    r.finished = done >= m_settings.spp;
+    out_regions[0] = r;
+    return 1;
+}
+
+void GpuBneeptRenderer::saveFrame(const std::string& path) {
+    std::lock_guard<std::mutex> lock(m_filmMutex);
+    check(phg_film_write_pfm(m_film.data(), m_settings.width, m_settings.height, path.c_str()), "phg_film_write_pfm");
+}
+
+GpuBneeptSettings GpuBneeptSettings::fromP2(const phg_p2_settings& p2) {
+    GpuBneeptSettings s;
+    s.width = p2.width;
+    s.height = p2.height;
+    s.spp = p2.spp;
+    s.max_bounces = p2.max_bounces;
+    s.estimator = p2.estimator;
+    s.filter = p2.filter;
+    return s;
 }
 
 }  // namespace phg

@@ -46,6 +46,18 @@ struct GpuBneeptSettings {
     uint64_t seed = 0;
     uint32_t chunk_spp = 4;         // samples per pixel per progressive chunk
     int32_t device = 0;
+    uint32_t estimator = PHG_ESTIMATOR_BNEEPT;
+    uint32_t filter = PHG_FILTER_BOX;
+
+    // Settings of a .p2 scene (frame size, samples, estimator, filter, bounce cap).
+    static GpuBneeptSettings fromP2(const phg_p2_settings& p2);
+};
+
+// Region update of asyncPollUpdatedRegions (RenderRegionStatus): the rectangle of pixels
+// that changed since the last poll, and whether rendering has finished.
+struct RenderRegionStatus {
+    uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    bool finished = false;
 };
 
 class GpuBneeptRenderer {
@@ -68,6 +80,11 @@ public:
     void asyncPeekFrame(std::size_t layerIndex, HdrRgbFrame& out_frame);
     // Renderer::getObservationInfo: names of the stats above.
     std::vector<std::string> getObservationInfo() const;
+    // Renderer::asyncPollUpdatedRegions: the frame region updated since the last poll (the
+    // whole frame after each finished chunk); returns the number of regions written (0 or 1).
+    std::size_t asyncPollUpdatedRegions(RenderRegionStatus* out_regions, std::size_t capacity);
+    // phSaveFrame: the developed frame as a .pfm file (PfmFile::save layout).
+    void saveFrame(const std::string& path);
 
 private:
     void check(phg_status st, const char* what);
@@ -78,6 +95,7 @@ private:
     std::vector<double> m_film;          // host copy of the last finished chunk's film
     mutable std::mutex m_filmMutex;
     std::atomic<uint64_t> m_sppDone{0};
+    std::atomic<uint64_t> m_sppPolled{0};
     std::atomic<uint64_t> m_elapsedMs{0};
     std::atomic<double> m_pathsPerSecond{0.0};
 };

@@ -56,7 +56,7 @@ class phg_camera(C.Structure):
 class phg_batch_job(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("estimator", C.c_uint32),
-                ("camera", phg_camera)]
+                ("camera", phg_camera), ("filter", C.c_uint32), ("pad2", C.c_uint32 * 3)]
 
 
 class phg_scene_desc(C.Structure):
@@ -109,7 +109,8 @@ class phg_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("spp_begin", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("spp_per_pass", C.c_uint32), ("seed", C.c_uint64),
                 ("flags", C.c_uint32), ("guide_prob", C.c_float), ("guide_lr", C.c_float),
-                ("guide_batch", C.c_uint32), ("guide_steps", C.c_uint32), ("estimator", C.c_uint32)]
+                ("guide_batch", C.c_uint32), ("guide_steps", C.c_uint32), ("estimator", C.c_uint32),
+                ("filter", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class phg_stats(C.Structure):
@@ -142,7 +143,7 @@ EXPORTED = ["phg_create", "phg_destroy", "phg_last_error", "phg_upload_scene", "
             "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_gradients",
             "phg_guide_train", "phg_scene_generate", "phg_scene_free", "phg_upload_scenes", "phg_render_batch",
             "phg_batch_film_read", "phg_set_tuning", "phg_get_tuning", "phg_autotune_batch", "phg_scene_parse_p2",
-            "phg_scene_load_p2"]
+            "phg_scene_load_p2", "phg_film_develop", "phg_film_write_pfm"]
 
 _lib = None
 
@@ -175,6 +176,8 @@ def lib() -> C.CDLL:
     L.phg_scene_generate.argtypes = [C.POINTER(phg_gen_opts), C.POINTER(C.POINTER(phg_scene_host))]
     L.phg_scene_free.argtypes = [C.POINTER(phg_scene_host)]
     L.phg_scene_free.restype = None
+    L.phg_film_develop.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+    L.phg_film_write_pfm.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_char_p]
     L.phg_scene_parse_p2.argtypes = [C.c_char_p, C.POINTER(phg_p2_settings), C.POINTER(C.POINTER(phg_scene_host)),
                                      C.c_char_p, C.c_uint32]
     L.phg_scene_load_p2.argtypes = [C.c_char_p, C.POINTER(phg_p2_settings), C.POINTER(C.POINTER(phg_scene_host)),
@@ -223,7 +226,7 @@ def _job_array(jobs):
         cam = j.get("camera")
         arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
                                j.get("seed", 0), JOB_CAMERA if cam else 0, j.get("estimator", 0),
-                               _camera_struct(cam) if cam else phg_camera())
+                               _camera_struct(cam) if cam else phg_camera(), j.get("filter", 0), (C.c_uint32 * 3)(0, 0, 0))
     return arr
 
 
@@ -480,11 +483,11 @@ class Context:
     def render(self, spp: int, max_bounces: int = 10000, seed: int = 0, spp_begin: int = 0,
                spp_per_pass: int = 0, count_traversal: bool = False, guided: bool = False, train: bool = False,
                guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4,
-               graph: bool = False, estimator: int = 0):
+               graph: bool = False, estimator: int = 0, filter: int = 0):
         flags = (RENDER_COUNT_TRAVERSAL if count_traversal else 0) | (RENDER_GUIDED if guided else 0) | \
             (RENDER_TRAIN if train else 0) | (RENDER_GRAPH if graph else 0)
         o = phg_render_opts(self.width, self.height, spp, spp_begin, max_bounces, spp_per_pass, seed, flags,
-                            guide_prob, guide_lr, guide_batch, guide_steps, estimator)
+                            guide_prob, guide_lr, guide_batch, guide_steps, estimator, filter, 0)
         self._check(self._L.phg_render(self._h, C.byref(o)), "phg_render")
 
     # ---- guiding network
@@ -524,6 +527,15 @@ class Context:
         s = phg_stats()
         self._check(self._L.phg_get_stats(self._h, C.byref(s)), "phg_get_stats")
         return {k: getattr(s, k) for k, _ in phg_stats._fields_ if k != "pad"}
+
+
+def write_pfm(film: np.ndarray, path: str):
+    """Develop an (H, W, 8) FP64 film and write it as a .pfm file (phg_film_write_pfm)."""
+    f = np.ascontiguousarray(film, np.float64)
+    h, w = f.shape[:2]
+    st = lib().phg_film_write_pfm(f.ctypes.data, w, h, path.encode())
+    if st != PHG_OK:
+        raise PhgError(f"phg_film_write_pfm: {STATUS_NAMES.get(st, st)}")
 
 
 def develop(film: np.ndarray) -> np.ndarray:

@@ -389,6 +389,25 @@ def test_film_gaussian_weighted_mean():
     assert w > 0 and np.allclose(out[0, 0, 0], out[0, 0, 1]) and 0.3 < out[0, 0, 0] < 0.7
 
 
+def test_film_filter_shapes_closed_forms():
+    """One sample at the centre of pixel 2 of a 5 x 1 film: the weights at offsets -2..2 are
+    f(dx) f(0).  Mitchell-Netravali (b = c = 1/3): f(0) = (6 - 2b)/6, f(1) = b/6, f(2) = 0;
+    Blackman-Harris (radius 2): 1 at the centre, 0.35875 - 0.14128 at distance 1;
+    submerged Gaussian (sigma 0.5): (exp(-2 dx^2) - exp(-16)) relative to the centre."""
+    def w(fid):
+        s, _ = splat(fid, 5, 1, [[2.5, 0.5]], [[1, 1, 1]])
+        return s[0, :, 3]
+    b = 1 / 3
+    mn = w(4)
+    assert np.allclose(mn, [0, b / 6 * (6 - 2 * b) / 6, ((6 - 2 * b) / 6) ** 2, b / 6 * (6 - 2 * b) / 6, 0], atol=1e-15)
+    bh = w(2)
+    assert np.isclose(bh[2], 1.0) and np.allclose(bh[[1, 3]], 0.35875 - 0.14128)
+    assert np.allclose(bh[[0, 4]], 0.35875 - 0.48829 + 0.14128 - 0.01168, atol=1e-12)
+    g = w(1)
+    e = np.exp(-16.0)
+    assert np.allclose(g[[1, 3]] / g[2], (np.exp(-2.0) - e) / (1 - e))
+
+
 # ---------------------------------------------------------------- camera (PinholeCamera.cpp)
 def test_camera_rays_horizontal_fov():
     cam = orc.orc_camera()

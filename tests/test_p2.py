@@ -132,3 +132,22 @@ def test_reference_cornell_scene_loads():
     assert np.allclose(sc.emitters["radiance"][0], 500 / (9 * np.pi), rtol=1e-5)
     v = sc.verts.reshape(-1, 3)
     assert np.allclose(v.min(axis=0), -5) and np.allclose(v.max(axis=0), 5)
+
+
+def test_pfm_writer_layout(tmp_path):
+    """phg_film_write_pfm writes what PfmFile::save writes: 'PF', '<w> <h>', '-1' lines, then
+    developed float32 RGB, bottom row first."""
+    rng = np.random.default_rng(3)
+    H, W = 3, 4
+    film = np.zeros((H, W, 8))
+    film[..., :3] = rng.random((H, W, 3)) * 5
+    film[..., 3] = rng.integers(1, 9, (H, W))
+    film[1, 2, 3] = 0.0                                    # no samples: develops to 0
+    path = str(tmp_path / "frame.pfm")
+    phg.write_pfm(film, path)
+    data = open(path, "rb").read()
+    header = b"PF\n4 3\n-1\n"
+    assert data[:len(header)] == header
+    px = np.frombuffer(data[len(header):], "<f4").reshape(H, W, 3)
+    expect = np.where(film[..., 3:4] > 0, film[..., :3] / np.where(film[..., 3:4] > 0, film[..., 3:4], 1), 0)
+    assert np.allclose(px, expect.astype(np.float32)) and np.all(px[1, 2] == 0)
