@@ -94,6 +94,8 @@ struct Ctx {
     DevBuf<float> g_train;       // collected training records
     DevBuf<float> g_q;           // staging for the guide entry points
     DevBuf<uint32_t> g_counters;
+    DevBuf<uint16_t> g_ws_x, g_ws_dz0, g_ws_dz1;   // training-step scratch (GuideWorkspace)
+    DevBuf<float> g_ws_part;
     bool g_init = false, g_ready = false;
     uint32_t g_steps = 0, g_pass_salt = 0;
     float g_keep = 1.0f;
@@ -183,15 +185,37 @@ static phg_status guide_load(Ctx* c, const float* host_params) {
     return PHG_OK;
 }
 
+// Scratch of training steps of up to nrec records.
+static phg_status guide_workspace(Ctx* c, uint32_t nrec, GuideWorkspace* ws) {
+    const uint32_t cap = (nrec + 31u) & ~31u;
+    const uint32_t parts = c->cu_count * 4, slices = 32;
+    PHG_HIP(c, c->g_ws_x.ensure((size_t)kGuideNetLayers * cap * 64));
+    PHG_HIP(c, c->g_ws_dz0.ensure((size_t)cap * 64));
+    PHG_HIP(c, c->g_ws_dz1.ensure((size_t)cap * 64));
+    const size_t part_floats = (size_t)kGuideNetLayers * (parts + 4) * (64 * 64 + 64);
+    PHG_HIP(c, c->g_ws_part.ensure(part_floats + (size_t)kGuideNetLayers * slices * (64 * 64 + 64)));
+    ws->X = c->g_ws_x.p; ws->dz0 = c->g_ws_dz0.p; ws->dz1 = c->g_ws_dz1.p; ws->partial = c->g_ws_part.p;
+    ws->partial2 = c->g_ws_part.p + part_floats;
+    ws->slices = slices;
+    ws->cap_records = cap;
+    ws->max_parts = parts;
+    ws->max_blocks = c->cu_count * 4;
+    return PHG_OK;
+}
+
 // `steps` Adam steps over device records, minibatches taken round robin
 static phg_status guide_train_device(Ctx* c, const float* d_rec, uint32_t nrec, uint32_t batch, float lr, uint32_t steps) {
     if (nrec == 0 || steps == 0) return PHG_OK;
     const uint32_t bs = std::min(batch ? batch : nrec, nrec);
-    const uint32_t grid = std::min<uint32_t>(c->cu_count, (bs + 31) / 32);
+    GuideWorkspace ws{};
+    {
+        const phg_status st = guide_workspace(c, bs, &ws);
+        if (st != PHG_OK) return st;
+    }
     uint32_t off = 0;
     for (uint32_t k = 0; k < steps; ++k) {
         const uint32_t n = std::min(bs, nrec - off);
-        PHG_HIP(c, guide_gradients(guide_net(c), d_rec + (size_t)off * kGuideRecordFloats, n, c->g_grads.p, grid, c->stream));
+        PHG_HIP(c, guide_gradients(guide_net(c), d_rec + (size_t)off * kGuideRecordFloats, n, c->g_grads.p, ws, c->stream));
         PHG_HIP(c, guide_adam(guide_net(c), c->g_grads.p, c->g_m.p, c->g_v.p, lr, (int)(++c->g_steps), c->stream));
         PHG_HIP(c, guide_prepare(guide_net(c), c->stream));
         off += n;
@@ -247,6 +271,7 @@ phg_status phg_destroy(void* ctx) {
     c->spill.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
     c->q_nvis.free_(); c->q_ntri.free_();
     c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
+    c->g_ws_x.free_(); c->g_ws_dz0.free_(); c->g_ws_dz1.free_(); c->g_ws_part.free_();
     c->g_wn.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
@@ -1099,8 +1124,12 @@ phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, floa
     PHG_HIP(c, c->g_q.ensure(n * kGuideRecordFloats));
     PHG_HIP(c, hipMemcpyAsync(c->g_q.p, records, n * kGuideRecordFloats * sizeof(float), hipMemcpyHostToDevice, c->stream));
     PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
-    const uint32_t grid = std::min<uint32_t>(c->cu_count, (uint32_t)((n + 31) / 32));
-    PHG_HIP(c, guide_gradients(guide_net(c), c->g_q.p, (uint32_t)n, c->g_grads.p, grid, c->stream));
+    GuideWorkspace ws{};
+    {
+        const phg_status st = guide_workspace(c, (uint32_t)n, &ws);
+        if (st != PHG_OK) return st;
+    }
+    PHG_HIP(c, guide_gradients(guide_net(c), c->g_q.p, (uint32_t)n, c->g_grads.p, ws, c->stream));
     PHG_HIP(c, hipMemcpyAsync(grads, c->g_grads.p, kGuideParamCount * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
     PHG_HIP(c, hipStreamSynchronize(c->stream));

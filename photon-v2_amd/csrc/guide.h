@@ -28,14 +28,30 @@ struct GuideNetDev {
 constexpr uint32_t kGuideParamCount = 19240;
 constexpr uint32_t kGuideRecordFloats = 16;
 constexpr uint32_t kGuideOutputs = 40;
+constexpr uint32_t kGuideNetLayers = 5;     // dense layers of the network (scratch sizing)
 
 hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st);
 hipError_t guide_infer_features(const GuideNetDev& net, const float* features, uint32_t n, float* out, int raw,
                                 uint32_t grid, hipStream_t st);
 hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const phg_ray* rays, const TriRec* tris,
                             const GuideScene& gs, const uint32_t* count_ptr, float* out, uint32_t grid, hipStream_t st);
-hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nrec, float* grads, uint32_t grid,
-                           hipStream_t st);
+// Scratch of a training step (device memory): layer inputs X (5 x cap x 64 bf16), two dZ
+// buffers (cap x 64 bf16 each), per-wave dW / db partials (5 x (max_parts + 4) x 4160 floats)
+// and their first-level sums (5 x slices x 4160 floats).
+struct GuideWorkspace {
+    uint16_t* X;
+    uint16_t* dz0;
+    uint16_t* dz1;
+    float* partial;
+    float* partial2;
+    uint32_t cap_records;    // records per step it holds (multiple of 32)
+    uint32_t max_parts;      // waves of the backward kernels
+    uint32_t max_blocks;     // blocks of the forward kernel
+    uint32_t slices;         // first-level reduction slices
+};
+// Gradient of the mean record loss over nrec records, written to grads (19240 floats).
+hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nrec, float* grads,
+                           const GuideWorkspace& ws, hipStream_t st);
 hipError_t guide_adam(const GuideNetDev& net, float* grads, float* m, float* v, float lr, int step, hipStream_t st);
 hipError_t guide_fill(float* p, uint32_t n, float v, hipStream_t st);
 // training records of a finished pass: incident radiance along each recorded sampled direction

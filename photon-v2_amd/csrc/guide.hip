@@ -8,6 +8,8 @@
 // plain ([in][out], B operand of the backward product dX = dZ W^T), both 16-byte reads.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "guide.h"
 #include "guide_device.h"
 
@@ -24,7 +26,7 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-__device__ __forceinline__ int layer_out(int l) { return l == kGuideLayers - 1 ? kGuideOut : kGuideHid; }
+__host__ __device__ __forceinline__ int layer_out(int l) { return l == kGuideLayers - 1 ? kGuideOut : kGuideHid; }
 
 // fp32 master parameters -> bf16 transposed / plain copies (rows padded to 64) and fp32 biases
 __global__ void k_guide_prepare(const float* __restrict__ params, uint16_t* __restrict__ wt_bits,
@@ -140,114 +142,175 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideNetDev net, InferSourc
     }
 }
 
-// Training: gradients of the mean record loss -w log(a q + (1 - a) b) over a batch of records
-// (9 features, direction xyz, weight w, BSDF density b, selection probability a, pad).  One wave per block, 32 records per iteration;
-// per-block gradient accumulator in LDS, added to the global gradient at the end.
-__global__ __launch_bounds__(64) void k_guide_train(GuideNetDev net, const float* __restrict__ rec, uint32_t nrec,
-                                                    float inv_batch, float* __restrict__ grads) {
-    __shared__ __attribute__((aligned(16))) __bf16 s_act[kGuideLayers][32 * 64];  // inputs of each layer
-    __shared__ __attribute__((aligned(16))) __bf16 s_dz[2][32 * 64];
-    __shared__ float s_out[32 * kGuideOutPad];
-    __shared__ float s_g[kGuideParams];
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    for (int i = lane; i < kGuideParams; i += 64) s_g[i] = 0.0f;
-    wave_sync();
-    for (uint32_t base = blockIdx.x * 32; base < nrec; base += gridDim.x * 32) {
-        // encode 32 records: lane handles row lane % 32, encoding half lane / 32
-        {
-            const int row = lane & 31, half = lane >> 5;
-            const uint32_t s = base + row;
-            float x[9];
-            for (int k = 0; k < 9; ++k) x[k] = s < nrec ? rec[kGuideRecordFloats * (size_t)s + k] : 0.0f;
-            for (int j = 32 * half; j < 32 * half + 32; ++j) s_act[0][row * 64 + j] = (__bf16)guide_encode(x, j);
-        }
+// Training step as batch GEMMs over the whole minibatch (N records):
+//   forward   X_0 = encode(features); Z_l = X_l W_l + b_l; X_{l+1} = relu(Z_l)   (store X_l)
+//   output    dZ_4 = d(mean record loss) / dZ_4                                  (per record)
+//   backward  dW_l = X_l^T dZ_l, db_l = column sums of dZ_l,
+//             dZ_{l-1} = (dZ_l W_l^T) * [X_l > 0]                                (layer by layer)
+// X_l and dZ_l are bf16 [record][64] in global memory (the matrix-core operands); every wave
+// of the backward kernel reduces dW over its own contiguous run of records in registers and
+// writes one partial, and k_train_reduce sums the partials, so no atomics are involved and
+// the result does not depend on scheduling.
+
+// Forward over 16-record tiles (4 waves per block, weights in LDS), storing every layer's
+// input and the output-layer gradient.
+__global__ __launch_bounds__(256) void k_train_forward(GuideNetDev net, const float* __restrict__ rec, uint32_t nrec,
+                                                       uint32_t npad, float inv_batch, __bf16* __restrict__ X,
+                                                       __bf16* __restrict__ dz_out) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_wt[kGuideLayers * 64 * 64];
+    __shared__ float s_bias[kGuideLayers * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 s_act[4][2][16 * 64];
+    __shared__ float s_out[4][16 * kGuideOutPad];
+    for (int i = threadIdx.x; i < kGuideLayers * 64 * 64 / 8; i += 256)
+        reinterpret_cast<bf16x8*>(s_wt)[i] = reinterpret_cast<const bf16x8*>(net.wt)[i];
+    for (int i = threadIdx.x; i < kGuideLayers * 64; i += 256) s_bias[i] = net.bias[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    for (uint32_t tile = blockIdx.x * 4 + wave; tile * 16 < npad; tile += gridDim.x * 4) {
+        const uint32_t q = tile * 16 + r;
+        float x[9];
+        if (q < nrec) for (int k = 0; k < 9; ++k) x[k] = rec[kGuideRecordFloats * (size_t)q + k];
+        else for (int k = 0; k < 9; ++k) x[k] = 0.0f;
+        __bf16* a0 = s_act[wave][0];
+        for (int j = 16 * g; j < 16 * g + 16; ++j) a0[r * 64 + j] = q < nrec ? (__bf16)guide_encode(x, j) : (__bf16)0.0f;
         wave_sync();
-        // forward, two 16-row tiles per layer
         for (int l = 0; l < kGuideLayers; ++l) {
+            // store this layer's input tile: 16 rows x 64 bf16 = 128 x 16 bytes, two per lane
+            const __bf16* src = s_act[wave][l & 1];
+            __bf16* dst = X + ((size_t)l * npad + (size_t)tile * 16) * 64;
+            reinterpret_cast<bf16x8*>(dst)[lane] = reinterpret_cast<const bf16x8*>(src)[lane];
+            reinterpret_cast<bf16x8*>(dst)[lane + 64] = reinterpret_cast<const bf16x8*>(src)[lane + 64];
             const bool last = l == kGuideLayers - 1;
-            for (int t = 0; t < 2; ++t)
-                layer_forward(s_act[l] + t * 16 * 64, reinterpret_cast<const __bf16*>(net.wt) + l * 4096, net.bias + l * 64,
-                              last ? nullptr : s_act[l + 1] + t * 16 * 64, s_out + t * 16 * kGuideOutPad, last);
+            layer_forward(src, s_wt + l * 4096, s_bias + l * 64, s_act[wave][(l + 1) & 1], s_out[wave], last);
             wave_sync();
         }
-        // output gradient per record
-        if (lane < 32) {
-            const uint32_t s = base + lane;
+        // output-layer gradient of each record, one lane per record
+        if (g == 0) {
             float gr[kGuideOut];
-            if (s < nrec) {
-                const float* R = rec + kGuideRecordFloats * (size_t)s;
-                guide_output_grad(s_out + lane * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], R[13], R[14], gr);
+            if (q < nrec) {
+                const float* R = rec + kGuideRecordFloats * (size_t)q;
+                guide_output_grad(s_out[wave] + r * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], R[13], R[14], gr);
             } else {
                 for (int k = 0; k < kGuideOut; ++k) gr[k] = 0.0f;
             }
-            for (int k = 0; k < 64; ++k) s_dz[0][lane * 64 + k] = (__bf16)(k < kGuideOut ? gr[k] * inv_batch : 0.0f);
-            for (int k = 0; k < kGuideOut; ++k) atomicAdd(&s_g[kGuideParamOffsets[9] + k], gr[k] * inv_batch);
+            __bf16* d = dz_out + (size_t)q * 64;
+            for (int k = 0; k < 64; ++k) d[k] = (__bf16)(k < kGuideOut ? gr[k] * inv_batch : 0.0f);
         }
         wave_sync();
-        int cur = 0;
-        for (int l = kGuideLayers - 1; l >= 0; --l) {
-            const int nout = layer_out(l);
-            const __bf16* dz = s_dz[cur];
-            const __bf16* X = s_act[l];
-            // dW_l = X^T dz over the 32 records: M = in (4 tiles), N = out, K = 32
-            for (int mt = 0; mt < 4; ++mt) {
-                bf16x8 a;
+    }
+}
+
+// Backward through layer l for a contiguous run of 32-record sub-tiles per wave: dW_l and db_l
+// partials in registers, and (l > 0) dZ_{l-1}.  Sub-tiles of X_l and dZ_l are staged in LDS
+// so that the transposed operands of X_l^T dZ_l are LDS gathers.
+__global__ __launch_bounds__(256) void k_train_backward(const __bf16* __restrict__ wn_l, const __bf16* __restrict__ X_l,
+                                                        const __bf16* __restrict__ dz, __bf16* __restrict__ dz_prev,
+                                                        uint32_t npad, uint32_t chunk, int l, float* __restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_w[64 * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[4][32 * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dz[4][32 * 64];
+    for (int i = threadIdx.x; i < 64 * 64 / 8; i += 256)
+        reinterpret_cast<bf16x8*>(s_w)[i] = reinterpret_cast<const bf16x8*>(wn_l)[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const uint32_t wid = blockIdx.x * 4 + wave;
+    const int nout = layer_out(l), ntiles = (nout + 15) / 16;
+    f32x4 acc[4][4];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) a[j] = X[(8 * g + j) * 64 + mt * 16 + r];
-                for (int nt = 0; nt * 16 < nout; ++nt) {
-                    bf16x8 b;
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) b[j] = dz[(8 * g + j) * 64 + nt * 16 + r];
-                    f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-                    c = mfma16(a, b, c);
-                    const int col = nt * 16 + r;
-                    if (col < nout) {
+        for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float acc_b = 0.0f;
+    __bf16* sx = s_x[wave];
+    __bf16* sd = s_dz[wave];
+    const uint32_t begin = wid * chunk, end = min(begin + chunk, npad);
+    for (uint32_t base = begin; base < end; base += 32) {
+        // stage 32 records of X_l and dZ_l: 32 x 64 bf16 = 256 x 16 bytes each, 4 per lane
+        const bf16x8* gx = reinterpret_cast<const bf16x8*>(X_l + (size_t)base * 64);
+        const bf16x8* gd = reinterpret_cast<const bf16x8*>(dz + (size_t)base * 64);
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const int in = mt * 16 + 4 * g + i;
-                            atomicAdd(&s_g[kGuideParamOffsets[2 * l] + in * nout + col], c[i]);
-                        }
-                    }
-                }
-            }
-            // db_l for hidden layers (the output layer's bias gradient was added above)
-            if (l < kGuideLayers - 1) {
-                float acc = 0.0f;
-                for (int s = 0; s < 32; ++s) acc += (float)dz[s * 64 + lane];
-                atomicAdd(&s_g[kGuideParamOffsets[2 * l + 1] + lane], acc);
-            }
-            if (l > 0) {
-                // dX = dz W^T (M = 32 records, N = 64 in, K = 64 out), then the ReLU mask of X
-                __bf16* dz_next = s_dz[cur ^ 1];
-                const __bf16* wn_l = reinterpret_cast<const __bf16*>(net.wn) + l * 4096;
-                for (int t = 0; t < 2; ++t) {
-                    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(dz + (t * 16 + r) * 64 + 8 * g);
-                    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(dz + (t * 16 + r) * 64 + 32 + 8 * g);
-                    for (int nt = 0; nt < 4; ++nt) {
-                        const int col = nt * 16 + r;  // input index of layer l
-                        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(wn_l + col * 64 + 8 * g);
-                        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(wn_l + col * 64 + 32 + 8 * g);
-                        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-                        c = mfma16(a0, b0, c);
-                        c = mfma16(a1, b1, c);
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const int row = t * 16 + 4 * g + i;
-                            const bool active = (float)X[row * 64 + col] > 0.0f;
-                            dz_next[row * 64 + col] = (__bf16)(active ? c[i] : 0.0f);
-                        }
-                    }
-                }
-                cur ^= 1;
-            }
-            wave_sync();
+        for (int k = 0; k < 4; ++k) {
+            reinterpret_cast<bf16x8*>(sx)[lane + 64 * k] = gx[lane + 64 * k];
+            reinterpret_cast<bf16x8*>(sd)[lane + 64 * k] = gd[lane + 64 * k];
         }
+        wave_sync();
+        // dW_l += X^T dZ: A = X^T (16 inputs x 32 records), B = dZ (32 records x 16 outputs)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            bf16x8 a;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[j] = sx[(8 * g + j) * 64 + mt * 16 + r];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if (nt >= ntiles) continue;
+                bf16x8 b;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) b[j] = sd[(8 * g + j) * 64 + nt * 16 + r];
+                acc[mt][nt] = mfma16(a, b, acc[mt][nt]);
+            }
+        }
+        // db_l: column sums of dZ (one column per lane)
+        for (int s = 0; s < 32; ++s) acc_b += (float)sd[s * 64 + lane];
+        if (l > 0) {
+            // dZ_{l-1} = (dZ W^T) masked by X_l > 0: rows = records (2 tiles), cols = inputs
+            for (int t = 0; t < 2; ++t) {
+                const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(sd + (t * 16 + r) * 64 + 8 * g);
+                const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(sd + (t * 16 + r) * 64 + 32 + 8 * g);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const int col = nt * 16 + r;
+                    const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(s_w + col * 64 + 8 * g);
+                    const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(s_w + col * 64 + 32 + 8 * g);
+                    f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+                    c = mfma16(a0, b0, c);
+                    c = mfma16(a1, b1, c);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int row = t * 16 + 4 * g + i;
+                        const bool active = (float)sx[row * 64 + col] > 0.0f;
+                        dz_prev[(size_t)(base + row) * 64 + col] = (__bf16)(active ? c[i] : 0.0f);
+                    }
+                }
+            }
+        }
+        wave_sync();
     }
-    wave_sync();
-    for (int i = lane; i < kGuideParams; i += 64) {
-        const float v = s_g[i];
-        if (v != 0.0f) atomicAdd(&grads[i], v);
-    }
+    // this wave's partial: dW [64 in][64 out] then db [64]
+    float* P = partial + (size_t)wid * (64 * 64 + 64);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) P[(mt * 16 + 4 * g + i) * 64 + nt * 16 + r] = acc[mt][nt][i];
+    P[64 * 64 + lane] = acc_b;
+}
+
+// First-level sums of the waves' partials, all layers in one launch: grid (17, slices, 5);
+// slice s of layer l adds partials s, s + slices, ... in that order.
+__global__ void k_train_reduce_parts(const float* __restrict__ partial, size_t layer_stride, uint32_t nparts,
+                                     float* __restrict__ partial2) {
+    const int l = blockIdx.z;
+    const uint32_t s = blockIdx.y, slices = gridDim.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 64 * 64 + 64) return;
+    const float* P = partial + (size_t)l * layer_stride;
+    float acc = 0.0f;
+    for (uint32_t p = s; p < nparts; p += slices) acc += P[(size_t)p * (64 * 64 + 64) + e];
+    partial2[((size_t)l * slices + s) * (64 * 64 + 64) + e] = acc;
+}
+
+// grads[layer l] = sum of the slices (dW entries of real outputs, then biases); grid (17, 1, 5).
+__global__ void k_train_reduce(const float* __restrict__ partial2, uint32_t slices, float* __restrict__ grads) {
+    const int l = blockIdx.z;
+    const int nout = layer_out(l);
+    const int n = 64 * nout + nout;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int src = i < 64 * nout ? (i / nout) * 64 + (i % nout) : 64 * 64 + (i - 64 * nout);
+    float s = 0.0f;
+    for (uint32_t p = 0; p < slices; ++p) s += partial2[((size_t)l * slices + p) * (64 * 64 + 64) + src];
+    grads[kGuideParamOffsets[2 * l] + i] = s;
 }
 
 // Adam (Kingma & Ba 2015) on the fp32 master parameters; clears the gradient.
@@ -340,10 +403,33 @@ hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const ph
     return hipGetLastError();
 }
 
-hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nrec, float* grads, uint32_t grid,
-                           hipStream_t st) {
+hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nrec, float* grads,
+                           const GuideWorkspace& ws, hipStream_t st) {
     if (nrec == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(64), 0, st, net, rec, nrec, 1.0f / (float)nrec, grads);
+    if (nrec > ws.cap_records) return hipErrorInvalidValue;
+    const uint32_t npad = (nrec + 31u) & ~31u;
+    __bf16* X = reinterpret_cast<__bf16*>(ws.X);
+    __bf16* dz[2] = {reinterpret_cast<__bf16*>(ws.dz0), reinterpret_cast<__bf16*>(ws.dz1)};
+    const uint32_t fwd_blocks = std::min<uint32_t>(ws.max_blocks, (npad / 16 + 3) / 4);
+    hipLaunchKernelGGL(k_train_forward, dim3(fwd_blocks), dim3(256), 0, st, net, rec, nrec, npad, 1.0f / (float)nrec, X,
+                       dz[0]);
+    // backward: waves x chunk covers npad in whole 32-record sub-tiles, at least 4 per wave
+    const uint32_t tiles = npad / 32;
+    const uint32_t waves = std::max<uint32_t>(1u, std::min<uint32_t>(ws.max_parts, tiles / 4));
+    const uint32_t chunk = ((tiles + waves - 1) / waves) * 32;
+    const uint32_t used = (npad + chunk - 1) / chunk;
+    const uint32_t blocks = (used + 3) / 4;
+    int cur = 0;
+    for (int l = kGuideLayers - 1; l >= 0; --l) {
+        hipLaunchKernelGGL(k_train_backward, dim3(blocks), dim3(256), 0, st,
+                           reinterpret_cast<const __bf16*>(net.wn) + l * 4096, X + (size_t)l * npad * 64, dz[cur],
+                           dz[cur ^ 1], npad, chunk, l, ws.partial + (size_t)l * (ws.max_parts + 4) * (64 * 64 + 64));
+        cur ^= 1;
+    }
+    const uint32_t eblocks = (64 * 64 + 64 + 255) / 256;
+    hipLaunchKernelGGL(k_train_reduce_parts, dim3(eblocks, ws.slices, kGuideLayers), dim3(256), 0, st, ws.partial,
+                       (size_t)(ws.max_parts + 4) * (64 * 64 + 64), blocks * 4, ws.partial2);
+    hipLaunchKernelGGL(k_train_reduce, dim3(eblocks, 1, kGuideLayers), dim3(256), 0, st, ws.partial2, ws.slices, grads);
     return hipGetLastError();
 }
 
