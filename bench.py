@@ -122,6 +122,29 @@ def batch_leg(args):
 SCALE_VIEWS = [(2048, 2048), (1920, 1080), (1600, 900), (1280, 720), (1024, 1024), (960, 540), (800, 800), (640, 360)]
 
 
+# Algorithmic multiply-add work of the guiding MLP (64-64-64-64-64-40, 2 flops per multiply-add):
+# inference = forward; a training record = forward + dW for every layer + dZ propagation
+# through layers 4..1.
+MLP_FWD_FLOPS = 2 * (4 * 64 * 64 + 64 * 40)
+MLP_TRAIN_FLOPS = 2 * MLP_FWD_FLOPS + 2 * (64 * 40 + 3 * 64 * 64)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_utilisation(st):
+    """Matrix-core utilisation of the guiding MLP over a render's inference and training time."""
+    out = {"peak_tflops": MFMA_BF16_PEAK_TFLOPS, "dtype": "bf16 (fp32 accumulation)"}
+    if st["guide_infer_ms"] > 0:
+        t = st["guide_queries"] * MLP_FWD_FLOPS / (st["guide_infer_ms"] / 1e3) / 1e12
+        out["infer"] = {"queries": st["guide_queries"], "ms": round(st["guide_infer_ms"], 2), "tflops": round(t, 2),
+                        "frac": round(t / MFMA_BF16_PEAK_TFLOPS, 4)}
+    if st["guide_train_ms"] > 0:
+        t = st["guide_records"] * MLP_TRAIN_FLOPS / (st["guide_train_ms"] / 1e3) / 1e12
+        out["train"] = {"records": st["guide_records"], "ms": round(st["guide_train_ms"], 2), "tflops": round(t, 2),
+                        "frac": round(t / MFMA_BF16_PEAK_TFLOPS, 4),
+                        "timed": "record collection + gradient kernels + Adam (HIP events)"}
+    return out
+
+
 def scale_leg(args):
     """20M-triangle terrain world, 8 views of uneven resolution, guided with one shared network."""
     import numpy as np
@@ -171,7 +194,8 @@ def scale_leg(args):
             "spp": args.scale_spp, "max_bounces": 12, "paths": paths,
             "guided_trained": {"seconds": round(g_s, 4), "msamples_per_s": round(paths / g_s / 1e6, 3),
                                "guide_infer_ms": round(gs["guide_infer_ms"], 2),
-                               "guide_train_ms": round(gs["guide_train_ms"], 2), "guide_steps": gs["guide_steps"]},
+                               "guide_train_ms": round(gs["guide_train_ms"], 2), "guide_steps": gs["guide_steps"],
+                               "mlp": mlp_utilisation(gs)},
             "plain_graph": {"seconds": round(p_s, 4), "msamples_per_s": round(paths / p_s / 1e6, 3),
                             "mrays_per_s": round((ps["rays_primary"] + ps["rays_extension"] + ps["rays_shadow"]) / p_s / 1e6, 3)},
             "k_extend_roofline": {"achieved_gbps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBPS, 4),
@@ -202,9 +226,10 @@ def convergence_leg(args):
         plain = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, target=0.005)
     with phg.Context(0) as ctx:
         ctx.upload(scene)
+        gtot = {}
         guided = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, guided=True,
-                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=16, target=0.005)
-        gst = ctx.stats()
+                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=16, target=0.005,
+                                totals=gtot)
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
     for name, cur in (("plain", plain), ("guided", guided)):
         print(f"[bench] {name}: " + " ".join(f"{s}spp/{t:.2f}s/{e:.2e}" for s, t, e in cur), file=sys.stderr,
@@ -215,7 +240,7 @@ def convergence_leg(args):
             "guided_s": None if tg is None else round(tg, 4),
             "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
             "guided": {"guide_prob": 0.5, "train_spp": args.conv_train_spp, "steps_per_pass": 16, "lr": 1e-3,
-                       "batch": 1 << 18},
+                       "batch": 1 << 18, "mlp": mlp_utilisation(gtot)},
             "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
             "guided_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in guided]}
 

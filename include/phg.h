@@ -144,7 +144,7 @@ typedef struct phg_render_opts {
     uint32_t guide_steps;      /* PHG_RENDER_TRAIN: Adam steps after each pass */
     uint32_t estimator;        /* PHG_ESTIMATOR_*; guiding needs BNEEPT */
     uint32_t filter;           /* PHG_FILTER_*: film reconstruction filter (SampleFilter.cpp:8-56) */
-    uint32_t pad;
+    uint32_t guide_depth;      /* PHG_RENDER_GUIDED: path vertices (from the first hit) that sample the lobes; 0 = all */
 } phg_render_opts;
 
 enum {
@@ -180,7 +180,7 @@ typedef struct phg_stats {
     double guide_infer_ms;        /* GPU time in guiding-network inference (last render) */
     double guide_train_ms;        /* GPU time in guiding-network training (last render) */
     uint64_t guide_queries;       /* network evaluations (last render) */
-    uint64_t guide_records;       /* training records used (last render) */
+    uint64_t guide_records;       /* records processed by training steps (last render; a record per step counts once) */
     uint32_t guide_steps;         /* Adam steps taken so far on this context */
     uint32_t batch_passes;        /* wavefront passes of the last phg_render_batch */
     double graph_ms;              /* GPU time of the replayed pass graphs (PHG_RENDER_GRAPH / PHG_BATCH_GRAPH);
@@ -271,7 +271,7 @@ typedef struct phg_batch_opts {
     float guide_prob;          /* PHG_BATCH_GUIDED: probability of sampling the learned lobes */
     float guide_lr;            /* PHG_BATCH_TRAIN: Adam learning rate (0 = 1e-3) */
     uint32_t guide_batch;      /* PHG_BATCH_TRAIN: records per step (0 = 2^18) */
-    uint32_t pad;
+    uint32_t guide_depth;      /* PHG_BATCH_GUIDED: path vertices that sample the lobes (0 = all) */
 } phg_batch_opts;
 
 typedef struct phg_batch_result {

@@ -204,7 +204,8 @@ static phg_status guide_workspace(Ctx* c, uint32_t nrec, GuideWorkspace* ws) {
 }
 
 // `steps` Adam steps over device records, minibatches taken round robin
-static phg_status guide_train_device(Ctx* c, const float* d_rec, uint32_t nrec, uint32_t batch, float lr, uint32_t steps) {
+static phg_status guide_train_device(Ctx* c, const float* d_rec, uint32_t nrec, uint32_t batch, float lr, uint32_t steps,
+                                     uint64_t* processed = nullptr) {
     if (nrec == 0 || steps == 0) return PHG_OK;
     const uint32_t bs = std::min(batch ? batch : nrec, nrec);
     GuideWorkspace ws{};
@@ -219,6 +220,7 @@ static phg_status guide_train_device(Ctx* c, const float* d_rec, uint32_t nrec, 
         PHG_HIP(c, guide_adam(guide_net(c), c->g_grads.p, c->g_m.p, c->g_v.p, lr, (int)(++c->g_steps), c->stream));
         PHG_HIP(c, guide_prepare(guide_net(c), c->stream));
         off += n;
+        if (processed) *processed += n;
         if (off >= nrec) off = 0;
     }
     return PHG_OK;
@@ -658,7 +660,7 @@ struct RenderMode {
     bool count_traversal = false, guided = false, train = false;
     bool graph = false;           // replay a captured bounce loop per pass (bounded depth, unguided)
     float guide_prob = 0.5f, guide_lr = 1e-3f;
-    uint32_t guide_batch = 1u << 18, guide_steps = 4;
+    uint32_t guide_batch = 1u << 18, guide_steps = 4, guide_depth = 0;
 };
 
 struct PassPlan {
@@ -789,6 +791,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         pp.film_threads = pl.film_threads;
         pp.max_bounces = pl.max_bounces;
         pp.guide_prob = (guided && c->g_ready) ? m.guide_prob : 0.0f;
+        pp.guide_depth = m.guide_depth;
         pp.record = train ? 1u : 0u;
         pp.rec_bounces = rec_bounces;
         pp.segs = c->d_segs.p;
@@ -819,7 +822,8 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         float ms = 0; PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
         for (uint32_t b = 0;; ++b) {
-            const bool use_guide = guided && pp.guide_prob > 0.0f && b < pl.max_bounces;
+            const bool use_guide = guided && pp.guide_prob > 0.0f && b < pl.max_bounces &&
+                                   (m.guide_depth == 0 || b < m.guide_depth);
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
             PHG_HIP(c, launch_extend(sc, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
@@ -866,12 +870,12 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             PHG_HIP(c, hipStreamSynchronize(c->stream));
             const uint32_t nrec = std::min(cnt[0], train_cap);
             const uint32_t steps = m.guide_steps ? m.guide_steps : 4;
-            phg_status gs = guide_train_device(c, c->g_train.p, nrec, batch, m.guide_lr > 0 ? m.guide_lr : 1e-3f, steps);
+            phg_status gs = guide_train_device(c, c->g_train.p, nrec, batch, m.guide_lr > 0 ? m.guide_lr : 1e-3f, steps,
+                                               &g_records);
             if (gs != PHG_OK) return gs;
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));
             PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_gtrain += ms;
-            g_records += nrec;
             // keep about as many records as the next pass's steps consume
             const double want = (double)steps * batch;
             c->g_keep = cnt[1] > 0 ? (float)std::min(1.0, 1.25 * want / (double)cnt[1]) : 1.0f;
@@ -972,6 +976,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     m.train = (o->flags & PHG_RENDER_TRAIN) != 0;
     m.graph = (o->flags & PHG_RENDER_GRAPH) != 0;
     m.guide_prob = o->guide_prob; m.guide_lr = o->guide_lr; m.guide_batch = o->guide_batch; m.guide_steps = o->guide_steps;
+    m.guide_depth = o->guide_depth;
     c->g_scene = c->scenes[0].box;
     return render_passes(c, jobs, passes, c->film.p, m, nullptr);
 }
@@ -1052,6 +1057,7 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
         }
         m.guide_prob = bo->guide_prob; m.guide_lr = bo->guide_lr; m.guide_batch = bo->guide_batch;
         m.guide_steps = bo->guide_steps;
+        m.guide_depth = bo->guide_depth;
         c->g_scene = c->scenes[in[0].scene].box;
     }
     std::vector<double> done(n, 0.0);

@@ -82,7 +82,7 @@ struct PassParams {
     float guide_prob;        // guided shading: probability of sampling the learned lobes
     uint32_t record;         // guided shading: write training records
     uint32_t rec_bounces;    // bounces with a record slot
-    uint32_t pad;
+    uint32_t guide_depth;    // guided shading at bounces < guide_depth only (0 = every bounce)
     const PassSeg* segs;     // num_segs entries, ascending path_offset and pix_offset
 };
 

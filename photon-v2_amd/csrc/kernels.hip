@@ -356,7 +356,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(SceneDev sc, const PassPa
 #pragma unroll
                 for (int k = 0; k < (GUIDED ? 12 : 8); ++k) u[k] = s.u();
                 const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideOut : nullptr;
-                const float alpha = (GUIDED && pp.guide_prob > 0.0f && guide_lobes_ok(lob)) ? pp.guide_prob : 0.0f;
+                const bool guide_here = pp.guide_depth == 0 || bounce < pp.guide_depth;
+                const float alpha = (GUIDED && guide_here && pp.guide_prob > 0.0f && guide_lobes_ok(lob)) ? pp.guide_prob : 0.0f;
                 const phg_material m = sc.mats[tr.mat];
                 if (m.type == PHG_MAT_ABSORBER) stop = true;
                 if (!stop) {

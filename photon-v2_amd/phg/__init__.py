@@ -79,7 +79,7 @@ class phg_bvh_opts(C.Structure):
 class phg_batch_opts(C.Structure):
     _fields_ = [("paths_per_pass", C.c_uint32), ("order", C.c_uint32), ("flags", C.c_uint32),
                 ("guide_steps", C.c_uint32), ("guide_prob", C.c_float), ("guide_lr", C.c_float),
-                ("guide_batch", C.c_uint32), ("pad", C.c_uint32)]
+                ("guide_batch", C.c_uint32), ("guide_depth", C.c_uint32)]
 
 
 class phg_batch_result(C.Structure):
@@ -110,7 +110,7 @@ class phg_render_opts(C.Structure):
                 ("max_bounces", C.c_uint32), ("spp_per_pass", C.c_uint32), ("seed", C.c_uint64),
                 ("flags", C.c_uint32), ("guide_prob", C.c_float), ("guide_lr", C.c_float),
                 ("guide_batch", C.c_uint32), ("guide_steps", C.c_uint32), ("estimator", C.c_uint32),
-                ("filter", C.c_uint32), ("pad", C.c_uint32)]
+                ("filter", C.c_uint32), ("guide_depth", C.c_uint32)]
 
 
 class phg_stats(C.Structure):
@@ -406,14 +406,15 @@ class Context:
 
     def render_batch(self, jobs, paths_per_pass: int = 0, order: int = BATCH_ORDER_GIVEN,
                      count_traversal: bool = False, graph: bool = False, guided: bool = False, train: bool = False,
-                     guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4):
+                     guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4,
+                     guide_depth: int = 0):
         """jobs: sequence of dicts with scene, width, height, spp, max_bounces, seed.
         Returns per job (done_ms, paths)."""
         arr = _job_array(jobs)
         res = (phg_batch_result * len(jobs))()
         flags = (BATCH_COUNT_TRAVERSAL if count_traversal else 0) | (BATCH_GRAPH if graph else 0) | \
             (BATCH_GUIDED if guided else 0) | (BATCH_TRAIN if train else 0)
-        opts = phg_batch_opts(paths_per_pass, order, flags, guide_steps, guide_prob, guide_lr, guide_batch, 0)
+        opts = phg_batch_opts(paths_per_pass, order, flags, guide_steps, guide_prob, guide_lr, guide_batch, guide_depth)
         self._check(self._L.phg_render_batch(self._h, arr, len(jobs), C.byref(opts), res), "phg_render_batch")
         self._batch_shapes = [(j["height"], j["width"]) for j in jobs]
         return [(r.done_ms, r.paths) for r in res]
@@ -483,11 +484,11 @@ class Context:
     def render(self, spp: int, max_bounces: int = 10000, seed: int = 0, spp_begin: int = 0,
                spp_per_pass: int = 0, count_traversal: bool = False, guided: bool = False, train: bool = False,
                guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4,
-               graph: bool = False, estimator: int = 0, filter: int = 0):
+               graph: bool = False, estimator: int = 0, filter: int = 0, guide_depth: int = 0):
         flags = (RENDER_COUNT_TRAVERSAL if count_traversal else 0) | (RENDER_GUIDED if guided else 0) | \
             (RENDER_TRAIN if train else 0) | (RENDER_GRAPH if graph else 0)
         o = phg_render_opts(self.width, self.height, spp, spp_begin, max_bounces, spp_per_pass, seed, flags,
-                            guide_prob, guide_lr, guide_batch, guide_steps, estimator, filter, 0)
+                            guide_prob, guide_lr, guide_batch, guide_steps, estimator, filter, guide_depth)
         self._check(self._L.phg_render(self._h, C.byref(o)), "phg_render")
 
     # ---- guiding network

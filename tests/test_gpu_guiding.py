@@ -211,7 +211,10 @@ def test_guided_furnace_stays_unbiased(albedo, radiance):
     assert abs(mean - expect) < 4 * se + 1e-4
 
 
-def test_guided_matches_plain_in_expectation():
+@pytest.mark.parametrize("guide_depth", [0, 2])
+def test_guided_matches_plain_in_expectation(guide_depth):
+    """Guided renders (lobes at every vertex, or only at the first two) agree with plain ones
+    per pixel within 99% confidence intervals."""
     sc = phg.Scene.generate(phg.GEN_CORNELL)
     W = H = 32
     with phg.Context(0) as c:
@@ -219,8 +222,9 @@ def test_guided_matches_plain_in_expectation():
         c.film_clear(W, H)
         c.render(16, max_bounces=10000, seed=3, spp_per_pass=1, guided=True, train=True, guide_batch=8192)
         c.film_clear(W, H)
-        c.render(256, max_bounces=10000, seed=4, spp_per_pass=4, guided=True)
+        c.render(256, max_bounces=10000, seed=4, spp_per_pass=4, guided=True, guide_depth=guide_depth)
         guided = c.film()
+        st = c.stats()
         c.film_clear(W, H)
         c.render(256, max_bounces=10000, seed=5, spp_per_pass=4)
         plain = c.film()
@@ -232,3 +236,6 @@ def test_guided_matches_plain_in_expectation():
           flush=True)
     assert frac < 0.02
     assert abs(mg - mp) < 0.01 * mp
+    if guide_depth:
+        # the network is queried at the first guide_depth bounces only
+        assert 0 < st["guide_queries"] <= guide_depth * st["paths"]
