@@ -300,6 +300,7 @@ def main():
     else:
         ctx.upload(scene)
     upload_s = time.time() - t0
+    launch_tuning = tuned if tuned else ctx.tuning()
     st0 = ctx.stats()
     print(f"[bench] BVH: {st0['bvh_nodes']} nodes, depth {st0['bvh_depth']}, build {st0['bvh_build_ms']:.0f} ms, "
           f"upload {upload_s:.1f}s", file=sys.stderr, flush=True)
@@ -385,7 +386,7 @@ def main():
                                f"{B} bounces", "width": W, "height": H, "spp": spp, "max_bounces": B,
                    "triangles": scene.num_tris, "lights": int(len(scene.emitters))},
         "mode": "each pass replays one captured HIP graph of generate, 9 bounces and film",
-        "tuning": tuned if tuned else ctx.tuning(),
+        "tuning": launch_tuning,
         "mrays_per_s": round(sum(rays) / graph_s / 1e6, 3),
         "mrays_per_s_by_class": {"primary": round(rays[0] / graph_s / 1e6, 3),
                                  "extension": round(rays[1] / graph_s / 1e6, 3),
