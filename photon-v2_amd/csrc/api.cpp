@@ -467,6 +467,11 @@ static phg_status upload_arena(Ctx* c, std::vector<HostScene>& hs) {
     }
     if (emitters.empty()) emitters.push_back(EmitterRec{});
     if (etris.empty()) etris.push_back(EmitTri{});
+    // traversal addresses nodes by 32-bit byte offsets
+    if ((uint64_t)nodes.size() * sizeof(WideNode) > 0xFFFFFFFFull) {
+        c->err = "BVH of " + std::to_string(nodes.size()) + " nodes exceeds the 4 GiB node arena";
+        return PHG_ERR_OUT_OF_MEMORY;
+    }
     PHG_HIP(c, hipSetDevice(c->device));
     PHG_HIP(c, hipStreamSynchronize(c->stream));
     PHG_HIP(c, c->nodes.ensure(nodes.size()));

@@ -49,10 +49,14 @@ struct TraversalStack {
 // finishes, instead of waiting for the slowest ray of the wave.
 template <int RING, bool ANY, bool COUNT>
 struct Traversal {
-    V3 o, d, rcp;
+    V3 o, d;
+    // slab setup, axes permuted so that axis 0 is the dominant axis of d: origin and
+    // reciprocal direction per permuted axis, and the byte offsets (within a node) of the
+    // near and far bound rows of each permuted axis -- the rows of the low bounds when the
+    // direction component is >= 0, of the high bounds when it is negative
+    float op0, op1, op2, rp0, rp1, rp2;
+    uint32_t nr0, fr0, nr1, fr1, nr2, fr2;
     float tmin, tb;
-    bool nx, ny, nz;
-    int kdom;
     int32_t cur;
     TraceResult r;
     TraversalStack<RING> st;
@@ -63,11 +67,22 @@ struct Traversal {
 
     PHG_HD void begin(V3 o_, V3 d_, float tmin_, float tmax_, int32_t root) {
         o = o_; d = d_; tmin = tmin_; tb = tmax_;
-        rcp = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-        nx = d.x < 0.0f; ny = d.y < 0.0f; nz = d.z < 0.0f;
         // dominant axis of the direction, chosen as the watertight test permutes it
         const float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
-        kdom = adx > ady ? (adx > adz ? 0 : 2) : (ady > adz ? 1 : 2);
+        const int kdom = adx > ady ? (adx > adz ? 0 : 2) : (ady > adz ? 1 : 2);
+        // permuted axes: kdom, then the other two in increasing order
+        const uint32_t a0 = (uint32_t)kdom, a1 = kdom == 0 ? 1u : 0u, a2 = kdom == 2 ? 1u : 2u;
+        op0 = kdom == 0 ? o.x : (kdom == 1 ? o.y : o.z);
+        op1 = kdom == 0 ? o.y : o.x;
+        op2 = kdom == 2 ? o.y : o.z;
+        const float d0 = kdom == 0 ? d.x : (kdom == 1 ? d.y : d.z);
+        const float d1 = kdom == 0 ? d.y : d.x;
+        const float d2 = kdom == 2 ? d.y : d.z;
+        rp0 = 1.0f / d0; rp1 = 1.0f / d1; rp2 = 1.0f / d2;
+        // TAABB3D.ipp:405-407: the near slab is the low bound unless the component is negative
+        nr0 = 32u * a0 + (d0 < 0.0f ? 16u : 0u); fr0 = 32u * a0 + (d0 < 0.0f ? 0u : 16u);
+        nr1 = 32u * a1 + (d1 < 0.0f ? 16u : 0u); fr1 = 32u * a1 + (d1 < 0.0f ? 0u : 16u);
+        nr2 = 32u * a2 + (d2 < 0.0f ? 16u : 0u); fr2 = 32u * a2 + (d2 < 0.0f ? 0u : 16u);
         r.t = 0.0f; r.tri = -1; r.a = 0.0f; r.b = 0.0f; r.c = 0.0f;
         st.sp = 0;
         cur = root;
@@ -77,35 +92,39 @@ struct Traversal {
                      uint32_t* ntri, uint32_t* overflow) {
         constexpr float mult = 1.1920928955078125e-07f * 2.0f + 1.0f;  // TAABB3D.ipp:417
         if (cur >= 0) {
-            const float4* p = reinterpret_cast<const float4*>(nodes + cur);
-            const float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-            const int4 ch = reinterpret_cast<const int4*>(p)[6];
+            // node rows: lx, hx, ly, hy, lz, hz (4 children each), children; the near and far
+            // rows of each axis are fetched directly through the ray's row offsets
+            const char* nb = reinterpret_cast<const char*>(nodes);
+            const uint32_t off = (uint32_t)cur * (uint32_t)sizeof(WideNode);
+            const float4 n0 = *reinterpret_cast<const float4*>(nb + (off + nr0));
+            const float4 f0 = *reinterpret_cast<const float4*>(nb + (off + fr0));
+            const float4 n1 = *reinterpret_cast<const float4*>(nb + (off + nr1));
+            const float4 f1 = *reinterpret_cast<const float4*>(nb + (off + fr1));
+            const float4 n2 = *reinterpret_cast<const float4*>(nb + (off + nr2));
+            const float4 f2 = *reinterpret_cast<const float4*>(nb + (off + fr2));
+            const int4 ch = *reinterpret_cast<const int4*>(nb + (off + 96u));
             if (COUNT) (*nvis)++;
-            const float bl[4][6] = {{lx.x, hx.x, ly.x, hy.x, lz.x, hz.x}, {lx.y, hx.y, ly.y, hy.y, lz.y, hz.y},
-                                    {lx.z, hx.z, ly.z, hy.z, lz.z, hz.z}, {lx.w, hx.w, ly.w, hy.w, lz.w, hz.w}};
+            const float near0[4] = {n0.x, n0.y, n0.z, n0.w}, far0[4] = {f0.x, f0.y, f0.z, f0.w};
+            const float near1[4] = {n1.x, n1.y, n1.z, n1.w}, far1[4] = {f1.x, f1.y, f1.z, f1.w};
+            const float near2[4] = {n2.x, n2.y, n2.z, n2.w}, far2[4] = {f2.x, f2.y, f2.z, f2.w};
             const int32_t cc[4] = {ch.x, ch.y, ch.z, ch.w};
             float key[4];
             int32_t kid[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                // robust slab (TAABB3D.ipp:390-420): entry t0 over [tmin, ...), exit t1
-                float t0 = tmin, t1 = kFltMax;
-                const float mx = ((nx ? bl[k][1] : bl[k][0]) - o.x) * rcp.x;
-                const float Mx = ((nx ? bl[k][0] : bl[k][1]) - o.x) * rcp.x;
-                const float my = ((ny ? bl[k][3] : bl[k][2]) - o.y) * rcp.y;
-                const float My = ((ny ? bl[k][2] : bl[k][3]) - o.y) * rcp.y;
-                const float mz = ((nz ? bl[k][5] : bl[k][4]) - o.z) * rcp.z;
-                const float Mz = ((nz ? bl[k][4] : bl[k][5]) - o.z) * rcp.z;
-                t0 = mx > t0 ? mx : t0; t1 = Mx < t1 ? Mx : t1;
-                t0 = my > t0 ? my : t0; t1 = My < t1 ? My : t1;
-                t0 = mz > t0 ? mz : t0; t1 = Mz < t1 ? Mz : t1;
+                // robust slab (TAABB3D.ipp:390-420): entry t0 over [tmin, ...), exit t1.
+                // fmaxf / fminf return the other operand when one is NaN, which is the
+                // reference's "safe max / min"; the axis order does not change the result.
+                const float m0 = (near0[k] - op0) * rp0, M0 = (far0[k] - op0) * rp0;
+                const float m1 = (near1[k] - op1) * rp1, M1 = (far1[k] - op1) * rp1;
+                const float m2 = (near2[k] - op2) * rp2, M2 = (far2[k] - op2) * rp2;
+                const float t0 = fmaxf(fmaxf(fmaxf(tmin, m0), m1), m2);
+                const float t1 = fminf(fminf(fminf(kFltMax, M0), M1), M2);
                 // The watertight t of a triangle in this box is a convex combination of its
                 // vertices' distances along the dominant axis, so its rounding error scales
                 // with the box's parametric span on that axis, not with t itself: widen the
                 // test against the current closest hit by 2^-19 of that span.
-                const float sm = kdom == 0 ? mx : (kdom == 1 ? my : mz);
-                const float sM = kdom == 0 ? Mx : (kdom == 1 ? My : Mz);
-                const float slack = fmaxf(fabsf(sm), fabsf(sM)) * 0x1p-19f;
+                const float slack = fmaxf(fabsf(m0), fabsf(M0)) * 0x1p-19f;
                 const bool hit = (t0 <= t1 * mult + slack) && (t0 <= tb + slack) && cc[k] != kEmptyChild;
                 key[k] = hit ? t0 : INFINITY;
                 kid[k] = cc[k];
