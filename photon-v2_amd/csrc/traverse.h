@@ -154,18 +154,26 @@ struct Traversal {
             cur = kid[0];
             return false;
         }
-        const uint32_t first = (uint32_t)cur & ((1u << kLeafFirstBits) - 1u);
-        const uint32_t cnt = (((uint32_t)cur >> kLeafFirstBits) & 7u) + 1u;
-        if (COUNT) (*ntri) += cnt;
-        for (uint32_t i = 0; i < cnt; ++i) {
+        // one triangle of the leaf per step, so that a wave whose lanes are split between
+        // inner nodes and leaves pays for one triangle test per iteration, not a whole leaf;
+        // the rest of the leaf stays in cur (leaf flag, triangles left - 1, next triangle)
+        const uint32_t u = (uint32_t)cur;
+        const uint32_t first = u & ((1u << kLeafFirstBits) - 1u);
+        const uint32_t rest = (u >> kLeafFirstBits) & 7u;
+        if (COUNT) (*ntri)++;
+        {
             V3 a, b, c;
-            load_tri(tris, first + i, &a, &b, &c);
+            load_tri(tris, first, &a, &b, &c);
             float t, ba, bb, bc;
             if (wt_intersect(a, b, c, o, d, tmin, tb, &t, &ba, &bb, &bc)) {
                 tb = t;
-                r.t = t; r.tri = (int32_t)(first + i); r.a = ba; r.b = bb; r.c = bc;
+                r.t = t; r.tri = (int32_t)first; r.a = ba; r.b = bb; r.c = bc;
                 if (ANY) return true;
             }
+        }
+        if (rest > 0) {
+            cur = (int32_t)((u & 0x80000000u) | ((rest - 1u) << kLeafFirstBits) | (first + 1u));
+            return false;
         }
         if (st.sp == 0) return true;
         cur = st.pop();
