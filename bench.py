@@ -23,7 +23,7 @@ Reported beside the throughput:
                 area lights), depth 12, one network shared by all views and trained online, in
                 shared passes; plain captured-graph passes of the same views beside it.
   convergence   time to relMSE 0.01 on the occluded-light room (~199k triangles, light behind
-                a wall with a 0.1-wide slit, GGX conductors), guided vs plain BNEEPT: both
+                a wall with a 0.1-wide slit, GGX conductors), 1024^2, guided vs plain BNEEPT: both
                 render progressively against a plain reference of the same scene; the guided
                 run trains its network online (included in its time).
 """
@@ -265,8 +265,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch and BVH parameters (no autotuner)")
     ap.add_argument("--tuning", default=None, metavar="LEAF,WIDTH,BLOCK,PATHS",
                     help="fixed BVH leaf size, BVH width, traversal block and paths per pass instead of the autotuner")
-    ap.add_argument("--conv-res", type=int, default=256)
-    ap.add_argument("--conv-ref-spp", type=int, default=16384)
+    ap.add_argument("--conv-res", type=int, default=1024)
+    ap.add_argument("--conv-ref-spp", type=int, default=4096)
     ap.add_argument("--conv-max-spp", type=int, default=8192)
     ap.add_argument("--conv-train-spp", type=int, default=512)
     args = ap.parse_args()

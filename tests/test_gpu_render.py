@@ -105,6 +105,31 @@ def test_progressive_passes_equal_single_pass(ctx):
     assert np.array_equal(once, per1)
 
 
+def test_relmse_falls_as_one_over_spp(ctx):
+    """relMSE against a converged reference (16 384 spp, its own per-pixel variance
+    subtracted) falls as 1/spp: the log-log slope over 16..1024 spp is -1 and relMSE x spp
+    stays put.  A bias in the estimator, or samples of one pixel that are not independent,
+    would flatten the slope."""
+    from phg import convergence as cv
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    W = H = 48
+    ctx.upload(sc)
+    ctx.film_clear(W, H)
+    ctx.render(16384, max_bounces=10000, seed=101)
+    R, vR = cv.mean_and_var_of_mean(ctx.film())
+    spps, errs = [16, 64, 256, 1024], []
+    for n in spps:
+        ctx.film_clear(W, H)
+        ctx.render(n, max_bounces=10000, seed=7)
+        m, _ = cv.mean_and_var_of_mean(ctx.film())
+        errs.append(cv.relmse(m, R, vR))
+    slope = np.polyfit(np.log(spps), np.log(errs), 1)[0]
+    prod = np.array(errs) * np.array(spps)
+    print(f"[render] relMSE {['%.3e' % e for e in errs]} slope={slope:.3f} relMSE*spp={np.round(prod, 4)}", flush=True)
+    assert -1.15 < slope < -0.85
+    assert prod.max() / prod.min() < 1.6
+
+
 def test_stats_report_paths_and_rays(ctx):
     sc = phg.Scene.generate(phg.GEN_CORNELL)
     gpu_render(ctx, sc, 32, 32, 4, 5, 2, count_traversal=True)
