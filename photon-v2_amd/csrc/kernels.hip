@@ -13,6 +13,7 @@
 #include "kernels.h"
 #include "pt_device.h"
 #include "traverse.h"
+#include "pixel_order.h"
 #include "guide_device.h"
 
 namespace phg {
@@ -107,10 +108,12 @@ __device__ __forceinline__ uint32_t find_seg(const PassSeg* __restrict__ segs, u
 }
 
 // pixel and sample index of pass path p within its segment's job
-__device__ __forceinline__ void path_ids(const PassSeg& sg, uint32_t p, uint32_t* pixel, uint32_t* sample) {
+__device__ __forceinline__ void path_ids(const PassSeg& sg, const JobDev& job, uint32_t p, uint32_t* pixel,
+                                         uint32_t* sample) {
     const uint32_t local = p - sg.path_offset;
-    *pixel = local % sg.npix;
-    *sample = sg.sample_begin + local / sg.npix;
+    const uint32_t s = local / sg.npix;
+    *pixel = tile_pixel(local - s * sg.npix, job.width, job.height);
+    *sample = sg.sample_begin + s;
 }
 
 // Extension and shadow rays in the path queues: (origin, root node of the job's BVH as int
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         const PassSeg sg = pp.segs[si];
         const JobDev& job = sc.jobs[sg.job];
         uint32_t pixel, sample;
-        path_ids(sg, i, &pixel, &sample);
+        path_ids(sg, job, i, &pixel, &sample);
         Stream s = stream_init(path_key(job.seed, pixel, sample, job.npix));
         const float rx = (float)(pixel % job.width) + s.u();
         const float ry = (float)(pixel / job.width) + s.u();
@@ -352,7 +355,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
             if (!stop && bounce >= job.max_bounces) stop = true;
             if (!stop) {
                 uint32_t pixel, sample;
-                path_ids(sg, pid, &pixel, &sample);
+                path_ids(sg, job, pid, &pixel, &sample);
                 const uint64_t key = path_key(job.seed, pixel, sample, job.npix);
                 Stream s = stream_resume(key, pb.rng[pid]);
                 float u[GUIDED ? 12 : 8];
@@ -538,6 +541,7 @@ __device__ void film_gather(const JobDev& job, const PassSeg& sg, const PathBuff
     for (int qy = max(py - 2, 0); qy <= min(py + 2, H - 1); ++qy)
         for (int qx = max(px - 2, 0); qx <= min(px + 2, W - 1); ++qx) {
             const uint32_t q = (uint32_t)(qy * W + qx);
+            const uint32_t slot = tile_slot((uint32_t)qx, (uint32_t)qy, job.width, job.height);
             for (uint32_t s = 0; s < sg.samples; ++s) {
                 Stream st = stream_init(path_key(job.seed, q, sg.sample_begin + s, job.npix));
                 const double xp = (double)((float)qx + st.u()), yp = (double)((float)qy + st.u());
@@ -545,7 +549,7 @@ __device__ void film_gather(const JobDev& job, const PassSeg& sg, const PathBuff
                 if (px < (int)ceil(xp - 2.5) || px > (int)floor(xp + 1.5)) continue;
                 if (py < (int)ceil(yp - 2.5) || py > (int)floor(yp + 1.5)) continue;
                 const double w = film_filter(job.filter, (double)px - (xp - 0.5), (double)py - (yp - 0.5));
-                const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + q];
+                const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + slot];
                 f0.x += w * L.x; f0.y += w * L.y; f1.x += w * L.z; f1.y += w;
                 f2.x += w * (double)L.x * L.x; f2.y += w * (double)L.y * L.y; f3.x += w * (double)L.z * L.z;
             }
@@ -574,8 +578,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_film(SceneDev sc, const PassPar
         }
         double2* f = reinterpret_cast<double2*>(pb.film + job.film_base + 8 * (size_t)x);
         double2 f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
+        const uint32_t py = x / job.width;
+        const uint32_t slot = tile_slot(x - py * job.width, py, job.width, job.height);
         for (uint32_t s = 0; s < sg.samples; ++s) {
-            const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + x];
+            const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + slot];
             f0.x += L.x; f0.y += L.y; f1.x += L.z; f1.y += 1.0;
             f2.x += (double)L.x * L.x; f2.y += (double)L.y * L.y; f3.x += (double)L.z * L.z;
         }

@@ -23,6 +23,7 @@ def lib():
         _lib.emu_guide_pdf.argtypes = [vp, C.c_uint32, vp, vp]
         _lib.emu_guide_sample.argtypes = [vp, C.c_uint32, vp, vp]
         _lib.emu_guide_encode.argtypes = [vp, C.c_uint32, vp]
+        _lib.emu_pixel_order.argtypes = [C.c_uint32, C.c_uint32, vp, vp]
     return _lib
 
 
@@ -77,3 +78,11 @@ def guide_encode(x9):
     out = np.zeros((n, 64), np.float32)
     lib().emu_guide_encode(x9.ctypes.data, n, out.ctypes.data)
     return out
+
+
+def pixel_order(W, H):
+    """(pixel of each path slot, slot of each pixel) of a W x H job."""
+    p = np.zeros(W * H, np.uint32)
+    q = np.zeros(W * H, np.uint32)
+    lib().emu_pixel_order(W, H, p.ctypes.data, q.ctypes.data)
+    return p, q

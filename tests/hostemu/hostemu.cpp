@@ -74,3 +74,13 @@ extern "C" void emu_guide_encode(const float* x9, uint32_t n, float* out64) {
     for (uint32_t i = 0; i < n; ++i)
         for (int j = 0; j < 64; ++j) out64[64 * (size_t)i + j] = guide_encode(x9 + 9 * (size_t)i, j);
 }
+
+#include "../../photon-v2_amd/csrc/pixel_order.h"
+
+// Pixel of every path slot of a W x H job (the product's slot order), and the slot
+// tile_slot gives back for each pixel.
+extern "C" void emu_pixel_order(uint32_t W, uint32_t H, uint32_t* pixel_of_slot, uint32_t* slot_of_pixel) {
+    for (uint32_t q = 0; q < W * H; ++q) pixel_of_slot[q] = phg::tile_pixel(q, W, H);
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) slot_of_pixel[y * W + x] = phg::tile_slot(x, y, W, H);
+}
