@@ -88,7 +88,8 @@ struct Ctx {
     DevBuf<uint32_t> q_nvis, q_ntri;
     // guiding network
     DevBuf<float> g_params, g_m, g_v, g_grads, g_bias;
-    DevBuf<uint16_t> g_wt, g_wn;
+    DevBuf<uint16_t> g_wt, g_wn, g_wp;
+    DevBuf<float> g_bp;
     DevBuf<float> g_lobes;       // per queued extension ray
     DevBuf<float4> g_rec;        // per path, per recorded bounce
     DevBuf<float> g_train;       // collected training records
@@ -136,6 +137,7 @@ static GuideNetDev guide_net(Ctx* c) {
     GuideNetDev n{};
     n.params = c->g_params.p; n.wt = c->g_wt.p; n.wn = c->g_wn.p; n.bias = c->g_bias.p;
     n.params_mut = c->g_params.p; n.wt_mut = c->g_wt.p; n.wn_mut = c->g_wn.p; n.bias_mut = c->g_bias.p;
+    n.wp = c->g_wp.p; n.bp = c->g_bp.p; n.wp_mut = c->g_wp.p; n.bp_mut = c->g_bp.p;
     return n;
 }
 
@@ -167,6 +169,8 @@ static phg_status guide_alloc(Ctx* c) {
     PHG_HIP(c, c->g_bias.ensure(5 * 64));
     PHG_HIP(c, c->g_wt.ensure(5 * 4096));
     PHG_HIP(c, c->g_wn.ensure(5 * 4096));
+    PHG_HIP(c, c->g_wp.ensure(5 * 4096));
+    PHG_HIP(c, c->g_bp.ensure(5 * 64));
     PHG_HIP(c, c->g_counters.ensure(4));
     c->g_init = true;
     return PHG_OK;
@@ -274,7 +278,7 @@ phg_status phg_destroy(void* ctx) {
     c->q_nvis.free_(); c->q_ntri.free_();
     c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
     c->g_ws_x.free_(); c->g_ws_dz0.free_(); c->g_ws_dz1.free_(); c->g_ws_part.free_();
-    c->g_wn.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
+    c->g_wn.free_(); c->g_wp.free_(); c->g_bp.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;

@@ -23,6 +23,10 @@ struct GuideNetDev {
     uint16_t* wt_mut;
     uint16_t* wn_mut;
     float* bias_mut;
+    const uint16_t* wp;      // 5 x 4096 bf16 in the matrix-core operand order of inference
+    const float* bp;         // 5 x 64 fp32 biases in the same output order
+    uint16_t* wp_mut;
+    float* bp_mut;
 };
 
 constexpr uint32_t kGuideParamCount = 19240;

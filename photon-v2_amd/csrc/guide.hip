@@ -29,11 +29,45 @@ __device__ __forceinline__ void wave_sync() {
 __host__ __device__ __forceinline__ int layer_out(int l) { return l == kGuideLayers - 1 ? kGuideOut : kGuideHid; }
 
 // fp32 master parameters -> bf16 transposed / plain copies (rows padded to 64) and fp32 biases
+// Inference keeps activations in registers: a layer is computed transposed, Z^T = W^T X^T,
+// so that the lane holding output row 16 mt + 4 g + i of query r (the matrix-core result
+// layout) feeds it to the next layer as the B operand.  The input (k) order of the next layer
+// is therefore permuted: position j (0..7) of lane group g in k-chunk kc is input
+// 16 (2 kc + j / 4) + 4 g + j % 4, and the weights are stored in that order.  The last
+// layer's 40 outputs are moved to rows where lane group g holds lobes 2g and 2g + 1 whole
+// (slots s = 0..9 at rows 16 (s / 4) + 4 g + s % 4; 8 of the 48 rows stay zero).
+__host__ __device__ inline int infer_in_index(int kc, int g, int j) { return 16 * (2 * kc + (j >> 2)) + 4 * g + (j & 3); }
+// logical output (0..39) at padded output row R of the last layer, or -1
+__host__ __device__ inline int infer_out_logical(int R) {
+    const int g = (R & 15) >> 2, s = 4 * (R >> 4) + (R & 3);
+    if (s >= 10) return -1;
+    const int lobe = 2 * g + (s >= 5 ? 1 : 0), p = s >= 5 ? s - 5 : s;
+    return 5 * lobe + p;
+}
+
 __global__ void k_guide_prepare(const float* __restrict__ params, uint16_t* __restrict__ wt_bits,
-                                uint16_t* __restrict__ wn_bits, float* __restrict__ bias) {
+                                uint16_t* __restrict__ wn_bits, float* __restrict__ bias, uint16_t* __restrict__ wp_bits,
+                                float* __restrict__ bp) {
     __bf16* wt = reinterpret_cast<__bf16*>(wt_bits);
     __bf16* wn = reinterpret_cast<__bf16*>(wn_bits);
+    __bf16* wp = reinterpret_cast<__bf16*>(wp_bits);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kGuideLayers * 64 * 64) {
+        // wp index ((((l * 4 + mt) * 2 + kc) * 16 + m) * 4 + g) * 8 + j: output row 16 mt + m
+        // of layer l, input infer_in_index(kc, g, j)
+        const int l = i / 4096, rem = i % 4096;
+        const int mt = rem >> 10, kc = (rem >> 9) & 1, m = (rem >> 5) & 15, g = (rem >> 3) & 3, j = rem & 7;
+        const int R = 16 * mt + m, in = infer_in_index(kc, g, j);
+        const int nout = layer_out(l);
+        const int o = l == kGuideLayers - 1 ? infer_out_logical(R) : R;
+        const float* W = params + kGuideParamOffsets[2 * l];
+        wp[i] = (__bf16)(o >= 0 && o < nout ? W[in * nout + o] : 0.0f);
+    }
+    if (i < kGuideLayers * 64) {
+        const int l = i / 64, R = i % 64;
+        const int o = l == kGuideLayers - 1 ? infer_out_logical(R) : R;
+        bp[i] = o >= 0 && o < layer_out(l) ? params[kGuideParamOffsets[2 * l + 1] + o] : 0.0f;
+    }
     if (i < kGuideLayers * 64 * 64) {
         const int l = i / 4096, rem = i % 4096, a = rem / 64, b = rem % 64;  // wt: a = out, b = in
         const int nout = layer_out(l);
@@ -102,43 +136,98 @@ __device__ __forceinline__ void query_features(const InferSource& src, uint32_t 
 }
 
 // Inference: decoded mixture (8 x (mu, kappa, weight)) per query, or raw outputs.
-// Block = 4 waves, each wave a 16-query tile per iteration; weights staged in LDS.
-__global__ __launch_bounds__(256) void k_guide_infer(GuideNetDev net, InferSource src, const uint32_t* count_ptr,
-                                                     uint32_t count, float* __restrict__ out, int raw) {
-    __shared__ __attribute__((aligned(16))) __bf16 s_wt[kGuideLayers * 64 * 64];
-    __shared__ float s_bias[kGuideLayers * 64];
-    __shared__ __attribute__((aligned(16))) __bf16 s_act[4][2][16 * 64];
-    __shared__ float s_out[4][16 * kGuideOutPad];
+// Each wave takes 64 queries per iteration: every lane computes one query's features, then
+// four tiles of 16 queries go through the 5 layers with the activations in registers (each
+// lane group g of a tile holds 16 of a query's 64 values, see infer_in_index); the permuted
+// weights (40 KB) are staged in LDS once per block.  The last layer leaves lobes 2g and 2g + 1
+// of query r in lane 16 g + r, which decodes them; the softmax over the 8 lobe weights
+// reduces across the 4 lanes of a query.
+constexpr int kInferWaves = 8;
+__global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev net, InferSource src,
+                                                                  const uint32_t* count_ptr, uint32_t count,
+                                                                  float* __restrict__ out, int raw) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_w[kGuideLayers * 64 * 64];
+    __shared__ float s_b[kGuideLayers * 64];
     const uint32_t n = count_ptr ? *count_ptr : count;
-    for (int i = threadIdx.x; i < kGuideLayers * 64 * 64 / 8; i += 256)
-        reinterpret_cast<bf16x8*>(s_wt)[i] = reinterpret_cast<const bf16x8*>(net.wt)[i];  // bf16 bit patterns
-    for (int i = threadIdx.x; i < kGuideLayers * 64; i += 256) s_bias[i] = net.bias[i];
+    for (int i = threadIdx.x; i < kGuideLayers * 64 * 64 / 8; i += 64 * kInferWaves)
+        reinterpret_cast<bf16x8*>(s_w)[i] = reinterpret_cast<const bf16x8*>(net.wp)[i];  // bf16 bit patterns
+    for (int i = threadIdx.x; i < kGuideLayers * 64; i += 64 * kInferWaves) s_b[i] = net.bp[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    for (uint32_t tile = blockIdx.x * 4 + wave; tile * 16 < n; tile += gridDim.x * 4) {
-        const uint32_t q = tile * 16 + r;
+    const uint32_t step = gridDim.x * kInferWaves * 64u;
+    for (uint32_t base = (blockIdx.x * kInferWaves + wave) * 64u; base < n; base += step) {
         float x[9];
-        if (q < n) query_features(src, q, x);
+        const uint32_t qf = base + (uint32_t)lane;
+        if (qf < n) query_features(src, qf, x);
         else for (int k = 0; k < 9; ++k) x[k] = 0.0f;
-        __bf16* a0 = s_act[wave][0];
-        for (int j = 16 * g; j < 16 * g + 16; ++j) a0[r * 64 + j] = (__bf16)guide_encode(x, j);
-        wave_sync();
-        for (int l = 0; l < kGuideLayers; ++l) {
-            const bool last = l == kGuideLayers - 1;
-            layer_forward(s_act[wave][l & 1], s_wt + l * 4096, s_bias + l * 64, s_act[wave][(l + 1) & 1], s_out[wave], last);
-            wave_sync();
-        }
-        if (g == 0 && q < n) {
-            const float* o = s_out[wave] + r * kGuideOutPad;
-            float* dst = out + (size_t)q * kGuideOut;
-            if (raw) { for (int k = 0; k < kGuideOut; ++k) dst[k] = o[k]; }
-            else {
-                float lobes[kGuideOut];
-                guide_decode(o, lobes);
-                for (int k = 0; k < kGuideOut; ++k) dst[k] = lobes[k];
+        for (int T = 0; T < 4; ++T) {
+            if (base + 16u * T >= n) break;  // whole wave
+            const uint32_t q = base + 16u * T + (uint32_t)r;
+            float xf[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) xf[k] = __shfl(x[k], 16 * T + r);
+            bf16x8 bop[2];
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) bop[kc][j] = (__bf16)guide_encode(xf, infer_in_index(kc, g, j));
+            f32x4 acc[4];
+#pragma unroll
+            for (int l = 0; l < kGuideLayers; ++l) {
+                const int nmt = l == kGuideLayers - 1 ? kGuideOutPad / 16 : 4;
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    if (mt >= nmt) continue;
+                    acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                    for (int kc = 0; kc < 2; ++kc) {
+                        const bf16x8 a = *reinterpret_cast<const bf16x8*>(
+                            s_w + l * 4096 + (((mt * 2 + kc) * 16 + r) * 4 + g) * 8);
+                        acc[mt] = mfma16(a, bop[kc], acc[mt]);
+                    }
+                }
+                if (l < kGuideLayers - 1) {
+#pragma unroll
+                    for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const int mt = 2 * kc + (j >> 2), i = j & 3;
+                            const float v = acc[mt][i] + s_b[l * 64 + 16 * mt + 4 * g + i];
+                            bop[kc][j] = (__bf16)fmaxf(v, 0.0f);
+                        }
+                }
+            }
+            // slots s = 0..9 of this lane: logical outputs 10 g + s (lobes 2g, 2g + 1)
+            float o[10];
+#pragma unroll
+            for (int s2 = 0; s2 < 10; ++s2)
+                o[s2] = acc[s2 >> 2][s2 & 3] + s_b[(kGuideLayers - 1) * 64 + 16 * (s2 >> 2) + 4 * g + (s2 & 3)];
+            float* dst = out + (size_t)q * kGuideOut + 10 * g;
+            if (raw) {
+                if (q < n)
+                    for (int s2 = 0; s2 < 10; ++s2) dst[s2] = o[s2];
+                continue;
+            }
+            float mx = fmaxf(o[4], o[9]);
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const float ea = expf(o[4] - mx), eb = expf(o[9] - mx);
+            float sum = ea + eb;
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
+            if (q < n) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const float* oh = o + 5 * h;
+                    const float rl = sqrtf(oh[0] * oh[0] + oh[1] * oh[1] + oh[2] * oh[2]);
+                    float* L = dst + 5 * h;
+                    if (rl > 0.0f) { L[0] = oh[0] / rl; L[1] = oh[1] / rl; L[2] = oh[2] / rl; }
+                    else { L[0] = 0.0f; L[1] = 0.0f; L[2] = 1.0f; }
+                    L[3] = expf(clampf(oh[3], kKappaLogMin, kKappaLogMax));
+                    L[4] = (h == 0 ? ea : eb) / sum;
+                }
             }
         }
-        wave_sync();
     }
 }
 
@@ -383,7 +472,7 @@ hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const Guid
 
 hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_prepare, dim3((kGuideLayers * 4096 + 255) / 256), dim3(256), 0, st,
-                       net.params, net.wt_mut, net.wn_mut, net.bias_mut);
+                       net.params, net.wt_mut, net.wn_mut, net.bias_mut, net.wp_mut, net.bp_mut);
     return hipGetLastError();
 }
 
@@ -391,7 +480,10 @@ hipError_t guide_infer_features(const GuideNetDev& net, const float* features, u
                                 uint32_t grid, hipStream_t st) {
     InferSource src{};
     src.features = features;
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(256), 0, st, net, src, (const uint32_t*)nullptr, n, out, raw);
+    const uint32_t need = (n + 64u * kInferWaves - 1u) / (64u * kInferWaves);
+    if (need == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_infer, dim3(std::min(grid, need)), dim3(64 * kInferWaves), 0, st, net, src,
+                       (const uint32_t*)nullptr, n, out, raw);
     return hipGetLastError();
 }
 
@@ -399,7 +491,7 @@ hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const ph
                             const GuideScene& gs, const uint32_t* count_ptr, float* out, uint32_t grid, hipStream_t st) {
     InferSource src{};
     src.hits = hits; src.rays = rays; src.tris = tris; src.gs = gs;
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(256), 0, st, net, src, count_ptr, 0u, out, 0);
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, net, src, count_ptr, 0u, out, 0);
     return hipGetLastError();
 }
 

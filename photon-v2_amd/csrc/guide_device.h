@@ -32,13 +32,20 @@ PHG_HD void guide_features(const GuideScene& gs, V3 P, V3 N, V3 V, float* x) {
     x[6] = 0.5f * (V.x + 1.0f); x[7] = 0.5f * (V.y + 1.0f); x[8] = 0.5f * (V.z + 1.0f);
 }
 
-// encoding value j (0..63) of the 9 features: raw, then sin/cos at frequencies pi, 2pi, 4pi
+// encoding value j (0..63) of the 9 features: raw, then sin/cos at frequencies pi, 2pi, 4pi.
+// On the device the hardware sine/cosine (v_sin_f32 / v_cos_f32, argument in revolutions)
+// serve: the arguments stay within 2 revolutions and every value is rounded to bf16 next.
 PHG_HD float guide_encode(const float* x, int j) {
     if (j < 9) return x[j];
     if (j == 63) return 1.0f;
     const int r = j - 9, band = r / 18, q = r % 18, i = q >> 1;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float rev = (float)(1 << band) * 0.5f * x[i];
+    return (q & 1) ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+#else
     const float a = (float)(1 << band) * kPi * x[i];
     return (q & 1) ? cosf(a) : sinf(a);
+#endif
 }
 
 // kappa / (2 pi (1 - exp(-2 kappa))), limit 1 / (4 pi) as kappa -> 0
