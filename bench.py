@@ -44,18 +44,26 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bneept_procedural1M_traffic.js
 
 
 def measured_traffic(kernel_prefix, config):
-    """HBM bytes per launch of a kernel from the committed FETCH_SIZE/WRITE_SIZE profile of
-    this workload (rocprofv3 counters, copy-calibrated) under launch configuration `config`
-    ("leaf,width,block,paths_per_pass"), or None when that configuration was not profiled."""
+    """(HBM bytes per launch, profiled configuration) of a kernel from the committed
+    FETCH_SIZE/WRITE_SIZE profile of this workload (rocprofv3 counters, copy-calibrated) under
+    launch configuration `config` ("leaf,width,block,paths_per_pass").  When that exact
+    configuration was not profiled, one with the same BVH and pass size and another block size
+    stands in (the block size moves a launch's traffic by under 1%); (None, None) otherwise."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             prof = json.load(f)
     except (OSError, ValueError):
-        return None
-    for name, v in prof.get("configs", {}).get(config, {}).get("kernels", {}).items():
-        if name.removeprefix("void ").startswith(kernel_prefix):
-            return round(v["traffic_bytes"])
-    return None
+        return None, None
+    if not config:
+        return None, None
+    configs = prof.get("configs", {})
+    leaf, width, _, paths = config.split(",")
+    keys = [config] + sorted(k for k in configs if k != config and (lambda p: (p[0], p[1], p[3]) == (leaf, width, paths))(k.split(",")))
+    for key in keys:
+        for name, v in configs.get(key, {}).get("kernels", {}).items():
+            if name.removeprefix("void ").startswith(kernel_prefix):
+                return round(v["traffic_bytes"]), key
+    return None, None
 
 
 METRIC = "Msamples/s at 1 GPU; time to relMSE 0.01, guided vs plain GPU BNEEPT"
@@ -304,6 +312,7 @@ def main():
     launch_tuning = tuned if tuned else ctx.tuning()
     traffic_key = (f"{tuned['max_leaf_tris']},{tuned['width']},{tuned['trace_block']},{tuned['paths_per_pass']}"
                    if tuned else None)
+    traffic_bytes, traffic_profiled = measured_traffic("phg::k_extend", traffic_key)
     st0 = ctx.stats()
     print(f"[bench] BVH: {st0['bvh_nodes']} nodes, depth {st0['bvh_depth']}, build {st0['bvh_build_ms']:.0f} ms, "
           f"upload {upload_s:.1f}s", file=sys.stderr, flush=True)
@@ -398,10 +407,10 @@ def main():
                   "note": "same passes with a host round trip per bounce and HIP events around every launch "
                           "(the roofline's kernel times come from these steps)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic("phg::k_extend", traffic_key),
-                     "traffic_source": f"profiles/bneept_procedural1M_traffic.json, launch configuration {traffic_key} "
-                                       "(rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch: bytes leaving L2, "
-                                       "served by Infinity Cache or HBM)",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes,
+                     "traffic_source": f"profiles/bneept_procedural1M_traffic.json, launch configuration "
+                                       f"{traffic_profiled} (this run: {traffic_key}; rocprofv3 FETCH_SIZE x2.0 + "
+                                       "WRITE_SIZE x1.0 per launch: bytes leaving L2, served by Infinity Cache or HBM)",
                      "kernel": "k_extend (closest-hit traversal)",
                      "timed_in": "the eager steps: HIP events on the library stream around each launch",
                      "launches": trace_launch, "avg_launch_ms": round(trace_ms / max(trace_launch, 1), 4),
