@@ -46,6 +46,13 @@ struct SceneInfo {
     GuideScene box{};             // bounds, for the guiding network's position features
 };
 
+// Paths per wavefront pass unless tuned: large passes keep every bounce launch full (the
+// path state is 216 B per path, 14 GB at 64M paths).  Renders that train the guiding network
+// use passes of at most kTrainPassPaths unless told otherwise: the network is updated after
+// every pass, and each pass keeps 768 B of training records per path.
+constexpr uint32_t kDefaultPassPaths = 32u << 20;
+constexpr uint32_t kTrainPassPaths = 4u << 20;
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -108,8 +115,9 @@ struct Ctx {
     uint32_t trace_block = 128;         // threads per block of the path traversal kernels
     uint32_t trace_blocks_per_cu = 8;   // resident blocks per compute unit, from the occupancy query
     uint32_t shade_blocks_per_cu = 4;
-    uint32_t paths_per_pass = 4u << 20; // paths in flight per wavefront pass
+    uint32_t paths_per_pass = kDefaultPassPaths; // paths in flight per wavefront pass
 };
+
 
 #define PHG_HIP(ctx, expr)                                                                        \
     do {                                                                                          \
@@ -955,9 +963,13 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
     if (o->seed >= (1ull << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "seed must be below 2^24");
     if ((uint64_t)o->spp_begin + o->spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "sample index range too large");
     if (o->spp == 0) return PHG_OK;
-    // samples per pass: fill ~4M paths in flight
+    // samples per pass: fill the tuned number of paths in flight
     uint32_t k = o->spp_per_pass;
-    if (k == 0) k = (uint32_t)std::max<uint64_t>(1, (uint64_t)c->paths_per_pass / npix);
+    if (k == 0) {
+        uint64_t ppp = c->paths_per_pass;
+        if (o->flags & PHG_RENDER_TRAIN) ppp = std::min<uint64_t>(ppp, kTrainPassPaths);
+        k = (uint32_t)std::max<uint64_t>(1, ppp / npix);
+    }
     k = std::min(k, o->spp);
     if (npix * k > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
     if (o->estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown estimator");
@@ -995,7 +1007,8 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
     Ctx* c = static_cast<Ctx*>(ctx);
     if (!c || !in || n == 0) return PHG_ERR_INVALID_ARG;
     if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
-    const uint64_t cap = bo && bo->paths_per_pass ? bo->paths_per_pass : (uint64_t)c->paths_per_pass;
+    uint64_t cap = bo && bo->paths_per_pass ? bo->paths_per_pass : (uint64_t)c->paths_per_pass;
+    if (!(bo && bo->paths_per_pass) && bo && (bo->flags & PHG_BATCH_TRAIN)) cap = std::min<uint64_t>(cap, kTrainPassPaths);
     if (cap > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass too large");
     std::vector<JobDev> jobs(n);
     uint64_t film_doubles = 0;
@@ -1179,7 +1192,7 @@ phg_status phg_set_tuning(void* ctx, const phg_tuning* t) {
     c->trace_block = block;
     c->trace_blocks_per_cu = tb;
     c->shade_blocks_per_cu = sb;
-    c->paths_per_pass = t->paths_per_pass ? t->paths_per_pass : (4u << 20);
+    c->paths_per_pass = t->paths_per_pass ? t->paths_per_pass : kDefaultPassPaths;
     return PHG_OK;
 }
 
@@ -1250,14 +1263,14 @@ phg_status phg_autotune_batch(void* ctx, const phg_scene_desc* scenes, uint32_t 
         bo.width = width;
         return phg_upload_scenes(c, sscenes.data(), (uint32_t)sscenes.size(), &bo);
     };
-    phg_tune_trial cur{4, 4, 128, 4u << 20, 0.0};
+    phg_tune_trial cur{4, 4, 128, kDefaultPassPaths, 0.0};
     phg_status st = upload(cur.max_leaf_tris, cur.width);
     if (st != PHG_OK) return st;
     // stage 1: launch parameters on the default BVH
     phg_tune_trial top{};
     top.msamples_per_s = -1.0;
     for (uint32_t block : {64u, 128u, 256u})
-        for (uint32_t ppp : {1u << 20, 2u << 20, 4u << 20, 8u << 20}) {
+        for (uint32_t ppp : {2u << 20, 4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20}) {
             phg_tune_trial t = cur;
             t.trace_block = block;
             t.paths_per_pass = ppp;
