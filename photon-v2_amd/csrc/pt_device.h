@@ -242,7 +242,12 @@ PHG_HD V3 ggx_sample_vh(const SurfaceHit& X, V3 N, V3 V, float alpha, float u0, 
 // DielectricFresnel.cpp:18-42)
 PHG_HD RGB fresnel_schlick(const phg_material& m, float cosI) {
     const float c = fabsf(cosI);
-    const float p = (float)pow((double)(1.0f - c), 5.0);
+    // the reference's std::pow(float, int) evaluates in double; (1 - c)^2 is exact in double
+    // and two more roundings leave the float result equal to the rounded pow except within a
+    // few double ulps of a float rounding boundary -- without a double-precision pow (its
+    // 64-bit temporaries set the shading kernel's register peak)
+    const double x = (double)(1.0f - c), x2 = x * x;
+    const float p = (float)(x2 * x2 * x);
     return rgb3((1.0f - m.rgb[0]) * p + m.rgb[0], (1.0f - m.rgb[1]) * p + m.rgb[1], (1.0f - m.rgb[2]) * p + m.rgb[2]);
 }
 PHG_HD float fresnel_dielectric(float etaOuter, float etaInner, float cosThetaI) {
