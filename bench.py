@@ -182,7 +182,7 @@ def scale_leg(args):
         rays1 = cs["rays_primary"] + cs["rays_extension"]
         nodes_per_ray, tris_per_ray = cs["trace_node_visits"] / rays1, cs["trace_tri_tests"] / rays1
         t0 = time.perf_counter()
-        gres = ctx.render_batch(jobs, guided=True, train=True, guide_steps=8)
+        gres = ctx.render_batch(jobs, guided=True, train=True, guide_steps=32)
         g_s = time.perf_counter() - t0
         gs = ctx.stats()
         paths = sum(p for _, p in gres)
@@ -237,7 +237,7 @@ def convergence_leg(args):
         ctx.upload(scene)
         gtot = {}
         guided = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, guided=True,
-                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=16, target=0.005,
+                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=64, target=0.005,
                                 totals=gtot)
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
     for name, cur in (("plain", plain), ("guided", guided)):
@@ -248,7 +248,8 @@ def convergence_leg(args):
             "relmse_target": 0.01, "plain_s": None if tp is None else round(tp, 4),
             "guided_s": None if tg is None else round(tg, 4),
             "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
-            "guided": {"guide_prob": 0.5, "train_spp": args.conv_train_spp, "steps_per_pass": 16, "lr": 1e-3,
+            "guided": {"guide_prob": 0.5, "train_spp": args.conv_train_spp, "steps_per_pass": 64,
+                       "training_pass_paths": "<= 16M (16 spp at 1024^2)", "lr": 1e-3,
                        "batch": 1 << 18, "mlp": mlp_utilisation(gtot)},
             "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
             "guided_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in guided]}
@@ -276,7 +277,7 @@ def main():
     ap.add_argument("--conv-res", type=int, default=1024)
     ap.add_argument("--conv-ref-spp", type=int, default=4096)
     ap.add_argument("--conv-max-spp", type=int, default=8192)
-    ap.add_argument("--conv-train-spp", type=int, default=512)
+    ap.add_argument("--conv-train-spp", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)

@@ -291,7 +291,7 @@ phg_status phg_batch_film_read(void* ctx, uint32_t job, double* out, uint64_t ou
  * per block of the traversal kernels (64, 128 or 256; 0 = 128); paths_per_pass: paths in
  * flight per wavefront pass (2^16..2^26; 0 = 32M), the default for phg_render passes and
  * batches that do not set their own (renders and batches that train the guiding network use
- * at most 4M unless they set their own).  Results do not depend on either.  phg_get_tuning also
+ * at most 16M unless they set their own).  Results do not depend on either.  phg_get_tuning also
  * reports the resident traversal blocks per compute unit that size the persistent grid. */
 typedef struct phg_tuning {
     uint32_t trace_block;

@@ -49,9 +49,9 @@ struct SceneInfo {
 // Paths per wavefront pass unless tuned: large passes keep every bounce launch full (the
 // path state is 216 B per path, 14 GB at 64M paths).  Renders that train the guiding network
 // use passes of at most kTrainPassPaths unless told otherwise: the network is updated after
-// every pass, and each pass keeps 768 B of training records per path.
+// every pass, and each pass keeps 768 B of training records per path (12 GB at 16M).
 constexpr uint32_t kDefaultPassPaths = 32u << 20;
-constexpr uint32_t kTrainPassPaths = 4u << 20;
+constexpr uint32_t kTrainPassPaths = 16u << 20;
 
 struct Ctx {
     int device = 0;

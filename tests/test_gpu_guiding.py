@@ -243,19 +243,19 @@ def test_guided_matches_plain_in_expectation(guide_depth):
 
 def test_training_renders_keep_small_passes():
     """A render that trains the network updates it after every pass; without an explicit pass
-    size its passes hold at most 4M paths (64 samples of a 512^2 image = 4 passes of 16
+    size its passes hold at most 16M paths (64 samples of a 1024^2 image = 4 passes of 16
     samples), whatever the context's tuned pass size."""
     sc = phg.Scene.generate(phg.GEN_CORNELL)
     with phg.Context(0) as c:
         c.set_tuning(trace_block=128, paths_per_pass=64 << 20)
         c.upload(sc)
-        c.film_clear(512, 512)
+        c.film_clear(1024, 1024)
         c.render(64, max_bounces=5, seed=3, guided=True, train=True, guide_steps=2, guide_batch=4096)
         assert c.stats()["guide_steps"] == 4 * 2
         # a plain render on the same context uses the tuned size: one pass of at most 6 bounce
         # launches (depth 5) instead of four
-        c.film_clear(512, 512)
+        c.film_clear(1024, 1024)
         c.render(64, max_bounces=5, seed=3)
         st = c.stats()
-        assert st["rays_primary"] == 64 * 512 * 512
+        assert st["rays_primary"] == 64 * 1024 * 1024
         assert st["trace_launches"] <= 6
