@@ -238,7 +238,7 @@ def convergence_leg(args):
         gtot = {}
         guided = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, guided=True,
                                 train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=64, target=0.005,
-                                totals=gtot)
+                                guide_depth=1, totals=gtot)
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
     for name, cur in (("plain", plain), ("guided", guided)):
         print(f"[bench] {name}: " + " ".join(f"{s}spp/{t:.2f}s/{e:.2e}" for s, t, e in cur), file=sys.stderr,
@@ -248,7 +248,7 @@ def convergence_leg(args):
             "relmse_target": 0.01, "plain_s": None if tp is None else round(tp, 4),
             "guided_s": None if tg is None else round(tg, 4),
             "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
-            "guided": {"guide_prob": 0.5, "train_spp": args.conv_train_spp, "steps_per_pass": 64,
+            "guided": {"guide_prob": 0.5, "guide_depth": 1, "train_spp": args.conv_train_spp, "steps_per_pass": 64,
                        "training_pass_paths": "<= 16M (16 spp at 1024^2)", "lr": 1e-3,
                        "batch": 1 << 18, "mlp": mlp_utilisation(gtot)},
             "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
