@@ -849,7 +849,11 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
                                             c->counts.p + b % kCountSlots, c->g_lobes.p, c->cu_count * 2, c->stream));
             }
             PHG_HIP(c, hipEventRecord(c->ev[4], c->stream));
-            PHG_HIP(c, launch_shade(sc, ppd, pb, b, grid_shade, guided || train, c->stream));
+            // the guided shading variant where lobes are sampled or training records written;
+            // deeper bounces of a guided render shade as plain BNEEPT (the same estimator with
+            // no lobe term, fewer random numbers per vertex)
+            const bool guided_bounce = train || (guided && (m.guide_depth == 0 || b < m.guide_depth));
+            PHG_HIP(c, launch_shade(sc, ppd, pb, b, grid_shade, guided_bounce, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
             PHG_HIP(c, launch_shadow(sc, ppd, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[3], c->stream));
