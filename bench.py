@@ -172,6 +172,9 @@ def scale_leg(args):
         t0 = time.time()
         ctx.upload_many([scene])
         up_s = time.time() - t0
+        # the plain passes run 64M paths at a time (the headline's tuned pass size; tuning this
+        # scene would rebuild its 20M-triangle BVH per trial); training passes stay smaller
+        ctx.set_tuning(trace_block=128, paths_per_pass=64 << 20)
         st0 = ctx.stats()
         ntris = scene.num_tris
         del scene
