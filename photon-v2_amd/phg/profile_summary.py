@@ -1,10 +1,11 @@
 """Summaries of rocprofv3 result databases (rocpd SQLite) for the profiles/ folder.
 
-    python -m phg.profile_summary stats  <kernel_trace.db>
+    python -m phg.profile_summary stats  <kernel_trace.db> [--skip K]
     python -m phg.profile_summary pmc    <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> <copy_bytes>
     python -m phg.profile_summary l2     <tcc.db>
 
-`stats` prints per-kernel call counts and durations; `pmc` prints per-kernel FETCH_SIZE and
+`stats` prints per-kernel call counts and durations (with --skip K, leaving out each kernel's
+first K dispatches, e.g. those of a counted warm-up render); `pmc` prints per-kernel FETCH_SIZE and
 WRITE_SIZE per dispatch, scaled by what the same counters read for a plain device copy of
 `copy_bytes` bytes, and emits a JSON line with the HBM traffic per launch of each kernel;
 `l2` prints each kernel's L2 hit rate from TCC_HIT_sum / (TCC_HIT_sum + TCC_MISS_sum).
@@ -27,9 +28,24 @@ def short(name):
     return re.sub(r"\(.*", "", name)
 
 
-def kernel_stats(path):
-    rows = _rows(path, "select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
-                       "from kernels group by name order by sum(duration) desc")
+def kernel_stats(path, skip=0):
+    if skip <= 0:
+        rows = _rows(path, "select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                           "from kernels group by name order by sum(duration) desc")
+    else:
+        per = {}
+        try:
+            ordered = _rows(path, "select name, duration from kernels order by start")
+        except sqlite3.OperationalError:  # no start column in this schema: dispatch (row) order
+            ordered = _rows(path, "select name, duration from kernels order by rowid")
+        for name, d in ordered:
+            per.setdefault(name, []).append(d)
+        rows = []
+        for name, ds in per.items():
+            ds = ds[skip:]
+            if ds:
+                rows.append((name, len(ds), sum(ds), sum(ds) / len(ds), min(ds), max(ds)))
+        rows.sort(key=lambda r: -r[2])
     total = sum(r[2] for r in rows) or 1
     return [{"kernel": short(n), "calls": c, "total_ms": s / 1e6, "avg_ms": a / 1e6, "min_ms": lo / 1e6,
              "max_ms": hi / 1e6, "percent": 100.0 * s / total} for n, c, s, a, lo, hi in rows]
@@ -43,7 +59,8 @@ def counter(path, name):
 
 def main(argv):
     if len(argv) >= 2 and argv[0] == "stats":
-        st = kernel_stats(argv[1])
+        skip = int(argv[argv.index("--skip") + 1]) if "--skip" in argv else 0
+        st = kernel_stats(argv[1], skip)
         print(f"{'kernel':60s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'min ms':>9s} {'max ms':>9s} {'%':>6s}")
         for r in st:
             print(f"{r['kernel'][:60]:60s} {r['calls']:6d} {r['total_ms']:10.3f} {r['avg_ms']:9.4f} "
