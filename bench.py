@@ -231,6 +231,9 @@ def convergence_leg(args):
         chunk = 1024
         for s0 in range(0, args.conv_ref_spp, chunk):
             ctx.render(min(chunk, args.conv_ref_spp - s0), seed=7, spp_begin=s0)
+            if args.conv_ref_spp > 8 * chunk and (s0 // chunk) % 8 == 7:
+                print(f"[bench] convergence reference: {s0 + chunk} spp, {time.perf_counter() - t0:.1f}s",
+                      file=sys.stderr, flush=True)
         ref_s = time.perf_counter() - t0
         R, vR = cv.mean_and_var_of_mean(ctx.film())
         print(f"[bench] convergence reference: {args.conv_ref_spp} spp at {res}x{res} in {ref_s:.1f}s "
@@ -278,7 +281,10 @@ def main():
     ap.add_argument("--tuning", default=None, metavar="LEAF,WIDTH,BLOCK,PATHS",
                     help="fixed BVH leaf size, BVH width, traversal block and paths per pass instead of the autotuner")
     ap.add_argument("--conv-res", type=int, default=1024)
-    ap.add_argument("--conv-ref-spp", type=int, default=4096)
+    ap.add_argument("--conv-ref-spp", type=int, default=4096,
+                    help="spp of the plain reference; its per-pixel variance of the mean is subtracted from every "
+                         "relMSE, so 4096 spp estimates the error against the converged image without bias "
+                         "(65536 spp at 1024^2 takes about 4 minutes)")
     ap.add_argument("--conv-max-spp", type=int, default=8192)
     ap.add_argument("--conv-train-spp", type=int, default=128)
     args = ap.parse_args()

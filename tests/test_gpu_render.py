@@ -106,8 +106,8 @@ def test_progressive_passes_equal_single_pass(ctx):
 
 
 def test_relmse_falls_as_one_over_spp(ctx):
-    """relMSE against a converged reference (16 384 spp, its own per-pixel variance
-    subtracted) falls as 1/spp: the log-log slope over 16..1024 spp is -1 and relMSE x spp
+    """relMSE against a converged reference (65 536 spp accumulated in the FP64 film, its
+    own per-pixel variance subtracted) falls as 1/spp: the log-log slope over 16..1024 spp is -1 and relMSE x spp
     stays put.  A bias in the estimator, or samples of one pixel that are not independent,
     would flatten the slope."""
     from phg import convergence as cv
@@ -115,7 +115,8 @@ def test_relmse_falls_as_one_over_spp(ctx):
     W = H = 48
     ctx.upload(sc)
     ctx.film_clear(W, H)
-    ctx.render(16384, max_bounces=10000, seed=101)
+    for s0 in range(0, 65536, 8192):
+        ctx.render(8192, max_bounces=10000, seed=101, spp_begin=s0)
     R, vR = cv.mean_and_var_of_mean(ctx.film())
     spps, errs = [16, 64, 256, 1024], []
     for n in spps:
