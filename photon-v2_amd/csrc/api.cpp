@@ -141,6 +141,23 @@ static inline float hluminance(const float* c) {
 
 static phg_status set_err(Ctx* c, phg_status s, const std::string& msg) { c->err = msg; return s; }
 
+// Runs the body of an exported call: nothing throws across the C ABI.  Host allocations that
+// fail (std::bad_alloc) become PHG_ERR_OUT_OF_MEMORY, any other exception PHG_ERR_INVALID_ARG;
+// the message goes to phg_last_error.
+template <typename F>
+static phg_status guarded(void* ctx, F&& body) {
+    Ctx* c = static_cast<Ctx*>(ctx);
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "host memory allocation failed";
+        return PHG_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return PHG_ERR_INVALID_ARG;
+    }
+}
+
 static GuideNetDev guide_net(Ctx* c) {
     GuideNetDev n{};
     n.params = c->g_params.p; n.wt = c->g_wt.p; n.wn = c->g_wn.p; n.bias = c->g_bias.p;
@@ -246,51 +263,55 @@ using namespace phg;
 extern "C" {
 
 phg_status phg_create(void** out_ctx, const phg_device_opts* opts) {
-    if (!out_ctx) return PHG_ERR_INVALID_ARG;
-    *out_ctx = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PHG_ERR_NO_DEVICE;
-    const int dev = opts ? opts->device : 0;
-    if (dev < 0 || dev >= ndev) return PHG_ERR_INVALID_ARG;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return PHG_ERR_NO_DEVICE;
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PHG_ERR_NO_DEVICE;
-    Ctx* c = new Ctx();
-    c->device = dev;
-    c->cu_count = (uint32_t)prop.multiProcessorCount;
-    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return PHG_ERR_HIP;
-    }
-    if (resident_blocks(c->trace_block, &c->trace_blocks_per_cu, &c->shade_blocks_per_cu) != hipSuccess) {
-        (void)hipStreamDestroy(c->stream);
-        delete c;
-        return PHG_ERR_HIP;
-    }
-    for (auto& e : c->ev) (void)hipEventCreate(&e);
-    *out_ctx = c;
-    return PHG_OK;
+    return guarded(nullptr, [&]() -> phg_status {
+        if (!out_ctx) return PHG_ERR_INVALID_ARG;
+        *out_ctx = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PHG_ERR_NO_DEVICE;
+        const int dev = opts ? opts->device : 0;
+        if (dev < 0 || dev >= ndev) return PHG_ERR_INVALID_ARG;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return PHG_ERR_NO_DEVICE;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PHG_ERR_NO_DEVICE;
+        Ctx* c = new Ctx();
+        c->device = dev;
+        c->cu_count = (uint32_t)prop.multiProcessorCount;
+        if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return PHG_ERR_HIP;
+        }
+        if (resident_blocks(c->trace_block, &c->trace_blocks_per_cu, &c->shade_blocks_per_cu) != hipSuccess) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return PHG_ERR_HIP;
+        }
+        for (auto& e : c->ev) (void)hipEventCreate(&e);
+        *out_ctx = c;
+        return PHG_OK;
+    });
 }
 
 phg_status phg_destroy(void* ctx) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c) return PHG_ERR_INVALID_ARG;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    c->nodes.free_(); c->tris.free_(); c->mats.free_(); c->emitters.free_(); c->emit_cdf.free_(); c->emit_tris.free_();
-    c->film.free_(); c->rng.free_(); c->T.free_(); c->Wem.free_(); c->prevP.free_(); c->L.free_(); c->hits.free_();
-    c->shadow_c.free_(); c->ext0.free_(); c->ext1.free_(); c->shadow.free_(); c->ext_pid0.free_(); c->ext_pid1.free_();
-    c->shadow_pid.free_(); c->path_seg.free_(); c->jobs.free_(); c->d_pass.free_(); c->d_segs.free_(); c->bfilm.free_();
-    if (c->h_pass) (void)hipHostFree(c->h_pass);
-    c->spill.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
-    c->q_nvis.free_(); c->q_ntri.free_();
-    c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
-    c->g_ws_x.free_(); c->g_ws_dz0.free_(); c->g_ws_dz1.free_(); c->g_ws_part.free_();
-    c->g_wn.free_(); c->g_wp.free_(); c->g_bp.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
-    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c) return PHG_ERR_INVALID_ARG;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        c->nodes.free_(); c->tris.free_(); c->mats.free_(); c->emitters.free_(); c->emit_cdf.free_(); c->emit_tris.free_();
+        c->film.free_(); c->rng.free_(); c->T.free_(); c->Wem.free_(); c->prevP.free_(); c->L.free_(); c->hits.free_();
+        c->shadow_c.free_(); c->ext0.free_(); c->ext1.free_(); c->shadow.free_(); c->ext_pid0.free_(); c->ext_pid1.free_();
+        c->shadow_pid.free_(); c->path_seg.free_(); c->jobs.free_(); c->d_pass.free_(); c->d_segs.free_(); c->bfilm.free_();
+        if (c->h_pass) (void)hipHostFree(c->h_pass);
+        c->spill.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
+        c->q_nvis.free_(); c->q_ntri.free_();
+        c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
+        c->g_ws_x.free_(); c->g_ws_dz0.free_(); c->g_ws_dz1.free_(); c->g_ws_part.free_();
+        c->g_wn.free_(); c->g_wp.free_(); c->g_bp.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
+        for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return PHG_OK;
+    });
 }
 
 const char* phg_last_error(void* ctx) {
@@ -555,15 +576,19 @@ static phg_status upload_scenes(Ctx* c, const phg_scene_desc* scenes, uint32_t n
 extern "C" {
 
 phg_status phg_upload_scene(void* ctx, const phg_scene_desc* s, const phg_bvh_opts* bo) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !s) return PHG_ERR_INVALID_ARG;
-    return upload_scenes(c, s, 1, bo);
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !s) return PHG_ERR_INVALID_ARG;
+        return upload_scenes(c, s, 1, bo);
+    });
 }
 
 phg_status phg_upload_scenes(void* ctx, const phg_scene_desc* scenes, uint32_t n, const phg_bvh_opts* bo) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !scenes || n == 0) return PHG_ERR_INVALID_ARG;
-    return upload_scenes(c, scenes, n, bo);
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !scenes || n == 0) return PHG_ERR_INVALID_ARG;
+        return upload_scenes(c, scenes, n, bo);
+    });
 }
 
 static uint32_t trace_grid(const Ctx* c) { return c->cu_count * c->trace_blocks_per_cu; }
@@ -631,41 +656,49 @@ static phg_status do_intersect(Ctx* c, const phg_ray* rays, uint64_t n, phg_hit*
 }
 
 phg_status phg_intersect(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits, int query, int ptr_kind) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c) return PHG_ERR_INVALID_ARG;
-    return do_intersect(c, rays, n, hits, query, ptr_kind, nullptr, nullptr);
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c) return PHG_ERR_INVALID_ARG;
+        return do_intersect(c, rays, n, hits, query, ptr_kind, nullptr, nullptr);
+    });
 }
 
 phg_status phg_intersect_counted(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits, uint32_t* node_visits,
                                  uint32_t* tri_tests, int ptr_kind) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c) return PHG_ERR_INVALID_ARG;
-    if (!node_visits || !tri_tests) return set_err(c, PHG_ERR_INVALID_ARG, "counter outputs required");
-    return do_intersect(c, rays, n, hits, PHG_QUERY_CLOSEST, ptr_kind, node_visits, tri_tests);
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c) return PHG_ERR_INVALID_ARG;
+        if (!node_visits || !tri_tests) return set_err(c, PHG_ERR_INVALID_ARG, "counter outputs required");
+        return do_intersect(c, rays, n, hits, PHG_QUERY_CLOSEST, ptr_kind, node_visits, tri_tests);
+    });
 }
 
 phg_status phg_film_clear(void* ctx, uint32_t width, uint32_t height) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || width == 0 || height == 0) return PHG_ERR_INVALID_ARG;
-    PHG_HIP(c, hipSetDevice(c->device));
-    const uint64_t n = 8ull * width * height;
-    PHG_HIP(c, c->film.ensure(n));
-    PHG_HIP(c, launch_fill_zero(c->film.p, n, c->stream));
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    c->film_w = width; c->film_h = height;
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || width == 0 || height == 0) return PHG_ERR_INVALID_ARG;
+        PHG_HIP(c, hipSetDevice(c->device));
+        const uint64_t n = 8ull * width * height;
+        PHG_HIP(c, c->film.ensure(n));
+        PHG_HIP(c, launch_fill_zero(c->film.p, n, c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        c->film_w = width; c->film_h = height;
+        return PHG_OK;
+    });
 }
 
 phg_status phg_film_read(void* ctx, double* out, uint64_t out_len) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !out) return PHG_ERR_INVALID_ARG;
-    const uint64_t n = 8ull * c->film_w * c->film_h;
-    if (n == 0) return set_err(c, PHG_ERR_INVALID_ARG, "film not initialised");
-    if (out_len < n) return set_err(c, PHG_ERR_INVALID_ARG, "output buffer too small");
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, hipMemcpyAsync(out, c->film.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !out) return PHG_ERR_INVALID_ARG;
+        const uint64_t n = 8ull * c->film_w * c->film_h;
+        if (n == 0) return set_err(c, PHG_ERR_INVALID_ARG, "film not initialised");
+        if (out_len < n) return set_err(c, PHG_ERR_INVALID_ARG, "output buffer too small");
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, hipMemcpyAsync(out, c->film.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        return PHG_OK;
+    });
 }
 
 }  // extern "C"
@@ -957,361 +990,385 @@ static JobDev make_job(const SceneInfo& si, uint32_t width, uint32_t height, uin
 extern "C" {
 
 phg_status phg_render(void* ctx, const phg_render_opts* o) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !o) return PHG_ERR_INVALID_ARG;
-    if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
-    if (o->width != c->film_w || o->height != c->film_h)
-        return set_err(c, PHG_ERR_INVALID_ARG, "film size differs from render size; call phg_film_clear first");
-    const uint64_t npix = (uint64_t)o->width * o->height;
-    if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "image size out of range");
-    if (o->seed >= (1ull << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "seed must be below 2^24");
-    if ((uint64_t)o->spp_begin + o->spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "sample index range too large");
-    if (o->spp == 0) return PHG_OK;
-    // samples per pass: fill the tuned number of paths in flight
-    uint32_t k = o->spp_per_pass;
-    if (k == 0) {
-        uint64_t ppp = c->paths_per_pass;
-        if (o->flags & PHG_RENDER_TRAIN) ppp = std::min<uint64_t>(ppp, kTrainPassPaths);
-        k = (uint32_t)std::max<uint64_t>(1, ppp / npix);
-    }
-    k = std::min(k, o->spp);
-    if (npix * k > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
-    if (o->estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown estimator");
-    if (o->estimator != PHG_ESTIMATOR_BNEEPT && (o->flags & (PHG_RENDER_GUIDED | PHG_RENDER_TRAIN)))
-        return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
-    if (o->filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown filter");
-    std::vector<JobDev> jobs{
-        make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0, o->estimator, o->filter)};
-    std::vector<PassPlan> passes;
-    for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
-        const uint32_t kk = std::min(k, o->spp - s0);
-        PassPlan pl;
-        PassSeg sg{};
-        sg.job = 0; sg.sample_begin = o->spp_begin + s0; sg.samples = kk; sg.path_offset = 0;
-        sg.pix_offset = 0; sg.npix = (uint32_t)npix;
-        pl.segs.push_back(sg);
-        pl.paths = (uint32_t)(npix * kk);
-        pl.film_threads = (uint32_t)npix;
-        pl.max_bounces = o->max_bounces;
-        passes.push_back(std::move(pl));
-    }
-    RenderMode m;
-    m.count_traversal = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) != 0;
-    m.guided = (o->flags & PHG_RENDER_GUIDED) != 0;
-    m.train = (o->flags & PHG_RENDER_TRAIN) != 0;
-    m.graph = (o->flags & PHG_RENDER_GRAPH) != 0;
-    m.guide_prob = o->guide_prob; m.guide_lr = o->guide_lr; m.guide_batch = o->guide_batch; m.guide_steps = o->guide_steps;
-    m.guide_depth = o->guide_depth;
-    c->g_scene = c->scenes[0].box;
-    return render_passes(c, jobs, passes, c->film.p, m, nullptr);
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !o) return PHG_ERR_INVALID_ARG;
+        if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
+        if (o->width != c->film_w || o->height != c->film_h)
+            return set_err(c, PHG_ERR_INVALID_ARG, "film size differs from render size; call phg_film_clear first");
+        const uint64_t npix = (uint64_t)o->width * o->height;
+        if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "image size out of range");
+        if (o->seed >= (1ull << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "seed must be below 2^24");
+        if ((uint64_t)o->spp_begin + o->spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "sample index range too large");
+        if (o->spp == 0) return PHG_OK;
+        // samples per pass: fill the tuned number of paths in flight
+        uint32_t k = o->spp_per_pass;
+        if (k == 0) {
+            uint64_t ppp = c->paths_per_pass;
+            if (o->flags & PHG_RENDER_TRAIN) ppp = std::min<uint64_t>(ppp, kTrainPassPaths);
+            k = (uint32_t)std::max<uint64_t>(1, ppp / npix);
+        }
+        k = std::min(k, o->spp);
+        if (npix * k > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "too many paths per pass");
+        if (o->estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown estimator");
+        if (o->estimator != PHG_ESTIMATOR_BNEEPT && (o->flags & (PHG_RENDER_GUIDED | PHG_RENDER_TRAIN)))
+            return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
+        if (o->filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown filter");
+        std::vector<JobDev> jobs{
+            make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0, o->estimator, o->filter)};
+        std::vector<PassPlan> passes;
+        for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
+            const uint32_t kk = std::min(k, o->spp - s0);
+            PassPlan pl;
+            PassSeg sg{};
+            sg.job = 0; sg.sample_begin = o->spp_begin + s0; sg.samples = kk; sg.path_offset = 0;
+            sg.pix_offset = 0; sg.npix = (uint32_t)npix;
+            pl.segs.push_back(sg);
+            pl.paths = (uint32_t)(npix * kk);
+            pl.film_threads = (uint32_t)npix;
+            pl.max_bounces = o->max_bounces;
+            passes.push_back(std::move(pl));
+        }
+        RenderMode m;
+        m.count_traversal = (o->flags & PHG_RENDER_COUNT_TRAVERSAL) != 0;
+        m.guided = (o->flags & PHG_RENDER_GUIDED) != 0;
+        m.train = (o->flags & PHG_RENDER_TRAIN) != 0;
+        m.graph = (o->flags & PHG_RENDER_GRAPH) != 0;
+        m.guide_prob = o->guide_prob; m.guide_lr = o->guide_lr; m.guide_batch = o->guide_batch; m.guide_steps = o->guide_steps;
+        m.guide_depth = o->guide_depth;
+        c->g_scene = c->scenes[0].box;
+        return render_passes(c, jobs, passes, c->film.p, m, nullptr);
+    });
 }
 
 phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, const phg_batch_opts* bo,
                             phg_batch_result* results) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !in || n == 0) return PHG_ERR_INVALID_ARG;
-    if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
-    uint64_t cap = bo && bo->paths_per_pass ? bo->paths_per_pass : (uint64_t)c->paths_per_pass;
-    if (!(bo && bo->paths_per_pass) && bo && (bo->flags & PHG_BATCH_TRAIN)) cap = std::min<uint64_t>(cap, kTrainPassPaths);
-    if (cap > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass too large");
-    std::vector<JobDev> jobs(n);
-    uint64_t film_doubles = 0;
-    for (uint32_t j = 0; j < n; ++j) {
-        const phg_batch_job& b = in[j];
-        const uint64_t npix = (uint64_t)b.width * b.height;
-        if (b.scene >= c->scenes.size()) return set_err(c, PHG_ERR_INVALID_ARG, "job scene index out of range");
-        if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "job image size out of range");
-        if (b.seed >= (1u << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "job seed must be below 2^24");
-        if (b.spp == 0 || (uint64_t)b.spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "job spp out of range");
-        if (b.estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job estimator");
-        if (b.filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job filter");
-        jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles, b.estimator,
-                           b.filter);
-        if (b.flags & PHG_JOB_CAMERA) jobs[j].cam = b.camera;
-        film_doubles += 8 * npix;
-    }
-    // pass plan: jobs in the requested order, each pass filled first-fit with whole samples
-    std::vector<uint32_t> order(n);
-    for (uint32_t j = 0; j < n; ++j) order[j] = j;
-    if (bo && bo->order == PHG_BATCH_ORDER_SHORTEST_FIRST)
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-            return (uint64_t)jobs[a].npix * in[a].spp < (uint64_t)jobs[b].npix * in[b].spp;
-        });
-    std::vector<uint32_t> left(n), next(n, 0);
-    for (uint32_t j = 0; j < n; ++j) left[j] = in[j].spp;
-    std::vector<PassPlan> passes;
-    size_t first = 0;
-    while (first < n) {
-        PassPlan pl;
-        uint64_t used = 0;
-        for (size_t oi = first; oi < n && pl.segs.size() < kMaxSegs; ++oi) {
-            const uint32_t j = order[oi];
-            if (left[j] == 0) continue;
-            const uint64_t npix = jobs[j].npix;
-            uint64_t fit = (cap - std::min(cap, used)) / npix;
-            if (fit == 0 && used == 0) fit = 1;  // a job larger than a pass: one sample per pass
-            if (fit == 0) continue;
-            const uint32_t take = (uint32_t)std::min<uint64_t>(fit, left[j]);
-            PassSeg sg{};
-            sg.job = j; sg.sample_begin = next[j]; sg.samples = take; sg.path_offset = (uint32_t)used;
-            sg.pix_offset = pl.film_threads; sg.npix = (uint32_t)npix;
-            pl.segs.push_back(sg);
-            used += (uint64_t)take * npix;
-            pl.film_threads += (uint32_t)npix;
-            pl.max_bounces = std::max(pl.max_bounces, in[j].max_bounces);
-            next[j] += take; left[j] -= take;
-        }
-        pl.paths = (uint32_t)used;
-        passes.push_back(std::move(pl));
-        while (first < n && left[order[first]] == 0) ++first;
-    }
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, c->bfilm.ensure(film_doubles));
-    PHG_HIP(c, launch_fill_zero(c->bfilm.p, film_doubles, c->stream));
-    c->bfilm_jobs.assign(jobs.begin(), jobs.end());
-    RenderMode m;
-    m.count_traversal = bo && (bo->flags & PHG_BATCH_COUNT_TRAVERSAL);
-    m.graph = bo && (bo->flags & PHG_BATCH_GRAPH);
-    m.guided = bo && (bo->flags & PHG_BATCH_GUIDED);
-    m.train = bo && (bo->flags & PHG_BATCH_TRAIN);
-    if (m.guided || m.train) {
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !in || n == 0) return PHG_ERR_INVALID_ARG;
+        if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
+        uint64_t cap = bo && bo->paths_per_pass ? bo->paths_per_pass : (uint64_t)c->paths_per_pass;
+        if (!(bo && bo->paths_per_pass) && bo && (bo->flags & PHG_BATCH_TRAIN)) cap = std::min<uint64_t>(cap, kTrainPassPaths);
+        if (cap > 0xffffffffull) return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass too large");
+        std::vector<JobDev> jobs(n);
+        uint64_t film_doubles = 0;
         for (uint32_t j = 0; j < n; ++j) {
-            if (in[j].scene != in[0].scene)
-                return set_err(c, PHG_ERR_UNSUPPORTED, "guided batches render one scene (the network is shared)");
-            if (in[j].estimator != PHG_ESTIMATOR_BNEEPT)
-                return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
+            const phg_batch_job& b = in[j];
+            const uint64_t npix = (uint64_t)b.width * b.height;
+            if (b.scene >= c->scenes.size()) return set_err(c, PHG_ERR_INVALID_ARG, "job scene index out of range");
+            if (npix == 0 || npix > (1ull << 26)) return set_err(c, PHG_ERR_INVALID_ARG, "job image size out of range");
+            if (b.seed >= (1u << 24)) return set_err(c, PHG_ERR_INVALID_ARG, "job seed must be below 2^24");
+            if (b.spp == 0 || (uint64_t)b.spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "job spp out of range");
+            if (b.estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job estimator");
+            if (b.filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job filter");
+            jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles, b.estimator,
+                               b.filter);
+            if (b.flags & PHG_JOB_CAMERA) jobs[j].cam = b.camera;
+            film_doubles += 8 * npix;
         }
-        m.guide_prob = bo->guide_prob; m.guide_lr = bo->guide_lr; m.guide_batch = bo->guide_batch;
-        m.guide_steps = bo->guide_steps;
-        m.guide_depth = bo->guide_depth;
-        c->g_scene = c->scenes[in[0].scene].box;
-    }
-    std::vector<double> done(n, 0.0);
-    const phg_status st = render_passes(c, jobs, passes, c->bfilm.p, m, &done);
-    if (st != PHG_OK) return st;
-    c->stats.batch_passes = (uint32_t)passes.size();
-    if (results)
-        for (uint32_t j = 0; j < n; ++j) {
-            results[j].done_ms = done[j];
-            results[j].paths = (uint64_t)jobs[j].npix * in[j].spp;
+        // pass plan: jobs in the requested order, each pass filled first-fit with whole samples
+        std::vector<uint32_t> order(n);
+        for (uint32_t j = 0; j < n; ++j) order[j] = j;
+        if (bo && bo->order == PHG_BATCH_ORDER_SHORTEST_FIRST)
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                return (uint64_t)jobs[a].npix * in[a].spp < (uint64_t)jobs[b].npix * in[b].spp;
+            });
+        std::vector<uint32_t> left(n), next(n, 0);
+        for (uint32_t j = 0; j < n; ++j) left[j] = in[j].spp;
+        std::vector<PassPlan> passes;
+        size_t first = 0;
+        while (first < n) {
+            PassPlan pl;
+            uint64_t used = 0;
+            for (size_t oi = first; oi < n && pl.segs.size() < kMaxSegs; ++oi) {
+                const uint32_t j = order[oi];
+                if (left[j] == 0) continue;
+                const uint64_t npix = jobs[j].npix;
+                uint64_t fit = (cap - std::min(cap, used)) / npix;
+                if (fit == 0 && used == 0) fit = 1;  // a job larger than a pass: one sample per pass
+                if (fit == 0) continue;
+                const uint32_t take = (uint32_t)std::min<uint64_t>(fit, left[j]);
+                PassSeg sg{};
+                sg.job = j; sg.sample_begin = next[j]; sg.samples = take; sg.path_offset = (uint32_t)used;
+                sg.pix_offset = pl.film_threads; sg.npix = (uint32_t)npix;
+                pl.segs.push_back(sg);
+                used += (uint64_t)take * npix;
+                pl.film_threads += (uint32_t)npix;
+                pl.max_bounces = std::max(pl.max_bounces, in[j].max_bounces);
+                next[j] += take; left[j] -= take;
+            }
+            pl.paths = (uint32_t)used;
+            passes.push_back(std::move(pl));
+            while (first < n && left[order[first]] == 0) ++first;
         }
-    return PHG_OK;
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, c->bfilm.ensure(film_doubles));
+        PHG_HIP(c, launch_fill_zero(c->bfilm.p, film_doubles, c->stream));
+        c->bfilm_jobs.assign(jobs.begin(), jobs.end());
+        RenderMode m;
+        m.count_traversal = bo && (bo->flags & PHG_BATCH_COUNT_TRAVERSAL);
+        m.graph = bo && (bo->flags & PHG_BATCH_GRAPH);
+        m.guided = bo && (bo->flags & PHG_BATCH_GUIDED);
+        m.train = bo && (bo->flags & PHG_BATCH_TRAIN);
+        if (m.guided || m.train) {
+            for (uint32_t j = 0; j < n; ++j) {
+                if (in[j].scene != in[0].scene)
+                    return set_err(c, PHG_ERR_UNSUPPORTED, "guided batches render one scene (the network is shared)");
+                if (in[j].estimator != PHG_ESTIMATOR_BNEEPT)
+                    return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
+            }
+            m.guide_prob = bo->guide_prob; m.guide_lr = bo->guide_lr; m.guide_batch = bo->guide_batch;
+            m.guide_steps = bo->guide_steps;
+            m.guide_depth = bo->guide_depth;
+            c->g_scene = c->scenes[in[0].scene].box;
+        }
+        std::vector<double> done(n, 0.0);
+        const phg_status st = render_passes(c, jobs, passes, c->bfilm.p, m, &done);
+        if (st != PHG_OK) return st;
+        c->stats.batch_passes = (uint32_t)passes.size();
+        if (results)
+            for (uint32_t j = 0; j < n; ++j) {
+                results[j].done_ms = done[j];
+                results[j].paths = (uint64_t)jobs[j].npix * in[j].spp;
+            }
+        return PHG_OK;
+    });
 }
 
 phg_status phg_batch_film_read(void* ctx, uint32_t job, double* out, uint64_t out_len) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !out) return PHG_ERR_INVALID_ARG;
-    if (job >= c->bfilm_jobs.size()) return set_err(c, PHG_ERR_INVALID_ARG, "no such job in the last batch");
-    const JobDev& j = c->bfilm_jobs[job];
-    const uint64_t n = 8ull * j.npix;
-    if (out_len < n) return set_err(c, PHG_ERR_INVALID_ARG, "output buffer too small");
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, hipMemcpyAsync(out, c->bfilm.p + j.film_base, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !out) return PHG_ERR_INVALID_ARG;
+        if (job >= c->bfilm_jobs.size()) return set_err(c, PHG_ERR_INVALID_ARG, "no such job in the last batch");
+        const JobDev& j = c->bfilm_jobs[job];
+        const uint64_t n = 8ull * j.npix;
+        if (out_len < n) return set_err(c, PHG_ERR_INVALID_ARG, "output buffer too small");
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, hipMemcpyAsync(out, c->bfilm.p + j.film_base, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        return PHG_OK;
+    });
 }
 
 phg_status phg_guide_set_params(void* ctx, const float* params, uint32_t n) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !params) return PHG_ERR_INVALID_ARG;
-    if (n != kGuideParamCount) return set_err(c, PHG_ERR_INVALID_ARG, "expected PHG_GUIDE_PARAMS values");
-    PHG_HIP(c, hipSetDevice(c->device));
-    phg_status st = guide_load(c, params);
-    if (st == PHG_OK) c->g_ready = true;
-    return st;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !params) return PHG_ERR_INVALID_ARG;
+        if (n != kGuideParamCount) return set_err(c, PHG_ERR_INVALID_ARG, "expected PHG_GUIDE_PARAMS values");
+        PHG_HIP(c, hipSetDevice(c->device));
+        phg_status st = guide_load(c, params);
+        if (st == PHG_OK) c->g_ready = true;
+        return st;
+    });
 }
 
 phg_status phg_guide_get_params(void* ctx, float* params, uint32_t n) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !params) return PHG_ERR_INVALID_ARG;
-    if (n != kGuideParamCount) return set_err(c, PHG_ERR_INVALID_ARG, "expected PHG_GUIDE_PARAMS values");
-    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, hipMemcpyAsync(params, c->g_params.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !params) return PHG_ERR_INVALID_ARG;
+        if (n != kGuideParamCount) return set_err(c, PHG_ERR_INVALID_ARG, "expected PHG_GUIDE_PARAMS values");
+        if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, hipMemcpyAsync(params, c->g_params.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        return PHG_OK;
+    });
 }
 
 phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !features || !out || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
-    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
-    if (n == 0) return PHG_OK;
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, c->g_q.ensure(n * (9 + kGuideOutputs)));
-    float* d_in = c->g_q.p;
-    float* d_out = c->g_q.p + n * 9;
-    PHG_HIP(c, hipMemcpyAsync(d_in, features, n * 9 * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    PHG_HIP(c, guide_infer_features(guide_net(c), d_in, (uint32_t)n, d_out, 1, c->cu_count * 2, c->stream));
-    PHG_HIP(c, hipMemcpyAsync(out, d_out, n * kGuideOutputs * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !features || !out || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
+        if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+        if (n == 0) return PHG_OK;
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, c->g_q.ensure(n * (9 + kGuideOutputs)));
+        float* d_in = c->g_q.p;
+        float* d_out = c->g_q.p + n * 9;
+        PHG_HIP(c, hipMemcpyAsync(d_in, features, n * 9 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        PHG_HIP(c, guide_infer_features(guide_net(c), d_in, (uint32_t)n, d_out, 1, c->cu_count * 2, c->stream));
+        PHG_HIP(c, hipMemcpyAsync(out, d_out, n * kGuideOutputs * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        return PHG_OK;
+    });
 }
 
 phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, float* grads) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !records || !grads || n == 0 || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
-    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, c->g_q.ensure(n * kGuideRecordFloats));
-    PHG_HIP(c, hipMemcpyAsync(c->g_q.p, records, n * kGuideRecordFloats * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
-    GuideWorkspace ws{};
-    {
-        const phg_status st = guide_workspace(c, (uint32_t)n, &ws);
-        if (st != PHG_OK) return st;
-    }
-    PHG_HIP(c, guide_gradients(guide_net(c), c->g_q.p, (uint32_t)n, c->g_grads.p, ws, c->stream));
-    PHG_HIP(c, hipMemcpyAsync(grads, c->g_grads.p, kGuideParamCount * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !records || !grads || n == 0 || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
+        if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, c->g_q.ensure(n * kGuideRecordFloats));
+        PHG_HIP(c, hipMemcpyAsync(c->g_q.p, records, n * kGuideRecordFloats * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
+        GuideWorkspace ws{};
+        {
+            const phg_status st = guide_workspace(c, (uint32_t)n, &ws);
+            if (st != PHG_OK) return st;
+        }
+        PHG_HIP(c, guide_gradients(guide_net(c), c->g_q.p, (uint32_t)n, c->g_grads.p, ws, c->stream));
+        PHG_HIP(c, hipMemcpyAsync(grads, c->g_grads.p, kGuideParamCount * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        PHG_HIP(c, guide_fill(c->g_grads.p, kGuideParamCount, 0.0f, c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        return PHG_OK;
+    });
 }
 
 phg_status phg_guide_train(void* ctx, const float* records, uint64_t n, float lr, uint32_t steps) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !records || n == 0 || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
-    if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, c->g_q.ensure(n * kGuideRecordFloats));
-    PHG_HIP(c, hipMemcpyAsync(c->g_q.p, records, n * kGuideRecordFloats * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    phg_status st = guide_train_device(c, c->g_q.p, (uint32_t)n, (uint32_t)n, lr, steps);
-    if (st != PHG_OK) return st;
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    c->g_ready = true;
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !records || n == 0 || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
+        if (!c->g_init) return set_err(c, PHG_ERR_INVALID_ARG, "guiding network not initialised");
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, c->g_q.ensure(n * kGuideRecordFloats));
+        PHG_HIP(c, hipMemcpyAsync(c->g_q.p, records, n * kGuideRecordFloats * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        phg_status st = guide_train_device(c, c->g_q.p, (uint32_t)n, (uint32_t)n, lr, steps);
+        if (st != PHG_OK) return st;
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        c->g_ready = true;
+        return PHG_OK;
+    });
 }
 
 phg_status phg_set_tuning(void* ctx, const phg_tuning* t) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !t) return PHG_ERR_INVALID_ARG;
-    const uint32_t block = t->trace_block ? t->trace_block : 128;
-    if (block != 64 && block != 128 && block != 256) return set_err(c, PHG_ERR_INVALID_ARG, "trace_block must be 64, 128 or 256");
-    if (t->paths_per_pass != 0 && (t->paths_per_pass < (1u << 16) || t->paths_per_pass > (1u << 26)))
-        return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass must be in [2^16, 2^26]");
-    PHG_HIP(c, hipSetDevice(c->device));
-    PHG_HIP(c, hipStreamSynchronize(c->stream));
-    uint32_t tb = 0, sb = 0;
-    PHG_HIP(c, resident_blocks(block, &tb, &sb));
-    c->trace_block = block;
-    c->trace_blocks_per_cu = tb;
-    c->shade_blocks_per_cu = sb;
-    c->paths_per_pass = t->paths_per_pass ? t->paths_per_pass : kDefaultPassPaths;
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !t) return PHG_ERR_INVALID_ARG;
+        const uint32_t block = t->trace_block ? t->trace_block : 128;
+        if (block != 64 && block != 128 && block != 256) return set_err(c, PHG_ERR_INVALID_ARG, "trace_block must be 64, 128 or 256");
+        if (t->paths_per_pass != 0 && (t->paths_per_pass < (1u << 16) || t->paths_per_pass > (1u << 26)))
+            return set_err(c, PHG_ERR_INVALID_ARG, "paths_per_pass must be in [2^16, 2^26]");
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        uint32_t tb = 0, sb = 0;
+        PHG_HIP(c, resident_blocks(block, &tb, &sb));
+        c->trace_block = block;
+        c->trace_blocks_per_cu = tb;
+        c->shade_blocks_per_cu = sb;
+        c->paths_per_pass = t->paths_per_pass ? t->paths_per_pass : kDefaultPassPaths;
+        return PHG_OK;
+    });
 }
 
 phg_status phg_get_tuning(void* ctx, phg_tuning* t) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !t) return PHG_ERR_INVALID_ARG;
-    std::memset(t, 0, sizeof(*t));
-    t->trace_block = c->trace_block;
-    t->paths_per_pass = c->paths_per_pass;
-    t->trace_blocks_per_cu = c->trace_blocks_per_cu;
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !t) return PHG_ERR_INVALID_ARG;
+        std::memset(t, 0, sizeof(*t));
+        t->trace_block = c->trace_block;
+        t->paths_per_pass = c->paths_per_pass;
+        t->trace_blocks_per_cu = c->trace_blocks_per_cu;
+        return PHG_OK;
+    });
 }
 
 phg_status phg_autotune_batch(void* ctx, const phg_scene_desc* scenes, uint32_t nscenes, const phg_batch_job* jobs,
                               uint32_t njobs, const phg_autotune_opts* ao, phg_tune_trial* best, phg_tune_trial* trials,
                               uint32_t max_trials, uint32_t* ntrials) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !scenes || nscenes == 0 || !jobs || njobs == 0 || !best) return PHG_ERR_INVALID_ARG;
-    for (uint32_t j = 0; j < njobs; ++j)
-        if (jobs[j].scene >= nscenes) return set_err(c, PHG_ERR_INVALID_ARG, "job scene index out of range");
-    const uint32_t want = ao && ao->sample_jobs ? ao->sample_jobs : 8;
-    const uint32_t sspp = ao && ao->sample_spp ? ao->sample_spp : 4;
-    const uint32_t reps = ao && ao->repeats ? ao->repeats : 2;
-    // sample: jobs spread evenly over the work-size order, with their scenes renumbered
-    std::vector<uint32_t> order(njobs);
-    for (uint32_t j = 0; j < njobs; ++j) order[j] = j;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return (uint64_t)jobs[a].width * jobs[a].height * jobs[a].spp < (uint64_t)jobs[b].width * jobs[b].height * jobs[b].spp;
-    });
-    const uint32_t ns = std::min(want, njobs);
-    std::vector<phg_batch_job> sjobs;
-    std::vector<phg_scene_desc> sscenes;
-    std::vector<int64_t> remap(nscenes, -1);
-    for (uint32_t k = 0; k < ns; ++k) {
-        const uint32_t j = order[ns > 1 ? (uint64_t)k * (njobs - 1) / (ns - 1) : 0];
-        phg_batch_job b = jobs[j];
-        if (remap[b.scene] < 0) { remap[b.scene] = (int64_t)sscenes.size(); sscenes.push_back(scenes[b.scene]); }
-        b.scene = (uint32_t)remap[b.scene];
-        b.spp = std::min(b.spp, sspp);
-        sjobs.push_back(b);
-    }
-    uint64_t paths = 0;
-    for (const phg_batch_job& b : sjobs) paths += (uint64_t)b.width * b.height * b.spp;
-    std::vector<phg_tune_trial> all;
-    auto measure = [&](phg_tune_trial& t) -> phg_status {
-        phg_tuning tu{};
-        tu.trace_block = t.trace_block;
-        tu.paths_per_pass = t.paths_per_pass;
-        phg_status st = phg_set_tuning(c, &tu);
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !scenes || nscenes == 0 || !jobs || njobs == 0 || !best) return PHG_ERR_INVALID_ARG;
+        for (uint32_t j = 0; j < njobs; ++j)
+            if (jobs[j].scene >= nscenes) return set_err(c, PHG_ERR_INVALID_ARG, "job scene index out of range");
+        const uint32_t want = ao && ao->sample_jobs ? ao->sample_jobs : 8;
+        const uint32_t sspp = ao && ao->sample_spp ? ao->sample_spp : 4;
+        const uint32_t reps = ao && ao->repeats ? ao->repeats : 2;
+        // sample: jobs spread evenly over the work-size order, with their scenes renumbered
+        std::vector<uint32_t> order(njobs);
+        for (uint32_t j = 0; j < njobs; ++j) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return (uint64_t)jobs[a].width * jobs[a].height * jobs[a].spp < (uint64_t)jobs[b].width * jobs[b].height * jobs[b].spp;
+        });
+        const uint32_t ns = std::min(want, njobs);
+        std::vector<phg_batch_job> sjobs;
+        std::vector<phg_scene_desc> sscenes;
+        std::vector<int64_t> remap(nscenes, -1);
+        for (uint32_t k = 0; k < ns; ++k) {
+            const uint32_t j = order[ns > 1 ? (uint64_t)k * (njobs - 1) / (ns - 1) : 0];
+            phg_batch_job b = jobs[j];
+            if (remap[b.scene] < 0) { remap[b.scene] = (int64_t)sscenes.size(); sscenes.push_back(scenes[b.scene]); }
+            b.scene = (uint32_t)remap[b.scene];
+            b.spp = std::min(b.spp, sspp);
+            sjobs.push_back(b);
+        }
+        uint64_t paths = 0;
+        for (const phg_batch_job& b : sjobs) paths += (uint64_t)b.width * b.height * b.spp;
+        std::vector<phg_tune_trial> all;
+        auto measure = [&](phg_tune_trial& t) -> phg_status {
+            phg_tuning tu{};
+            tu.trace_block = t.trace_block;
+            tu.paths_per_pass = t.paths_per_pass;
+            phg_status st = phg_set_tuning(c, &tu);
+            if (st != PHG_OK) return st;
+            phg_batch_opts bo{};
+            bo.flags = PHG_BATCH_GRAPH;
+            double best_ms = INFINITY;
+            for (uint32_t r = 0; r <= reps; ++r) {  // the first render warms up
+                const auto t0 = std::chrono::steady_clock::now();
+                st = phg_render_batch(c, sjobs.data(), (uint32_t)sjobs.size(), &bo, nullptr);
+                if (st != PHG_OK) return st;
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (r > 0) best_ms = std::min(best_ms, ms);
+            }
+            t.msamples_per_s = (double)paths / (best_ms * 1e3);
+            all.push_back(t);
+            return PHG_OK;
+        };
+        auto upload = [&](uint32_t leaf, uint32_t width) -> phg_status {
+            phg_bvh_opts bo{};
+            bo.max_leaf_tris = leaf;
+            bo.width = width;
+            return phg_upload_scenes(c, sscenes.data(), (uint32_t)sscenes.size(), &bo);
+        };
+        phg_tune_trial cur{4, 4, 128, kDefaultPassPaths, 0.0};
+        phg_status st = upload(cur.max_leaf_tris, cur.width);
         if (st != PHG_OK) return st;
-        phg_batch_opts bo{};
-        bo.flags = PHG_BATCH_GRAPH;
-        double best_ms = INFINITY;
-        for (uint32_t r = 0; r <= reps; ++r) {  // the first render warms up
-            const auto t0 = std::chrono::steady_clock::now();
-            st = phg_render_batch(c, sjobs.data(), (uint32_t)sjobs.size(), &bo, nullptr);
-            if (st != PHG_OK) return st;
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            if (r > 0) best_ms = std::min(best_ms, ms);
-        }
-        t.msamples_per_s = (double)paths / (best_ms * 1e3);
-        all.push_back(t);
+        // stage 1: launch parameters on the default BVH
+        phg_tune_trial top{};
+        top.msamples_per_s = -1.0;
+        for (uint32_t block : {64u, 128u, 256u})
+            for (uint32_t ppp : {2u << 20, 4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20}) {
+                phg_tune_trial t = cur;
+                t.trace_block = block;
+                t.paths_per_pass = ppp;
+                st = measure(t);
+                if (st != PHG_OK) return st;
+                if (t.msamples_per_s > top.msamples_per_s) top = t;
+            }
+        // stage 2: BVH shape with the best launch parameters
+        for (uint32_t leaf : {2u, 4u, 8u})
+            for (uint32_t width : {2u, 4u}) {
+                if (leaf == 4 && width == 4) continue;  // measured in stage 1
+                st = upload(leaf, width);
+                if (st != PHG_OK) return st;
+                phg_tune_trial t = top;
+                t.max_leaf_tris = leaf;
+                t.width = width;
+                st = measure(t);
+                if (st != PHG_OK) return st;
+                if (t.msamples_per_s > top.msamples_per_s) top = t;
+            }
+        *best = top;
+        phg_tuning tu{};
+        tu.trace_block = top.trace_block;
+        tu.paths_per_pass = top.paths_per_pass;
+        st = phg_set_tuning(c, &tu);
+        if (st != PHG_OK) return st;
+        if (trials)
+            for (uint32_t i = 0; i < std::min<uint32_t>(max_trials, (uint32_t)all.size()); ++i) trials[i] = all[i];
+        if (ntrials) *ntrials = (uint32_t)all.size();
         return PHG_OK;
-    };
-    auto upload = [&](uint32_t leaf, uint32_t width) -> phg_status {
-        phg_bvh_opts bo{};
-        bo.max_leaf_tris = leaf;
-        bo.width = width;
-        return phg_upload_scenes(c, sscenes.data(), (uint32_t)sscenes.size(), &bo);
-    };
-    phg_tune_trial cur{4, 4, 128, kDefaultPassPaths, 0.0};
-    phg_status st = upload(cur.max_leaf_tris, cur.width);
-    if (st != PHG_OK) return st;
-    // stage 1: launch parameters on the default BVH
-    phg_tune_trial top{};
-    top.msamples_per_s = -1.0;
-    for (uint32_t block : {64u, 128u, 256u})
-        for (uint32_t ppp : {2u << 20, 4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20}) {
-            phg_tune_trial t = cur;
-            t.trace_block = block;
-            t.paths_per_pass = ppp;
-            st = measure(t);
-            if (st != PHG_OK) return st;
-            if (t.msamples_per_s > top.msamples_per_s) top = t;
-        }
-    // stage 2: BVH shape with the best launch parameters
-    for (uint32_t leaf : {2u, 4u, 8u})
-        for (uint32_t width : {2u, 4u}) {
-            if (leaf == 4 && width == 4) continue;  // measured in stage 1
-            st = upload(leaf, width);
-            if (st != PHG_OK) return st;
-            phg_tune_trial t = top;
-            t.max_leaf_tris = leaf;
-            t.width = width;
-            st = measure(t);
-            if (st != PHG_OK) return st;
-            if (t.msamples_per_s > top.msamples_per_s) top = t;
-        }
-    *best = top;
-    phg_tuning tu{};
-    tu.trace_block = top.trace_block;
-    tu.paths_per_pass = top.paths_per_pass;
-    st = phg_set_tuning(c, &tu);
-    if (st != PHG_OK) return st;
-    if (trials)
-        for (uint32_t i = 0; i < std::min<uint32_t>(max_trials, (uint32_t)all.size()); ++i) trials[i] = all[i];
-    if (ntrials) *ntrials = (uint32_t)all.size();
-    return PHG_OK;
+    });
 }
 
 phg_status phg_get_stats(void* ctx, phg_stats* out) {
-    Ctx* c = static_cast<Ctx*>(ctx);
-    if (!c || !out) return PHG_ERR_INVALID_ARG;
-    *out = c->stats;
-    return PHG_OK;
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c || !out) return PHG_ERR_INVALID_ARG;
+        *out = c->stats;
+        return PHG_OK;
+    });
 }
 
 }  // extern "C"

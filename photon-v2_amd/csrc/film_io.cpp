@@ -5,6 +5,7 @@
 // bottom row first.
 #include <cstdint>
 #include <cstdio>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -24,7 +25,12 @@ extern "C" phg_status phg_film_develop(const double* film, uint32_t width, uint3
 
 extern "C" phg_status phg_film_write_pfm(const double* film, uint32_t width, uint32_t height, const char* path) {
     if (!film || !path || width == 0 || height == 0) return PHG_ERR_INVALID_ARG;
-    std::vector<float> rgb(3ull * width * height);
+    std::vector<float> rgb;
+    try {
+        rgb.resize(3ull * width * height);
+    } catch (const std::bad_alloc&) {  // nothing throws across the C ABI
+        return PHG_ERR_OUT_OF_MEMORY;
+    }
     phg_status st = phg_film_develop(film, width, height, rgb.data());
     if (st != PHG_OK) return st;
     FILE* fp = std::fopen(path, "wb");

@@ -657,9 +657,15 @@ extern "C" phg_status phg_scene_parse_p2(const char* text, phg_p2_settings* sett
 extern "C" phg_status phg_scene_load_p2(const char* path, phg_p2_settings* settings, phg_scene_host** out, char* err,
                                         uint32_t err_len) {
     if (!path || !out) return PHG_ERR_INVALID_ARG;
-    std::ifstream f(path, std::ios::binary);
-    if (!f) return fail(err, err_len, std::string("cannot open ") + path, PHG_ERR_INVALID_ARG);
-    std::stringstream ss;
-    ss << f.rdbuf();
-    return phg_scene_parse_p2(ss.str().c_str(), settings, out, err, err_len);
+    std::string text;
+    try {  // nothing throws across the C ABI
+        std::ifstream f(path, std::ios::binary);
+        if (!f) return fail(err, err_len, std::string("cannot open ") + path, PHG_ERR_INVALID_ARG);
+        std::stringstream ss;
+        ss << f.rdbuf();
+        text = ss.str();
+    } catch (const std::bad_alloc&) {
+        return fail(err, err_len, "out of memory", PHG_ERR_OUT_OF_MEMORY);
+    }
+    return phg_scene_parse_p2(text.c_str(), settings, out, err, err_len);
 }
