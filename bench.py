@@ -243,7 +243,7 @@ def convergence_leg(args):
         ctx.upload(scene)
         gtot = {}
         guided = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, guided=True,
-                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=64, target=0.005,
+                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=32, target=0.005,
                                 guide_depth=1, totals=gtot)
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
     for name, cur in (("plain", plain), ("guided", guided)):
@@ -254,8 +254,8 @@ def convergence_leg(args):
             "relmse_target": 0.01, "plain_s": None if tp is None else round(tp, 4),
             "guided_s": None if tg is None else round(tg, 4),
             "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
-            "guided": {"guide_prob": 0.5, "guide_depth": 1, "train_spp": args.conv_train_spp, "steps_per_pass": 64,
-                       "training_pass_paths": "<= 16M (16 spp at 1024^2)", "lr": 1e-3,
+            "guided": {"guide_prob": 0.5, "guide_depth": 1, "train_spp": args.conv_train_spp, "steps_per_pass": 32,
+                       "training_pass_paths": "<= 16M (16 spp at 1024^2)", "records": "first path vertex", "lr": 1e-3,
                        "batch": 1 << 18, "mlp": mlp_utilisation(gtot)},
             "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
             "guided_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in guided]}
@@ -286,7 +286,7 @@ def main():
                          "relMSE, so 4096 spp estimates the error against the converged image without bias "
                          "(65536 spp at 1024^2 takes about 4 minutes)")
     ap.add_argument("--conv-max-spp", type=int, default=8192)
-    ap.add_argument("--conv-train-spp", type=int, default=128)
+    ap.add_argument("--conv-train-spp", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus must be 1)", file=sys.stderr)

@@ -144,7 +144,8 @@ typedef struct phg_render_opts {
     uint32_t guide_steps;      /* PHG_RENDER_TRAIN: Adam steps after each pass */
     uint32_t estimator;        /* PHG_ESTIMATOR_*; guiding needs BNEEPT */
     uint32_t filter;           /* PHG_FILTER_*: film reconstruction filter (SampleFilter.cpp:8-56) */
-    uint32_t guide_depth;      /* PHG_RENDER_GUIDED: path vertices (from the first hit) that sample the lobes; 0 = all */
+    uint32_t guide_depth;      /* PHG_RENDER_GUIDED: path vertices (from the first hit) that sample the lobes; 0 = all.
+                                  PHG_RENDER_TRAIN keeps training records at those vertices only */
 } phg_render_opts;
 
 enum {
@@ -271,7 +272,8 @@ typedef struct phg_batch_opts {
     float guide_prob;          /* PHG_BATCH_GUIDED: probability of sampling the learned lobes */
     float guide_lr;            /* PHG_BATCH_TRAIN: Adam learning rate (0 = 1e-3) */
     uint32_t guide_batch;      /* PHG_BATCH_TRAIN: records per step (0 = 2^18) */
-    uint32_t guide_depth;      /* PHG_BATCH_GUIDED: path vertices that sample the lobes (0 = all) */
+    uint32_t guide_depth;      /* PHG_BATCH_GUIDED: path vertices that sample the lobes (0 = all); training
+                                  records are kept at those vertices only */
 } phg_batch_opts;
 
 typedef struct phg_batch_result {

@@ -782,6 +782,8 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     for (const PassPlan& pl : passes) max_b = std::max(max_b, pl.max_bounces);
     if (train) {
         rec_bounces = std::min<uint32_t>(kGuideRecBounces, max_b + 1);
+        // the network is queried at the first guide_depth vertices only: fit it to those
+        if (m.guide_depth > 0) rec_bounces = std::min(rec_bounces, m.guide_depth);
         PHG_HIP(c, c->g_rec.ensure((size_t)rec_bounces * cap * kRecFloat4));
         // fresh records for every step of a pass (no record is reused within the pass)
         train_cap = std::max<uint32_t>(4u, m.guide_steps ? m.guide_steps : 4u) * batch;
@@ -882,10 +884,10 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
                                             c->counts.p + b % kCountSlots, c->g_lobes.p, c->cu_count * 2, c->stream));
             }
             PHG_HIP(c, hipEventRecord(c->ev[4], c->stream));
-            // the guided shading variant where lobes are sampled or training records written;
-            // deeper bounces of a guided render shade as plain BNEEPT (the same estimator with
-            // no lobe term, fewer random numbers per vertex)
-            const bool guided_bounce = train || (guided && (m.guide_depth == 0 || b < m.guide_depth));
+            // the guided shading variant where lobes are sampled or training records written
+            // (records are kept for bounces < guide_depth); deeper bounces shade as plain
+            // BNEEPT (the same estimator with no lobe term, fewer random numbers per vertex)
+            const bool guided_bounce = (train || guided) && (m.guide_depth == 0 || b < m.guide_depth);
             PHG_HIP(c, launch_shade(sc, ppd, pb, b, grid_shade, guided_bounce, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
             PHG_HIP(c, launch_shadow(sc, ppd, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
