@@ -184,6 +184,7 @@ def scale_leg(args):
         cs = ctx.stats()
         rays1 = cs["rays_primary"] + cs["rays_extension"]
         nodes_per_ray, tris_per_ray = cs["trace_node_visits"] / rays1, cs["trace_tri_tests"] / rays1
+        conv = scale_convergence(args, ctx, jobs) if args.scale_ref_spp > 0 else None
         t0 = time.perf_counter()
         gres = ctx.render_batch(jobs, guided=True, train=True, guide_steps=32)
         g_s = time.perf_counter() - t0
@@ -213,7 +214,40 @@ def scale_leg(args):
             "k_extend_roofline": {"achieved_gbps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBPS, 4),
                                   "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3),
                                   "timed_in": "eager plain render of the same views (HIP events)"},
-            "bvh_build_s": round(st0["bvh_build_ms"] / 1e3, 2)}
+            "bvh_build_s": round(st0["bvh_build_ms"] / 1e3, 2), "convergence": conv}
+
+
+def scale_convergence(args, ctx, jobs):
+    """Time to relMSE 0.01 over all 8 views, plain (captured bounce loops) and guided (one
+    network for all views, trained on the first samples, lobes at the first path vertex),
+    against a plain reference per view with its own variance subtracted.  Runs before any
+    other guided render on this context, so the network starts untrained."""
+    from phg import convergence as cv
+    t0 = time.perf_counter()
+    for s0 in range(0, args.scale_ref_spp, 256):
+        ctx.render_batch([dict(j, spp=min(256, args.scale_ref_spp - s0), seed=5000 + 16 * s0 + k)
+                          for k, j in enumerate(jobs)], graph=True)
+        films = [ctx.batch_film(k) for k in range(len(jobs))]
+        acc = films if s0 == 0 else [a + b for a, b in zip(acc, films)]
+        print(f"[bench] scale reference: {s0 + 256} spp, {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    refs = [cv.mean_and_var_of_mean(f) for f in acc]
+    del acc, films
+    ref_s = time.perf_counter() - t0
+    plain = cv.progressive_batch(ctx, jobs, args.scale_conv_max_spp, refs, seed_base=100000, graph=True, target=0.005)
+    gtot = {}
+    guided = cv.progressive_batch(ctx, jobs, args.scale_conv_max_spp, refs, seed_base=200000, guided=True,
+                                  train_spp=16, guide_steps=32, guide_depth=1, target=0.005, totals=gtot)
+    tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
+    for name, cur in (("plain", plain), ("guided", guided)):
+        print(f"[bench] scale {name}: " + " ".join(f"{s}spp/{t:.2f}s/{e:.2e}" for s, t, e in cur), file=sys.stderr,
+              flush=True)
+    return {"reference_spp": args.scale_ref_spp, "reference_s": round(ref_s, 2), "relmse_target": 0.01,
+            "plain_s": None if tp is None else round(tp, 4), "guided_s": None if tg is None else round(tg, 4),
+            "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
+            "guided": {"guide_prob": 0.5, "guide_depth": 1, "train_spp": 16, "steps_per_pass": 32,
+                       "mlp": mlp_utilisation(gtot)},
+            "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
+            "guided_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in guided]}
 
 
 def convergence_leg(args):
@@ -277,6 +311,9 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=48)
     ap.add_argument("--no-scale", action="store_true", help="skip the 20M-triangle multi-view leg")
     ap.add_argument("--scale-spp", type=int, default=16)
+    ap.add_argument("--scale-ref-spp", type=int, default=1024,
+                    help="spp per view of the 8-view reference for time to relMSE 0.01 (0 skips it)")
+    ap.add_argument("--scale-conv-max-spp", type=int, default=1024)
     ap.add_argument("--no-tune", action="store_true", help="default launch and BVH parameters (no autotuner)")
     ap.add_argument("--tuning", default=None, metavar="LEAF,WIDTH,BLOCK,PATHS",
                     help="fixed BVH leaf size, BVH width, traversal block and paths per pass instead of the autotuner")

@@ -81,3 +81,47 @@ def progressive(ctx, width: int, height: int, max_spp: int, ref_mean, ref_var, s
             break
         nxt *= 2
     return out
+
+
+def progressive_batch(ctx, jobs, max_spp: int, refs, seed_base: int = 1000, guided: bool = False,
+                      train_spp: int = 0, guide_prob: float = 0.5, guide_steps: int = 16, guide_lr: float = 1e-3,
+                      guide_depth: int = 0, graph: bool = False, target: float | None = None,
+                      totals: dict | None = None):
+    """progressive() for several jobs rendered together (phg_render_batch), each against its
+    own reference (mean, variance of the mean).  Checkpoints at 1, 2, 4, ... max_spp samples
+    per pixel of every job; each increment renders with seeds of its own (independent samples)
+    and its films are added on the host (films hold sums, so the sum is the film of all the
+    samples).  relMSE is the mean over every pixel and channel of all jobs.  Guided renders
+    train on the increments below `train_spp`.  Returns the (spp, seconds, relMSE) checkpoints."""
+    films = None
+    total, spp, nxt, inc, out = 0.0, 0, 1, 0, []
+    while spp < max_spp:
+        end = min(nxt, max_spp)
+        while spp < end:
+            train = guided and spp < train_spp
+            stop = min(end, train_spp) if train else end
+            batch = [dict(j, spp=stop - spp, seed=(seed_base + inc * len(jobs) + k) % (1 << 24))
+                     for k, j in enumerate(jobs)]
+            t0 = time.perf_counter()
+            ctx.render_batch(batch, graph=graph, guided=guided, train=train, guide_prob=guide_prob,
+                             guide_steps=guide_steps, guide_lr=guide_lr, guide_depth=guide_depth)
+            total += time.perf_counter() - t0
+            if totals is not None:
+                st = ctx.stats()
+                for k in ("guide_infer_ms", "guide_queries", "guide_train_ms", "guide_records"):
+                    totals[k] = totals.get(k, 0) + st[k]
+            f = [ctx.batch_film(k) for k in range(len(jobs))]
+            films = f if films is None else [a + b for a, b in zip(films, f)]
+            spp, inc = stop, inc + 1
+        num, cnt = 0.0, 0
+        for film, (R, vR) in zip(films, refs):
+            m, _ = mean_and_var_of_mean(film)
+            d2 = (m - R) ** 2 - vR
+            num += float(np.sum(d2 / (R * R + EPS)))
+            cnt += d2.size
+        e = num / cnt
+        out.append((spp, total, e))
+        if target is not None and e <= target:
+            break
+        nxt *= 2
+    return out

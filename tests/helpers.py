@@ -75,6 +75,20 @@ def pixel_z(film_a, film_b):
     return z
 
 
+def image_mean_z(film_a, film_b):
+    """Per-channel z statistic of the difference of two independent renders' image means
+    (mean over pixels of the per-pixel means), from the per-pixel variances of the mean."""
+    def mv(f):
+        n = f[..., 3:4]
+        m = f[..., 0:3] / n
+        var = np.maximum(f[..., 4:7] - n * m * m, 0) / np.maximum(n - 1, 1) / n
+        P = m.shape[0] * m.shape[1]
+        return m.reshape(-1, 3).mean(0), var.reshape(-1, 3).sum(0) / (P * P)
+    ma, va = mv(film_a)
+    mb, vb = mv(film_b)
+    return (ma - mb) / np.sqrt(va + vb)
+
+
 def scene_from_tris(verts, mat_type=1, rgb=(0.5, 0.5, 0.5), roughness=0.3, ior=1.5, camera=None):
     """phg.Scene with one material for all triangles and no emitters."""
     import phg

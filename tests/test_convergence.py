@@ -47,3 +47,39 @@ def test_time_to_interpolates_log_log():
     assert abs(cv.time_to(curve, 0.01) - 0.1) < 1e-9
     assert cv.time_to(curve, 1e-4) is None
     assert cv.time_to(curve, 2.0) == curve[0][1]
+
+
+class FakeBatchContext:
+    """Stands in for phg.Context in progressive_batch: each job's film holds `spp` Gaussian
+    samples per pixel around the job's truth, drawn from the job's seed."""
+
+    def __init__(self, truths, sd):
+        self.truths, self.sd, self.films, self.seeds = truths, sd, [], []
+
+    def render_batch(self, jobs, **kw):
+        self.films = []
+        for j in jobs:
+            self.seeds.append(j["seed"])
+            rng = np.random.default_rng(j["seed"])
+            t = self.truths[j["scene"]]
+            self.films.append(film_from_samples(t + rng.normal(0, self.sd, (j["spp"],) + t.shape)))
+
+    def batch_film(self, k):
+        return self.films[k]
+
+
+def test_progressive_batch_sums_independent_increments():
+    rng = np.random.default_rng(2)
+    truths = [rng.uniform(0.5, 2.0, (6, 9, 3)), rng.uniform(0.5, 2.0, (4, 5, 3))]
+    jobs = [dict(scene=0, spp=1, seed=0), dict(scene=1, spp=1, seed=0)]
+    ctx = FakeBatchContext(truths, 0.8)
+    refs = [(t, np.zeros_like(t)) for t in truths]
+    curve = cv.progressive_batch(ctx, jobs, 256, refs, seed_base=10)
+    assert [c[0] for c in curve] == [1, 2, 4, 8, 16, 32, 64, 128, 256]
+    assert len(set(ctx.seeds)) == len(ctx.seeds)           # every increment of every job: its own seed
+    # against the exact truth relMSE falls as 1/spp: E[relMSE] = mean(sd^2 / spp / (t^2 + eps))
+    w = np.concatenate([(0.8 ** 2 / (t * t + cv.EPS)).ravel() for t in truths]).mean()
+    prod = np.array([e * s for s, _, e in curve])
+    assert np.all(np.abs(prod / w - 1.0) < 0.35)
+    slope = np.polyfit(np.log([c[0] for c in curve]), np.log([c[2] for c in curve]), 1)[0]
+    assert -1.15 < slope < -0.85

@@ -9,7 +9,7 @@ import pytest
 
 import phg
 from oracle import guiding_oracle as go
-from helpers import furnace_scene, pixel_z
+from helpers import furnace_scene, image_mean_z, pixel_z
 
 pytestmark = pytest.mark.gpu
 
@@ -211,11 +211,19 @@ def test_guided_furnace_stays_unbiased(albedo, radiance):
     assert abs(mean - expect) < 4 * se + 1e-4
 
 
-@pytest.mark.parametrize("guide_depth", [0, 2])
-def test_guided_matches_plain_in_expectation(guide_depth):
+@pytest.mark.parametrize("kind,guide_depth", [("cornell", 0), ("cornell", 2), ("ggx", 0)])
+def test_guided_matches_plain_in_expectation(kind, guide_depth):
     """Guided renders (lobes at every vertex, or only at the first two) agree with plain ones
-    per pixel within 99% confidence intervals."""
-    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    per pixel within 99% confidence intervals: the Lambertian Cornell box, and a scene of GGX
+    conductors and dielectrics of roughness 0.05..0.6, where the lobes are sampled at the
+    rougher conductors and the other surfaces sample their BSDF alone."""
+    if kind == "cornell":
+        sc = phg.Scene.generate(phg.GEN_CORNELL)
+    else:
+        sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=6, target_tris=30_000, num_instances=12, num_lights=6,
+                                subdiv=3)
+        rough = sc.materials["roughness"][sc.materials["type"] == phg.MAT_CONDUCTOR]
+        assert np.any(rough >= 0.25) and np.any(rough < 0.25)
     W = H = 32
     with phg.Context(0) as c:
         c.upload(sc)
@@ -232,10 +240,14 @@ def test_guided_matches_plain_in_expectation(guide_depth):
     frac = np.mean(np.abs(z) > 2.5758)
     mg = guided[..., :3].sum() / guided[..., 3].sum()
     mp = plain[..., :3].sum() / plain[..., 3].sum()
-    print(f"[guide] cornell guided vs plain: outside_99CI={frac:.4f} mean_guided={mg:.5f} mean_plain={mp:.5f}",
+    print(f"[guide] {kind} guided vs plain: outside_99CI={frac:.4f} mean_guided={mg:.5f} mean_plain={mp:.5f}",
           flush=True)
+    zm = image_mean_z(guided, plain)
+    print(f"[guide] {kind} image-mean z per channel {np.round(zm, 2)}", flush=True)
     assert frac < 0.02
-    assert abs(mg - mp) < 0.01 * mp
+    assert np.all(np.abs(zm) < 4.0)
+    if kind == "cornell":
+        assert abs(mg - mp) < 0.01 * mp
     if guide_depth:
         # the network is queried at the first guide_depth bounces only
         assert 0 < st["guide_queries"] <= guide_depth * st["paths"]
