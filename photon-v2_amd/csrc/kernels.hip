@@ -375,8 +375,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                 const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideOut : nullptr;
                 const bool guide_here = pp.guide_depth == 0 || bounce < pp.guide_depth;
                 const phg_material m = sc.mats[tr.mat];
-                const float alpha = (GUIDED && guide_here && pp.guide_prob > 0.0f && guide_lobes_ok(lob) &&
-                                     guide_samples_material(m)) ? pp.guide_prob : 0.0f;
+                const float alpha = (GUIDED && guide_here && pp.guide_prob > 0.0f && guide_samples_material(m) &&
+                                     guide_lobes_ok(lob)) ? pp.guide_prob : 0.0f;
                 if (m.type == PHG_MAT_ABSORBER) stop = true;
                 if (!stop) {
                     const RGB T = rgb3(T4.x, T4.y, T4.z);
