@@ -33,6 +33,7 @@ import os
 import platform
 import sys
 import time
+import traceback
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for p in (os.path.join(ROOT, "photon-v2_amd"), ROOT):
@@ -424,9 +425,18 @@ def main():
                "sample": f"every {step}th row of the same {W}x{H} image at {spp} spp ({cst['paths']} paths, "
                          f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}"}
 
-    batch = None if args.no_batch else batch_leg(args)
-    scale = None if args.no_scale else scale_leg(args)
-    conv = None if args.no_conv else convergence_leg(args)
+    def leg(fn):
+        """An auxiliary leg that fails is reported in its JSON field (traceback on stderr); the
+        headline line is still printed."""
+        try:
+            return fn(args)
+        except Exception as e:  # noqa: BLE001
+            traceback.print_exc()
+            return {"error": f"{type(e).__name__}: {e}"}
+
+    batch = None if args.no_batch else leg(batch_leg)
+    scale = None if args.no_scale else leg(scale_leg)
+    conv = None if args.no_conv else leg(convergence_leg)
 
     out = {
         "metric": METRIC,
