@@ -148,6 +148,10 @@ MLP_TRAIN_FLOPS = 2 * MLP_FWD_FLOPS + 2 * (64 * 40 + 3 * 64 * 64)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
+def ordinal(n: int) -> str:
+    return f"{n}{'th' if 10 <= n % 100 <= 20 else {1: 'st', 2: 'nd', 3: 'rd'}.get(n % 10, 'th')}"
+
+
 def mlp_utilisation(st):
     """Matrix-core utilisation of the guiding MLP over a render's inference and training time."""
     out = {"peak_tflops": MFMA_BF16_PEAK_TFLOPS, "dtype": "bf16 (fp32 accumulation)"}
@@ -430,8 +434,15 @@ def main():
         _, cst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=threads, row_step=step)
         cpu = {"value": cst["paths"] / (cst["render_ms"] / 1e3) / 1e6, "unit": "Msamples/s", "cores": threads,
                "kind": "port",
-               "sample": f"every {step}th row of the same {W}x{H} image at {spp} spp ({cst['paths']} paths, "
+               "sample": f"every {ordinal(step)} row of the same {W}x{H} image at {spp} spp ({cst['paths']} paths, "
                          f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}"}
+        # one core, on a sample of about a third of the time
+        per_path_1 = per_path_s * threads
+        step1 = int(max(1, min(1024, round(W * H * spp / max(args.cpu_seconds / 3 / per_path_1, 1.0)))))
+        _, sst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=1, row_step=step1)
+        cpu["single_core"] = {"value": sst["paths"] / (sst["render_ms"] / 1e3) / 1e6, "unit": "Msamples/s",
+                              "sample": f"every {ordinal(step1)} row ({sst['paths']} paths, {sst['render_ms'] / 1e3:.1f} s), "
+                                        "1 thread"}
 
     def leg(fn):
         """An auxiliary leg that fails is reported in its JSON field (traceback on stderr); the
