@@ -467,6 +467,11 @@ static bool refract_dir(float etaOuter, float etaInner, V3 I, V3 N, V3* out) {
 // TranslucentMicrofacet.cpp:362-366
 static inline float reflect_prob(float F) { return std::clamp(F, 0.0001f, 1.0f - 0.0001f); }
 
+// Planted mistakes for the tests that show a check is sensitive (orc_set_fault); 0 in every
+// other use.  1: transmission without the (eta_t / eta_i)^2 radiance factor; 2: transmission
+// weighted by F instead of 1 - F; 3: transmission weight not divided by its pick probability.
+static int g_fault = 0;
+
 // BsdfHelper.h:14-45
 static inline bool half_vec_same_hemi(V3 L, V3 V, V3 N, V3* H) {
     V3 h = L + V;
@@ -526,10 +531,11 @@ static BsdfEval bsdf_eval(const orc_material& m, const SurfaceHit& X, V3 L, V3 V
             if (dot(N, H) < 0.0f) H = H * -1.0f;
             const float hov = dot(H, V);
             const float hol = dot(H, L);
-            const float F = 1.0f - fresnel_dielectric(m.ior_outer, m.ior_inner, hol);
+            const float Fr = fresnel_dielectric(m.ior_outer, m.ior_inner, hol);
+            const float F = g_fault == 2 ? Fr : 1.0f - Fr;
             const float D = ggx_D(N, H, a);
             const float G = ggx_G(N, H, L, V, a);
-            const float transport = etaT / etaI;
+            const float transport = g_fault == 1 ? 1.0f : etaT / etaI;
             const float iorTerm = transport * etaI / (etaI * hol + etaT * hov);
             if (!std::isfinite(iorTerm)) return e;
             const float dotTerm = std::fabs(hol * hov / nolnov);
@@ -634,12 +640,13 @@ static BsdfSample bsdf_sample(const orc_material& m, const SurfaceHit& X, V3 V, 
             if (!refract_dir(m.ior_outer, m.ior_inner, V, H, &R) || !opposite_hemi(X, V, R)) return s;
             L = R;
             F = 1.0f - fresnel_dielectric(m.ior_outer, m.ior_inner, dot(H, L));
+            if (g_fault == 2) F = 1.0f - F;
             float etaI = m.ior_outer, etaT = m.ior_inner;
             if (dot(N, L) < 0.0f) std::swap(etaI, etaT);
             const float rel = etaT / etaI;
             s.relIor = rel;
-            F = F * (rel * rel);
-            F = F / (1.0f - rp);
+            if (g_fault != 1) F = F * (rel * rel);
+            if (g_fault != 3) F = F / (1.0f - rp);
         }
         const float pdf = ggx_vndf_pdf(N, H, V, a);
         const float nov = dot(N, V), hov = dot(H, V);
@@ -1029,6 +1036,9 @@ void orc_pcg32_generate(uint64_t seq, uint64_t init_state, int use_default, uint
 }
 
 uint64_t orc_moremur(uint64_t v) { return moremur(v); }
+
+// Test-only: plant one of the mistakes listed at g_fault (0 restores the restatement).
+void orc_set_fault(int fault) { orc::g_fault = fault; }
 
 void orc_stream_samples(uint64_t key, uint32_t n, float* out) {
     Stream s(key);

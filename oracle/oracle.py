@@ -69,6 +69,8 @@ def lib() -> C.CDLL:
     L.orc_pcg32_generate.argtypes = [u64, u64, C.c_int, u64, u32, vp]
     L.orc_moremur.argtypes = [u64]
     L.orc_moremur.restype = u64
+    L.orc_set_fault.argtypes = [C.c_int]
+    L.orc_set_fault.restype = None
     L.orc_stream_samples.argtypes = [u64, u32, vp]
     L.orc_intersect_triangle.argtypes = [vp, vp, vp, vp]
     L.orc_intersect_brute.argtypes = [vp, u32, vp, u32, vp, vp, vp, vp, u32]

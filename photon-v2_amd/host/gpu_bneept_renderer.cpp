@@ -29,7 +29,10 @@ void GpuBneeptRenderer::doUpdate(const phg_scene_desc& scene) {
     std::lock_guard<std::mutex> lock(m_filmMutex);
     m_film.assign(8ull * m_settings.width * m_settings.height, 0.0);
     m_sppDone = 0;
-    m_sppPolled = 0;
+    {
+        std::lock_guard<std::mutex> rl(m_regionMutex);
+        m_regionQueue.clear();
+    }
     m_elapsedMs = 0;
     m_pathsPerSecond = 0.0;
 }
@@ -55,31 +58,50 @@ void GpuBneeptRenderer::doRender() {
             m_film.swap(staging);
         }
         m_sppDone = s + o.spp;
+        {
+            // every tile's work unit of this chunk is done
+            std::lock_guard<std::mutex> rl(m_regionMutex);
+            for (int64_t y = 0; y < (int64_t)m_settings.height; y += kTile)
+                for (int64_t x = 0; x < (int64_t)m_settings.width; x += kTile) {
+                    RenderRegionStatus r;
+                    r.region = Region{x, y, std::min<int64_t>(x + kTile, m_settings.width),
+                                      std::min<int64_t>(y + kTile, m_settings.height)};
+                    r.status = ERegionStatus::Finished;
+                    m_regionQueue.push_back(r);
+                }
+        }
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         m_elapsedMs = (uint64_t)ms;
         m_pathsPerSecond = ms > 0 ? (double)m_sppDone * m_settings.width * m_settings.height / (ms / 1e3) : 0.0;
     }
 }
 
-void GpuBneeptRenderer::develop_locked(HdrRgbFrame& out) {
-    out.width = m_settings.width;
-    out.height = m_settings.height;
-    out.rgb.assign(3ull * out.width * out.height, 0.0f);
-    for (size_t i = 0; i < (size_t)out.width * out.height; ++i) {
-        const double w = m_film[8 * i + 3];
-        const double rw = w == 0.0 ? 0.0 : 1.0 / w;  // HdrRgbFilm.cpp:186-193
-        for (int k = 0; k < 3; ++k) out.rgb[3 * i + k] = (float)(m_film[8 * i + k] * rw);
+void GpuBneeptRenderer::develop_locked(const Region& region, HdrRgbFrame& out) {
+    const int64_t W = m_settings.width, H = m_settings.height;
+    if (out.width != m_settings.width || out.height != m_settings.height || out.rgb.size() != 3ull * W * H) {
+        out.width = m_settings.width;
+        out.height = m_settings.height;
+        out.rgb.assign(3ull * W * H, 0.0f);
     }
+    const int64_t x0 = std::max<int64_t>(region.x0, 0), x1 = std::min<int64_t>(region.x1, W);
+    const int64_t y0 = std::max<int64_t>(region.y0, 0), y1 = std::min<int64_t>(region.y1, H);
+    for (int64_t y = y0; y < y1; ++y)
+        for (int64_t x = x0; x < x1; ++x) {
+            const size_t i = (size_t)(y * W + x);
+            const double w = m_film[8 * i + 3];
+            const double rw = w == 0.0 ? 0.0 : 1.0 / w;  // HdrRgbFilm.cpp:186-193
+            for (int k = 0; k < 3; ++k) out.rgb[3 * i + k] = (float)(m_film[8 * i + k] * rw);
+        }
 }
 
 void GpuBneeptRenderer::retrieveFrame(std::size_t layerIndex, HdrRgbFrame& out_frame) {
-    if (layerIndex != 0) throw std::out_of_range("GpuBneeptRenderer has one layer");
-    std::lock_guard<std::mutex> lock(m_filmMutex);
-    develop_locked(out_frame);
+    asyncPeekFrame(layerIndex, Region{0, 0, m_settings.width, m_settings.height}, out_frame);
 }
 
-void GpuBneeptRenderer::asyncPeekFrame(std::size_t layerIndex, HdrRgbFrame& out_frame) {
-    retrieveFrame(layerIndex, out_frame);
+void GpuBneeptRenderer::asyncPeekFrame(std::size_t layerIndex, const Region& region, HdrRgbFrame& out_frame) {
+    if (layerIndex != 0) throw std::out_of_range("GpuBneeptRenderer has one layer");
+    std::lock_guard<std::mutex> lock(m_filmMutex);
+    develop_locked(region, out_frame);
 }
 
 RenderStats GpuBneeptRenderer::asyncQueryRenderStats() const {
@@ -102,15 +124,14 @@ std::vector<std::string> GpuBneeptRenderer::getObservationInfo() const {
 }
 
 std::size_t GpuBneeptRenderer::asyncPollUpdatedRegions(RenderRegionStatus* out_regions, std::size_t capacity) {
-    const uint64_t done = m_sppDone.load();
-    if (capacity == 0 || !out_regions || done == m_sppPolled.load()) return 0;
-    m_sppPolled = done;
-    RenderRegionStatus r;
-    r.x1 = m_settings.width;
-    r.y1 = m_settings.height;
-    r.finished = done >= m_settings.spp;
-    out_regions[0] = r;
-    return 1;
+    if (capacity == 0 || !out_regions) return 0;
+    std::lock_guard<std::mutex> rl(m_regionMutex);
+    std::size_t n = 0;
+    while (n < capacity && !m_regionQueue.empty()) {
+        out_regions[n++] = m_regionQueue.front();
+        m_regionQueue.pop_front();
+    }
+    return n;
 }
 
 void GpuBneeptRenderer::saveFrame(const std::string& path) {

@@ -11,6 +11,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <deque>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -53,11 +54,20 @@ struct GpuBneeptSettings {
     static GpuBneeptSettings fromP2(const phg_p2_settings& p2);
 };
 
-// Region update of asyncPollUpdatedRegions (RenderRegionStatus): the rectangle of pixels
-// that changed since the last poll, and whether rendering has finished.
+// A rectangle of pixels [x0, x1) x [y0, y1), raster y up (Region = TAABB2D<int64>,
+// Core/Scheduler/Region.h:8).
+struct Region {
+    int64_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    int64_t area() const { return x1 > x0 && y1 > y0 ? (x1 - x0) * (y1 - y0) : 0; }
+};
+
+// Core/Renderer/ERegionStatus.h: later entries take precedence when regions merge.
+enum class ERegionStatus { Invalid = 0, Finished = 1, Updating = 2 };
+
+// Core/Renderer/RenderRegionStatus.h: a region and what happened to it.
 struct RenderRegionStatus {
-    uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-    bool finished = false;
+    Region region;
+    ERegionStatus status = ERegionStatus::Invalid;
 };
 
 class GpuBneeptRenderer {
@@ -76,26 +86,36 @@ public:
     // Renderer::asyncQueryRenderStats / asyncQueryRenderProgress: safe during doRender.
     RenderStats asyncQueryRenderStats() const;
     RenderProgress asyncQueryRenderProgress() const;
-    // Renderer::asyncPeekFrame: the frame as of the last finished chunk (region = whole image).
-    void asyncPeekFrame(std::size_t layerIndex, HdrRgbFrame& out_frame);
+    // Renderer::asyncPeekFrame: the pixels of `region` as of the last finished chunk, developed
+    // into out_frame (sized to the whole frame if it is not already; pixels outside the region
+    // are left as they are), as EqualSamplingRenderer::asyncPeekFrame develops the main film
+    // over a region (EqualSamplingRenderer.cpp:217-234).
+    void asyncPeekFrame(std::size_t layerIndex, const Region& region, HdrRgbFrame& out_frame);
     // Renderer::getObservationInfo: names of the stats above.
     std::vector<std::string> getObservationInfo() const;
-    // Renderer::asyncPollUpdatedRegions: the frame region updated since the last poll (the
-    // whole frame after each finished chunk); returns the number of regions written (0 or 1).
+    // Renderer::asyncPollUpdatedRegions: regions whose work finished since the last poll, oldest
+    // first, up to `capacity` of them; returns how many were written.  The frame is covered by
+    // 128 x 128 tiles (EqualSamplingRenderer's block size, EqualSamplingRenderer.cpp:61); when
+    // a chunk of samples finishes, every tile's work unit of that chunk has finished and each
+    // tile is queued as ERegionStatus::Finished (EqualSamplingRenderer::asyncAddUpdatedRegion,
+    // EqualSamplingRenderer.cpp:243-258).
     std::size_t asyncPollUpdatedRegions(RenderRegionStatus* out_regions, std::size_t capacity);
     // phSaveFrame: the developed frame as a .pfm file (PfmFile::save layout).
     void saveFrame(const std::string& path);
 
 private:
     void check(phg_status st, const char* what);
-    void develop_locked(HdrRgbFrame& out);
+    void develop_locked(const Region& region, HdrRgbFrame& out);
+
+    static constexpr int64_t kTile = 128;
 
     GpuBneeptSettings m_settings;
     void* m_ctx = nullptr;
     std::vector<double> m_film;          // host copy of the last finished chunk's film
     mutable std::mutex m_filmMutex;
     std::atomic<uint64_t> m_sppDone{0};
-    std::atomic<uint64_t> m_sppPolled{0};
+    std::deque<RenderRegionStatus> m_regionQueue;   // guarded by m_regionMutex
+    std::mutex m_regionMutex;
     std::atomic<uint64_t> m_elapsedMs{0};
     std::atomic<double> m_pathsPerSecond{0.0};
 };

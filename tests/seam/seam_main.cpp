@@ -1,11 +1,17 @@
 // Drives the renderer seam (photon-v2_amd/host/gpu_bneept_renderer.h) the way the engine's
 // front ends drive ph::Renderer: load a .p2 scene, update, render on a worker thread while the
-// caller polls progress, statistics and updated regions, then save the frame.
+// caller polls progress, statistics and updated regions, then peek a region and save the frame.
 //   seam_main <scene.p2> <out.pfm>
-// Prints one line per poll and a final summary line.
+// Prints one line per poll that saw regions and a final summary line; exits non-zero when the
+// worker thread failed, a tile never reported its work finished, or a region peek differs from
+// the same pixels of the whole frame.
+#include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <exception>
+#include <set>
 #include <thread>
+#include <utility>
 
 #include "../../include/phg.h"
 #include "../../photon-v2_amd/host/gpu_bneept_renderer.h"
@@ -29,26 +35,74 @@ int main(int argc, char** argv) {
         phg::GpuBneeptRenderer r(s);
         r.doUpdate(scene->desc);
         phg_scene_free(scene);
-        std::thread worker([&] { r.doRender(); });
+        // an exception on the worker thread is carried to this thread (it would otherwise end
+        // the process through std::terminate)
+        std::exception_ptr worker_error;
+        std::atomic<bool> worker_done{false};
+        std::thread worker([&] {
+            try {
+                r.doRender();
+            } catch (...) {
+                worker_error = std::current_exception();
+            }
+            worker_done = true;
+        });
         uint64_t polls = 0, regions = 0;
-        bool finished = false;
-        while (!finished) {
-            phg::RenderRegionStatus reg[4];
-            const std::size_t n = r.asyncPollUpdatedRegions(reg, 4);
+        std::set<std::pair<int64_t, int64_t>> finished_tiles;
+        auto poll = [&]() {
+            phg::RenderRegionStatus reg[16];
+            const std::size_t n = r.asyncPollUpdatedRegions(reg, 16);
+            for (std::size_t k = 0; k < n; ++k)
+                if (reg[k].status == phg::ERegionStatus::Finished)
+                    finished_tiles.insert({reg[k].region.x0, reg[k].region.y0});
             regions += n;
-            if (n && reg[0].finished) finished = true;
-            const phg::RenderProgress p = r.asyncQueryRenderProgress();
-            const phg::RenderStats st = r.asyncQueryRenderStats();
-            if (n) std::printf("poll %llu: %llu/%llu spp, %.0f paths/s\n", (unsigned long long)polls,
-                               (unsigned long long)p.work_done, (unsigned long long)p.total_work, st.paths_per_second);
+            if (n) {
+                const phg::RenderProgress p = r.asyncQueryRenderProgress();
+                const phg::RenderStats st = r.asyncQueryRenderStats();
+                std::printf("poll %llu: %zu regions, %llu/%llu spp, %.0f paths/s\n", (unsigned long long)polls, n,
+                            (unsigned long long)p.work_done, (unsigned long long)p.total_work, st.paths_per_second);
+            }
             ++polls;
-            if (!finished) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+            return n;
+        };
+        while (!worker_done) {
+            poll();
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
         }
         worker.join();
+        if (worker_error) std::rethrow_exception(worker_error);
+        while (poll() > 0) {}
+        const int64_t tiles = ((s.width + 127) / 128) * ((s.height + 127) / 128);
+        if ((int64_t)finished_tiles.size() != tiles) {
+            std::fprintf(stderr, "error: %zu of %lld tiles reported finished\n", finished_tiles.size(),
+                         (long long)tiles);
+            return 1;
+        }
+        // a peek of a region develops exactly those pixels of the frame
+        phg::HdrRgbFrame whole, part;
+        r.retrieveFrame(0, whole);
+        part.width = s.width;
+        part.height = s.height;
+        part.rgb.assign(3ull * s.width * s.height, -1.0f);
+        const phg::Region reg{3, 5, (int64_t)s.width / 2, (int64_t)s.height - 2};
+        r.asyncPeekFrame(0, reg, part);
+        for (int64_t y = 0; y < (int64_t)s.height; ++y)
+            for (int64_t x = 0; x < (int64_t)s.width; ++x) {
+                const bool in = x >= reg.x0 && x < reg.x1 && y >= reg.y0 && y < reg.y1;
+                for (int k = 0; k < 3; ++k) {
+                    const size_t i = 3 * (size_t)(y * s.width + x) + k;
+                    if (in ? part.rgb[i] != whole.rgb[i] : part.rgb[i] != -1.0f) {
+                        std::fprintf(stderr, "error: region peek differs at (%lld, %lld)\n", (long long)x,
+                                     (long long)y);
+                        return 1;
+                    }
+                }
+            }
         r.saveFrame(argv[2]);
         const phg::RenderProgress p = r.asyncQueryRenderProgress();
-        std::printf("done %u x %u, %llu spp, %llu region updates, estimator %u, filter %u\n", s.width, s.height,
-                    (unsigned long long)p.work_done, (unsigned long long)regions, s.estimator, s.filter);
+        std::printf("done %u x %u, %llu spp, %llu region updates over %lld tiles, estimator %u, filter %u\n", s.width,
+                    s.height, (unsigned long long)p.work_done, (unsigned long long)regions, (long long)tiles,
+                    s.estimator, s.filter);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

@@ -44,8 +44,9 @@ def check_parity(gpu, ref, name, min_identical):
 
 
 def test_cornell_matches_oracle(ctx):
+    """The Cornell-box configuration at its full size: 256^2, 16 spp, depth 5."""
     sc = phg.Scene.generate(phg.GEN_CORNELL)
-    W = H = 64
+    W = H = 256
     print("[render] case=cornell start", flush=True)
     gpu = gpu_render(ctx, sc, W, H, 16, 5, 11)
     ref, _ = orc.render(sc, W, H, 16, max_bounces=5, seed=11)
@@ -233,8 +234,9 @@ def checkerboard_lights(n=12):
 
 def test_renderer_seam_renders_a_p2_file(ctx, tmp_path):
     """The C++ renderer seam (GpuBneeptRenderer) driven like the engine's front ends: .p2 load,
-    update, render on a worker thread while polling progress / regions, save the frame as
-    .pfm.  The saved frame equals a direct render of the same settings."""
+    update, render on a worker thread while polling progress and updated regions (every tile
+    reported finished), peek a region (exactly those pixels developed), save the frame as .pfm.
+    The saved frame equals a direct render of the same settings."""
     import os
     import subprocess
     here = os.path.dirname(os.path.abspath(__file__))
@@ -245,7 +247,11 @@ def test_renderer_seam_renders_a_p2_file(ctx, tmp_path):
     r = subprocess.run([exe, scene_path, out], capture_output=True, text=True, timeout=120)
     print(r.stdout[-400:], r.stderr[-400:], flush=True)
     assert r.returncode == 0
-    assert "done 64 x 48, 16 spp, 4 region updates" in r.stdout
+    import re
+    m = re.search(r"done 64 x 48, 16 spp, (\d+) region updates over (\d+) tiles", r.stdout)
+    assert m, r.stdout[-400:]
+    # one finished 128 x 128 tile per chunk of 4 samples (how many polls see them varies)
+    assert int(m.group(2)) == 1 and int(m.group(1)) == 4
     data = open(out, "rb").read()
     header = b"PF\n64 48\n-1\n"
     assert data.startswith(header)
@@ -253,3 +259,25 @@ def test_renderer_seam_renders_a_p2_file(ctx, tmp_path):
     sc, st = phg.Scene.parse_p2(path=scene_path)
     film = gpu_render(ctx, sc, 64, 48, 16, st["max_bounces"], 17)
     assert np.array_equal(img, phg.develop(film).astype(np.float32))
+
+
+@pytest.mark.parametrize("shape,camera,expect,bar", [("sphere", "outside", 0.5, 1e-3), ("cube", "outside", 0.5, 1e-3),
+                                                    ("sphere", "inside", 1.125, 2.5e-3)])
+def test_glass_furnace(ctx, shape, camera, expect, bar):
+    """The reference's gray furnace with a glass sphere and a glass cube
+    (Main/RenderTest/tests/test_gray_furnace_box.py:31-37,48-49,79-80: image mean 0.5 within
+    1e-3), plus the camera inside the glass, where every pixel is 0.5 * 1.5^2 (the
+    transmission lobe's (eta_t / eta_i)^2 radiance factor).  The same render on the oracle at a
+    smaller size agrees path for path."""
+    from helpers import film_mean_se, glass_furnace_scene
+    sc = glass_furnace_scene(shape, camera=camera)
+    print(f"[render] case=glass_furnace_{shape}_{camera} start", flush=True)
+    film = gpu_render(ctx, sc, 256, 256, 256, 10000, 61)
+    mean, se = film_mean_se(film)
+    print(f"[render] case=glass_furnace_{shape}_{camera} mean={mean:.6f} se={se:.2e} expect={expect}", flush=True)
+    assert abs(mean - expect) < bar
+    gpu = gpu_render(ctx, sc, 32, 32, 16, 10000, 62)
+    ref, _ = orc.render(sc, 32, 32, 16, max_bounces=10000, seed=62)
+    ident = np.all(np.isclose(gpu[..., :3], ref[..., :3], rtol=1e-5, atol=1e-7), axis=-1).mean()
+    print(f"[render] case=glass_furnace_{shape}_{camera} identical_pixels={ident:.4f}", flush=True)
+    assert ident > 0.8
