@@ -210,6 +210,16 @@ phg_status phg_upload_scene(void* ctx, const phg_scene_desc* scene, const phg_bv
 phg_status phg_intersect(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits,
                          int query, int ptr_kind);
 
+/* Allocation-free variant of phg_intersect for device pointers.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or allocating library routine: its scratch (error flag and traversal-stack spill)
+ * is a file-scope device array of 16 MiB per GPU, held until the process ends, and its grid is
+ * sized to fit that array (a BVH whose per-block spill exceeds it is PHG_ERR_UNSUPPORTED).
+ * Results equal phg_intersect's.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (from several contexts or threads) race; calls are synchronous. */
+phg_status phg_intersect_noalloc(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits, int query);
+
 /* Same as phg_intersect(PHG_QUERY_CLOSEST) but also returns per-ray node visits and
  * triangle tests (for the algorithmic-byte roofline). */
 phg_status phg_intersect_counted(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits,

@@ -116,6 +116,7 @@ struct Ctx {
     uint32_t trace_blocks_per_cu = 8;   // resident blocks per compute unit, from the occupancy query
     uint32_t shade_blocks_per_cu = 4;
     uint32_t paths_per_pass = kDefaultPassPaths; // paths in flight per wavefront pass
+    void* noalloc_scratch = nullptr;    // this device's copy of the allocation-free query's scratch
 };
 
 
@@ -280,7 +281,9 @@ phg_status phg_create(void** out_ctx, const phg_device_opts* opts) {
             delete c;
             return PHG_ERR_HIP;
         }
-        if (resident_blocks(c->trace_block, &c->trace_blocks_per_cu, &c->shade_blocks_per_cu) != hipSuccess) {
+        if (resident_blocks(c->trace_block, &c->trace_blocks_per_cu, &c->shade_blocks_per_cu) != hipSuccess ||
+            noalloc_scratch_address(dev, &c->noalloc_scratch) != hipSuccess) {
+            (void)hipGetLastError();
             (void)hipStreamDestroy(c->stream);
             delete c;
             return PHG_ERR_HIP;
@@ -660,6 +663,42 @@ phg_status phg_intersect(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hi
         Ctx* c = static_cast<Ctx*>(ctx);
         if (!c) return PHG_ERR_INVALID_ARG;
         return do_intersect(c, rays, n, hits, query, ptr_kind, nullptr, nullptr);
+    });
+}
+
+phg_status phg_intersect_noalloc(void* ctx, const phg_ray* rays, uint64_t n, phg_hit* hits, int query) {
+    return guarded(ctx, [&]() -> phg_status {
+        Ctx* c = static_cast<Ctx*>(ctx);
+        if (!c) return PHG_ERR_INVALID_ARG;
+        if (!c->have_scene) return set_err(c, PHG_ERR_NO_SCENE, "no scene uploaded");
+        if (n == 0) return PHG_OK;
+        if (!rays || !hits || (query != PHG_QUERY_CLOSEST && query != PHG_QUERY_ANY))
+            return set_err(c, PHG_ERR_INVALID_ARG, "bad intersect arguments");
+        // the grid is sized to the scratch: blocks of kTraceBlock lanes, each lane with its
+        // share of the traversal-stack spill (the kernel streams any number of rays per lane)
+        const size_t lane_entries = std::max<uint32_t>(spill_cap(c), 1u);
+        const size_t block_bytes = (size_t)kTraceBlock * lane_entries * sizeof(int32_t);
+        const size_t fit_blocks = (kNoAllocScratchBytes - kNoAllocFlagBytes) / block_bytes;
+        if (fit_blocks == 0)
+            return set_err(c, PHG_ERR_UNSUPPORTED, "BVH too deep: one block's traversal-stack spill exceeds the "
+                                                   "allocation-free query's scratch");
+        const uint32_t grid = (uint32_t)std::min<size_t>(trace_threads(c) / kTraceBlock, fit_blocks);
+        uint32_t* d_flags = static_cast<uint32_t*>(c->noalloc_scratch);
+        int32_t* d_spill = reinterpret_cast<int32_t*>(static_cast<unsigned char*>(c->noalloc_scratch) + kNoAllocFlagBytes);
+        PHG_HIP(c, hipSetDevice(c->device));
+        PHG_HIP(c, launch_clear_counts(d_flags, 1, c->stream));   // a fill kernel, no memset node
+        PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+        PHG_HIP(c, launch_intersect(c->sc, rays, n, hits, query == PHG_QUERY_ANY, nullptr, nullptr, d_flags, d_spill,
+                                    spill_cap(c), grid, c->stream));
+        PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+        uint32_t flags = 0;
+        PHG_HIP(c, hipMemcpyAsync(&flags, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        PHG_HIP(c, hipStreamSynchronize(c->stream));
+        float ms = 0.0f;
+        PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        if (query == PHG_QUERY_ANY) c->stats.shadow_ms = ms; else c->stats.trace_ms = ms;
+        if (flags) return set_err(c, PHG_ERR_UNSUPPORTED, "traversal stack overflow");
+        return PHG_OK;
     });
 }
 

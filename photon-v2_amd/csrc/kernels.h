@@ -114,6 +114,12 @@ struct PathBuffers {
     uint32_t* path_seg;      // per path: its pass segment
 };
 
+// Size of the device scratch array of the allocation-free ray query; its address on `device`
+// (looked up on the first call per device, which makes that device current).
+constexpr size_t kNoAllocScratchBytes = 16u << 20;
+constexpr size_t kNoAllocFlagBytes = 256;
+hipError_t noalloc_scratch_address(int device, void** out);
+
 hipError_t launch_intersect(const SceneDev& sc, const phg_ray* rays, uint64_t n, phg_hit* hits, int any,
                             uint32_t* nvis, uint32_t* ntri, uint32_t* flags, int32_t* spill, uint32_t spill_cap,
                             uint32_t grid, hipStream_t st);

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "kernels.h"
 #include "pt_device.h"
@@ -609,6 +610,34 @@ __global__ void k_clear_counts(uint32_t* counts, uint32_t n) {
 __global__ void k_fill_double(double* p, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------------------
+// Scratch of the allocation-free ray query (phg_intersect_noalloc): its error flag in the
+// first 256 bytes, then the traversal-stack spill of its grid.  Each GPU holds its own copy at
+// its own address, looked up once per device with that device current; host code uses only
+// that address, never the array's name (which is a host-side stand-in).
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch_address(int device, void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices || !out) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        if (e != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return hipSuccess;
 }
 
 // ---------------------------------------------------------------------------------------

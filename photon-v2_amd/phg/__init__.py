@@ -143,7 +143,7 @@ EXPORTED = ["phg_create", "phg_destroy", "phg_last_error", "phg_upload_scene", "
             "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_gradients",
             "phg_guide_train", "phg_scene_generate", "phg_scene_free", "phg_upload_scenes", "phg_render_batch",
             "phg_batch_film_read", "phg_set_tuning", "phg_get_tuning", "phg_autotune_batch", "phg_scene_parse_p2",
-            "phg_scene_load_p2", "phg_film_develop", "phg_film_write_pfm"]
+            "phg_scene_load_p2", "phg_film_develop", "phg_film_write_pfm", "phg_intersect_noalloc"]
 
 _lib = None
 
@@ -164,6 +164,7 @@ def lib() -> C.CDLL:
     L.phg_upload_scene.argtypes = [vp, C.POINTER(phg_scene_desc), C.POINTER(phg_bvh_opts)]
     L.phg_intersect.argtypes = [vp, vp, C.c_uint64, vp, C.c_int, C.c_int]
     L.phg_intersect_counted.argtypes = [vp, vp, C.c_uint64, vp, vp, vp, C.c_int]
+    L.phg_intersect_noalloc.argtypes = [vp, vp, C.c_uint64, vp, C.c_int]
     L.phg_film_clear.argtypes = [vp, C.c_uint32, C.c_uint32]
     L.phg_film_read.argtypes = [vp, vp, C.c_uint64]
     L.phg_render.argtypes = [vp, C.POINTER(phg_render_opts)]
@@ -476,6 +477,11 @@ class Context:
         """Device-pointer variant (e.g. torch tensors' data_ptr())."""
         self._check(self._L.phg_intersect(self._h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
                                           QUERY_ANY if any_hit else QUERY_CLOSEST, PTR_DEVICE), "phg_intersect")
+
+    def intersect_noalloc(self, rays_ptr: int, n: int, hits_ptr: int, any_hit: bool = False):
+        """phg_intersect_noalloc on device pointers: no device allocation by the library."""
+        self._check(self._L.phg_intersect_noalloc(self._h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
+                                                  QUERY_ANY if any_hit else QUERY_CLOSEST), "phg_intersect_noalloc")
 
     def film_clear(self, width: int, height: int):
         self._check(self._L.phg_film_clear(self._h, width, height), "phg_film_clear")

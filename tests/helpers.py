@@ -233,3 +233,43 @@ def film_mean_se(film):
     mean = film[..., 0].sum() / n
     var = max(film[..., 4].sum() / n - mean * mean, 0.0)
     return mean, np.sqrt(var / n)
+
+
+class DeviceArray:
+    """A caller-owned device buffer for tests of the device-pointer entry points, allocated with
+    the HIP runtime directly (hipMalloc / hipMemcpy / hipFree through ctypes)."""
+    _hip = None
+
+    @classmethod
+    def _lib(cls):
+        import ctypes as C
+        if cls._hip is None:
+            import os
+            path = "/opt/rocm/lib/libamdhip64.so"
+            h = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, host: np.ndarray):
+        import ctypes as C
+        self.host = np.ascontiguousarray(host)
+        self.ptr = C.c_void_p()
+        h = self._lib()
+        assert h.hipMalloc(C.byref(self.ptr), max(self.host.nbytes, 1)) == 0
+        assert h.hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1) == 0   # host to device
+
+    def read(self) -> np.ndarray:
+        out = np.empty_like(self.host)
+        h = self._lib()
+        assert h.hipDeviceSynchronize() == 0
+        assert h.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0               # device to host
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._lib().hipFree(self.ptr)
+            self.ptr = None
