@@ -4,8 +4,9 @@ Workload (one step): plain BNEEPT over the 1M-triangle procedural scene — 64 d
 icospheres + tessellated ground (2^20 triangles) + 16 area-light quads, GGX conductors and
 dielectrics — at 1024x1024, 64 samples per pixel, 8 bounces, FP32 shading, FP64 film.
 Inputs are resident on the GPU before timing starts (scene and BVH uploaded once).
-The headline value times the production mode: each pass (4M paths) replays one captured
-HIP graph of the whole bounce loop.  The same steps also run eagerly (one host round trip
+The headline value times the production mode: the autotuner's pass size (64M paths on this
+workload, so the 64-spp render is one pass) and each pass replays one captured HIP graph of
+the whole bounce loop.  The same steps also run eagerly (one host round trip
 per bounce) with HIP events around every kernel launch; those give the kernel times.
 
 Reported beside the throughput:
@@ -79,6 +80,23 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor()
+
+
+def lscpu():
+    """Model name and core counts as lscpu reports them (empty when lscpu is missing)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return {}
+    keep = {"Model name": "model", "CPU(s)": "cpus", "Thread(s) per core": "threads_per_core",
+            "Core(s) per socket": "cores_per_socket", "Socket(s)": "sockets"}
+    info = {}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keep:
+            info[keep[k.strip()]] = v.strip()
+    return info
 
 
 def batch_leg(args):
@@ -424,22 +442,25 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as orc
+        # the timed CPU build: -O3 -march=native for this host, built here (outside any timing)
+        orc.build_native(force=True)
         threads = os.cpu_count() or 1
         # probe, then size the row sample to about args.cpu_seconds of CPU work
         step = 64
-        _, pst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=threads, row_step=256)
+        _, pst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=threads, row_step=256, native=True)
         per_path_s = pst["render_ms"] / 1e3 / max(pst["paths"], 1)
         want_paths = args.cpu_seconds / max(per_path_s, 1e-9)
         step = int(max(1, min(256, round(W * H * spp / max(want_paths, 1.0)))))
-        _, cst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=threads, row_step=step)
+        _, cst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=threads, row_step=step, native=True)
         cpu = {"value": cst["paths"] / (cst["render_ms"] / 1e3) / 1e6, "unit": "Msamples/s", "cores": threads,
                "kind": "port",
                "sample": f"every {ordinal(step)} row of the same {W}x{H} image at {spp} spp ({cst['paths']} paths, "
-                         f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}"}
+                         f"{cst['render_ms'] / 1e3:.1f} s), oracle BNEEPT, {threads} threads on {cpu_model()}",
+               "build": f"g++ {orc.NATIVE_FLAGS}", "lscpu": lscpu()}
         # one core, on a sample of about a third of the time
         per_path_1 = per_path_s * threads
         step1 = int(max(1, min(1024, round(W * H * spp / max(args.cpu_seconds / 3 / per_path_1, 1.0)))))
-        _, sst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=1, row_step=step1)
+        _, sst = orc.render(scene, W, H, spp, max_bounces=B, seed=1, nthreads=1, row_step=step1, native=True)
         cpu["single_core"] = {"value": sst["paths"] / (sst["render_ms"] / 1e3) / 1e6, "unit": "Msamples/s",
                               "sample": f"every {ordinal(step1)} row ({sst['paths']} paths, {sst['render_ms'] / 1e3:.1f} s), "
                                         "1 thread"}

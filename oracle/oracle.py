@@ -51,20 +51,29 @@ class orc_render_stats(C.Structure):
                 ("paths", C.c_uint64), ("render_ms", C.c_double)]
 
 
-_lib = None
+NATIVE_LIB_PATH = os.path.join(_HERE, "_build", "liboracle_native.so")
+NATIVE_FLAGS = "-O3 -march=native -ffp-contract=off -fno-fast-math"
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} missing; build it with `make -C oracle`")
-    L = C.CDLL(LIB_PATH)
+def build_native(force: bool = False):
+    """The timed CPU-baseline build (-O3 -march=native) for the host this runs on; force
+    rebuilds it even when a build (perhaps for another host) is already there."""
+    subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE, "native"], check=True)
+
+
+def lib(native: bool = False) -> C.CDLL:
+    """The oracle library; native=True loads the -O3 -march=native build (build_native())."""
+    if native in _libs:
+        return _libs[native]
+    path = NATIVE_LIB_PATH if native else LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing; build it with `make -C oracle{' native' if native else ''}`")
+    L = C.CDLL(path)
     vp, f, u32, u64, i32 = C.c_void_p, C.c_float, C.c_uint32, C.c_uint64, C.c_int32
     L.orc_pcg32_generate.argtypes = [u64, u64, C.c_int, u64, u32, vp]
     L.orc_moremur.argtypes = [u64]
@@ -104,7 +113,7 @@ def lib() -> C.CDLL:
     L.orc_render_paths.argtypes = [C.POINTER(orc_scene), C.POINTER(orc_render_opts), vp, vp, u32, vp]
     L.orc_film_splat.argtypes = [C.c_int, C.c_int64, C.c_int64, vp, vp, u32, vp]
     L.orc_film_develop.argtypes = [C.c_int64, C.c_int64, vp, vp]
-    _lib = L
+    _libs[native] = L
     return L
 
 
@@ -164,10 +173,10 @@ def render_filtered(scene, width, height, spp, filter, max_bounces=10000, seed=0
 
 
 def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_step=1,
-           estimator=0):
+           estimator=0, native=False):
     """FP64 film (H, W, 8) of the oracle render (estimator 0 BNEEPT, 1 BVPT, 2 BVPTDL), plus
-    ray counts."""
-    L = lib()
+    ray counts.  native=True runs the -O3 -march=native build."""
+    L = lib(native)
     sv = SceneView(scene)
     film = np.zeros((height, width, 8), np.float64)
     o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, row_step,

@@ -543,3 +543,14 @@ def test_oracle_bvpt_matches_bneept_in_expectation(estimator, cap):
     ma, mb = a[..., :3].sum() / a[..., 3].sum(), b[..., :3].sum() / b[..., 3].sum()
     assert frac < 0.03, frac
     assert abs(ma - mb) < 0.03 * mb, (ma, mb)
+
+
+def test_native_build_renders_the_same_film():
+    """The timed CPU baseline is the same source built -O3 -march=native (contraction off): it
+    renders the oracle's film bit for bit."""
+    import phg
+    orc.build_native()
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    a, _ = orc.render(sc, 24, 24, 4, max_bounces=5, seed=3)
+    b, _ = orc.render(sc, 24, 24, 4, max_bounces=5, seed=3, native=True)
+    assert np.array_equal(a, b)
