@@ -43,14 +43,24 @@ enum {
     PHG_MAT_DIELECTRIC = 3  /* TranslucentMicrofacet + exact dielectric Fresnel via AbradedTranslucent */
 };
 
+/* Conductor Fresnel (ConductiveInterfaceInfo::genFresnelEffect,
+ * Actor/Material/Component/ConductiveInterfaceInfo.cpp): Schlick's approximation from F0
+ * (SchlickApproxConductorFresnel.cpp:35-43) or the exact conductor-dielectric formula from the
+ * complex index of refraction n + ik inside and the real index outside
+ * (ExactConductorFresnel.cpp:66-101). */
+enum { PHG_FRESNEL_SCHLICK = 0, PHG_FRESNEL_EXACT = 1 };
+
 typedef struct phg_material {
     uint32_t type;
-    float rgb[3];        /* albedo (Lambert) or F0 (conductor) */
+    float rgb[3];        /* albedo (Lambert) or F0 (conductor, Schlick) */
     float roughness;     /* GGX alpha = max(roughness^2, 0.001) */
-    float ior_outer;     /* dielectric only */
-    float ior_inner;
-    float pad;
-} phg_material;          /* 32 bytes */
+    float ior_outer;     /* dielectric; conductor with PHG_FRESNEL_EXACT: the outer index */
+    float ior_inner;     /* dielectric only */
+    uint32_t fresnel;    /* conductor: PHG_FRESNEL_SCHLICK (F0 = rgb) or PHG_FRESNEL_EXACT (n, k) */
+    float n[3];          /* conductor with PHG_FRESNEL_EXACT: inner index, real part per channel */
+    float k[3];          /* ... imaginary part per channel */
+    float pad[2];
+} phg_material;          /* 64 bytes */
 
 /* Area-light emitter group (DiffuseSurfaceEmitter / GroupedDiffuseSurfaceEmitter):
  * constant front-face radiance over triangles emitter_tris[tri_begin, tri_begin+tri_count). */

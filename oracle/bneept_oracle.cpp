@@ -440,6 +440,36 @@ static inline RGB fresnel_schlick(const orc_material& m, float cosI) {
     const float p = static_cast<float>(std::pow(1.0f - c, 5));
     return RGB{(1.0f - m.rgb[0]) * p + m.rgb[0], (1.0f - m.rgb[1]) * p + m.rgb[1], (1.0f - m.rgb[2]) * p + m.rgb[2]};
 }
+// ExactConductorFresnel.cpp:66-101 (Lagarde's conductor formula), per channel: the
+// reflectance of a conductor of complex index n + ik under a dielectric of index eta_outer;
+// setIors (:103-116) precomputes en2 - ek2 and 4 en2 ek2 with en2 = (n / eta_outer)^2 (the
+// square as a product).  The sign of cosI does not matter.
+static inline float fresnel_conductor_exact1(float etaOuter, float n, float k, float cosThetaI) {
+    const float en = n / etaOuter, ek = k / etaOuter;
+    const float en2 = en * en, ek2 = ek * ek;
+    const float e2_sub_k2 = en2 - ek2, four_e2k2 = en2 * ek2 * 4.0f;
+    const float cosI = std::fabs(cosThetaI);
+    const float cosI2 = cosI * cosI;
+    const float sinI2 = 1.0f - cosI * cosI;
+    const float t0 = e2_sub_k2 - sinI2;
+    const float a2plusb2 = std::sqrt(t0 * t0 + four_e2k2);
+    const float a = std::sqrt((a2plusb2 + t0) * 0.5f);
+    const float t1 = a2plusb2 + cosI2;
+    const float t2 = a * (2.0f * cosI);
+    const float t3 = a2plusb2 * cosI2 + sinI2 * sinI2;
+    const float t4 = t2 * sinI2;
+    const float Rs = (t1 - t2) / (t1 + t2);
+    const float Rp = Rs * ((t3 - t4) / (t3 + t4));
+    return (Rs + Rp) * 0.5f;
+}
+// the conductor's Fresnel term as its interface info selects it (ConductiveInterfaceInfo.cpp)
+static inline RGB fresnel_conductor(const orc_material& m, float cosI) {
+    if (m.fresnel == 1)
+        return RGB{fresnel_conductor_exact1(m.ior_outer, m.n[0], m.k[0], cosI),
+                   fresnel_conductor_exact1(m.ior_outer, m.n[1], m.k[1], cosI),
+                   fresnel_conductor_exact1(m.ior_outer, m.n[2], m.k[2], cosI)};
+    return fresnel_schlick(m, cosI);
+}
 // ExactDielectricFresnel.cpp:12-41
 static inline float fresnel_dielectric(float etaOuter, float etaInner, float cosThetaI) {
     float etaI = etaOuter, etaT = etaInner, cosI = cosThetaI;
@@ -503,7 +533,7 @@ static BsdfEval bsdf_eval(const orc_material& m, const SurfaceHit& X, V3 L, V3 V
         if (!half_vec_same_hemi(L, V, N, &H)) return e;
         const float hol = dot(H, L);
         const float a = mat_alpha(m);
-        const RGB F = fresnel_schlick(m, hol);
+        const RGB F = fresnel_conductor(m, hol);
         const float D = ggx_D(N, H, a);
         const float G = ggx_G(N, H, L, V, a);
         e.f = muls(F, D * G / (4.0f * std::fabs(nov * nol)));
@@ -617,7 +647,7 @@ static BsdfSample bsdf_sample(const orc_material& m, const SurfaceHit& X, V3 V, 
         const float nov = dot(N, V);
         const float hol = dot(H, L);
         const float dotTerms = std::fabs(hol / nov);
-        const RGB F = fresnel_schlick(m, hol);
+        const RGB F = fresnel_conductor(m, hol);
         const float G = ggx_G(N, H, L, V, a);
         const float D = ggx_D(N, H, a);
         const float pdf = ggx_vndf_pdf(N, H, V, a);
@@ -1211,6 +1241,12 @@ void orc_fresnel_schlick(const float* f0_3, float cos_i, float* out3) {
     const RGB F = fresnel_schlick(m, cos_i);
     out3[0] = F.r; out3[1] = F.g; out3[2] = F.b;
 }
+void orc_fresnel_conductor_exact(float eta_outer, const float* n3, const float* k3, float cos_i, float* out3) {
+    orc_material m{}; m.fresnel = 1; m.ior_outer = eta_outer;
+    for (int c = 0; c < 3; ++c) { m.n[c] = n3[c]; m.k[c] = k3[c]; }
+    const RGB F = fresnel_conductor(m, cos_i);
+    out3[0] = F.r; out3[1] = F.g; out3[2] = F.b;
+}
 int orc_half_vector(const float* L3, const float* V3_, float* out_H3) {
     V3 h = mk(L3[0], L3[1], L3[2]) + mk(V3_[0], V3_[1], V3_[2]);
     if (is_zero(h)) return 0;
@@ -1382,6 +1418,19 @@ static double filter_eval(int filter, double x, double y) {
 }
 static double filter_half(int filter) { return filter == 0 ? 0.5 : filter == 3 ? 0.5 : 2.0; }
 
+// Raster coordinate of a sample of a filtered film.  TSamplingFilm::updateSampleDimensions
+// (TSamplingFilm.ipp:103-134; soft edge on, softness 1) expands the image's pixel centres by
+// the filter half size into the sample window, and ReceiverSamplingWork::doWork
+// (ReceiverSamplingWork.cpp:76-101) spreads a batch's W x H raster samples over that window
+// (sampleToSurface of a W x H stratified stage): stratum x of n with offset u in [0, 1) lands at
+// lo + ((x + u) / n) * extent, in FP64.  With a unit-size filter the window is the image and
+// the position is x + u (float, as the camera takes it).
+static double window_raster(uint32_t x, float u, uint32_t n, double half) {
+    const double lo = 0.5 - half;
+    const double ext = ((double)n - 0.5 + half) - lo;
+    return lo + (((double)x + (double)u) / (double)n) * ext;
+}
+
 void orc_film_splat(int filter, int64_t W, int64_t H, const double* xy, const float* rgb3, uint32_t n, double* sensors4) {
     const double hs = filter_half(filter);
     for (uint32_t i = 0; i < n; ++i) {
@@ -1401,7 +1450,8 @@ void orc_film_splat(int filter, int64_t W, int64_t H, const double* xy, const fl
 }
 
 // Render with a reconstruction filter: every path's radiance is splatted at its raster
-// position (pixel + the first two numbers of its stream) into the 8-double film layout
+// position (window_raster of its pixel and the first two numbers of its stream) into the
+// 8-double film layout
 // (sum w r, g, b, sum w, sum w r^2, g^2, b^2, 0), samples in (pixel, sample) order.
 // filter: 0 box, 1 gaussian, 2 Blackman-Harris, 4 Mitchell-Netravali.
 int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int filter, double* film) {
@@ -1413,6 +1463,7 @@ int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int 
     std::vector<double> xy(2 * ns);
     std::vector<float> rgb(3 * ns);
     std::atomic<uint64_t> next{0};
+    const double hs = filter_half(filter);
     auto worker = [&]() {
         orc::PathStats st;
         for (uint64_t pix = next.fetch_add(1); pix < npix; pix = next.fetch_add(1)) {
@@ -1420,12 +1471,13 @@ int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int 
             for (uint32_t k = 0; k < opt->spp; ++k) {
                 const uint64_t i = pix * opt->spp + k;
                 orc::Stream sf(orc::path_key(opt->seed, pix, opt->spp_begin + k, npix));
-                const float rx = (float)x + sf.u();
-                const float ry = (float)y + sf.u();
+                const float ux = sf.u(), uy = sf.u();
+                double xr = (double)((float)x + ux), yr = (double)((float)y + uy);
+                if (hs != 0.5) { xr = window_raster(x, ux, W, hs); yr = window_raster(y, uy, H, hs); }
                 orc::V3 o, d;
-                orc::camera_ray(scene->camera, W, H, rx, ry, &o, &d);
+                orc::camera_ray(scene->camera, W, H, (float)xr, (float)yr, &o, &d);
                 const orc::RGB L = orc::estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st);
-                xy[2 * i] = rx; xy[2 * i + 1] = ry;
+                xy[2 * i] = xr; xy[2 * i + 1] = yr;
                 rgb[3 * i] = L.r; rgb[3 * i + 1] = L.g; rgb[3 * i + 2] = L.b;
             }
         }
@@ -1433,7 +1485,6 @@ int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int 
     std::vector<std::thread> th;
     for (uint32_t i = 0; i < std::max(1u, opt->nthreads); ++i) th.emplace_back(worker);
     for (auto& t : th) t.join();
-    const double hs = filter_half(filter);
     for (uint64_t i = 0; i < ns; ++i) {
         const double xp = xy[2 * i], yp = xy[2 * i + 1];
         const double fminx = std::max(xp - hs, 0.0), fminy = std::max(yp - hs, 0.0);

@@ -19,7 +19,10 @@ typedef struct orc_material {
     float roughness;   /* alpha = max(roughness^2, 0.001) */
     float ior_outer;
     float ior_inner;
-    float pad;
+    uint32_t fresnel;  /* conductor: 0 Schlick from rgb, 1 exact from ior_outer, n, k */
+    float n[3];
+    float k[3];
+    float pad[2];
 } orc_material;
 
 typedef struct orc_emitter {

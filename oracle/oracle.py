@@ -20,7 +20,8 @@ FLT_MAX = float(np.finfo(np.float32).max)
 
 class orc_material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("rgb", C.c_float * 3), ("roughness", C.c_float),
-                ("ior_outer", C.c_float), ("ior_inner", C.c_float), ("pad", C.c_float)]
+                ("ior_outer", C.c_float), ("ior_inner", C.c_float), ("fresnel", C.c_uint32),
+                ("n", C.c_float * 3), ("k", C.c_float * 3), ("pad", C.c_float * 2)]
 
 
 class orc_emitter(C.Structure):
@@ -101,6 +102,7 @@ def lib(native: bool = False) -> C.CDLL:
     L.orc_fresnel_dielectric.argtypes = [f, f, f]
     L.orc_fresnel_dielectric.restype = f
     L.orc_fresnel_schlick.argtypes = [vp, f, vp]
+    L.orc_fresnel_conductor_exact.argtypes = [f, vp, vp, f, vp]
     L.orc_half_vector.argtypes = [vp, vp, vp]
     L.orc_basis_from_y.argtypes = [vp, vp, vp]
     L.orc_pwc_distribution.argtypes = [vp, u32, vp, u32, vp, vp, vp]
@@ -121,13 +123,19 @@ def f32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, np.float32))
 
 
-def material(mtype: int, rgb=(0.5, 0.5, 0.5), roughness=0.3, ior_outer=1.0, ior_inner=1.5) -> orc_material:
+def material(mtype: int, rgb=(0.5, 0.5, 0.5), roughness=0.3, ior_outer=1.0, ior_inner=1.5, n=None,
+             k=None) -> orc_material:
+    """A material record; a conductor given n and k uses the exact conductor Fresnel."""
     m = orc_material()
     m.type = mtype
     m.rgb[:] = [float(x) for x in rgb]
     m.roughness = roughness
     m.ior_outer = ior_outer
     m.ior_inner = ior_inner
+    if n is not None:
+        m.fresnel = 1
+        m.n[:] = [float(x) for x in n]
+        m.k[:] = [float(x) for x in k]
     return m
 
 

@@ -355,8 +355,14 @@ static phg_status prepare_scene(const phg_scene_desc* s, const BvhBuildParams& b
         if (s->tri_material[t] >= s->num_materials) return scene_fail(h, PHG_ERR_INVALID_ARG, "triangle material id out of range");
         if (s->tri_emitter[t] >= (int32_t)s->num_emitters) return scene_fail(h, PHG_ERR_INVALID_ARG, "triangle emitter id out of range");
     }
-    for (uint32_t m = 0; m < s->num_materials; ++m)
-        if (s->materials[m].type > PHG_MAT_DIELECTRIC) return scene_fail(h, PHG_ERR_INVALID_ARG, "unknown material type");
+    for (uint32_t m = 0; m < s->num_materials; ++m) {
+        const phg_material& mt = s->materials[m];
+        if (mt.type > PHG_MAT_DIELECTRIC) return scene_fail(h, PHG_ERR_INVALID_ARG, "unknown material type");
+        if (mt.type == PHG_MAT_CONDUCTOR && mt.fresnel > PHG_FRESNEL_EXACT)
+            return scene_fail(h, PHG_ERR_INVALID_ARG, "unknown conductor Fresnel");
+        if (mt.type == PHG_MAT_CONDUCTOR && mt.fresnel == PHG_FRESNEL_EXACT && !(mt.ior_outer > 0.0f))
+            return scene_fail(h, PHG_ERR_INVALID_ARG, "exact conductor Fresnel needs a positive outer index");
+    }
     for (uint32_t e = 0; e < s->num_emitters; ++e) {
         const phg_emitter& em = s->emitters[e];
         if (em.tri_count == 0 || (uint64_t)em.tri_begin + em.tri_count > s->num_emitter_tris)

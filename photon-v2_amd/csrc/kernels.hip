@@ -147,8 +147,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         uint32_t pixel, sample;
         path_ids(sg, job, i, &pixel, &sample);
         Stream s = stream_init(path_key(job.seed, pixel, sample, job.npix));
-        const float rx = (float)(pixel % job.width) + s.u();
-        const float ry = (float)(pixel / job.width) + s.u();
+        const uint32_t px = pixel % job.width, py = pixel / job.width;
+        const float ux = s.u(), uy = s.u();
+        float rx = (float)px + ux, ry = (float)py + uy;
+        if (job.filter != PHG_FILTER_BOX) {
+            rx = (float)window_raster(px, ux, job.width, kFilterHalf);
+            ry = (float)window_raster(py, uy, job.height, kFilterHalf);
+        }
         // PinholeCamera::genReceiveRayDir with the horizontal-FOV sensor of SingleLensObserver
         const phg_camera& c = job.cam;
         const float cx = (2.0f * (rx / (float)job.width) - 1.0f) * c.tan_half_fov;
@@ -544,21 +549,34 @@ __device__ double film_filter(uint32_t f, double x, double y) {
 
 // Filtered film, gathered per pixel: every sample of this pass whose filter footprint covers
 // the pixel centre (the range HdrRgbFilm::addRgbSample splats to, HdrRgbFilm.cpp:82-122, with
-// half size 2) adds its radiance times the filter weight.  A sample's raster position is its
-// pixel plus the first two numbers of its path's stream, recomputed here.
+// half size 2) adds its radiance times the filter weight.  A sample's raster position comes
+// from its stratum (the pixel that keys its stream) and the first two numbers of its stream,
+// placed in the soft-edged sample window (window_raster); only strata whose part of the
+// window reaches within the footprint of this pixel are visited.
 __device__ void film_gather(const JobDev& job, const PassSeg& sg, const PathBuffers& pb, uint32_t x) {
     const int W = (int)job.width, H = (int)job.height;
     const int px = (int)(x % job.width), py = (int)(x / job.width);
     double2* f = reinterpret_cast<double2*>(pb.film + job.film_base + 8 * (size_t)x);
     double2 f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
-    for (int qy = max(py - 2, 0); qy <= min(py + 2, H - 1); ++qy)
-        for (int qx = max(px - 2, 0); qx <= min(px + 2, W - 1); ++qx) {
+    // samples that splat to px lie in [px - 1.5, px + 2.5]; stratum q covers
+    // [lo + q s, lo + (q + 1) s] with lo = 0.5 - 2 and s = (n + 3) / n
+    const double lo = 0.5 - kFilterHalf;
+    const double sx = ((double)W + 2.0 * kFilterHalf - 1.0) / (double)W;
+    const double sy = ((double)H + 2.0 * kFilterHalf - 1.0) / (double)H;
+    const int qx0 = max((int)floor(((double)px - 1.5 - lo) / sx) - 1, 0);
+    const int qx1 = min((int)floor(((double)px + 2.5 - lo) / sx) + 1, W - 1);
+    const int qy0 = max((int)floor(((double)py - 1.5 - lo) / sy) - 1, 0);
+    const int qy1 = min((int)floor(((double)py + 2.5 - lo) / sy) + 1, H - 1);
+    for (int qy = qy0; qy <= qy1; ++qy)
+        for (int qx = qx0; qx <= qx1; ++qx) {
             const uint32_t q = (uint32_t)(qy * W + qx);
             const uint32_t slot = tile_slot((uint32_t)qx, (uint32_t)qy, job.width, job.height);
             for (uint32_t s = 0; s < sg.samples; ++s) {
                 Stream st = stream_init(path_key(job.seed, q, sg.sample_begin + s, job.npix));
-                const double xp = (double)((float)qx + st.u()), yp = (double)((float)qy + st.u());
-                // pixels the sample splats to: ceil(p - 2.5) .. floor(p + 1.5)
+                const float ux = st.u(), uy = st.u();
+                const double xp = window_raster((uint32_t)qx, ux, job.width, kFilterHalf);
+                const double yp = window_raster((uint32_t)qy, uy, job.height, kFilterHalf);
+                // pixels the sample splats to: ceil(p - 2.5) .. floor(p + 1.5), within the image
                 if (px < (int)ceil(xp - 2.5) || px > (int)floor(xp + 1.5)) continue;
                 if (py < (int)ceil(yp - 2.5) || py > (int)floor(yp + 1.5)) continue;
                 const double w = film_filter(job.filter, (double)px - (xp - 0.5), (double)py - (yp - 0.5));

@@ -418,24 +418,34 @@ struct Loader {
                 r = std::sqrt(std::max(alpha, 0.0));
             } else if (map && map->value != "squared") warn();
             if (m->type == "abraded-opaque") {
+                // ConductiveInterfaceInfo::genFresnelEffect (ConductiveInterfaceInfo.cpp): with
+                // the complex inner index given, `fresnel exact` selects ExactConductorFresnel
+                // and `schlick` (the default) Schlick with the F0 of that index; without it,
+                // Schlick with f0 (an `exact` request then falls back, with a warning)
                 F3 f0 = f3(1.0f, 1.0f, 1.0f);
                 const Clause* fr = m->find("fresnel");
-                if (m->find("f0")) {
-                    f0 = color_of(m->find("f0"), f0);
-                } else if (m->find("ior-inner-n") && m->find("ior-inner-k")) {
-                    // normal-incidence reflectance of the conductor (Schlick's F0)
+                const bool exact = fr && fr->value == "exact";
+                if (fr && fr->value != "exact" && fr->value != "schlick") warn();
+                if (m->find("ior-inner-n") && m->find("ior-inner-k")) {
                     F3 n{}, k{};
                     vec3(*m, "ior-inner-n", &n);
                     vec3(*m, "ior-inner-k", &k);
                     const float eo = (float)real_or(*m, "ior-outer", 1.0);
-                    auto F0 = [&](float nn, float kk) {
-                        return ((nn - eo) * (nn - eo) + kk * kk) / ((nn + eo) * (nn + eo) + kk * kk);
-                    };
-                    f0 = f3(F0(n.x, k.x), F0(n.y, k.y), F0(n.z, k.z));
-                    warn();  // exact conductor Fresnel approximated by Schlick with this F0
+                    if (exact) {
+                        id = s.add_exact_conductor(n, k, (float)r, eo);
+                    } else {
+                        // SchlickApproxConductorFresnel(iorOuter, n, k): normal-incidence reflectance
+                        auto F0 = [&](float nn, float kk) {
+                            return ((nn - eo) * (nn - eo) + kk * kk) / ((nn + eo) * (nn + eo) + kk * kk);
+                        };
+                        id = s.add_material(PHG_MAT_CONDUCTOR, f3(F0(n.x, k.x), F0(n.y, k.y), F0(n.z, k.z)), (float)r, 1,
+                                            1);
+                    }
+                } else {
+                    if (m->find("f0")) f0 = color_of(m->find("f0"), f0);
+                    if (exact) warn();
+                    id = s.add_material(PHG_MAT_CONDUCTOR, f0, (float)r, 1, 1);
                 }
-                if (fr && fr->value != "schlick") warn();
-                id = s.add_material(PHG_MAT_CONDUCTOR, f0, (float)r, 1, 1);
             } else {
                 const float eo = (float)real_or(*m, "ior-outer", 1.0);
                 const float ei = (float)real_or(*m, "ior-inner", 1.5);

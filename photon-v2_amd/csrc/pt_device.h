@@ -250,6 +250,34 @@ PHG_HD RGB fresnel_schlick(const phg_material& m, float cosI) {
     const float p = (float)(x2 * x2 * x);
     return rgb3((1.0f - m.rgb[0]) * p + m.rgb[0], (1.0f - m.rgb[1]) * p + m.rgb[1], (1.0f - m.rgb[2]) * p + m.rgb[2]);
 }
+// ExactConductorFresnel.cpp:66-101 per channel, with setIors' precomputation (:103-116):
+// en2 = (n / eta_outer)^2, ek2 = (k / eta_outer)^2 (squares as products).
+PHG_HD float fresnel_conductor_exact1(float etaOuter, float n, float k, float cosThetaI) {
+    const float en = n / etaOuter, ek = k / etaOuter;
+    const float en2 = en * en, ek2 = ek * ek;
+    const float e2_sub_k2 = en2 - ek2, four_e2k2 = en2 * ek2 * 4.0f;
+    const float cosI = fabsf(cosThetaI);
+    const float cosI2 = cosI * cosI;
+    const float sinI2 = 1.0f - cosI * cosI;
+    const float t0 = e2_sub_k2 - sinI2;
+    const float a2plusb2 = sqrtf(t0 * t0 + four_e2k2);
+    const float a = sqrtf((a2plusb2 + t0) * 0.5f);
+    const float t1 = a2plusb2 + cosI2;
+    const float t2 = a * (2.0f * cosI);
+    const float t3 = a2plusb2 * cosI2 + sinI2 * sinI2;
+    const float t4 = t2 * sinI2;
+    const float Rs = (t1 - t2) / (t1 + t2);
+    const float Rp = Rs * ((t3 - t4) / (t3 + t4));
+    return (Rs + Rp) * 0.5f;
+}
+// the conductor's Fresnel term as its interface info selects it (ConductiveInterfaceInfo.cpp)
+PHG_HD RGB fresnel_conductor(const phg_material& m, float cosI) {
+    if (m.fresnel == PHG_FRESNEL_EXACT)
+        return rgb3(fresnel_conductor_exact1(m.ior_outer, m.n[0], m.k[0], cosI),
+                    fresnel_conductor_exact1(m.ior_outer, m.n[1], m.k[1], cosI),
+                    fresnel_conductor_exact1(m.ior_outer, m.n[2], m.k[2], cosI));
+    return fresnel_schlick(m, cosI);
+}
 PHG_HD float fresnel_dielectric(float etaOuter, float etaInner, float cosThetaI) {
     float etaI = etaOuter, etaT = etaInner, cosI = cosThetaI;
     if (cosThetaI < 0.0f) { const float t = etaI; etaI = etaT; etaT = t; cosI = fabsf(cosI); }
@@ -305,7 +333,7 @@ PHG_HD BsdfEval bsdf_eval(const phg_material& m, const SurfaceHit& X, V3 L, V3 V
         if (!half_vec_same_hemi(L, V, N, &H)) return e;
         const float hol = dot(H, L);
         const float a = mat_alpha(m);
-        const RGB F = fresnel_schlick(m, hol);
+        const RGB F = fresnel_conductor(m, hol);
         const float D = ggx_D(N, H, a);
         const float G = ggx_G(N, H, L, V, a);
         e.f = muls(F, D * G / (4.0f * fabsf(nov * nol)));
@@ -415,7 +443,7 @@ PHG_HD BsdfSample bsdf_sample(const phg_material& m, const SurfaceHit& X, V3 V, 
         const float nov = dot(N, V);
         const float hol = dot(H, L);
         const float dotTerms = fabsf(hol / nov);
-        const RGB F = fresnel_schlick(m, hol);
+        const RGB F = fresnel_conductor(m, hol);
         const float G = ggx_G(N, H, L, V, a);
         const float D = ggx_D(N, H, a);
         const float pdf = ggx_vndf_pdf(N, H, V, a);
@@ -553,5 +581,18 @@ PHG_HD uint32_t cdf_sample(const float* cdf, uint32_t n, float u) {
     return lo - 1;
 }
 PHG_HD float mis_power(float p0, float p1) { return (p0 * p0) / (p0 * p0 + p1 * p1); }
+
+// ---- film sample window (TSamplingFilm::updateSampleDimensions, TSamplingFilm.ipp:103-134) ---
+// A film with a reconstruction filter of half size `half` (pixels) draws its samples over the
+// image's pixel centres expanded by `half` (soft edge, softness 1), a batch's n strata per
+// axis spread evenly over that window (ReceiverSamplingWork.cpp:76-101): stratum x with offset
+// u lands at lo + ((x + u) / n) * extent, in FP64.  A unit-size (box) filter's window is the
+// image itself and its samples sit at x + u in FP32.
+constexpr double kFilterHalf = 2.0;   // Gaussian, Mitchell-Netravali and Blackman-Harris: 4 x 4 pixels
+PHG_HD double window_raster(uint32_t x, float u, uint32_t n, double half) {
+    const double lo = 0.5 - half;
+    const double ext = ((double)n - 0.5 + half) - lo;
+    return lo + (((double)x + (double)u) / (double)n) * ext;
+}
 
 }  // namespace phg

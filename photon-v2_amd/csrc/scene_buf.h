@@ -79,6 +79,18 @@ struct SceneBuf {
         mats.push_back(m);
         return (uint32_t)mats.size() - 1;
     }
+    // conductor with the exact Fresnel of a complex inner index n + ik (outer index ior_out)
+    uint32_t add_exact_conductor(F3 n, F3 k, float rough, float ior_out) {
+        phg_material m{};
+        m.type = PHG_MAT_CONDUCTOR;
+        m.rgb[0] = m.rgb[1] = m.rgb[2] = 1.0f;
+        m.roughness = rough; m.ior_outer = ior_out; m.ior_inner = 1.0f;
+        m.fresnel = PHG_FRESNEL_EXACT;
+        m.n[0] = n.x; m.n[1] = n.y; m.n[2] = n.z;
+        m.k[0] = k.x; m.k[1] = k.y; m.k[2] = k.z;
+        mats.push_back(m);
+        return (uint32_t)mats.size() - 1;
+    }
     void tri(F3 a, F3 b, F3 c, uint32_t mat, int32_t emitter = -1) {
         const float v[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
         verts.insert(verts.end(), v, v + 9);

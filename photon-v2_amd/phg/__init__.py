@@ -25,6 +25,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "HIP", 3: "OUT_OF_MEMORY", 4: "NO_
                 6: "UNSUPPORTED"}
 
 MAT_ABSORBER, MAT_LAMBERT, MAT_CONDUCTOR, MAT_DIELECTRIC = 0, 1, 2, 3
+FRESNEL_SCHLICK, FRESNEL_EXACT = 0, 1
 GEN_CORNELL, GEN_PROCEDURAL, GEN_OCCLUDED_ROOM, GEN_TERRAIN = 1, 2, 3, 4
 QUERY_CLOSEST, QUERY_ANY = 0, 1
 PTR_HOST, PTR_DEVICE = 0, 1
@@ -40,7 +41,8 @@ FLT_MAX = float(np.finfo(np.float32).max)
 
 class phg_material(C.Structure):
     _fields_ = [("type", C.c_uint32), ("rgb", C.c_float * 3), ("roughness", C.c_float),
-                ("ior_outer", C.c_float), ("ior_inner", C.c_float), ("pad", C.c_float)]
+                ("ior_outer", C.c_float), ("ior_inner", C.c_float), ("fresnel", C.c_uint32),
+                ("n", C.c_float * 3), ("k", C.c_float * 3), ("pad", C.c_float * 2)]
 
 
 class phg_emitter(C.Structure):
@@ -247,7 +249,8 @@ class Scene:
     camera: dict
 
     MAT_DTYPE = np.dtype([("type", "<u4"), ("rgb", "<f4", 3), ("roughness", "<f4"), ("ior_outer", "<f4"),
-                          ("ior_inner", "<f4"), ("pad", "<f4")])
+                          ("ior_inner", "<f4"), ("fresnel", "<u4"), ("n", "<f4", 3), ("k", "<f4", 3),
+                          ("pad", "<f4", 2)])
     EMIT_DTYPE = np.dtype([("radiance", "<f4", 3), ("tri_begin", "<u4"), ("tri_count", "<u4"), ("pad", "<u4", 3)])
 
     @property
@@ -269,7 +272,7 @@ class Scene:
             verts = np.ctypeslib.as_array(d.verts, shape=(T * 9,)).reshape(T, 9).copy()
             tri_mat = np.ctypeslib.as_array(d.tri_material, shape=(T,)).copy()
             tri_emit = np.ctypeslib.as_array(d.tri_emitter, shape=(T,)).copy()
-            mats = np.frombuffer(C.string_at(d.materials, M * 32), dtype=Scene.MAT_DTYPE).copy()
+            mats = np.frombuffer(C.string_at(d.materials, M * Scene.MAT_DTYPE.itemsize), dtype=Scene.MAT_DTYPE).copy()
             emits = np.frombuffer(C.string_at(d.emitters, E * 32), dtype=Scene.EMIT_DTYPE).copy() if E else \
                 np.zeros(0, Scene.EMIT_DTYPE)
             etris = np.ctypeslib.as_array(d.emitter_tris, shape=(K,)).copy() if K else np.zeros(0, np.uint32)
@@ -289,7 +292,7 @@ class Scene:
             verts = np.ctypeslib.as_array(d.verts, shape=(T * 9,)).reshape(T, 9).copy()
             tri_mat = np.ctypeslib.as_array(d.tri_material, shape=(T,)).copy()
             tri_emit = np.ctypeslib.as_array(d.tri_emitter, shape=(T,)).copy()
-            mats = np.frombuffer(C.string_at(d.materials, M * 32), dtype=Scene.MAT_DTYPE).copy()
+            mats = np.frombuffer(C.string_at(d.materials, M * Scene.MAT_DTYPE.itemsize), dtype=Scene.MAT_DTYPE).copy()
             emits = np.frombuffer(C.string_at(d.emitters, E * 32), dtype=Scene.EMIT_DTYPE).copy() if E else \
                 np.zeros(0, Scene.EMIT_DTYPE)
             etris = np.ctypeslib.as_array(d.emitter_tris, shape=(K,)).copy() if K else np.zeros(0, np.uint32)

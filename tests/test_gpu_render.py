@@ -281,3 +281,23 @@ def test_glass_furnace(ctx, shape, camera, expect, bar):
     ident = np.all(np.isclose(gpu[..., :3], ref[..., :3], rtol=1e-5, atol=1e-7), axis=-1).mean()
     print(f"[render] case=glass_furnace_{shape}_{camera} identical_pixels={ident:.4f}", flush=True)
     assert ident > 0.8
+
+
+def test_p2_exact_conductor_matches_oracle(ctx):
+    """The .p2 box with its sphere made an exact-Fresnel conductor (gold's complex index,
+    `fresnel exact`, ExactConductorFresnel.cpp:66-101) renders on the GPU as on the oracle."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes", "glossy_box.p2")
+    text = open(path).read()
+    old = 'material(abraded-opaque) @gold = [enum microsurface ggx] [real roughness 0.35] [spectrum f0 "1.0 0.765557 0.336057"];'
+    assert old in text
+    text = text.replace(old, 'material(abraded-opaque) @gold = [enum microsurface ggx] [real roughness 0.35] '
+                             '[enum fresnel exact] [spectrum ior-inner-n "0.143 0.374 1.442"] '
+                             '[spectrum ior-inner-k "3.983 2.385 1.603"];')
+    sc, st = phg.Scene.parse_p2(text)
+    assert np.count_nonzero(sc.materials["fresnel"] == phg.FRESNEL_EXACT) == 1 and st["warnings"] == 0
+    W, H, spp, B = st["width"], st["height"], 32, st["max_bounces"]
+    print(f"[render] case=p2_exact_conductor tris={sc.num_tris} {W}x{H} spp={spp}", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, spp, B, 19)
+    ref, _ = orc.render(sc, W, H, spp, max_bounces=B, seed=19)
+    check_parity(gpu, ref, "p2_exact_conductor", 0.5)

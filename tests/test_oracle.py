@@ -227,6 +227,69 @@ def test_fresnel_schlick_endpoints():
     assert np.allclose(out, exp, atol=1e-6)
 
 
+def _exact_conductor(eo, n, k, cos):
+    L = orc.lib()
+    out = np.zeros(3, np.float32)
+    n32, k32 = fp(n), fp(k)          # kept alive across the call
+    L.orc_fresnel_conductor_exact(C.c_float(eo), n32.ctypes.data, k32.ctypes.data, C.c_float(cos), out.ctypes.data)
+    return out
+
+
+def _complex_fresnel(eo, n, k, cos):
+    """Textbook reflectance of unpolarized light at a dielectric (index eo) / conductor
+    (index n + ik) interface, in complex arithmetic (independent of the formula under test)."""
+    eta = (np.asarray(n, np.float64) + 1j * np.asarray(k, np.float64)) / eo
+    c = abs(cos)
+    s2 = 1.0 - c * c
+    root = np.sqrt(eta * eta - s2)
+    rs = (c - root) / (c + root)
+    rp = (eta * eta * c - root) / (eta * eta * c + root)
+    return 0.5 * (np.abs(rs) ** 2 + np.abs(rp) ** 2)
+
+
+def test_fresnel_exact_conductor_closed_forms():
+    """ExactConductorFresnel (ExactConductorFresnel.cpp:66-101): normal incidence
+    ((n - eo)^2 + k^2) / ((n + eo)^2 + k^2); 1 at grazing; k = 0 reduces to the exact
+    dielectric Fresnel; every angle against the complex-arithmetic Fresnel equations; the sign
+    of cos does not matter."""
+    L = orc.lib()
+    gold_n, gold_k = [0.143, 0.374, 1.442], [3.983, 2.385, 1.603]
+    for eo in (1.0, 1.33):
+        f = _exact_conductor(eo, gold_n, gold_k, 1.0)
+        n, k = np.array(gold_n), np.array(gold_k)
+        assert np.allclose(f, ((n - eo) ** 2 + k ** 2) / ((n + eo) ** 2 + k ** 2), rtol=2e-6)
+        assert np.allclose(_exact_conductor(eo, gold_n, gold_k, 0.0), 1.0, atol=1e-6)
+        for cos in np.linspace(0.02, 1.0, 50):
+            got = _exact_conductor(eo, gold_n, gold_k, cos)
+            assert np.allclose(got, _complex_fresnel(eo, gold_n, gold_k, cos), rtol=2e-5, atol=2e-6), cos
+            assert np.array_equal(got, _exact_conductor(eo, gold_n, gold_k, -cos))
+    for cos in np.linspace(0.05, 1.0, 20):
+        d = L.orc_fresnel_dielectric(C.c_float(1.0), C.c_float(1.5), C.c_float(cos))
+        c = _exact_conductor(1.0, [1.5] * 3, [0.0] * 3, cos)
+        assert np.allclose(c, d, rtol=1e-5, atol=1e-7), cos
+
+
+def test_exact_conductor_sample_weight_matches_integral():
+    """A rough conductor with the exact Fresnel: the sampler's mean weight equals the
+    quadrature of its f cos (same check as the Schlick conductors)."""
+    m = orc.material(orc.ORC_MAT_CONDUCTOR, roughness=0.5, n=[0.2, 0.9, 1.4], k=[3.9, 2.4, 1.6])
+    N = fp([0, 1, 0])
+    V = fp([np.sin(0.7), np.cos(0.7), 0])
+    rng = np.random.default_rng(3)
+    u = rng.random((200_000, 3)).astype(np.float32)
+    Ls, w, _, ok = orc.bsdf_sample_batch(m, N, V, u)
+    mean_w = np.where(ok[:, None], w, 0).mean(0)
+    # quadrature of f cos over the hemisphere
+    th = (np.arange(400) + 0.5) / 400 * (np.pi / 2)
+    ph = (np.arange(800) + 0.5) / 800 * (2 * np.pi)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    Lq = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
+    fv, _, okq = orc.bsdf_eval_batch(m, N, V, Lq)
+    dw = (np.pi / 2 / 400) * (2 * np.pi / 800) * np.sin(T).reshape(-1)
+    integral = (np.where(okq[:, None], fv, 0) * Lq[:, 1:2] * dw[:, None]).sum(0)
+    assert np.allclose(mean_w, integral, rtol=0.02), (mean_w, integral)
+
+
 def test_half_vector_known_answers():
     # surface_optics_test.cpp:9-36
     L = orc.lib()
@@ -406,6 +469,39 @@ def test_film_filter_shapes_closed_forms():
     g = w(1)
     e = np.exp(-16.0)
     assert np.allclose(g[[1, 3]] / g[2], (np.exp(-2.0) - e) / (1 - e))
+
+
+@pytest.mark.parametrize("fid", [1, 2, 3])
+def test_filtered_film_soft_edge_weight_sums(fid):
+    """fid: PHG_FILTER_GAUSSIAN, _MN, _BH.  Soft-edged sampling window (TSamplingFilm.ipp:103-134): the W x H samples of a batch
+    are spread over the pixel centres expanded by the filter half size 2, i.e. a window of
+    (W + 3) x (H + 3) pixels, so the whole 4 x 4 footprint of every pixel, border and corner
+    pixels included, receives samples at the same density W H / ((W + 3)(H + 3)).  The expected
+    weight sum per pixel and sample is then that density times the filter's integral: the
+    closed forms (4 * 0.35875)^2 for Blackman-Harris and 1 for Mitchell-Netravali, a quadrature
+    for the submerged Gaussian.  Without the soft edge a corner pixel would get about a third
+    of an interior one's weight."""
+    import phg
+    W = H = 8
+    spp = 16384
+    sc = scene_from_tris(np.array([[0, 0, -50, 1, 0, -50, 0, 1, -50]], np.float32))   # behind the camera
+    film = orc.render_filtered(sc, W, H, spp, fid, max_bounces=0, seed=9)
+    wsum = film[..., 3] / spp
+    density = W * H / ((W + 3) * (H + 3))
+    if fid == 1:
+        x = (np.arange(4000) + 0.5) / 4000 * 4 - 2
+        X, Y = np.meshgrid(x, x)
+        g = np.maximum(np.exp(-(X * X + Y * Y) / 0.5) - np.exp(-16.0), 0).sum() * (4 / 4000) ** 2
+        integral = g
+    elif fid == 3:
+        integral = (4 * 0.35875) ** 2
+    else:
+        integral = 1.0
+    expect = density * integral
+    dev = np.abs(wsum / expect - 1)
+    print(f"[film] filter {fid}: expected {expect:.5f}, corner {wsum[0, 0]:.5f}, centre {wsum[4, 4]:.5f}, "
+          f"max deviation {dev.max():.4f}")
+    assert dev.max() < 0.02
 
 
 # ---------------------------------------------------------------- camera (PinholeCamera.cpp)

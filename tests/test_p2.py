@@ -109,6 +109,28 @@ def test_roughness_mappings_and_sRGB():
     assert np.allclose(srgb["rgb"], ((0.5 + 0.055) / 1.055) ** 2.4)
 
 
+def test_conductor_fresnel_selection():
+    """ConductiveInterfaceInfo::genFresnelEffect: with ior-inner-n / -k, `fresnel exact` is the
+    exact conductor Fresnel and `schlick` (the default) Schlick with that index's normal
+    reflectance; f0 alone is Schlick, and an exact request without the index falls back to it
+    with a warning."""
+    def mat(m):
+        sc, st = phg.Scene.parse_p2(one_model("geometry(rectangle) @g = [real width 1] [real height 1];", "", m))
+        return sc.materials[sc.tri_material[0]], st["warnings"]
+    nk = '[spectrum ior-inner-n "0.143 0.374 1.442"] [spectrum ior-inner-k "3.983 2.385 1.603"] [real ior-outer 1.2]'
+    ex, w = mat(f'material(abraded-opaque) @m = [real roughness 0.3] [enum fresnel exact] {nk};')
+    assert w == 0 and ex["type"] == phg.MAT_CONDUCTOR and ex["fresnel"] == phg.FRESNEL_EXACT
+    assert np.allclose(ex["n"], [0.143, 0.374, 1.442]) and np.allclose(ex["k"], [3.983, 2.385, 1.603])
+    assert np.isclose(ex["ior_outer"], 1.2)
+    for clause in ("[enum fresnel schlick] ", ""):
+        sl, w = mat(f'material(abraded-opaque) @m = [real roughness 0.3] {clause}{nk};')
+        n, k = np.array([0.143, 0.374, 1.442]), np.array([3.983, 2.385, 1.603])
+        assert w == 0 and sl["fresnel"] == phg.FRESNEL_SCHLICK
+        assert np.allclose(sl["rgb"], ((n - 1.2) ** 2 + k ** 2) / ((n + 1.2) ** 2 + k ** 2), rtol=1e-5)
+    fb, w = mat('material(abraded-opaque) @m = [enum fresnel exact] [spectrum f0 "0.9 0.8 0.7"];')
+    assert w == 1 and fb["fresnel"] == phg.FRESNEL_SCHLICK and np.allclose(fb["rgb"], [0.9, 0.8, 0.7])
+
+
 def test_syntax_errors_and_skipped_commands():
     with pytest.raises(phg.PhgError):
         phg.Scene.parse_p2("geometry(rectangle) @g = [real width 1")          # unterminated clause
