@@ -162,8 +162,11 @@ enum {
     PHG_RENDER_COUNT_TRAVERSAL = 1,  /* count node visits / triangle tests (slower) */
     PHG_RENDER_GUIDED = 2,           /* sample directions from the guiding network too (one-sample MIS with the BSDF) */
     PHG_RENDER_TRAIN = 4,            /* train the guiding network online on this render's radiance samples */
-    PHG_RENDER_GRAPH = 8             /* unguided, max_bounces < 62: each pass replays a captured HIP graph of the
-                                        whole bounce loop (no host round trip per bounce); same results */
+    PHG_RENDER_GRAPH = 8             /* captured HIP graphs, same results as the eager loop: an unguided render of
+                                        max_bounces < 62 replays one graph of the whole pass; any other (unbounded,
+                                        guided, training) replays graphs of 8 bounces -- extend, guide inference,
+                                        shade, shadow -- until the path queue is empty (one host round trip per 8
+                                        bounces instead of per bounce) */
 };
 
 typedef struct phg_stats {
@@ -279,7 +282,7 @@ enum { PHG_JOB_CAMERA = 1 };
 enum { PHG_BATCH_ORDER_GIVEN = 0, PHG_BATCH_ORDER_SHORTEST_FIRST = 1 };
 enum {
     PHG_BATCH_COUNT_TRAVERSAL = 1,
-    PHG_BATCH_GRAPH = 2,       /* as PHG_RENDER_GRAPH */
+    PHG_BATCH_GRAPH = 2,       /* as PHG_RENDER_GRAPH (guided and training batches too) */
     PHG_BATCH_GUIDED = 4,      /* as PHG_RENDER_GUIDED; all jobs must render the same scene (one network) */
     PHG_BATCH_TRAIN = 8        /* as PHG_RENDER_TRAIN, on the records of every job in the pass */
 };

@@ -51,6 +51,11 @@ struct SceneInfo {
 // use passes of at most kTrainPassPaths unless told otherwise: the network is updated after
 // every pass, and each pass keeps 768 B of training records per path (12 GB at 16M).
 constexpr uint32_t kDefaultPassPaths = 32u << 20;
+// Bounces per captured chunk of an unbounded or guided render: even (the ping-pong queue of
+// bounce m0 + k is that of k) and a divisor of the counter ring (the slot of m0 + k follows
+// from the device bounce base).
+constexpr uint32_t kChunkBounces = 8;
+static_assert(kChunkBounces % 2 == 0 && kCountSlots % kChunkBounces == 0, "bounce chunk must be even and divide the counter ring");
 constexpr uint32_t kTrainPassPaths = 16u << 20;
 
 struct Ctx {
@@ -102,6 +107,7 @@ struct Ctx {
     DevBuf<float> g_train;       // collected training records
     DevBuf<float> g_q;           // staging for the guide entry points
     DevBuf<uint32_t> g_counters;
+    DevBuf<uint32_t> g_blockcnt;     // per-block record counts of the collection scan
     DevBuf<uint16_t> g_ws_x, g_ws_dz0, g_ws_dz1;   // training-step scratch (GuideWorkspace)
     DevBuf<float> g_ws_part;
     bool g_init = false, g_ready = false;
@@ -309,7 +315,7 @@ phg_status phg_destroy(void* ctx) {
         c->q_nvis.free_(); c->q_ntri.free_();
         c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
         c->g_ws_x.free_(); c->g_ws_dz0.free_(); c->g_ws_dz1.free_(); c->g_ws_part.free_();
-        c->g_wn.free_(); c->g_wp.free_(); c->g_bp.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_();
+        c->g_wn.free_(); c->g_wp.free_(); c->g_bp.free_(); c->g_lobes.free_(); c->g_rec.free_(); c->g_train.free_(); c->g_q.free_(); c->g_counters.free_(); c->g_blockcnt.free_();
         for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -781,7 +787,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     PHG_HIP(c, c->ext0.ensure(cap)); PHG_HIP(c, c->ext1.ensure(cap)); PHG_HIP(c, c->shadow.ensure(cap));
     PHG_HIP(c, c->ext_pid0.ensure(cap)); PHG_HIP(c, c->ext_pid1.ensure(cap)); PHG_HIP(c, c->shadow_pid.ensure(cap));
     PHG_HIP(c, c->path_seg.ensure(cap));
-    PHG_HIP(c, c->counts.ensure(2 * kCountSlots));
+    PHG_HIP(c, c->counts.ensure(2 * kCountSlots + 1));   // + the bounce base of captured chunks
     PHG_HIP(c, c->tcount.ensure(8));
     PHG_HIP(c, c->errflags.ensure(1));
     PHG_HIP(c, c->jobs.ensure(jobs.size()));
@@ -805,6 +811,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     pb.hits = c->hits.p; pb.shadow = c->shadow.p; pb.shadow_pid = c->shadow_pid.p; pb.shadow_c = c->shadow_c.p;
     pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = film;
     pb.path_seg = c->path_seg.p;
+    pb.bounce_base = c->counts.p + 2 * kCountSlots;
     {
         const phg_status st = ensure_spill(c);
         if (st != PHG_OK) return st;
@@ -837,7 +844,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     pb.lobes = c->g_lobes.p;
     pb.rec = c->g_rec.p;
     double t_ginfer = 0, t_gtrain = 0;
-    uint64_t g_queries = 0, g_records = 0;
+    uint64_t g_records = 0;
     const uint32_t grid_trace = trace_grid(c);     // persistent-style grid-stride launches
     const uint32_t grid_shade = c->cu_count * c->shade_blocks_per_cu;
     double t_trace = 0, t_shadow = 0, t_shade = 0;
@@ -850,6 +857,52 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     // after the pass parameters are copied; timed as a whole (no per-kernel times).
     hipGraphExec_t gexec = nullptr;
     const bool use_graph = m.graph && !guided && !train && max_b + 2 < kCountSlots;
+    // Otherwise (unbounded depth, guided or training renders) a graph mode replays captured
+    // chunks of kChunkBounces bounces -- extend, inference where guided, shade, shadow, then
+    // the bounce base advanced -- until the extension queue is empty: one host round trip per
+    // chunk instead of per bounce.  Chunks differ only in which bounces sample the lobes, so
+    // a graph is captured per pattern (bit k: inference + guided shading at bounce m0 + k).
+    const bool use_chunks = m.graph && !use_graph;
+    struct ChunkGraphs {
+        std::vector<std::pair<uint32_t, hipGraphExec_t>> g;
+        ~ChunkGraphs() { for (auto& e : g) (void)hipGraphExecDestroy(e.second); }
+    } chunk_graphs;
+    auto chunk_graph = [&](uint32_t infer_mask, uint32_t guided_mask, hipGraphExec_t* out) -> phg_status {
+        const uint32_t key = infer_mask | (guided_mask << 16);
+        for (auto& e : chunk_graphs.g)
+            if (e.first == key) { *out = e.second; return PHG_OK; }
+        hipGraph_t graph = nullptr;
+        PHG_HIP(c, hipStreamSynchronize(c->stream));   // nothing of the pass in flight while capturing
+        PHG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = hipSuccess;
+        for (uint32_t k = 0; k < kChunkBounces && ce == hipSuccess; ++k) {
+            ce = launch_extend(sc, pb, k, grid_trace, c->trace_block, count_trav, c->stream);
+            if (ce == hipSuccess && (infer_mask >> k & 1u))
+                ce = guide_infer_hits(guide_net(c), pb.hits, pb.ext[k & 1u], sc.tris, c->g_scene, c->counts.p,
+                                      pb.bounce_base, k, c->g_lobes.p, c->tcount.p + 6, c->cu_count * 2, c->stream);
+            if (ce == hipSuccess) ce = launch_shade(sc, ppd, pb, k, grid_shade, (guided_mask >> k & 1u) != 0, c->stream);
+            if (ce == hipSuccess) ce = launch_shadow(sc, ppd, pb, k, grid_trace, c->trace_block, count_trav, c->stream);
+        }
+        if (ce == hipSuccess) ce = launch_advance_bounce(pb.bounce_base, kChunkBounces, c->stream);
+        const hipError_t ee = hipStreamEndCapture(c->stream, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            c->err = std::string("bounce-chunk capture: ") + hipGetErrorString(ce != hipSuccess ? ce : ee);
+            return PHG_ERR_HIP;
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            (void)hipGetLastError();
+            c->err = std::string("bounce-chunk graph: ") + hipGetErrorString(ie);
+            return PHG_ERR_HIP;
+        }
+        chunk_graphs.g.emplace_back(key, ex);
+        *out = ex;
+        return PHG_OK;
+    };
     if (use_graph) {
         hipGraph_t graph = nullptr;
         PHG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -913,11 +966,40 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             }
             continue;
         }
+        float ms = 0;
+        if (use_chunks) {
+            PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+            PHG_HIP(c, launch_generate(sc, ppd, pb, grid_shade, c->stream));
+            for (uint32_t m0 = 0;; m0 += kChunkBounces) {
+                uint32_t infer_mask = 0, guided_mask = 0;
+                for (uint32_t k = 0; k < kChunkBounces; ++k) {
+                    const uint32_t b = m0 + k;
+                    const bool in_depth = m.guide_depth == 0 || b < m.guide_depth;
+                    if (guided && pp.guide_prob > 0.0f && b < pl.max_bounces && in_depth) infer_mask |= 1u << k;
+                    if ((train || guided) && in_depth) guided_mask |= 1u << k;
+                }
+                hipGraphExec_t ex = nullptr;
+                const phg_status gs = chunk_graph(infer_mask, guided_mask, &ex);
+                if (gs != PHG_OK) return gs;
+                PHG_HIP(c, hipGraphLaunch(ex, c->stream));
+                uint32_t next = 0;
+                PHG_HIP(c, hipMemcpyAsync(&next, c->counts.p + (m0 + kChunkBounces) % kCountSlots, sizeof(uint32_t),
+                                          hipMemcpyDeviceToHost, c->stream));
+                PHG_HIP(c, hipStreamSynchronize(c->stream));
+                n_trace += kChunkBounces; n_shadow += kChunkBounces;
+                if (next == 0 || m0 + kChunkBounces > pl.max_bounces) break;
+            }
+            PHG_HIP(c, launch_film(sc, ppd, pb, grid_shade, c->stream));
+            PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+            PHG_HIP(c, hipStreamSynchronize(c->stream));
+            PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+            t_graph += ms;
+        } else {
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
         PHG_HIP(c, launch_generate(sc, ppd, pb, grid_shade, c->stream));
         PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
         PHG_HIP(c, hipStreamSynchronize(c->stream));
-        float ms = 0; PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
+        PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
         for (uint32_t b = 0;; ++b) {
             const bool use_guide = guided && pp.guide_prob > 0.0f && b < pl.max_bounces &&
                                    (m.guide_depth == 0 || b < m.guide_depth);
@@ -925,8 +1007,9 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             PHG_HIP(c, launch_extend(sc, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
             if (use_guide) {
-                PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], sc.tris, c->g_scene,
-                                            c->counts.p + b % kCountSlots, c->g_lobes.p, c->cu_count * 2, c->stream));
+                PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], sc.tris, c->g_scene, c->counts.p,
+                                            pb.bounce_base, b, c->g_lobes.p, c->tcount.p + 6, c->cu_count * 2,
+                                            c->stream));
             }
             PHG_HIP(c, hipEventRecord(c->ev[4], c->stream));
             // the guided shading variant where lobes are sampled or training records written
@@ -937,9 +1020,8 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             PHG_HIP(c, hipEventRecord(c->ev[2], c->stream));
             PHG_HIP(c, launch_shadow(sc, ppd, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[3], c->stream));
-            uint32_t next[2] = {0, 0};
-            PHG_HIP(c, hipMemcpyAsync(&next[0], c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-            PHG_HIP(c, hipMemcpyAsync(&next[1], c->counts.p + b % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            uint32_t next = 0;
+            PHG_HIP(c, hipMemcpyAsync(&next, c->counts.p + (b + 1) % kCountSlots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));
             float a = 0, bb = 0, cc = 0, gi = 0;
             PHG_HIP(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -947,15 +1029,15 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             PHG_HIP(c, hipEventElapsedTime(&bb, c->ev[4], c->ev[2]));
             PHG_HIP(c, hipEventElapsedTime(&cc, c->ev[2], c->ev[3]));
             t_trace += a; t_shade += bb; t_shadow += cc; t_ginfer += gi;
-            if (use_guide) g_queries += next[1];
             ++n_trace; ++n_shadow;
-            if (next[0] == 0 || b >= pl.max_bounces) break;
+            if (next == 0 || b >= pl.max_bounces) break;
         }
         PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
         PHG_HIP(c, launch_film(sc, ppd, pb, grid_shade, c->stream));
         PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
+        }
         if (job_done_ms) {
             const double now = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
             for (const PassSeg& sg : pl.segs) (*job_done_ms)[sg.job] = now;  // the last pass of a job wins
@@ -963,9 +1045,9 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         if (train) {
             // online training on this pass's radiance samples
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
-            PHG_HIP(c, hipMemsetAsync(c->g_counters.p, 0, 4 * sizeof(uint32_t), c->stream));
-            PHG_HIP(c, guide_collect(pb, pp, c->g_scene, c->g_train.p, train_cap, c->g_counters.p, c->g_keep,
-                                     0.0f, ++c->g_pass_salt, c->stream));
+            PHG_HIP(c, c->g_blockcnt.ensure(2 * (size_t)guide_collect_blocks(pp.paths)));
+            PHG_HIP(c, guide_collect(pb, pp, c->g_scene, c->g_train.p, train_cap, c->g_counters.p, c->g_blockcnt.p,
+                                     c->g_keep, 0.0f, ++c->g_pass_salt, c->stream));
             uint32_t cnt[2] = {0, 0};
             PHG_HIP(c, hipMemcpyAsync(cnt, c->g_counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));
@@ -1003,7 +1085,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     c->stats.shadow_launches = n_shadow;
     c->stats.guide_infer_ms = t_ginfer;
     c->stats.guide_train_ms = t_gtrain;
-    c->stats.guide_queries = g_queries;
+    c->stats.guide_queries = tc[6];
     c->stats.guide_records = g_records;
     c->stats.guide_steps = c->g_steps;
     c->stats.trace_ms = t_trace;

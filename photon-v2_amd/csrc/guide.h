@@ -37,8 +37,11 @@ constexpr uint32_t kGuideNetLayers = 5;     // dense layers of the network (scra
 hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st);
 hipError_t guide_infer_features(const GuideNetDev& net, const float* features, uint32_t n, float* out, int raw,
                                 uint32_t grid, hipStream_t st);
+// Lobes of the hits queued at bounce bounce_base[0] + bk (their count is that bounce's
+// extension-queue counter in `counts`); adds the number of queries to *queries.
 hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const phg_ray* rays, const TriRec* tris,
-                            const GuideScene& gs, const uint32_t* count_ptr, float* out, uint32_t grid, hipStream_t st);
+                            const GuideScene& gs, const uint32_t* counts, const uint32_t* bounce_base, uint32_t bk,
+                            float* out, unsigned long long* queries, uint32_t grid, hipStream_t st);
 // Scratch of a training step (device memory): layer inputs X (5 x cap x 64 bf16), two dZ
 // buffers (cap x 64 bf16 each), per-wave dW / db partials (5 x (max_parts + 4) x 4160 floats)
 // and their first-level sums (5 x slices x 4160 floats).
@@ -58,9 +61,13 @@ hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nr
                            const GuideWorkspace& ws, hipStream_t st);
 hipError_t guide_adam(const GuideNetDev& net, float* grads, float* m, float* v, float lr, int step, hipStream_t st);
 hipError_t guide_fill(float* p, uint32_t n, float v, hipStream_t st);
-// training records of a finished pass: incident radiance along each recorded sampled direction
+// Training records of a finished pass (incident radiance along each recorded sampled
+// direction), in path order: counters[0] = records kept (at most `capacity` are written),
+// counters[1] = records valid before the keep test; block_counts holds 2 x
+// guide_collect_blocks(paths) entries of scratch.
+uint32_t guide_collect_blocks(uint32_t paths);
 hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
-                         uint32_t capacity, uint32_t* counters, float keep_prob, float sel, uint32_t salt,
-                         hipStream_t st);
+                         uint32_t capacity, uint32_t* counters, uint32_t* block_counts, float keep_prob, float sel,
+                         uint32_t salt, hipStream_t st);
 
 }  // namespace phg

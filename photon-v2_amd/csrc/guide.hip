@@ -144,11 +144,15 @@ __device__ __forceinline__ void query_features(const InferSource& src, uint32_t 
 // reduces across the 4 lanes of a query.
 constexpr int kInferWaves = 8;
 __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev net, InferSource src,
-                                                                  const uint32_t* count_ptr, uint32_t count,
-                                                                  float* __restrict__ out, int raw) {
+                                                                  const uint32_t* counts, const uint32_t* bounce_base,
+                                                                  uint32_t bk, uint32_t count, float* __restrict__ out,
+                                                                  int raw, unsigned long long* queries) {
     __shared__ __attribute__((aligned(16))) __bf16 s_w[kGuideLayers * 64 * 64];
     __shared__ float s_b[kGuideLayers * 64];
-    const uint32_t n = count_ptr ? *count_ptr : count;
+    // queued hits of bounce bounce_base[0] + bk (the extension-queue count of that bounce)
+    const uint32_t n = counts ? counts[(bounce_base[0] + bk) % kCountSlots] : count;
+    if (queries && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(queries, (unsigned long long)n);
+    if (n == 0) return;   // whole block (a captured chunk runs past the last bounce of its paths)
     for (int i = threadIdx.x; i < kGuideLayers * 64 * 64 / 8; i += 64 * kInferWaves)
         reinterpret_cast<bf16x8*>(s_w)[i] = reinterpret_cast<const bf16x8*>(net.wp)[i];  // bf16 bit patterns
     for (int i = threadIdx.x; i < kGuideLayers * 64; i += 64 * kInferWaves) s_b[i] = net.bp[i];
@@ -426,13 +430,13 @@ __global__ void k_fill_float(float* p, uint32_t n, float v) {
 // density (BSDF x cosine x incident radiance, which depends on the outgoing direction the
 // network sees) over the sampling density.  Its luminance is the record's weight in the
 // KL-divergence gradient.  Records are kept with probability keep_prob.
-__global__ __launch_bounds__(256) void k_guide_collect(PathBuffers pb, PassParams pp, GuideScene gs, float* __restrict__ out,
-                                                       uint32_t capacity, uint32_t* __restrict__ counters, float keep_prob,
-                                                       float sel, uint32_t salt) {
-    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
-    if (pid >= pp.paths) return;
+//
+// The records a path keeps this pass, as a mask over its recorded bounces (and how many of
+// its records are valid before the keep test).
+__device__ __forceinline__ uint32_t collect_mask(const PathBuffers& pb, const PassParams& pp, uint32_t pid,
+                                                 float keep_prob, uint32_t salt, uint32_t* valid) {
     const float4 L = pb.L[pid];
-    uint32_t valid = 0;
+    uint32_t mask = 0, nvalid = 0;
     for (uint32_t b = 0; b < pp.rec_bounces; ++b) {
         const float4* rc = pb.rec + ((size_t)b * pp.paths + pid) * kRecFloat4;
         const float4 r0 = rc[0];
@@ -445,28 +449,104 @@ __global__ __launch_bounds__(256) void k_guide_collect(PathBuffers pb, PassParam
         const float lbb = T.z > 0.0f ? (L.z - lb.z) / T.z : 0.0f;
         const float w = luminance(rgb3(fmaxf(lr, 0.0f), fmaxf(lg, 0.0f), fmaxf(lbb, 0.0f)));
         if (!(w > 0.0f) || !__builtin_isfinite(w)) continue;
-        ++valid;
+        ++nvalid;
         uint32_t h = pid * 0x9E3779B1u ^ (b + 1u) * 0x85EBCA77u ^ salt * 0xC2B2AE3Du;
         h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12; h *= 0x297A2D39u; h ^= h >> 15;
         if ((float)(h >> 8) * 0x1p-24f >= keep_prob) continue;
-        const uint32_t slot = atomicAdd(&counters[0], 1u);
-        if (slot >= capacity) continue;
-        const float4 n4 = rc[1], v4 = rc[2], d4 = rc[3];
+        mask |= 1u << b;
+    }
+    *valid = nvalid;
+    return mask;
+}
+
+// exclusive prefix sum over a 256-thread block (returns this thread's prefix; *total = block sum)
+__device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t* s, uint32_t* total) {
+    const uint32_t t = threadIdx.x;
+    s[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t a = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += a;
+        __syncthreads();
+    }
+    const uint32_t incl = s[t];
+    *total = s[255];
+    __syncthreads();
+    return incl - v;
+}
+
+// Records are collected in path order, bounce by bounce within a path (no atomics, so the
+// training batches and therefore the trained network do not depend on scheduling):
+// count per block, scan the block counts, write.
+__global__ __launch_bounds__(256) void k_guide_collect_count(PathBuffers pb, PassParams pp, float keep_prob, uint32_t salt,
+                                                             uint32_t* __restrict__ block_counts) {
+    __shared__ uint32_t s[256];
+    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
+    uint32_t valid = 0, kept = 0;
+    if (pid < pp.paths) kept = (uint32_t)__popc(collect_mask(pb, pp, pid, keep_prob, salt, &valid));
+    uint32_t tk = 0, tv = 0;
+    block_exclusive_scan256(kept, s, &tk);
+    block_exclusive_scan256(valid, s, &tv);
+    if (threadIdx.x == 0) { block_counts[blockIdx.x] = tk; block_counts[gridDim.x + blockIdx.x] = tv; }
+}
+
+// One block: block_counts[0, n) become exclusive offsets; counters[0] = records kept,
+// counters[1] = records valid before the keep test.
+__global__ __launch_bounds__(256) void k_guide_collect_scan(uint32_t* __restrict__ block_counts, uint32_t n,
+                                                            uint32_t* __restrict__ counters) {
+    __shared__ uint32_t s[256];
+    const uint32_t per = (n + 255u) / 256u, b0 = min(threadIdx.x * per, n), b1 = min(b0 + per, n);
+    uint32_t sum = 0, vsum = 0;
+    for (uint32_t k = b0; k < b1; ++k) { sum += block_counts[k]; vsum += block_counts[n + k]; }
+    uint32_t total = 0, vtotal = 0;
+    uint32_t run = block_exclusive_scan256(sum, s, &total);
+    block_exclusive_scan256(vsum, s, &vtotal);
+    for (uint32_t k = b0; k < b1; ++k) { const uint32_t c = block_counts[k]; block_counts[k] = run; run += c; }
+    if (threadIdx.x == 0) { counters[0] = total; counters[1] = vtotal; }
+}
+
+__global__ __launch_bounds__(256) void k_guide_collect_write(PathBuffers pb, PassParams pp, GuideScene gs,
+                                                             float* __restrict__ out, uint32_t capacity,
+                                                             const uint32_t* __restrict__ block_offsets,
+                                                             float keep_prob, float sel, uint32_t salt) {
+    __shared__ uint32_t s[256];
+    const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
+    uint32_t valid = 0, mask = 0;
+    if (pid < pp.paths) mask = collect_mask(pb, pp, pid, keep_prob, salt, &valid);
+    uint32_t tk = 0;
+    uint32_t slot = block_offsets[blockIdx.x] + block_exclusive_scan256((uint32_t)__popc(mask), s, &tk);
+    for (uint32_t b = 0; mask != 0; ++b, mask >>= 1) {
+        if (!(mask & 1u)) continue;
+        const uint32_t k = slot++;
+        if (k >= capacity) break;
+        const float4* rc = pb.rec + ((size_t)b * pp.paths + pid) * kRecFloat4;
+        const float4 L = pb.L[pid];
+        const float4 r0 = rc[0], n4 = rc[1], v4 = rc[2], d4 = rc[3], T = rc[4], lb = rc[5];
+        const float lr = T.x > 0.0f ? (L.x - lb.x) / T.x : 0.0f;
+        const float lg = T.y > 0.0f ? (L.y - lb.y) / T.y : 0.0f;
+        const float lbb = T.z > 0.0f ? (L.z - lb.z) / T.z : 0.0f;
+        const float w = luminance(rgb3(fmaxf(lr, 0.0f), fmaxf(lg, 0.0f), fmaxf(lbb, 0.0f)));
         float x[9];
         guide_features(gs, mk(r0.x, r0.y, r0.z), mk(n4.x, n4.y, n4.z), mk(v4.x, v4.y, v4.z), x);
-        float* o = out + (size_t)slot * kGuideRecordFloats;
-        for (int k = 0; k < 9; ++k) o[k] = x[k];
+        float* o = out + (size_t)k * kGuideRecordFloats;
+        for (int q = 0; q < 9; ++q) o[q] = x[q];
         o[9] = d4.x; o[10] = d4.y; o[11] = d4.z; o[12] = w;
         o[13] = n4.w; o[14] = sel; o[15] = 0.0f;
     }
-    if (valid) atomicAdd(&counters[1], valid);
 }
 
+uint32_t guide_collect_blocks(uint32_t paths) { return (paths + 255u) / 256u; }
+
 hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
-                         uint32_t capacity, uint32_t* counters, float keep_prob, float sel, uint32_t salt,
-                         hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_collect, dim3((pp.paths + 255) / 256), dim3(256), 0, st, pb, pp, gs, out_rec, capacity,
-                       counters, keep_prob, sel, salt);
+                         uint32_t capacity, uint32_t* counters, uint32_t* block_counts, float keep_prob, float sel,
+                         uint32_t salt, hipStream_t st) {
+    const uint32_t nb = guide_collect_blocks(pp.paths);
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_collect_count, dim3(nb), dim3(256), 0, st, pb, pp, keep_prob, salt, block_counts);
+    hipLaunchKernelGGL(k_guide_collect_scan, dim3(1), dim3(256), 0, st, block_counts, nb, counters);
+    hipLaunchKernelGGL(k_guide_collect_write, dim3(nb), dim3(256), 0, st, pb, pp, gs, out_rec, capacity, block_counts,
+                       keep_prob, sel, salt);
     return hipGetLastError();
 }
 
@@ -483,15 +563,18 @@ hipError_t guide_infer_features(const GuideNetDev& net, const float* features, u
     const uint32_t need = (n + 64u * kInferWaves - 1u) / (64u * kInferWaves);
     if (need == 0) return hipSuccess;
     hipLaunchKernelGGL(k_guide_infer, dim3(std::min(grid, need)), dim3(64 * kInferWaves), 0, st, net, src,
-                       (const uint32_t*)nullptr, n, out, raw);
+                       (const uint32_t*)nullptr, (const uint32_t*)nullptr, 0u, n, out, raw,
+                       (unsigned long long*)nullptr);
     return hipGetLastError();
 }
 
 hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const phg_ray* rays, const TriRec* tris,
-                            const GuideScene& gs, const uint32_t* count_ptr, float* out, uint32_t grid, hipStream_t st) {
+                            const GuideScene& gs, const uint32_t* counts, const uint32_t* bounce_base, uint32_t bk,
+                            float* out, unsigned long long* queries, uint32_t grid, hipStream_t st) {
     InferSource src{};
     src.hits = hits; src.rays = rays; src.tris = tris; src.gs = gs;
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, net, src, count_ptr, 0u, out, 0);
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, net, src, counts, bounce_base, bk, 0u,
+                       out, 0, queries);
     return hipGetLastError();
 }
 

@@ -112,6 +112,8 @@ struct PathBuffers {
     int32_t* spill;          // traversal-stack spill, spill_cap entries per traversal thread
     uint32_t spill_cap;
     uint32_t* path_seg;      // per path: its pass segment
+    uint32_t* bounce_base;   // first bounce of the running chunk of bounces (the stage kernels
+                             // take their bounce as bounce_base[0] + their chunk offset)
 };
 
 // Size of the device scratch array of the allocation-free ray query; its address on `device`
@@ -130,14 +132,18 @@ hipError_t launch_generate(const SceneDev& sc, const PassParams* ppd, const Path
 // shading kernel (kShadeBlock threads) that one compute unit holds at once; sizes the
 // persistent grids.
 hipError_t resident_blocks(uint32_t trace_block, uint32_t* trace_blocks_per_cu, uint32_t* shade_blocks_per_cu);
-hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t block,
+// The stage kernels of bounce bounce_base[0] + bk (bk: offset within the running chunk; eager
+// loops keep bounce_base[0] = 0 and pass the bounce itself).
+hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bk, uint32_t grid, uint32_t block,
                          uint32_t count, hipStream_t st);
-hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bk, uint32_t grid,
                          uint32_t block, uint32_t count, hipStream_t st);
-hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bk, uint32_t grid,
                         bool guided, hipStream_t st);
 hipError_t launch_film(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t grid, hipStream_t st);
 hipError_t launch_clear_counts(uint32_t* counts, uint32_t n, hipStream_t st);
+// bounce_base[0] += chunk: ends a captured chunk of bounces
+hipError_t launch_advance_bounce(uint32_t* bounce_base, uint32_t chunk, hipStream_t st);
 hipError_t launch_fill_zero(double* p, uint64_t n, hipStream_t st);
 
 }  // namespace phg

@@ -139,7 +139,10 @@ __device__ __forceinline__ void queue_ray(phg_ray* q, uint32_t slot, V3 o, uint3
 __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb) {
     const PassParams pp = *ppd;
     const uint32_t stride = gridDim.x * kShadeBlock;
-    if (blockIdx.x == 0 && threadIdx.x == 0) pb.counts[0] = pp.paths;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.counts[0] = pp.paths;
+        pb.bounce_base[0] = 0u;
+    }
     for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < pp.paths; i += stride) {
         const uint32_t si = find_seg(pp.segs, pp.num_segs, i);
         const PassSeg sg = pp.segs[si];
@@ -202,7 +205,8 @@ __device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* _
 }
 
 template <int RING, uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bounce, uint32_t count_traversal) {
+__global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bk, uint32_t count_traversal) {
+    const uint32_t bounce = pb.bounce_base[0] + bk;
     __shared__ int32_t stack_mem[RING * BLOCK];
     int32_t* spill = pb.spill + (size_t)(xcd_block_id() * BLOCK + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[bounce % kCountSlots];
@@ -233,7 +237,8 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, PathBuffers pb, u
 
 template <int RING, uint32_t BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
-                                                  uint32_t bounce, uint32_t count_traversal) {
+                                                  uint32_t bk, uint32_t count_traversal) {
+    const uint32_t bounce = pb.bounce_base[0] + bk;
     const uint32_t rec_bounces = ppd->record ? ppd->rec_bounces : 0u, npaths = ppd->paths;
     __shared__ int32_t stack_mem[RING * BLOCK];
     int32_t* spill = pb.spill + (size_t)(xcd_block_id() * BLOCK + threadIdx.x) * pb.spill_cap;
@@ -283,7 +288,8 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, const PassParams*
 // measured 15% faster on the 1M-triangle, 1024^2 workload (5 or 6 waves spill more and lose).
 template <bool GUIDED>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
-                                                       uint32_t bounce) {
+                                                       uint32_t bk) {
+    const uint32_t bounce = pb.bounce_base[0] + bk;
     const PassParams pp = *ppd;
     const uint32_t n = pb.counts[bounce % kCountSlots];
     const uint32_t qin = bounce & 1u, qout = qin ^ 1u;
@@ -625,6 +631,10 @@ __global__ void k_clear_counts(uint32_t* counts, uint32_t n) {
     if (i < n) counts[i] = 0u;
 }
 
+__global__ void k_advance_bounce(uint32_t* bounce_base, uint32_t chunk) {
+    if (threadIdx.x == 0) bounce_base[0] += chunk;
+}
+
 __global__ void k_fill_double(double* p, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -690,32 +700,32 @@ hipError_t launch_generate(const SceneDev& sc, const PassParams* ppd, const Path
     return hipGetLastError();
 }
 
-hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bounce, uint32_t grid, uint32_t block,
+hipError_t launch_extend(const SceneDev& sc, const PathBuffers& pb, uint32_t bk, uint32_t grid, uint32_t block,
                          uint32_t count, hipStream_t st) {
     switch (block) {
-        case 64: hipLaunchKernelGGL((k_extend<kTraceRing, 64>), dim3(grid), dim3(64), 0, st, sc, pb, bounce, count); break;
-        case 128: hipLaunchKernelGGL((k_extend<kTraceRing, 128>), dim3(grid), dim3(128), 0, st, sc, pb, bounce, count); break;
-        case 256: hipLaunchKernelGGL((k_extend<kTraceRing, 256>), dim3(grid), dim3(256), 0, st, sc, pb, bounce, count); break;
+        case 64: hipLaunchKernelGGL((k_extend<kTraceRing, 64>), dim3(grid), dim3(64), 0, st, sc, pb, bk, count); break;
+        case 128: hipLaunchKernelGGL((k_extend<kTraceRing, 128>), dim3(grid), dim3(128), 0, st, sc, pb, bk, count); break;
+        case 256: hipLaunchKernelGGL((k_extend<kTraceRing, 256>), dim3(grid), dim3(256), 0, st, sc, pb, bk, count); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+hipError_t launch_shadow(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bk, uint32_t grid,
                          uint32_t block, uint32_t count, hipStream_t st) {
     switch (block) {
-        case 64: hipLaunchKernelGGL((k_shadow<kTraceRing, 64>), dim3(grid), dim3(64), 0, st, sc, ppd, pb, bounce, count); break;
-        case 128: hipLaunchKernelGGL((k_shadow<kTraceRing, 128>), dim3(grid), dim3(128), 0, st, sc, ppd, pb, bounce, count); break;
-        case 256: hipLaunchKernelGGL((k_shadow<kTraceRing, 256>), dim3(grid), dim3(256), 0, st, sc, ppd, pb, bounce, count); break;
+        case 64: hipLaunchKernelGGL((k_shadow<kTraceRing, 64>), dim3(grid), dim3(64), 0, st, sc, ppd, pb, bk, count); break;
+        case 128: hipLaunchKernelGGL((k_shadow<kTraceRing, 128>), dim3(grid), dim3(128), 0, st, sc, ppd, pb, bk, count); break;
+        case 256: hipLaunchKernelGGL((k_shadow<kTraceRing, 256>), dim3(grid), dim3(256), 0, st, sc, ppd, pb, bk, count); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bounce, uint32_t grid,
+hipError_t launch_shade(const SceneDev& sc, const PassParams* ppd, const PathBuffers& pb, uint32_t bk, uint32_t grid,
                         bool guided, hipStream_t st) {
-    if (guided) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb, bounce);
-    else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb, bounce);
+    if (guided) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb, bk);
+    else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kShadeBlock), 0, st, sc, ppd, pb, bk);
     return hipGetLastError();
 }
 
@@ -750,6 +760,11 @@ hipError_t launch_film(const SceneDev& sc, const PassParams* ppd, const PathBuff
 
 hipError_t launch_clear_counts(uint32_t* counts, uint32_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_clear_counts, dim3(blocks_for(n, 256)), dim3(256), 0, st, counts, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_bounce(uint32_t* bounce_base, uint32_t chunk, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance_bounce, dim3(1), dim3(64), 0, st, bounce_base, chunk);
     return hipGetLastError();
 }
 

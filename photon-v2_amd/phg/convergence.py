@@ -50,13 +50,14 @@ def time_to(curve, target: float):
 def progressive(ctx, width: int, height: int, max_spp: int, ref_mean, ref_var, seed: int = 1,
                 max_bounces: int = 10000, guided: bool = False, train_spp: int = 0, guide_prob: float = 0.5,
                 guide_steps: int = 16, guide_lr: float = 1e-3, target: float | None = None,
-                guide_depth: int = 0, totals: dict | None = None, train_spp_per_pass: int = 0):
+                guide_depth: int = 0, totals: dict | None = None, train_spp_per_pass: int = 0, graph: bool = False):
     """Render 1, 2, 4, ... max_spp samples per pixel progressively on one film; returns the
     (spp, seconds, relMSE) checkpoints.  Guided renders train the network on every pass of
     the first `train_spp` samples and keep sampling with it afterwards.  Stops early once
     `target` is reached.  `totals`, if given, accumulates the renders' guiding statistics
     (guide_infer_ms, guide_queries, guide_train_ms, guide_records).  `train_spp_per_pass` sets
-    the samples per pass of the training renders (0 = the library's choice)."""
+    the samples per pass of the training renders (0 = the library's choice); `graph` replays
+    captured bounce loops / chunks (PHG_RENDER_GRAPH)."""
     ctx.film_clear(width, height)
     total, spp, nxt, out = 0.0, 0, 1, []
     while spp < max_spp:
@@ -67,7 +68,7 @@ def progressive(ctx, width: int, height: int, max_spp: int, ref_mean, ref_var, s
             t0 = time.perf_counter()
             ctx.render(stop - spp, max_bounces=max_bounces, seed=seed, spp_begin=spp, guided=guided, train=train,
                        guide_prob=guide_prob, guide_steps=guide_steps, guide_lr=guide_lr, guide_depth=guide_depth,
-                       spp_per_pass=train_spp_per_pass if train else 0)
+                       spp_per_pass=train_spp_per_pass if train else 0, graph=graph)
             total += time.perf_counter() - t0
             if totals is not None:
                 st = ctx.stats()

@@ -212,3 +212,56 @@ def test_guided_batch_needs_one_scene():
         c.upload_many(sc)
         with pytest.raises(phg.PhgError):
             c.render_batch(JOBS[:2], guided=True)
+
+
+def test_captured_chunks_equal_eager_unbounded():
+    """Unbounded renders (the reference's 10000-bounce cap) replay captured chunks of 8
+    bounces until the extension queue drains: films and ray counts equal the eager loop's,
+    replay after replay, and a bounce cap that falls inside a chunk stops the paths there."""
+    sc = phg.Scene.generate(phg.GEN_OCCLUDED_ROOM, seed=3, target_tris=20_000)
+    W, H = 64, 48
+    with phg.Context(0) as c:
+        c.upload(sc)
+        for cap in (10000, 13):
+            films, stats = [], []
+            for graph in (False, True, False, True):
+                c.film_clear(W, H)
+                c.render(12, max_bounces=cap, seed=8, spp_per_pass=4, graph=graph)
+                films.append(c.film())
+                stats.append(c.stats())
+            print(f"[batch] chunks cap={cap} launches eager={stats[0]['trace_launches']} "
+                  f"graph={stats[1]['trace_launches']} graph_ms={stats[1]['graph_ms']:.2f}", flush=True)
+            assert stats[1]["graph_ms"] > 0
+            for f in films[1:]:
+                assert np.array_equal(f, films[0])
+            for k in ("rays_extension", "rays_shadow"):
+                assert stats[1][k] == stats[0][k], k
+
+
+@pytest.mark.parametrize("guide_depth", [0, 1, 11])
+def test_captured_chunks_equal_eager_guided(guide_depth):
+    """Guided renders in captured chunks (inference and guided shading inside the graph, at
+    the bounces guide_depth selects): with the same network the films and query counts equal
+    the eager loop's; training renders (records, collection and Adam steps after every pass)
+    reach the same network and films from the same start."""
+    sc = phg.Scene.generate(phg.GEN_OCCLUDED_ROOM, seed=3, target_tris=20_000)
+    W, H = 48, 40
+    out = {}
+    for graph in (False, True):
+        with phg.Context(0) as c:
+            c.upload(sc)
+            c.film_clear(W, H)
+            c.render(8, max_bounces=10000, seed=3, spp_per_pass=2, guided=True, train=True, guide_batch=4096,
+                     guide_steps=2, guide_depth=guide_depth, graph=graph)
+            trained = c.film()
+            params = c.guide_get_params()
+            c.film_clear(W, H)
+            c.render(16, max_bounces=10000, seed=4, spp_per_pass=4, guided=True, guide_depth=guide_depth, graph=graph)
+            out[graph] = (trained, params, c.film(), c.stats())
+    st_e, st_g = out[False][3], out[True][3]
+    print(f"[batch] guided chunks depth={guide_depth} queries eager={st_e['guide_queries']} "
+          f"graph={st_g['guide_queries']}", flush=True)
+    assert st_g["guide_queries"] == st_e["guide_queries"] > 0
+    assert np.array_equal(out[True][1], out[False][1])
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][2], out[False][2])
