@@ -279,7 +279,8 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, const PassParams*
 // every path whose extension ray of this bounce is queued: emission found by the previous
 // BSDF sample (MIS against NEE), then NEE at this vertex (shadow ray queued with its
 // contribution), then the BSDF sample that extends the path, throughput update and
-// Russian roulette.  Whole waves iterate together because the queue pushes use ballots.
+// Russian roulette.  The queue pushes ballot over the lanes that reach them (one atomic per
+// wave and queue).
 // GUIDED: the extension direction comes from one-sample MIS between the BSDF sampler and
 // the guiding network's lobes (probability guide_prob), and the combined density replaces
 // the BSDF pdf in both MIS weights against NEE, which keeps the estimator unbiased.
@@ -290,36 +291,30 @@ template <bool GUIDED>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
                                                        uint32_t bk) {
     const uint32_t bounce = pb.bounce_base[0] + bk;
-    const PassParams pp = *ppd;
+    const PassParams& pp = *ppd;   // fields read where used (uniform loads; no registers held)
     const uint32_t n = pb.counts[bounce % kCountSlots];
     const uint32_t qin = bounce & 1u, qout = qin ^ 1u;
     const uint32_t stride = gridDim.x * kShadeBlock;
     const uint32_t iend = (n + 63u) & ~63u;
     for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < iend; i += stride) {
-        bool push_ext = false, push_shadow = false;
-        uint32_t pid = 0;
-        V3 ext_o = mk(0.0f, 0.0f, 0.0f), ext_d = mk(0.0f, 0.0f, 0.0f);
-        V3 sh_o = mk(0.0f, 0.0f, 0.0f), sh_d = mk(0.0f, 0.0f, 0.0f);
-        float sh_t = 0.0f;
-        uint32_t root = 0;
-        RGB sh_c = rgb1(0.0f);
         if (i < n) {
-            pid = pb.ext_pid[qin][i];
+            const uint32_t pid = pb.ext_pid[qin][i];
             const float4 h = pb.hits[i];
             const int32_t tri = __float_as_int(h.x);
             const float4 rd = reinterpret_cast<const float4*>(pb.ext[qin] + i)[1];
             const V3 dir = mk(rd.x, rd.y, rd.z);
             bool stop = tri < 0;
-            float4 Lacc = pb.L[pid];
-            bool l_dirty = false;
             SurfaceHit X;
-            TriRec tr;
-            TriVerts tv;
+            uint32_t mat = 0;
+            int32_t emitter = -1;
             if (!stop) {
-                tr = sc.tris[tri];
-                tv.a = mk(tr.a[0], tr.a[1], tr.a[2]);
-                tv.b = mk(tr.b[0], tr.b[1], tr.b[2]);
-                tv.c = mk(tr.c[0], tr.c[1], tr.c[2]);
+                // the triangle's vertices are read again where the escape tests need them,
+                // instead of being held in registers through the NEE and BSDF stages
+                const TriRec& tr = sc.tris[tri];
+                mat = tr.mat;
+                emitter = tr.emitter;
+                TriVerts tv;
+                load_tri(sc.tris, (uint32_t)tri, &tv.a, &tv.b, &tv.c);
                 X = make_hit(tv.a, tv.b, tv.c, (uint32_t)tri, h.y, h.z, h.w);
                 if (!side_agreed(X, dir)) stop = true;  // SurfaceTracer.h:126-143
             }
@@ -327,9 +322,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
             const JobDev& job = sc.jobs[sg.job];
             const bool bvpt = job.estimator != PHG_ESTIMATOR_BNEEPT;
             float4 T4 = pb.T[pid];
-            if (!stop && tr.emitter >= 0) {
+            if (!stop && emitter >= 0) {
+                // the radiance so far is read and written here only (not held through the
+                // NEE and BSDF stages)
+                float4 Lacc = pb.L[pid];
+                bool l_dirty = false;
                 // DiffuseSurfaceEmitterBase::evalEnergyFromSurface: front face only
-                const EmitterRec em = sc.emitters[tr.emitter];
+                const EmitterRec em = sc.emitters[emitter];
                 const bool front = dot(neg(dir), X.Ns) > 0.0f;
                 const RGB Le = front ? rgb3(em.radiance[0], em.radiance[1], em.radiance[2]) : rgb1(0.0f);
                 if (bvpt) {
@@ -351,6 +350,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                     float neePdf = 0.0f;
                     const V3 e2t = Xp - X.P;
                     if (dot(e2t, X.Ns) > 0.0f) {
+                        TriVerts tv;
+                        load_tri(sc.tris, (uint32_t)tri, &tv.a, &tv.b, &tv.c);
                         const float pdfA = 1.0f / tri_area(tv.a, tv.b, tv.c);
                         float pdfW = pdfA_to_pdfW(pdfA, e2t, X.Ns);
                         const float gp = em.tri_count >= 2 ? 1.0f / (float)em.tri_count : 1.0f;
@@ -369,24 +370,25 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                         l_dirty = true;
                     }
                 }
+                if (l_dirty) pb.L[pid] = Lacc;
             }
             if (!stop && bounce > 0) {
                 const uint32_t flags = __float_as_uint(pb.prevP[pid].w);
                 if (flags & kFlagDieAfterEmission) stop = true;
             }
-            root = job.root;
             if (!stop && bounce >= job.max_bounces) stop = true;
             if (!stop) {
                 uint32_t pixel, sample;
                 path_ids(sg, job, pid, &pixel, &sample);
                 const uint64_t key = path_key(job.seed, pixel, sample, job.npix);
                 Stream s = stream_resume(key, pb.rng[pid]);
-                float u[GUIDED ? 12 : 8];
-#pragma unroll
-                for (int k = 0; k < (GUIDED ? 12 : 8); ++k) u[k] = s.u();
+                // the vertex's random numbers, drawn in stream order where they are first
+                // needed (NEE: 4, then BSDF: 3 + Russian roulette: 1, + 4 when guided), so that
+                // no more of them are live at once than the stage uses
+                const float u0 = s.u(), u1 = s.u(), u2 = s.u(), u3 = s.u();
                 const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideOut : nullptr;
                 const bool guide_here = pp.guide_depth == 0 || bounce < pp.guide_depth;
-                const phg_material m = sc.mats[tr.mat];
+                const phg_material& m = sc.mats[mat];
                 const float alpha = (GUIDED && guide_here && pp.guide_prob > 0.0f && guide_samples_material(m) &&
                                      guide_lobes_ok(lob)) ? pp.guide_prob : 0.0f;
                 if (m.type == PHG_MAT_ABSORBER) stop = true;
@@ -396,15 +398,15 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                     const V3 V = dir * -1.0f;
                     // ---- NEE (TDirectLightEstimator.ipp:70-102, BNEEPTEstimator.cpp:103-149)
                     if (!bvpt && job.num_emitters > 0) {
-                        const uint32_t ei = cdf_sample(sc.emit_cdf + job.cdf_begin, job.num_emitters, u[0]);
+                        const uint32_t ei = cdf_sample(sc.emit_cdf + job.cdf_begin, job.num_emitters, u0);
                         const EmitterRec em = sc.emitters[job.emitter_begin + ei];
-                        uint32_t k = (uint32_t)(u[1] * (float)em.tri_count);
+                        uint32_t k = (uint32_t)(u1 * (float)em.tri_count);
                         if (k >= em.tri_count) k = em.tri_count - 1u;
                         const EmitTri et = sc.emit_tris[em.tri_begin + k];
                         TriVerts te;
                         load_tri(sc.tris, et.tri, &te.a, &te.b, &te.c);
-                        const float A = sqrtf(u[2]);
-                        const float B = u[3];
+                        const float A = sqrtf(u2);
+                        const float B = u3;
                         const float e0 = 1.0f - A, e1 = A * (1.0f - B), e2 = B * A;
                         const float pdfA = 1.0f / et.area;
                         const SurfaceHit Xe = make_hit(te.a, te.b, te.c, et.tri, e0, e1, e2);
@@ -414,6 +416,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                             if (em.tri_count >= 2) pdf = pdf * (1.0f / (float)em.tri_count);
                             pdf = pdf * em.pick_pdf;
                             V3 so, sd; float st;
+                            TriVerts tv;
+                            load_tri(sc.tris, (uint32_t)tri, &tv.a, &tv.b, &tv.c);
                             if (pdf_ok(pdf) && side_agreed(X, Xe.P - X.P) && mutual_escape(X, tv, Xe, te, &so, &sd, &st)) {
                                 const V3 Ld = normalize(Xe.P - X.P);
                                 const BsdfEval be = bsdf_eval(m, X, Ld, V);
@@ -426,21 +430,33 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                                     weight = mulc(weight, T);
                                     weight = muls(weight, w / pdf);
                                     weight = rational_clamp(weight);
-                                    sh_c = rgb3(em.radiance[0] * weight.r, em.radiance[1] * weight.g, em.radiance[2] * weight.b);
-                                    if (!rgb_zero(sh_c)) { push_shadow = true; sh_o = so; sh_d = sd; sh_t = st; }
+                                    const RGB sh_c = rgb3(em.radiance[0] * weight.r, em.radiance[1] * weight.g,
+                                                          em.radiance[2] * weight.b);
+                                    if (!rgb_zero(sh_c)) {
+                                        // queued at once (its lanes ballot together), so the
+                                        // shadow ray is not carried through the BSDF stage
+                                        const uint32_t slot_s =
+                                            wave_push(&pb.counts[kCountSlots + bounce % kCountSlots], true);
+                                        queue_ray(pb.shadow, slot_s, so, job.root, sd, st);
+                                        pb.shadow_pid[slot_s] = pid;
+                                        pb.shadow_c[slot_s] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.0f);
+                                    }
                                 }
                             }
                         }
                     }
                     // ---- BSDF sample + extension (BNEEPTEstimator.cpp:151-255)
+                    const float u4 = s.u(), u5 = s.u(), u6 = s.u(), u7 = s.u();
+                    float u8 = 0.0f, u9 = 0.0f, u10 = 0.0f, u11 = 0.0f;
+                    if (GUIDED) { u8 = s.u(); u9 = s.u(); u10 = s.u(); u11 = s.u(); }
                     bool ok = false;
                     V3 Ls = mk(0.0f, 0.0f, 0.0f);
                     RGB ws = rgb1(0.0f);
                     float relIor = 1.0f, pdfc = 0.0f;
                     float bsdf_pdf_ls = 0.0f;  // BSDF density of the chosen direction (training records)
-                    if (GUIDED && u[8] < alpha) {
+                    if (GUIDED && u8 < alpha) {
                         // direction from the learned lobes, weighted by the combined density
-                        Ls = guide_sample(lob, u[9], u[10], u[11]);
+                        Ls = guide_sample(lob, u9, u10, u11);
                         const BsdfEval be = bsdf_eval(m, X, Ls, V);
                         float bp = bsdf_pdf(m, X, Ls, V);
                         if (!pdf_ok(bp)) bp = 0.0f;
@@ -457,7 +473,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                             ok = measurable(ws, cosl) && side_agreed(X, Ls);
                         }
                     } else {
-                        const BsdfSample bs = bsdf_sample(m, X, V, u[4], u[5], u[6]);
+                        const BsdfSample bs = bsdf_sample(m, X, V, u4, u5, u6);
                         if (bs.ok) {
                             Ls = bs.L; ws = bs.w; relIor = bs.relIor;
                             const float bp = bsdf_pdf(m, X, bs.L, V);
@@ -474,6 +490,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                     }
                     if (ok) {
                         V3 off, ud;
+                        TriVerts tv;
+                        load_tri(sc.tris, (uint32_t)tri, &tv.a, &tv.b, &tv.c);
                         iterative_offset(X, tv, Ls, &off, &ud);
                         const RGB wem = mulc(ws, T);
                         RGB Tn = mulc(T, ws);
@@ -481,7 +499,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                         bool die = false;
                         if (bounce >= 3) {  // RussianRoulette.h:30-55
                             const float surv = safe_clamp(luminance(muls(Tn, rrScale)), 0.0f, 0.95f);
-                            if (u[7] < surv) Tn = muls(Tn, 1.0f / surv);
+                            if (u7 < surv) Tn = muls(Tn, 1.0f / surv);
                             else die = true;
                         }
                         if (!die && rgb_zero(Tn)) die = true;
@@ -493,9 +511,9 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                             pb.Wem[pid] = make_float4(wem.r, wem.g, wem.b, pdfc);
                             pb.prevP[pid] = make_float4(X.P.x, X.P.y, X.P.z, __uint_as_float(die ? kFlagDieAfterEmission : 0u));
                             pb.rng[pid] = s.state;
-                            push_ext = true;
-                            ext_o = X.P + off;
-                            ext_d = ud;
+                            const uint32_t slot_e = wave_push(&pb.counts[(bounce + 1) % kCountSlots], true);
+                            queue_ray(pb.ext[qout], slot_e, X.P + off, job.root, ud, kFltMax);
+                            pb.ext_pid[qout][slot_e] = pid;
                             if (GUIDED && pp.record && bounce < pp.rec_bounces) {
                                 float4* rc = pb.rec + ((size_t)bounce * pp.paths + pid) * kRecFloat4;
                                 rc[0] = make_float4(X.P.x, X.P.y, X.P.z, pdfc);
@@ -503,24 +521,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                                 rc[2] = make_float4(V.x, V.y, V.z, 0.0f);
                                 rc[3] = make_float4(ud.x, ud.y, ud.z, 0.0f);
                                 rc[4] = make_float4(T.r, T.g, T.b, 0.0f);  // throughput before this vertex
-                                rc[5] = make_float4(Lacc.x, Lacc.y, Lacc.z, 0.0f);
+                                const float4 Lnow = pb.L[pid];   // radiance so far, this vertex's emission included
+                                rc[5] = make_float4(Lnow.x, Lnow.y, Lnow.z, 0.0f);
                             }
                         }
                     }
                 }
             }
-            if (l_dirty) pb.L[pid] = Lacc;
-        }
-        const uint32_t slot_e = wave_push(&pb.counts[(bounce + 1) % kCountSlots], push_ext);
-        if (push_ext) {
-            queue_ray(pb.ext[qout], slot_e, ext_o, root, ext_d, kFltMax);
-            pb.ext_pid[qout][slot_e] = pid;
-        }
-        const uint32_t slot_s = wave_push(&pb.counts[kCountSlots + bounce % kCountSlots], push_shadow);
-        if (push_shadow) {
-            queue_ray(pb.shadow, slot_s, sh_o, root, sh_d, sh_t);
-            pb.shadow_pid[slot_s] = pid;
-            pb.shadow_c[slot_s] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.0f);
         }
     }
 }

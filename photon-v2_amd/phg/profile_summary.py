@@ -4,6 +4,12 @@
     python -m phg.profile_summary pmc    <fetch.db> <write.db> <copy_fetch.db> <copy_write.db> <copy_bytes>
     python -m phg.profile_summary l2     <tcc.db>
 
+    python -m phg.profile_summary csvstats <kernel_trace.csv> [--skip K]
+    python -m phg.profile_summary counters <counter_collection.csv> [--skip K]
+
+`csvstats` / `counters` read rocprofv3's CSV output (--output-format csv): per-kernel
+durations from the kernel trace, and per-kernel averages over dispatches of every collected
+counter (rows of one dispatch and counter, one per hardware instance, summed first).
 `stats` prints per-kernel call counts and durations (with --skip K, leaving out each kernel's
 first K dispatches, e.g. those of a counted warm-up render); `pmc` prints per-kernel FETCH_SIZE and
 WRITE_SIZE per dispatch, scaled by what the same counters read for a plain device copy of
@@ -57,7 +63,81 @@ def counter(path, name):
     return {short(n): (c, a) for n, c, a in rows}
 
 
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def csv_kernel_stats(path, skip=0):
+    per = {}
+    rows = _csv_rows(path)
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    for r in rows:
+        per.setdefault(short(r["Kernel_Name"]), []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+    out = []
+    for k, ds in per.items():
+        ds = ds[skip:]
+        if ds:
+            out.append({"kernel": k, "calls": len(ds), "total_ms": sum(ds), "avg_ms": sum(ds) / len(ds),
+                        "min_ms": min(ds), "max_ms": max(ds)})
+    out.sort(key=lambda r: -r["total_ms"])
+    total = sum(r["total_ms"] for r in out) or 1.0
+    for r in out:
+        r["percent"] = 100.0 * r["total_ms"] / total
+    return out
+
+
+def csv_counters(path, skip=0):
+    """{kernel: {"dispatches": n, counter: average per dispatch}}"""
+    per_dispatch = {}
+    order = {}
+    for r in _csv_rows(path):
+        d = int(r["Dispatch_Id"])
+        k = short(r["Kernel_Name"])
+        order[d] = k
+        key = (d, r["Counter_Name"])
+        per_dispatch[key] = per_dispatch.get(key, 0.0) + float(r["Counter_Value"])
+    seen, keep = {}, set()
+    for d in sorted(order):
+        k = order[d]
+        seen[k] = seen.get(k, 0) + 1
+        if seen[k] > skip:
+            keep.add(d)
+    out = {}
+    for (d, c), v in per_dispatch.items():
+        if d not in keep:
+            continue
+        e = out.setdefault(order[d], {"dispatches": set()})
+        e["dispatches"].add(d)
+        e.setdefault(c, []).append(v)
+    res = {}
+    for k, e in out.items():
+        res[k] = {"dispatches": len(e["dispatches"])}
+        for c, vs in e.items():
+            if c != "dispatches":
+                res[k][c] = sum(vs) / len(vs)
+    return res
+
+
 def main(argv):
+    if len(argv) >= 2 and argv[0] == "csvstats":
+        skip = int(argv[argv.index("--skip") + 1]) if "--skip" in argv else 0
+        print(f"{'kernel':60s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'min ms':>9s} {'max ms':>9s} {'%':>6s}")
+        for r in csv_kernel_stats(argv[1], skip):
+            print(f"{r['kernel'][:60]:60s} {r['calls']:6d} {r['total_ms']:10.3f} {r['avg_ms']:9.4f} "
+                  f"{r['min_ms']:9.4f} {r['max_ms']:9.4f} {r['percent']:6.2f}")
+        return 0
+    if len(argv) >= 2 and argv[0] == "counters":
+        skip = int(argv[argv.index("--skip") + 1]) if "--skip" in argv else 0
+        res = csv_counters(argv[1], skip)
+        names = sorted({c for e in res.values() for c in e if c != "dispatches"})
+        print(f"{'kernel':36s} {'disp':>5s} " + " ".join(f"{n[:22]:>22s}" for n in names))
+        for k in sorted(res, key=lambda k: -res[k].get(names[0], 0) if names else 0):
+            e = res[k]
+            print(f"{k[:36]:36s} {e['dispatches']:5d} " + " ".join(f"{e.get(n, float('nan')):22.6g}" for n in names))
+        print(json.dumps(res))
+        return 0
     if len(argv) >= 2 and argv[0] == "stats":
         skip = int(argv[argv.index("--skip") + 1]) if "--skip" in argv else 0
         st = kernel_stats(argv[1], skip)
