@@ -42,30 +42,46 @@ for p in (os.path.join(ROOT, "photon-v2_amd"), ROOT):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBPS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "bneept_procedural1M_traffic.json")
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "bneept_procedural1M_counters.json")
+# VALU issue peak: one wave64 vector instruction per compute unit per clock (4 SIMDs, each
+# taking 4 clocks per wave64 instruction), 256 compute units at the 2.4 GHz peak engine clock
+VALU_PEAK_GINST = 256 * 2.4
 
 
-def measured_traffic(kernel_prefix, config):
-    """(HBM bytes per launch, profiled configuration) of a kernel from the committed
-    FETCH_SIZE/WRITE_SIZE profile of this workload (rocprofv3 counters, copy-calibrated) under
+def profiled_kernel(kernel_prefix, config):
+    """(per-launch counter figures, profiled configuration) of a kernel of this workload from
+    the committed rocprofv3 profile (profiles/bneept_procedural1M_counters.json: trace time,
+    VALU instructions, VALU busy fraction, lane utilization, bytes past L2, L2 hit rate) under
     launch configuration `config` ("leaf,width,block,paths_per_pass").  When that exact
-    configuration was not profiled, one with the same BVH and pass size and another block size
-    stands in (the block size moves a launch's traffic by under 1%); (None, None) otherwise."""
+    configuration was not profiled, one with the same pass size and block size stands in, then
+    one with the same pass size (the BVH leaf size and the block size move these figures by
+    under 2%); (None, None) otherwise."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(COUNTER_PROFILE) as f:
             prof = json.load(f)
     except (OSError, ValueError):
         return None, None
     if not config:
         return None, None
     configs = prof.get("configs", {})
-    leaf, width, _, paths = config.split(",")
-    keys = [config] + sorted(k for k in configs if k != config and (lambda p: (p[0], p[1], p[3]) == (leaf, width, paths))(k.split(",")))
-    for key in keys:
+    leaf, width, block, paths = config.split(",")
+    same_pb = sorted(k for k in configs if k.split(",")[2:] == [block, paths])
+    same_p = sorted(k for k in configs if k.split(",")[3] == paths)
+    for key in [config] + same_pb + same_p:
         for name, v in configs.get(key, {}).get("kernels", {}).items():
             if name.removeprefix("void ").startswith(kernel_prefix):
-                return round(v["traffic_bytes"]), key
+                return v, key
     return None, None
+
+
+# Algorithmic bytes of one shading-kernel vertex (k_shade), the most it moves: the queue entry
+# (path id 4, hit 16, ray direction 16), path state read (throughput, previous vertex 16 each,
+# generator state 8, segment index 4, radiance 16), triangle record 48 and material 64, next
+# event (emitter-pick CDF 16, emitter 32, emitter triangle 8, light triangle 48), path state
+# written (throughput, weight, previous vertex 16 each, generator state 8), and the two queue
+# pushes (extension ray 32 + id 4, shadow ray 32 + id 4 + contribution 16).
+SHADE_BYTES_PER_VERTEX = (4 + 16 + 16) + (16 + 16 + 8 + 4 + 16) + (48 + 64) + (16 + 32 + 8 + 48) + \
+    (16 + 16 + 16 + 8) + (32 + 4 + 32 + 4 + 16)
 
 
 METRIC = "Msamples/s at 1 GPU; time to relMSE 0.01, guided vs plain GPU BNEEPT"
@@ -267,7 +283,7 @@ def scale_convergence(args, ctx, jobs):
     plain = cv.progressive_batch(ctx, jobs, args.scale_conv_max_spp, refs, seed_base=100000, graph=True, target=0.005)
     gtot = {}
     guided = cv.progressive_batch(ctx, jobs, args.scale_conv_max_spp, refs, seed_base=200000, guided=True,
-                                  train_spp=16, guide_steps=32, guide_depth=1, target=0.005, totals=gtot)
+                                  train_spp=16, guide_steps=32, guide_depth=1, target=0.005, totals=gtot, graph=True)
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
     for name, cur in (("plain", plain), ("guided", guided)):
         print(f"[bench] scale {name}: " + " ".join(f"{s}spp/{t:.2f}s/{e:.2e}" for s, t, e in cur), file=sys.stderr,
@@ -303,14 +319,29 @@ def convergence_leg(args):
         R, vR = cv.mean_and_var_of_mean(ctx.film())
         print(f"[bench] convergence reference: {args.conv_ref_spp} spp at {res}x{res} in {ref_s:.1f}s "
               f"(its own relMSE {float(np.mean(vR / (R * R + cv.EPS))):.2e})", file=sys.stderr, flush=True)
-        plain = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, target=0.005)
+        plain = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, target=0.005, graph=True)
     with phg.Context(0) as ctx:
         ctx.upload(scene)
         gtot = {}
         guided = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, guided=True,
                                 train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=32, target=0.005,
-                                guide_depth=1, totals=gtot)
+                                guide_depth=1, totals=gtot, graph=True)
+        # the captured chunks carry no per-kernel times: the trained network's inference rate
+        # comes from one eager guided render of the same image
+        ctx.film_clear(res, res)
+        ctx.render(16, seed=99, guided=True, guide_depth=1)
+        ist = ctx.stats()
+        gtot["guide_infer_ms"], gtot["guide_queries"] = ist["guide_infer_ms"], ist["guide_queries"]
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
+    # per-sample error at the last checkpoint both curves reach: relMSE x spp (the variance of
+    # one sample per pixel, relative), and seconds per sample
+    common = max(s for s, _, _ in guided if any(s == q for q, _, _ in plain))
+    pe = next((t, e) for s, t, e in plain if s == common)
+    ge = next((t, e) for s, t, e in guided if s == common)
+    per_sample = {"spp": common, "plain_relmse_x_spp": round(pe[1] * common, 4),
+                  "guided_relmse_x_spp": round(ge[1] * common, 4),
+                  "variance_gain": round(pe[1] / ge[1], 4), "plain_s": round(pe[0], 4), "guided_s": round(ge[0], 4),
+                  "note": "guided time includes its training passes"}
     for name, cur in (("plain", plain), ("guided", guided)):
         print(f"[bench] {name}: " + " ".join(f"{s}spp/{t:.2f}s/{e:.2e}" for s, t, e in cur), file=sys.stderr,
               flush=True)
@@ -322,6 +353,8 @@ def convergence_leg(args):
             "guided": {"guide_prob": 0.5, "guide_depth": 1, "train_spp": args.conv_train_spp, "steps_per_pass": 32,
                        "training_pass_paths": "<= 16M (16 spp at 1024^2)", "records": "first path vertex", "lr": 1e-3,
                        "batch": 1 << 18, "mlp": mlp_utilisation(gtot)},
+            "per_sample": per_sample,
+            "mode": "both legs replay captured bounce chunks (8 bounces per graph) until the path queue drains",
             "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
             "guided_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in guided]}
 
@@ -390,7 +423,10 @@ def main():
     launch_tuning = tuned if tuned else ctx.tuning()
     traffic_key = (f"{tuned['max_leaf_tris']},{tuned['width']},{tuned['trace_block']},{tuned['paths_per_pass']}"
                    if tuned else None)
-    traffic_bytes, traffic_profiled = measured_traffic("phg::k_extend", traffic_key)
+    prof_extend, profiled_key = profiled_kernel("phg::k_extend", traffic_key)
+    prof_shadow, _ = profiled_kernel("phg::k_shadow", traffic_key)
+    prof_shade, _ = profiled_kernel("phg::k_shade<false>", traffic_key)
+    traffic_bytes = round(prof_extend["traffic_bytes"]) if prof_extend else None
     st0 = ctx.stats()
     print(f"[bench] BVH: {st0['bvh_nodes']} nodes, depth {st0['bvh_depth']}, build {st0['bvh_build_ms']:.0f} ms, "
           f"upload {upload_s:.1f}s", file=sys.stderr, flush=True)
@@ -478,6 +514,59 @@ def main():
     scale = None if args.no_scale else leg(scale_leg)
     conv = None if args.no_conv else leg(convergence_leg)
 
+    avg_launch_s = trace_ms / max(trace_launch, 1) / 1e3
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes, "cache_served": True,
+        "kernel": "k_extend (closest-hit traversal)",
+        "note": "achieved = algorithmic bytes (48 + 128 per node visit + 48 per triangle test, per ray) per "
+                "launch / launch time: this scene's nodes and triangles (~85 MB) are served by L2 and the 256 MB "
+                "Infinity Cache, so frac is not a fraction of what HBM delivers; hbm_measured is (traffic past L2 "
+                "per launch) / launch time, and valu (the bound that binds) the vector-instruction issue rate "
+                "against its peak",
+        "timed_in": "the eager steps: HIP events on the library stream around each launch",
+        "launches": trace_launch, "avg_launch_ms": round(avg_launch_s * 1e3, 4),
+        "bytes_per_launch": round(bytes_closest / max(trace_launch, 1)),
+        "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3),
+        "binding": "valu",
+    }
+    if prof_extend:
+        roofline["hbm_measured"] = {
+            "achieved": round(prof_extend["traffic_bytes"] / avg_launch_s / 1e9, 1), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(prof_extend["traffic_bytes"] / avg_launch_s / 1e9 / HBM_PEAK_GBPS, 4),
+            "l2_hit_rate": round(prof_extend["l2_hit_rate"], 4)}
+        ach_v = prof_extend["insts_valu"] / avg_launch_s / 1e9
+        roofline["valu"] = {
+            "achieved": round(ach_v, 1), "peak": VALU_PEAK_GINST, "unit": "G wave64 VALU instructions/s",
+            "frac": round(ach_v / VALU_PEAK_GINST, 4), "busy": round(prof_extend["valu_busy"], 4),
+            "lane_utilization": round(prof_extend["lane_utilization"], 4),
+            "insts_per_launch": round(prof_extend["insts_valu"])}
+        roofline["profile"] = (f"profiles/bneept_procedural1M_counters.json, launch configuration {profiled_key} "
+                               f"(this run: {traffic_key}); per-launch averages of rocprofv3 SQ_INSTS_VALU, "
+                               "SQ_ACTIVE_INST_VALU, SQ_THREAD_CYCLES_VALU, GRBM_GUI_ACTIVE, FETCH_SIZE x2.0 + "
+                               "WRITE_SIZE x1.0, TCC_HIT/MISS over the 64-spp render's launches")
+    kern = {}
+    for name, pr in (("k_shadow", prof_shadow), ("k_shade", prof_shade)):
+        if not pr:
+            continue
+        t = pr["trace_avg_ms"] / 1e3
+        e = {"avg_launch_ms": round(pr["trace_avg_ms"], 4),
+             "valu": {"frac": round(pr["insts_valu"] / t / 1e9 / VALU_PEAK_GINST, 4), "busy": round(pr["valu_busy"], 4),
+                      "lane_utilization": round(pr["lane_utilization"], 4)},
+             "hbm_measured": {"frac": round(pr["traffic_bytes"] / t / 1e9 / HBM_PEAK_GBPS, 4),
+                              "l2_hit_rate": round(pr["l2_hit_rate"], 4)}}
+        if name == "k_shade":
+            # vertices shaded per launch: every queued extension ray (camera rays included)
+            verts = (rays[0] + rays[1]) / max(trace_launch, 1)
+            alg = verts * SHADE_BYTES_PER_VERTEX / t / 1e9
+            e["algorithmic"] = {"bytes_per_vertex": SHADE_BYTES_PER_VERTEX, "vertices_per_launch": round(verts),
+                                "achieved": round(alg, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                                "frac": round(alg / HBM_PEAK_GBPS, 4)}
+        kern[name] = e
+    if kern:
+        roofline["other_kernels"] = kern
+        roofline["other_kernels_timed_in"] = "launch times from the committed kernel trace of the same configuration"
+
     out = {
         "metric": METRIC,
         "value": round(msps, 3),
@@ -503,19 +592,7 @@ def main():
         "eager": {"ms_per_step": round(total_s / args.steps * 1e3, 3), "msamples_per_s": round(msps_eager, 3),
                   "note": "same passes with a host round trip per bounce and HIP events around every launch "
                           "(the roofline's kernel times come from these steps)"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_bytes,
-                     "traffic_source": f"profiles/bneept_procedural1M_traffic.json, launch configuration "
-                                       f"{traffic_profiled} (this run: {traffic_key}; rocprofv3 FETCH_SIZE x2.0 + "
-                                       "WRITE_SIZE x1.0 per launch: bytes leaving L2, served by Infinity Cache or HBM)",
-                     "kernel": "k_extend (closest-hit traversal)",
-                     "note": "algorithmic bytes (48 + 128 per node visit + 48 per triangle test, per ray) can exceed "
-                             "the HBM rate: this scene's nodes and triangles (~85 MB) stay in L2 and the 256 MB "
-                             "Infinity Cache; `traffic` is what the kernel actually moves past L2 per launch",
-                     "timed_in": "the eager steps: HIP events on the library stream around each launch",
-                     "launches": trace_launch, "avg_launch_ms": round(trace_ms / max(trace_launch, 1), 4),
-                     "bytes_per_launch": round(bytes_closest / max(trace_launch, 1)),
-                     "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "batch": batch,
         "scale": scale,

@@ -6,6 +6,7 @@
 
     python -m phg.profile_summary csvstats <kernel_trace.csv> [--skip K]
     python -m phg.profile_summary counters <counter_collection.csv> [--skip K]
+    python -m phg.profile_summary profile <root> <leaf,width,block,paths> ...
 
 `csvstats` / `counters` read rocprofv3's CSV output (--output-format csv): per-kernel
 durations from the kernel trace, and per-kernel averages over dispatches of every collected
@@ -120,7 +121,60 @@ def csv_counters(path, skip=0):
     return res
 
 
+def counter_profile(root, configs, skip=9, cus=256, xcds=8, clock_ghz=2.4, copy_bytes=1 << 30):
+    """Per-kernel, per-launch figures of the headline workload for each launch configuration
+    (directories <root>/<leaf_width_block_paths>/{kt,valu,fetch,write,l2} of rocprofv3 CSV
+    output, and <root>/{cf,cw} for the copy calibration), skipping each kernel's first `skip`
+    launches (the counted 1-spp render): trace time, VALU and SALU instructions, VALU busy
+    fraction (SQ_ACTIVE_INST_VALU / CUs / GUI-active cycles of one XCD), VALU lane utilization
+    (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)), bytes past L2 (copy-calibrated
+    FETCH_SIZE and WRITE_SIZE) and L2 hit rate."""
+    import glob
+    import os
+
+    def one(path_glob):
+        hits = glob.glob(path_glob)
+        return hits[0] if hits else None
+    cf = csv_counters(one(f"{root}/cf/*counter_collection.csv"))
+    cw = csv_counters(one(f"{root}/cw/*counter_collection.csv"))
+    kib = copy_bytes / 1024.0
+    kf = kib / max(v["FETCH_SIZE"] for v in cf.values())
+    kw = kib / max(v["WRITE_SIZE"] for v in cw.values())
+    out = {"compute_units": cus, "xcds": xcds, "peak_engine_clock_ghz": clock_ghz, "fetch_scale": kf,
+           "write_scale": kw, "copy_bytes": copy_bytes, "skip_launches": skip, "configs": {}}
+    for cfg in configs:
+        d = os.path.join(root, cfg.replace(",", "_"))
+        kt = {r["kernel"]: r for r in csv_kernel_stats(one(f"{d}/kt/*kernel_trace.csv"), skip)}
+        valu = csv_counters(one(f"{d}/valu/*counter_collection.csv"), skip)
+        f = csv_counters(one(f"{d}/fetch/*counter_collection.csv"), skip)
+        w = csv_counters(one(f"{d}/write/*counter_collection.csv"), skip)
+        l2 = csv_counters(one(f"{d}/l2/*counter_collection.csv"), skip)
+        ks = {}
+        for k, v in valu.items():
+            if "phg::" not in k:
+                continue
+            gui = v["GRBM_GUI_ACTIVE"] / xcds
+            e = {"launches": v["dispatches"], "insts_valu": v["SQ_INSTS_VALU"], "insts_salu": v["SQ_INSTS_SALU"],
+                 "valu_busy": v["SQ_ACTIVE_INST_VALU"] / cus / gui,
+                 "lane_utilization": v["SQ_THREAD_CYCLES_VALU"] / (64.0 * v["SQ_ACTIVE_INST_VALU"])}
+            if k in kt:
+                e["trace_avg_ms"] = kt[k]["avg_ms"]
+            if k in f:
+                e["fetch_bytes"] = f[k]["FETCH_SIZE"] * kf * 1024
+                e["write_bytes"] = w.get(k, {}).get("WRITE_SIZE", 0.0) * kw * 1024
+                e["traffic_bytes"] = e["fetch_bytes"] + e["write_bytes"]
+            if k in l2:
+                h, m = l2[k].get("TCC_HIT_sum", 0.0), l2[k].get("TCC_MISS_sum", 0.0)
+                e["l2_hit_rate"] = h / (h + m) if h + m > 0 else None
+            ks[k] = e
+        out["configs"][cfg] = {"kernels": ks}
+    return out
+
+
 def main(argv):
+    if len(argv) >= 3 and argv[0] == "profile":
+        print(json.dumps(counter_profile(argv[1], argv[2:]), indent=1))
+        return 0
     if len(argv) >= 2 and argv[0] == "csvstats":
         skip = int(argv[argv.index("--skip") + 1]) if "--skip" in argv else 0
         print(f"{'kernel':60s} {'calls':>6s} {'total ms':>10s} {'avg ms':>9s} {'min ms':>9s} {'max ms':>9s} {'%':>6s}")
