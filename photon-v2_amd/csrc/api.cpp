@@ -83,6 +83,7 @@ struct Ctx {
     DevBuf<phg_ray> ext0, ext1, shadow;
     DevBuf<uint32_t> ext_pid0, ext_pid1, shadow_pid;
     DevBuf<uint32_t> counts;
+    DevBuf<uint32_t> fetch;       // region counters of the traversal kernels' work distribution
     DevBuf<uint32_t> path_seg;   // per path: its pass segment
     DevBuf<JobDev> jobs;         // job table of the current render
     DevBuf<PassParams> d_pass;   // pass parameters and segments, rewritten before every pass
@@ -311,7 +312,7 @@ phg_status phg_destroy(void* ctx) {
         c->shadow_c.free_(); c->ext0.free_(); c->ext1.free_(); c->shadow.free_(); c->ext_pid0.free_(); c->ext_pid1.free_();
         c->shadow_pid.free_(); c->path_seg.free_(); c->jobs.free_(); c->d_pass.free_(); c->d_segs.free_(); c->bfilm.free_();
         if (c->h_pass) (void)hipHostFree(c->h_pass);
-        c->spill.free_(); c->counts.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
+        c->spill.free_(); c->counts.free_(); c->fetch.free_(); c->tcount.free_(); c->errflags.free_(); c->q_rays.free_(); c->q_hits.free_();
         c->q_nvis.free_(); c->q_ntri.free_();
         c->g_params.free_(); c->g_m.free_(); c->g_v.free_(); c->g_grads.free_(); c->g_bias.free_(); c->g_wt.free_();
         c->g_ws_x.free_(); c->g_ws_dz0.free_(); c->g_ws_dz1.free_(); c->g_ws_part.free_();
@@ -788,6 +789,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     PHG_HIP(c, c->ext_pid0.ensure(cap)); PHG_HIP(c, c->ext_pid1.ensure(cap)); PHG_HIP(c, c->shadow_pid.ensure(cap));
     PHG_HIP(c, c->path_seg.ensure(cap));
     PHG_HIP(c, c->counts.ensure(2 * kCountSlots + 1));   // + the bounce base of captured chunks
+    PHG_HIP(c, c->fetch.ensure(kFetchWords));
     PHG_HIP(c, c->tcount.ensure(8));
     PHG_HIP(c, c->errflags.ensure(1));
     PHG_HIP(c, c->jobs.ensure(jobs.size()));
@@ -803,6 +805,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     PHG_HIP(c, hipMemsetAsync(c->tcount.p, 0, 8 * sizeof(unsigned long long), c->stream));
     PHG_HIP(c, hipMemsetAsync(c->errflags.p, 0, sizeof(uint32_t), c->stream));
     PHG_HIP(c, launch_clear_counts(c->counts.p, 2 * kCountSlots, c->stream));
+    PHG_HIP(c, launch_clear_counts(c->fetch.p, kFetchWords, c->stream));
     SceneDev sc = c->sc;
     sc.jobs = c->jobs.p;
     PathBuffers pb{};
@@ -812,6 +815,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = film;
     pb.path_seg = c->path_seg.p;
     pb.bounce_base = c->counts.p + 2 * kCountSlots;
+    pb.fetch = c->fetch.p;
     {
         const phg_status st = ensure_spill(c);
         if (st != PHG_OK) return st;

@@ -143,6 +143,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         pb.counts[0] = pp.paths;
         pb.bounce_base[0] = 0u;
     }
+    if (blockIdx.x == 0 && threadIdx.x < 8u) pb.fetch[fetch_slot(false, 0u) + threadIdx.x * kFetchStride] = 0u;
     for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < pp.paths; i += stride) {
         const uint32_t si = find_seg(pp.segs, pp.num_segs, i);
         const PassSeg sg = pp.segs[si];
@@ -179,52 +180,139 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
     }
 }
 
-// Every ray of a path queue, grid-stride over the lanes of the persistent grid: a lane starts
-// its next ray as soon as its current one finishes.  Queue rays are (origin, root node bits),
+// Every ray of a path queue, traced by the lanes of the persistent grid; a lane starts its
+// next ray as soon as its current one finishes.  Queue rays are (origin, root node bits),
 // (direction, tmax) with tmin = 0; done(i, result) consumes each finished ray.
+//
+// Two schedules.  A plain grid-stride split (lane l of wave w takes rays k S + 64 w + l) left
+// most of every launch's tail to a few waves: the sequencer favours older waves, so the
+// younger ones of each CU finished up to 2x later, and with S a multiple of the image's tile
+// rows each wave of the camera bounce saw the same tile columns every round.
+//  - coherent (the camera rays, queued in 8 x 8 pixel tiles): static rounds, wave w taking the
+//    64-ray group (w + 7 k) mod W of round k, so that a wave's rounds walk across the image;
+//    a wave's 64 lanes trace one tile together, which keeps their node fetches shared.
+//  - otherwise dynamic: the queue is split into 8 regions, one per XCD (blocks are dispatched
+//    to the XCDs round robin); a wave takes chunks of kFetchChunk consecutive rays from its
+//    XCD's region counter (one atomic per chunk) and gives them to its idle lanes in lane
+//    order; once its region is drained it takes chunks from the next kFetchSteal regions, then
+//    stops: every region is drained by its own XCD's waves, so stealing only evens out the
+//    end.  fetch: 8 region counters kFetchStride words apart (one 128-byte line each: atomics
+//    on one line serialize), zero at launch.
+// Which lane traces a ray does not change its result.
+constexpr uint32_t kFetchChunk = 256;
+constexpr uint32_t kFetchSteal = 2;
+constexpr uint32_t kRoundRotate = 7;
+static_assert(kFetchChunk % 64 == 0, "chunks keep 64-ray groups aligned");
+
 template <int RING, uint32_t BLOCK, bool ANY, bool COUNT, typename Done>
 __device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* __restrict__ queue, uint32_t n,
-                                            int32_t* stk, int32_t* spill, uint32_t spill_cap, uint32_t* nvis,
-                                            uint32_t* ntri, uint32_t* ovf, Done done) {
-    const uint32_t stride = gridDim.x * BLOCK;
-    uint32_t i = xcd_block_id() * BLOCK + threadIdx.x;
+                                            bool coherent, uint32_t* __restrict__ fetch, int32_t* stk,
+                                            int32_t* spill, uint32_t spill_cap, uint32_t* nvis, uint32_t* ntri,
+                                            uint32_t* ovf, Done done) {
     Traversal<RING, ANY, COUNT> tv;
     tv.attach(stk, BLOCK, spill, spill_cap);
+    const uint32_t lane = __lane_id();
+    uint32_t i = 0u;
     auto start = [&](uint32_t k) {
+        i = k;
         const float4* rp = reinterpret_cast<const float4*>(queue + k);
         const float4 r0 = rp[0], r1 = rp[1];
         tv.begin(mk(r0.x, r0.y, r0.z), mk(r1.x, r1.y, r1.z), 0.0f, r1.w, (int32_t)__float_as_uint(r0.w));
     };
-    if (i < n) start(i);
-    while (i < n) {
+    if (coherent) {
+        const uint32_t stride = gridDim.x * BLOCK, nw = stride / 64u, rot = kRoundRotate % nw;
+        uint32_t group = (xcd_block_id() * BLOCK + threadIdx.x) / 64u;
+        uint64_t base = 0u;
+        if ((uint64_t)group * 64u + lane >= n) return;
+        start(group * 64u + lane);
+        while (true) {
+            if (!tv.step(sc.nodes, sc.tris, nvis, ntri, ovf)) continue;
+            done(i, tv.r);
+            base += stride;
+            group += rot;
+            if (group >= nw) group -= nw;
+            const uint64_t k = base + (uint64_t)group * 64u + lane;
+            if (k >= n) break;
+            start((uint32_t)k);
+        }
+        return;
+    }
+    const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
+    // regions of whole 64-ray groups; region r is [r rsz, min(n, (r + 1) rsz))
+    const uint32_t rsz = ((n + 7u) / 8u + 63u) & ~63u;
+    // wave-uniform: the region being drained, the regions still to try, the wave's chunk
+    // (a grid of fewer than 8 blocks leaves regions without home waves: its waves try them all)
+    uint32_t reg = blockIdx.x & 7u, left = gridDim.x >= 8u ? 1u + kFetchSteal : 8u, cnext = 0u, cend = 0u;
+    bool has = false;
+    while (true) {
+        const uint64_t need = __ballot(!has);
+        if (need != 0ull) {
+            uint32_t m = (uint32_t)__popcll(need);
+            uint32_t rank = (uint32_t)__popcll(need & below);
+            while (m != 0u && left != 0u) {
+                if (cnext == cend) {
+                    const uint32_t first = (uint32_t)__ffsll((long long)__ballot(1)) - 1u;
+                    uint32_t got = 0u;
+                    if (lane == first) got = atomicAdd(&fetch[reg * kFetchStride], kFetchChunk);
+                    got = __builtin_amdgcn_readfirstlane(got);
+                    const uint32_t rbeg = reg * rsz;
+                    const uint32_t rend = rbeg + rsz < n ? rbeg + rsz : n;
+                    if (rbeg >= rend || got >= rend - rbeg) {
+                        reg = (reg + 1u) & 7u;
+                        --left;
+                        continue;
+                    }
+                    cnext = rbeg + got;
+                    cend = rend - cnext < kFetchChunk ? rend : cnext + kFetchChunk;
+                }
+                const uint32_t take = cend - cnext < m ? cend - cnext : m;
+                if (!has) {
+                    if (rank < take) {
+                        start(cnext + rank);
+                        has = true;
+                    } else {
+                        rank -= take;
+                    }
+                }
+                cnext += take;
+                m -= take;
+            }
+        }
+        if (!has) break;
         if (!tv.step(sc.nodes, sc.tris, nvis, ntri, ovf)) continue;
         done(i, tv.r);
-        i += stride;
-        if (i < n) start(i);
+        has = false;
     }
 }
 
+// The traversal kernels keep 7 waves per SIMD (at most 72 VGPRs): their persistent grids hold
+// 7 waves per SIMD, and a wave fewer measured 2-4% slower.
 template <int RING, uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bk, uint32_t count_traversal) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_extend(SceneDev sc, PathBuffers pb, uint32_t bk, uint32_t count_traversal) {
     const uint32_t bounce = pb.bounce_base[0] + bk;
     __shared__ int32_t stack_mem[RING * BLOCK];
     int32_t* spill = pb.spill + (size_t)(xcd_block_id() * BLOCK + threadIdx.x) * pb.spill_cap;
     const uint32_t n = pb.counts[bounce % kCountSlots];
-    // the counters the shade stage of this bounce pushes to start from zero
+    // the counters the shade stage of this bounce pushes to start from zero, and the region
+    // counters of this bounce's shadow rays and of the next bounce's extension rays
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         pb.counts[(bounce + 1) % kCountSlots] = 0u;
         pb.counts[kCountSlots + bounce % kCountSlots] = 0u;
         if (bounce > 0) atomicAdd(&pb.traversal_counters[4], (unsigned long long)n);
     }
+    if (blockIdx.x == 0 && threadIdx.x < 8u) {
+        pb.fetch[fetch_slot(false, bounce + 1u) + threadIdx.x * kFetchStride] = 0u;
+        pb.fetch[fetch_slot(true, bounce) + threadIdx.x * kFetchStride] = 0u;
+    }
     const uint32_t q = bounce & 1u;
     uint32_t nvis = 0, ntri = 0, ovf = 0;
     if (count_traversal)
-        trace_queue<RING, BLOCK, false, true>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, true>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(false, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        [&](uint32_t i, const TraceResult& tr) {
                                            pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                        });
     else
-        trace_queue<RING, BLOCK, false, false>(sc, pb.ext[q], n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, false>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(false, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                         [&](uint32_t i, const TraceResult& tr) {
                                             pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                         });
@@ -236,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(SceneDev sc, PathBuffers pb, u
 }
 
 template <int RING, uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_shadow(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb,
                                                   uint32_t bk, uint32_t count_traversal) {
     const uint32_t bounce = pb.bounce_base[0] + bk;
     const uint32_t rec_bounces = ppd->record ? ppd->rec_bounces : 0u, npaths = ppd->paths;
@@ -262,10 +350,10 @@ __global__ __launch_bounds__(BLOCK) void k_shadow(SceneDev sc, const PassParams*
         }
     };
     if (count_traversal)
-        trace_queue<RING, BLOCK, true, true>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, true>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(true, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                       unoccluded);
     else
-        trace_queue<RING, BLOCK, true, false>(sc, pb.shadow, n, stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, false>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(true, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        unoccluded);
     if (count_traversal) {
         atomicAdd(&pb.traversal_counters[2], (unsigned long long)nvis);
