@@ -114,16 +114,17 @@ struct PathBuffers {
     uint32_t* path_seg;      // per path: its pass segment
     uint32_t* bounce_base;   // first bounce of the running chunk of bounces (the stage kernels
                              // take their bounce as bounce_base[0] + their chunk offset)
-    uint32_t* fetch;         // work counters of the traversal kernels (fetch_slot), kFetchWords
+    uint32_t* fetch;         // work counters of the stage kernels (fetch_slot), kFetchWords
 };
 
-// Region counters of the dynamic work distribution of the traversal kernels: 8 per queue
-// (extension, shadow) and bounce, in a ring of kCountSlots bounces, each counter on a
-// 128-byte line of its own (kFetchStride words apart).
+// Region counters of the dynamic work distribution of the stage kernels: 8 per queue pass
+// (extension-ray traversal, shadow-ray traversal, shading) and bounce, in a ring of
+// kCountSlots bounces, each counter on a 128-byte line of its own (kFetchStride words apart).
+enum : uint32_t { kFetchExtend = 0, kFetchShadow = 1, kFetchShade = 2, kFetchKinds = 3 };
 constexpr uint32_t kFetchStride = 32;
-constexpr uint32_t kFetchWords = 2u * kCountSlots * 8u * kFetchStride;
-__host__ __device__ inline uint32_t fetch_slot(bool shadow, uint32_t bounce) {
-    return ((shadow ? kCountSlots : 0u) + bounce % kCountSlots) * 8u * kFetchStride;
+constexpr uint32_t kFetchWords = kFetchKinds * kCountSlots * 8u * kFetchStride;
+__host__ __device__ inline uint32_t fetch_slot(uint32_t kind, uint32_t bounce) {
+    return (kind * kCountSlots + bounce % kCountSlots) * 8u * kFetchStride;
 }
 
 // Size of the device scratch array of the allocation-free ray query; its address on `device`

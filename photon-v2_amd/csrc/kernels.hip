@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         pb.counts[0] = pp.paths;
         pb.bounce_base[0] = 0u;
     }
-    if (blockIdx.x == 0 && threadIdx.x < 8u) pb.fetch[fetch_slot(false, 0u) + threadIdx.x * kFetchStride] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 8u) pb.fetch[fetch_slot(kFetchExtend, 0u) + threadIdx.x * kFetchStride] = 0u;
     for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < pp.paths; i += stride) {
         const uint32_t si = find_seg(pp.segs, pp.num_segs, i);
         const PassSeg sg = pp.segs[si];
@@ -301,18 +301,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
         if (bounce > 0) atomicAdd(&pb.traversal_counters[4], (unsigned long long)n);
     }
     if (blockIdx.x == 0 && threadIdx.x < 8u) {
-        pb.fetch[fetch_slot(false, bounce + 1u) + threadIdx.x * kFetchStride] = 0u;
-        pb.fetch[fetch_slot(true, bounce) + threadIdx.x * kFetchStride] = 0u;
+        pb.fetch[fetch_slot(kFetchExtend, bounce + 1u) + threadIdx.x * kFetchStride] = 0u;
+        pb.fetch[fetch_slot(kFetchShadow, bounce) + threadIdx.x * kFetchStride] = 0u;
+        pb.fetch[fetch_slot(kFetchShade, bounce) + threadIdx.x * kFetchStride] = 0u;
     }
     const uint32_t q = bounce & 1u;
     uint32_t nvis = 0, ntri = 0, ovf = 0;
     if (count_traversal)
-        trace_queue<RING, BLOCK, false, true>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(false, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, true>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(kFetchExtend, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        [&](uint32_t i, const TraceResult& tr) {
                                            pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                        });
     else
-        trace_queue<RING, BLOCK, false, false>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(false, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, false>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(kFetchExtend, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                         [&](uint32_t i, const TraceResult& tr) {
                                             pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                         });
@@ -350,10 +351,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
         }
     };
     if (count_traversal)
-        trace_queue<RING, BLOCK, true, true>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(true, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, true>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(kFetchShadow, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                       unoccluded);
     else
-        trace_queue<RING, BLOCK, true, false>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(true, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, false>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(kFetchShadow, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        unoccluded);
     if (count_traversal) {
         atomicAdd(&pb.traversal_counters[2], (unsigned long long)nvis);
@@ -382,9 +383,30 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
     const PassParams& pp = *ppd;   // fields read where used (uniform loads; no registers held)
     const uint32_t n = pb.counts[bounce % kCountSlots];
     const uint32_t qin = bounce & 1u, qout = qin ^ 1u;
-    const uint32_t stride = gridDim.x * kShadeBlock;
-    const uint32_t iend = (n + 63u) & ~63u;
-    for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < iend; i += stride) {
+    // 64-vertex groups handed out dynamically, as the traversal kernels' incoherent queues:
+    // chunks of kFetchChunk vertices from the XCD's region of the queue, then from the next
+    // kFetchSteal regions (wave-uniform state; one atomic per chunk)
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t rsz = ((n + 7u) / 8u + 63u) & ~63u;
+    uint32_t* fetch = pb.fetch + fetch_slot(kFetchShade, bounce);
+    uint32_t reg = blockIdx.x & 7u, left = gridDim.x >= 8u ? 1u + kFetchSteal : 8u, cnext = 0u, cend = 0u;
+    while (left != 0u) {
+        if (cnext == cend) {
+            uint32_t got = 0u;
+            if (lane == 0u) got = atomicAdd(&fetch[reg * kFetchStride], kFetchChunk);
+            got = __builtin_amdgcn_readfirstlane(got);
+            const uint32_t rbeg = reg * rsz;
+            const uint32_t rend = rbeg + rsz < n ? rbeg + rsz : n;
+            if (rbeg >= rend || got >= rend - rbeg) {
+                reg = (reg + 1u) & 7u;
+                --left;
+                continue;
+            }
+            cnext = rbeg + got;
+            cend = rend - cnext < kFetchChunk ? rend : cnext + kFetchChunk;
+        }
+        const uint32_t i = cnext + lane;
+        cnext = cend - cnext < 64u ? cend : cnext + 64u;
         if (i < n) {
             const uint32_t pid = pb.ext_pid[qin][i];
             const float4 h = pb.hits[i];
