@@ -140,6 +140,15 @@ enum { PHG_ESTIMATOR_BNEEPT = 0, PHG_ESTIMATOR_BVPT = 1, PHG_ESTIMATOR_BVPTDL = 
  * filter the film's sums are weighted: sum w r, g, b, sum w, sum w r^2, g^2, b^2. */
 enum { PHG_FILTER_BOX = 0, PHG_FILTER_GAUSSIAN = 1, PHG_FILTER_MN = 2, PHG_FILTER_BH = 3 };
 
+/* Sample source of a job (EngineEnv/SampleSource: sample-source(uniform-random | stratified)).
+ * INDEPENDENT: every path draws its dimensions from its own PCG32 stream
+ * (SGUniformRandom). STRATIFIED: SGStratified (Core/SampleGenerator/SGStratified.cpp:16-154):
+ * sample s of every pixel forms one batch, and the five dimensions the estimator reads first at
+ * the first surface are stratified across the batch's pixels (2-D, 2-D and 1-D jittered strata
+ * in keyed shuffled order); the raster offset and all later dimensions are as INDEPENDENT.
+ * Per-pixel distributions are the same; the pixels of one batch are negatively correlated. */
+enum { PHG_SAMPLER_INDEPENDENT = 0, PHG_SAMPLER_STRATIFIED = 1 };
+
 typedef struct phg_render_opts {
     uint32_t width, height;
     uint32_t spp;              /* samples per pixel rendered by this call */
@@ -156,6 +165,7 @@ typedef struct phg_render_opts {
     uint32_t filter;           /* PHG_FILTER_*: film reconstruction filter (SampleFilter.cpp:8-56) */
     uint32_t guide_depth;      /* PHG_RENDER_GUIDED: path vertices (from the first hit) that sample the lobes; 0 = all.
                                   PHG_RENDER_TRAIN keeps training records at those vertices only */
+    uint32_t sampler;          /* PHG_SAMPLER_* (ReceiverSamplingWork.cpp:76-84 with the scene's sample source) */
 } phg_render_opts;
 
 enum {
@@ -275,7 +285,8 @@ typedef struct phg_batch_job {
     uint32_t estimator;        /* PHG_ESTIMATOR_* */
     phg_camera camera;
     uint32_t filter;           /* PHG_FILTER_* */
-    uint32_t pad2[3];
+    uint32_t sampler;          /* PHG_SAMPLER_* */
+    uint32_t pad2[2];
 } phg_batch_job;
 
 enum { PHG_JOB_CAMERA = 1 };
@@ -424,7 +435,7 @@ typedef struct phg_p2_settings {
     uint32_t max_bounces;      /* visualizer max-path-length if given, else 10000 (no cap) */
     uint32_t filter;           /* PHG_FILTER_*; the GPU film accumulates with a box filter */
     uint32_t warnings;         /* skipped commands / fields / approximations */
-    uint32_t pad;
+    uint32_t sampler;          /* PHG_SAMPLER_* of the sample-source */
 } phg_p2_settings;
 
 phg_status phg_scene_parse_p2(const char* text, phg_p2_settings* settings, phg_scene_host** out, char* err,

@@ -904,11 +904,83 @@ static void camera_ray(const orc_camera& c, uint32_t W, uint32_t H, float rx, fl
 }
 
 // ---------------------------------------------------------------------------------------
+// Stratified sample source (SGStratified.cpp:16-154), as this renderer defines it: sample s of
+// every pixel is one batch; the first vertex's dimensions u[0], u[2] (2-D stage A), u[3], u[4]
+// (2-D stage B) and u[5] (1-D stage C) are jittered strata over the batch's N pixels -- k x k
+// with k = floor(sqrt N), x fastest, whole passes then uniform (SGStratified.cpp:42-90,
+// 112-152), N strata in 1-D (SGStratified.cpp:24-40) -- in the order of a keyed bijection of the
+// pixels that stands for the reference's shuffle (SamplesND.h:101-113); each stratum's jitter is
+// the path's own stream value.
+// ---------------------------------------------------------------------------------------
+struct Strat {
+    bool on = false;
+    uint32_t pixel = 0, npix = 0, sample = 0;
+    uint64_t seed = 0;
+};
+
+// 8-round Feistel network over the smallest even bit count holding n, round function
+// moremur(key ^ (round << 32) ^ right half), repeated until the value lands below n, then a
+// cyclic shift by floor(n * (upper 32 bits of moremur(key ^ 0xD1B54A32D192ED03)) / 2^32)
+static uint32_t strat_perm(uint32_t i, uint32_t n, uint64_t key) {
+    if (n < 2) return 0;
+    uint32_t nb = 0;
+    for (uint32_t v = n - 1; v != 0; v >>= 1) ++nb;
+    if (nb % 2) ++nb;
+    const uint32_t h = nb / 2, lo_mask = (1u << h) - 1u;
+    uint32_t v = i;
+    for (;;) {
+        uint32_t left = v >> h, right = v & lo_mask;
+        for (uint32_t round = 0; round < 8; ++round) {
+            const uint32_t f = (uint32_t)moremur(key ^ ((uint64_t)round << 32) ^ right) & lo_mask;
+            const uint32_t t = left ^ f;
+            left = right;
+            right = t;
+        }
+        v = (left << h) | right;
+        if (v < n) break;
+    }
+    const uint64_t shift = ((moremur(key ^ 0xD1B54A32D192ED03ULL) >> 32) * (uint64_t)n) >> 32;
+    return (uint32_t)((v + shift) % n);
+}
+
+static uint64_t strat_stage_key(uint64_t seed, uint32_t sample, uint32_t stage) {
+    return moremur((seed << 40) ^ ((uint64_t)sample << 2) ^ (uint64_t)stage ^ 0x9E3779B97F4A7C15ULL);
+}
+
+static float strat_cell(uint32_t cell, uint32_t count, float jitter) {
+    const float v = (float)(((double)cell + (double)jitter) / (double)count);
+    return v < 1.0f ? v : std::nextafter(1.0f, 0.0f);
+}
+
+static uint32_t floor_sqrt(uint32_t n) {
+    uint64_t lo = 0, hi = 65536;   // lo^2 <= n < hi^2
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (mid * mid <= n) lo = mid; else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+static void strat_apply(const Strat& t, float* u) {
+    const uint32_t k = floor_sqrt(t.npix), cells = k * k, whole = (t.npix / cells) * cells;
+    const int dims[2][2] = {{0, 2}, {3, 4}};
+    for (uint32_t stage = 0; stage < 2; ++stage) {
+        const uint32_t r = strat_perm(t.pixel, t.npix, strat_stage_key(t.seed, t.sample, stage));
+        if (r >= whole) continue;   // the remainder after whole passes is uniform
+        const uint32_t cell = r % cells;
+        u[dims[stage][0]] = strat_cell(cell % k, k, u[dims[stage][0]]);
+        u[dims[stage][1]] = strat_cell(cell / k, k, u[dims[stage][1]]);
+    }
+    u[5] = strat_cell(strat_perm(t.pixel, t.npix, strat_stage_key(t.seed, t.sample, 2)), t.npix, u[5]);
+}
+
+// ---------------------------------------------------------------------------------------
 // BNEEPTEstimator::estimate (BNEEPTEstimator.cpp:38-262) with a bounce cap.
 // ---------------------------------------------------------------------------------------
 struct PathStats { uint64_t rays_primary = 0, rays_ext = 0, rays_shadow = 0; };
 
-static RGB estimate(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, PathStats* st) {
+static RGB estimate(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, PathStats* st,
+                    const Strat& strat) {
     RGB energy = rgb(0), T = rgb(1);
     float rrScale = 1.0f;
     Hit h;
@@ -926,6 +998,7 @@ static RGB estimate(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bou
         const orc_material& m = S.s->materials[S.s->tri_material[X.tri]];
         float u[8];
         for (int k = 0; k < 8; ++k) u[k] = sf.u();
+        if (nb == 0 && strat.on) strat_apply(strat, u);
         if (m.type == ORC_MAT_ABSORBER) break;  // absorber: no BSDF, path ends
         const V3 V = curDir * -1.0f;
         // NEE with MIS
@@ -993,7 +1066,8 @@ static RGB estimate(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bou
 // that extends the path, throughput update, Russian roulette from the 4th vertex on.  The
 // bounce cap is the loop bound.  Random numbers per vertex as in estimate(): 8, of which the
 // BSDF sample uses u[4..6] and Russian roulette u[7].
-static RGB estimate_bvpt(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, PathStats* st) {
+static RGB estimate_bvpt(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, PathStats* st,
+                         const Strat& strat) {
     RGB energy = rgb(0), T = rgb(1);
     float rrScale = 1.0f;
     Hit h;
@@ -1012,6 +1086,7 @@ static RGB estimate_bvpt(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t ma
         const orc_material& m = S.s->materials[S.s->tri_material[X.tri]];
         float u[8];
         for (int k = 0; k < 8; ++k) u[k] = sf.u();
+        if (nb == 0 && strat.on) strat_apply(strat, u);
         if (m.type == ORC_MAT_ABSORBER) break;
         const V3 V = curDir * -1.0f;
         const BsdfSample bs = bsdf_sample(m, X, V, u + 4);
@@ -1041,10 +1116,17 @@ static RGB estimate_bvpt(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t ma
 // and emission found by one BSDF sample from the first hit — BVPT with a cap of one bounce
 // (no Russian roulette happens before the 4th vertex).
 static RGB estimate_any(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max_bounces, uint32_t estimator,
-                        PathStats* st) {
-    if (estimator == 1) return estimate_bvpt(S, org, dir, sf, max_bounces, st);
-    if (estimator == 2) return estimate_bvpt(S, org, dir, sf, std::min(max_bounces, 1u), st);
-    return estimate(S, org, dir, sf, max_bounces, st);
+                        PathStats* st, const Strat& strat) {
+    if (estimator == 1) return estimate_bvpt(S, org, dir, sf, max_bounces, st, strat);
+    if (estimator == 2) return estimate_bvpt(S, org, dir, sf, std::min(max_bounces, 1u), st, strat);
+    return estimate(S, org, dir, sf, max_bounces, st, strat);
+}
+
+static Strat strat_of(const orc_render_opts* opt, uint64_t pix, uint32_t sample, uint64_t npix) {
+    Strat t;
+    t.on = opt->sampler == 1;
+    t.pixel = (uint32_t)pix; t.npix = (uint32_t)npix; t.sample = sample; t.seed = opt->seed;
+    return t;
 }
 
 }  // namespace orc
@@ -1066,6 +1148,22 @@ void orc_pcg32_generate(uint64_t seq, uint64_t init_state, int use_default, uint
 }
 
 uint64_t orc_moremur(uint64_t v) { return moremur(v); }
+
+// The stratified source's keyed bijection, and the first-vertex dimensions of one batch:
+// jitter5 holds the stream values u0, u2, u3, u4, u5 of every pixel; out5 receives them
+// stratified.
+uint32_t orc_strat_perm(uint32_t i, uint32_t n, uint64_t key) { return strat_perm(i, n, key); }
+uint64_t orc_strat_key(uint64_t seed, uint32_t sample, uint32_t stage) { return strat_stage_key(seed, sample, stage); }
+void orc_strat_batch(uint32_t npix, uint64_t seed, uint32_t sample, const float* jitter5, float* out5) {
+    for (uint32_t p = 0; p < npix; ++p) {
+        float u[8] = {jitter5[5 * p], 0.5f, jitter5[5 * p + 1], jitter5[5 * p + 2], jitter5[5 * p + 3], jitter5[5 * p + 4],
+                      0.5f, 0.5f};
+        Strat t; t.on = true; t.pixel = p; t.npix = npix; t.sample = sample; t.seed = seed;
+        strat_apply(t, u);
+        out5[5 * p] = u[0]; out5[5 * p + 1] = u[2]; out5[5 * p + 2] = u[3]; out5[5 * p + 3] = u[4];
+        out5[5 * p + 4] = u[5];
+    }
+}
 
 // Test-only: plant one of the mistakes listed at g_fault (0 restores the restatement).
 void orc_set_fault(int fault) { orc::g_fault = fault; }
@@ -1340,7 +1438,8 @@ int orc_render(const orc_scene* scene, const orc_render_opts* opt, double* film,
                         const float ry = (float)y + sf.u();
                         V3 o, d;
                         camera_ray(scene->camera, W, H, rx, ry, &o, &d);
-                        const RGB L = estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st);
+                        const RGB L = estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st,
+                                                   strat_of(opt, pix, s, npix));
                         f[0] += L.r; f[1] += L.g; f[2] += L.b; f[3] += 1.0;
                         f[4] += (double)L.r * L.r; f[5] += (double)L.g * L.g; f[6] += (double)L.b * L.b;
                         n_paths.fetch_add(1, std::memory_order_relaxed);
@@ -1376,7 +1475,8 @@ void orc_render_paths(const orc_scene* scene, const orc_render_opts* opt, const 
         const float ry = (float)(pix[i] / W) + sf.u();
         V3 o, d;
         camera_ray(scene->camera, W, H, rx, ry, &o, &d);
-        const RGB L = estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st);
+        const RGB L = estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st,
+                                   strat_of(opt, pix[i], smp[i], npix));
         out_rgb[3 * i] = L.r; out_rgb[3 * i + 1] = L.g; out_rgb[3 * i + 2] = L.b;
     }
 }
@@ -1476,7 +1576,8 @@ int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int 
                 if (hs != 0.5) { xr = window_raster(x, ux, W, hs); yr = window_raster(y, uy, H, hs); }
                 orc::V3 o, d;
                 orc::camera_ray(scene->camera, W, H, (float)xr, (float)yr, &o, &d);
-                const orc::RGB L = orc::estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st);
+                const orc::RGB L = orc::estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st,
+                                                     orc::strat_of(opt, pix, opt->spp_begin + k, npix));
                 xy[2 * i] = xr; xy[2 * i + 1] = yr;
                 rgb[3 * i] = L.r; rgb[3 * i + 1] = L.g; rgb[3 * i + 2] = L.b;
             }

@@ -44,7 +44,7 @@ class orc_scene(C.Structure):
 class orc_render_opts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp_begin", C.c_uint32), ("spp", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("nthreads", C.c_uint32), ("seed", C.c_uint64),
-                ("row_step", C.c_uint32), ("estimator", C.c_uint32)]
+                ("row_step", C.c_uint32), ("estimator", C.c_uint32), ("sampler", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class orc_render_stats(C.Structure):
@@ -169,40 +169,66 @@ FILTER_IDS = {0: 0, 1: 1, 2: 4, 3: 2}   # PHG_FILTER_* (box, gaussian, mn, bh) -
 
 
 def render_filtered(scene, width, height, spp, filter, max_bounces=10000, seed=0, spp_begin=0, nthreads=None,
-                    estimator=0):
+                    estimator=0, sampler=0):
     """Weighted FP64 film (H, W, 8) of an oracle render splatted with a PHG_FILTER_* filter."""
     L = lib()
     sv = SceneView(scene)
     film = np.zeros((height, width, 8), np.float64)
     o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, 1,
-                        estimator)
+                        estimator, sampler)
     L.orc_render_filtered(C.byref(sv.s), C.byref(o), FILTER_IDS[filter], film.ctypes.data)
     return film
 
 
 def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_step=1,
-           estimator=0, native=False):
-    """FP64 film (H, W, 8) of the oracle render (estimator 0 BNEEPT, 1 BVPT, 2 BVPTDL), plus
-    ray counts.  native=True runs the -O3 -march=native build."""
+           estimator=0, native=False, sampler=0):
+    """FP64 film (H, W, 8) of the oracle render (estimator 0 BNEEPT, 1 BVPT, 2 BVPTDL; sampler 0
+    independent, 1 stratified), plus ray counts.  native=True runs the -O3 -march=native build."""
     L = lib(native)
     sv = SceneView(scene)
     film = np.zeros((height, width, 8), np.float64)
     o = orc_render_opts(width, height, spp_begin, spp, max_bounces, nthreads or os.cpu_count() or 1, seed, row_step,
-                        estimator)
+                        estimator, sampler)
     st = orc_render_stats()
     L.orc_render(C.byref(sv.s), C.byref(o), film.ctypes.data, C.byref(st))
     return film, {"rays_primary": st.rays_primary, "rays_extension": st.rays_extension, "rays_shadow": st.rays_shadow,
                   "paths": st.paths, "render_ms": st.render_ms}
 
 
-def render_paths(scene, width, height, pixels, samples, max_bounces=10000, seed=0):
+def render_paths(scene, width, height, pixels, samples, max_bounces=10000, seed=0, sampler=0):
     L = lib()
     sv = SceneView(scene)
     pix = np.ascontiguousarray(pixels, np.uint32)
     smp = np.ascontiguousarray(samples, np.uint32)
     out = np.zeros((len(pix), 3), np.float32)
-    o = orc_render_opts(width, height, 0, 0, max_bounces, 1, seed, 0, 0)
+    o = orc_render_opts(width, height, 0, 0, max_bounces, 1, seed, 0, 0, sampler)
     L.orc_render_paths(C.byref(sv.s), C.byref(o), pix.ctypes.data, smp.ctypes.data, len(pix), out.ctypes.data)
+    return out
+
+
+def strat_perm(i, n, key):
+    """The stratified source's keyed bijection of [0, n)."""
+    L = lib()
+    L.orc_strat_perm.restype = C.c_uint32
+    L.orc_strat_perm.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64]
+    return L.orc_strat_perm(i, n, key)
+
+
+def strat_key(seed, sample, stage):
+    L = lib()
+    L.orc_strat_key.restype = C.c_uint64
+    L.orc_strat_key.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32]
+    return L.orc_strat_key(seed, sample, stage)
+
+
+def strat_batch(npix, seed, sample, jitter5):
+    """First-vertex dimensions (u0, u2, u3, u4, u5) of every pixel of one stratified batch, from
+    the pixels' stream values jitter5 (npix, 5)."""
+    L = lib()
+    j = np.ascontiguousarray(jitter5, np.float32).reshape(npix, 5)
+    out = np.zeros((npix, 5), np.float32)
+    L.orc_strat_batch.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.orc_strat_batch(npix, seed, sample, j.ctypes.data, out.ctypes.data)
     return out
 
 

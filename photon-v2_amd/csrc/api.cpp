@@ -1101,9 +1101,10 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
 }
 
 static JobDev make_job(const SceneInfo& si, uint32_t width, uint32_t height, uint32_t max_bounces, uint64_t seed,
-                       uint64_t film_base, uint32_t estimator, uint32_t filter) {
+                       uint64_t film_base, uint32_t estimator, uint32_t filter, uint32_t sampler) {
     JobDev j{};
     j.filter = filter;
+    j.sampler = sampler;
     // BVPTDL is BVPT capped at one bounce
     j.estimator = estimator == PHG_ESTIMATOR_BNEEPT ? PHG_ESTIMATOR_BNEEPT : PHG_ESTIMATOR_BVPT;
     if (estimator == PHG_ESTIMATOR_BVPTDL) max_bounces = std::min(max_bounces, 1u);
@@ -1147,8 +1148,9 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
         if (o->estimator != PHG_ESTIMATOR_BNEEPT && (o->flags & (PHG_RENDER_GUIDED | PHG_RENDER_TRAIN)))
             return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
         if (o->filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown filter");
+        if (o->sampler > PHG_SAMPLER_STRATIFIED) return set_err(c, PHG_ERR_INVALID_ARG, "unknown sampler");
         std::vector<JobDev> jobs{
-            make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0, o->estimator, o->filter)};
+            make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0, o->estimator, o->filter, o->sampler)};
         std::vector<PassPlan> passes;
         for (uint32_t s0 = 0; s0 < o->spp; s0 += k) {
             const uint32_t kk = std::min(k, o->spp - s0);
@@ -1194,8 +1196,9 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
             if (b.spp == 0 || (uint64_t)b.spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "job spp out of range");
             if (b.estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job estimator");
             if (b.filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job filter");
+            if (b.sampler > PHG_SAMPLER_STRATIFIED) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job sampler");
             jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles, b.estimator,
-                               b.filter);
+                               b.filter, b.sampler);
             if (b.flags & PHG_JOB_CAMERA) jobs[j].cam = b.camera;
             film_doubles += 8 * npix;
         }

@@ -495,7 +495,16 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                 // the vertex's random numbers, drawn in stream order where they are first
                 // needed (NEE: 4, then BSDF: 3 + Russian roulette: 1, + 4 when guided), so that
                 // no more of them are live at once than the stage uses
-                const float u0 = s.u(), u1 = s.u(), u2 = s.u(), u3 = s.u();
+                float u0 = s.u();
+                const float u1 = s.u();
+                float u2 = s.u(), u3 = s.u();
+                // the stratified source's first-vertex dimensions (stages (u0, u2), (u3, u4), u5)
+                const bool strat = bounce == 0u && job.sampler == PHG_SAMPLER_STRATIFIED;
+                if (strat) {
+                    u0 = strat_2d(pixel, job.npix, job.seed, sample, 0u, 0u, u0);
+                    u2 = strat_2d(pixel, job.npix, job.seed, sample, 0u, 1u, u2);
+                    u3 = strat_2d(pixel, job.npix, job.seed, sample, 1u, 0u, u3);
+                }
                 const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideOut : nullptr;
                 const bool guide_here = pp.guide_depth == 0 || bounce < pp.guide_depth;
                 const phg_material& m = sc.mats[mat];
@@ -556,7 +565,12 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                         }
                     }
                     // ---- BSDF sample + extension (BNEEPTEstimator.cpp:151-255)
-                    const float u4 = s.u(), u5 = s.u(), u6 = s.u(), u7 = s.u();
+                    float u4 = s.u(), u5 = s.u();
+                    const float u6 = s.u(), u7 = s.u();
+                    if (strat) {
+                        u4 = strat_2d(pixel, job.npix, job.seed, sample, 1u, 1u, u4);
+                        u5 = strat_1d(pixel, job.npix, job.seed, sample, u5);
+                    }
                     float u8 = 0.0f, u9 = 0.0f, u10 = 0.0f, u11 = 0.0f;
                     if (GUIDED) { u8 = s.u(); u9 = s.u(); u10 = s.u(); u11 = s.u(); }
                     bool ok = false;

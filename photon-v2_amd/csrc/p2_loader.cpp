@@ -537,11 +537,10 @@ struct Loader {
             if (!smp && r.category == "sample-source") smp = &r;
         }
         if (smp) st.spp = (uint32_t)std::max(1.0, real_or(*smp, "samples", 16));
-        // stratified and uniform-random sources give every pixel independent, uniformly
-        // distributed samples from batch to batch (SGStratified stratifies across the pixels of
-        // one batch, ReceiverSamplingWork.cpp:76-84), which is what the renderer draws; a
-        // Halton source places raster samples by a low-discrepancy sequence over the whole
-        // image (SGHalton.cpp) and is rendered with independent samples instead
+        // stratified (SGStratified) and uniform-random (SGUniformRandom) sources are rendered as
+        // such (PHG_SAMPLER_*); a Halton source places raster samples by a low-discrepancy
+        // sequence over the whole image (SGHalton.cpp) and is rendered with independent samples
+        st.sampler = smp && smp->type == "stratified" ? PHG_SAMPLER_STRATIFIED : PHG_SAMPLER_INDEPENDENT;
         if (smp && smp->type == "halton") warn();
         if (vis) {
             if (vis->type != "path-tracing") warn();

@@ -92,6 +92,82 @@ PHG_HD Stream stream_resume(uint64_t key, uint64_t state) {
     Stream s; s.state = state; s.inc = (key << 1u) | 1u; return s;
 }
 
+// ---- stratified sample source (Core/SampleGenerator/SGStratified.cpp:16-154) -------------
+// A batch of the reference's stratified source is one sample of every pixel: its raster stage
+// has one stratum per pixel (a uniform offset in the pixel, as the independent source draws),
+// and its 5-D ray stage -- the first five SampleFlow dimensions the estimator reads at the
+// first surface (ReceiverSamplingWork.cpp:76-84, SampleFlow.h:105-169) -- is generated as a
+// 2-D, a 2-D and a 1-D stage (SampleGenerator.cpp:81-135: SGStratified has no >= 3-D
+// generator), each a set of jittered strata handed to the batch's N samples in shuffled order
+// (SGStratified.cpp:24-90): k x k strata with k = floor(sqrt N), x fastest, in whole passes and
+// the rest uniform (reviseDimSizeHints, SGStratified.cpp:112-152), or N strata in 1-D.
+// Here the shuffle is strat_permute, a keyed bijection of the pixels (the reference shuffles
+// with its thread-local random source), and a jitter is the path's own stream value.  In this
+// renderer's stream order at the first vertex the stages are (u0 emitter pick, u2 light point
+// x), (u3 light point y, u4 BSDF direction x) and u5 (BSDF direction y); the triangle pick u1
+// is drawn from the reference's global random source (GroupedDiffuseSurfaceEmitter.cpp:68-69)
+// and stays independent, as do all later dimensions (SampleFlow.h:157-169).
+
+// Bijection of [0, n) keyed by key: an 8-round Feistel network on the smallest even number of
+// bits that holds n (round function: moremur of key, round and half), walked until the image
+// falls inside [0, n) (at most 4 steps on average), then rotated by a key-derived offset so
+// that every index's image is uniform over keys (fewer rounds leave pairs of images visibly
+// non-uniform on small n).
+PHG_HD uint32_t strat_permute(uint32_t i, uint32_t n, uint64_t key) {
+    if (n <= 1u) return 0u;
+    uint32_t bits = 0;
+    while (bits < 32u && ((n - 1u) >> bits) != 0u) ++bits;
+    bits += bits & 1u;
+    const uint32_t half = bits / 2u, mask = (1u << half) - 1u;
+    uint32_t x = i;
+    do {
+        uint32_t L = x >> half, R = x & mask;
+        for (uint32_t r = 0; r < 8u; ++r) {
+            const uint32_t F = (uint32_t)moremur(key ^ ((uint64_t)r << 32) ^ R) & mask;
+            const uint32_t nL = R;
+            R = L ^ F;
+            L = nL;
+        }
+        x = (L << half) | R;
+    } while (x >= n);
+    const uint32_t off = (uint32_t)(((moremur(key ^ 0xD1B54A32D192ED03ULL) >> 32) * (uint64_t)n) >> 32);
+    x += off;
+    return x >= n ? x - n : x;
+}
+
+// key of stratified stage `stage` (0, 1: the 2-D stages, 2: the 1-D stage) of sample batch
+// `sample` of a job with stream seed `seed` (< 2^24)
+PHG_HD uint64_t strat_key(uint64_t seed, uint32_t sample, uint32_t stage) {
+    return moremur((seed << 40) ^ ((uint64_t)sample << 2) ^ (uint64_t)stage ^ 0x9E3779B97F4A7C15ULL);
+}
+
+// (c + u) / k rounded to float, kept below 1 (u in (0, 1) is the stratum's jitter)
+PHG_HD float strat_value(uint32_t c, uint32_t k, float u) {
+    const float v = (float)(((double)c + (double)u) / (double)k);
+    return v < 1.0f ? v : 0x1.fffffep-1f;
+}
+
+PHG_HD uint32_t strat_isqrt(uint32_t n) {
+    uint32_t k = (uint32_t)sqrt((double)n);
+    while ((uint64_t)k * k > n) --k;
+    while ((uint64_t)(k + 1u) * (k + 1u) <= n) ++k;
+    return k;
+}
+
+// dimension `axis` (0: x, 1: y) of 2-D stage `stage` for the sample of `pixel` among npix
+PHG_HD float strat_2d(uint32_t pixel, uint32_t npix, uint64_t seed, uint32_t sample, uint32_t stage, uint32_t axis,
+                      float u) {
+    const uint32_t k = strat_isqrt(npix), per = k * k, passes = npix / per;
+    const uint32_t s = strat_permute(pixel, npix, strat_key(seed, sample, stage));
+    if (s >= passes * per) return u;   // beyond the whole passes: uniform
+    const uint32_t c = s % per;
+    return strat_value(axis == 0u ? c % k : c / k, k, u);
+}
+
+PHG_HD float strat_1d(uint32_t pixel, uint32_t npix, uint64_t seed, uint32_t sample, float u) {
+    return strat_value(strat_permute(pixel, npix, strat_key(seed, sample, 2u)), npix, u);
+}
+
 // ---- watertight ray/triangle (Math/Geometry/TWatertightTriangle.ipp:16-149) -------------
 PHG_HD bool wt_intersect(V3 va, V3 vb, V3 vc, V3 org, V3 dir, float minT, float maxT,
                                              float* out_t, float* out_a, float* out_b, float* out_c) {

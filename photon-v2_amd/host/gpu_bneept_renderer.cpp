@@ -51,6 +51,7 @@ void GpuBneeptRenderer::doRender() {
         o.seed = m_settings.seed;
         o.estimator = m_settings.estimator;
         o.filter = m_settings.filter;
+        o.sampler = m_settings.sampler;
         check(phg_render(m_ctx, &o), "phg_render");
         check(phg_film_read(m_ctx, staging.data(), staging.size()), "phg_film_read");
         {
@@ -147,6 +148,7 @@ GpuBneeptSettings GpuBneeptSettings::fromP2(const phg_p2_settings& p2) {
     s.max_bounces = p2.max_bounces;
     s.estimator = p2.estimator;
     s.filter = p2.filter;
+    s.sampler = p2.sampler;
     return s;
 }
 

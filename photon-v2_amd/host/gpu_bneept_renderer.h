@@ -49,8 +49,9 @@ struct GpuBneeptSettings {
     int32_t device = 0;
     uint32_t estimator = PHG_ESTIMATOR_BNEEPT;
     uint32_t filter = PHG_FILTER_BOX;
+    uint32_t sampler = PHG_SAMPLER_INDEPENDENT;
 
-    // Settings of a .p2 scene (frame size, samples, estimator, filter, bounce cap).
+    // Settings of a .p2 scene (frame size, samples, estimator, filter, sample source, bounce cap).
     static GpuBneeptSettings fromP2(const phg_p2_settings& p2);
 };
 

@@ -35,6 +35,7 @@ BATCH_COUNT_TRAVERSAL, BATCH_GRAPH, BATCH_GUIDED, BATCH_TRAIN = 1, 2, 4, 8
 JOB_CAMERA = 1
 PHG_ESTIMATOR_BNEEPT, PHG_ESTIMATOR_BVPT, PHG_ESTIMATOR_BVPTDL = 0, 1, 2
 PHG_FILTER_BOX, PHG_FILTER_GAUSSIAN, PHG_FILTER_MN, PHG_FILTER_BH = 0, 1, 2, 3
+SAMPLER_INDEPENDENT, SAMPLER_STRATIFIED = 0, 1
 GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -58,7 +59,7 @@ class phg_camera(C.Structure):
 class phg_batch_job(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("max_bounces", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32), ("estimator", C.c_uint32),
-                ("camera", phg_camera), ("filter", C.c_uint32), ("pad2", C.c_uint32 * 3)]
+                ("camera", phg_camera), ("filter", C.c_uint32), ("sampler", C.c_uint32), ("pad2", C.c_uint32 * 2)]
 
 
 class phg_scene_desc(C.Structure):
@@ -104,7 +105,7 @@ class phg_autotune_opts(C.Structure):
 
 class phg_p2_settings(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("estimator", C.c_uint32),
-                ("max_bounces", C.c_uint32), ("filter", C.c_uint32), ("warnings", C.c_uint32), ("pad", C.c_uint32)]
+                ("max_bounces", C.c_uint32), ("filter", C.c_uint32), ("warnings", C.c_uint32), ("sampler", C.c_uint32)]
 
 
 class phg_render_opts(C.Structure):
@@ -112,7 +113,7 @@ class phg_render_opts(C.Structure):
                 ("max_bounces", C.c_uint32), ("spp_per_pass", C.c_uint32), ("seed", C.c_uint64),
                 ("flags", C.c_uint32), ("guide_prob", C.c_float), ("guide_lr", C.c_float),
                 ("guide_batch", C.c_uint32), ("guide_steps", C.c_uint32), ("estimator", C.c_uint32),
-                ("filter", C.c_uint32), ("guide_depth", C.c_uint32)]
+                ("filter", C.c_uint32), ("guide_depth", C.c_uint32), ("sampler", C.c_uint32)]
 
 
 class phg_stats(C.Structure):
@@ -229,7 +230,8 @@ def _job_array(jobs):
         cam = j.get("camera")
         arr[i] = phg_batch_job(j["scene"], j["width"], j["height"], j["spp"], j.get("max_bounces", 10000),
                                j.get("seed", 0), JOB_CAMERA if cam else 0, j.get("estimator", 0),
-                               _camera_struct(cam) if cam else phg_camera(), j.get("filter", 0), (C.c_uint32 * 3)(0, 0, 0))
+                               _camera_struct(cam) if cam else phg_camera(), j.get("filter", 0), j.get("sampler", 0),
+                               (C.c_uint32 * 2)(0, 0))
     return arr
 
 
@@ -317,7 +319,7 @@ class Scene:
             rc = L.phg_scene_parse_p2(text.encode(), C.byref(st), C.byref(out), err, 512)
         if rc != PHG_OK:
             raise PhgError(f"p2: {STATUS_NAMES.get(rc, rc)}: {err.value.decode()}")
-        settings = {k: getattr(st, k) for k, _ in phg_p2_settings._fields_ if k != "pad"}
+        settings = {k: getattr(st, k) for k, _ in phg_p2_settings._fields_}
         return Scene._from_host(out), settings
 
     def desc(self) -> phg_scene_desc:
@@ -493,11 +495,11 @@ class Context:
     def render(self, spp: int, max_bounces: int = 10000, seed: int = 0, spp_begin: int = 0,
                spp_per_pass: int = 0, count_traversal: bool = False, guided: bool = False, train: bool = False,
                guide_prob: float = 0.5, guide_lr: float = 1e-3, guide_batch: int = 0, guide_steps: int = 4,
-               graph: bool = False, estimator: int = 0, filter: int = 0, guide_depth: int = 0):
+               graph: bool = False, estimator: int = 0, filter: int = 0, guide_depth: int = 0, sampler: int = 0):
         flags = (RENDER_COUNT_TRAVERSAL if count_traversal else 0) | (RENDER_GUIDED if guided else 0) | \
             (RENDER_TRAIN if train else 0) | (RENDER_GRAPH if graph else 0)
         o = phg_render_opts(self.width, self.height, spp, spp_begin, max_bounces, spp_per_pass, seed, flags,
-                            guide_prob, guide_lr, guide_batch, guide_steps, estimator, filter, guide_depth)
+                            guide_prob, guide_lr, guide_batch, guide_steps, estimator, filter, guide_depth, sampler)
         self._check(self._L.phg_render(self._h, C.byref(o)), "phg_render")
 
     # ---- guiding network

@@ -18,7 +18,7 @@ def scenes():
 
 
 JOBS = [dict(scene=0, width=64, height=48, spp=9, max_bounces=5, seed=11),
-        dict(scene=1, width=96, height=64, spp=5, max_bounces=8, seed=12),
+        dict(scene=1, width=96, height=64, spp=5, max_bounces=8, seed=12, sampler=phg.SAMPLER_STRATIFIED),
         dict(scene=2, width=40, height=40, spp=17, max_bounces=10000, seed=13),
         dict(scene=0, width=33, height=17, spp=3, max_bounces=2, seed=14),
         dict(scene=1, width=200, height=150, spp=2, max_bounces=4, seed=15)]
@@ -28,7 +28,7 @@ def single(scene, job):
     with phg.Context(0) as c:
         c.upload(scene)
         c.film_clear(job["width"], job["height"])
-        c.render(job["spp"], max_bounces=job["max_bounces"], seed=job["seed"])
+        c.render(job["spp"], max_bounces=job["max_bounces"], seed=job["seed"], sampler=job.get("sampler", 0))
         return c.film()
 
 

@@ -53,6 +53,22 @@ def test_cornell_matches_oracle(ctx):
     check_parity(gpu, ref, "cornell", 0.9)
 
 
+def test_stratified_source_matches_oracle(ctx):
+    """sample-source(stratified) (SGStratified.cpp:16-154): the first vertex's five dimensions
+    are stratified across each batch's pixels.  The GPU draws the oracle's strata (same parity
+    bar as the independent source), the film is not the independent one, and per-pixel sample
+    counts are unchanged."""
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    W = H = 64
+    print("[render] case=cornell_stratified start", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, 16, 5, 11, sampler=phg.SAMPLER_STRATIFIED)
+    ref, _ = orc.render(sc, W, H, 16, max_bounces=5, seed=11, sampler=1)
+    check_parity(gpu, ref, "cornell_stratified", 0.9)
+    ind = gpu_render(ctx, sc, W, H, 16, 5, 11)
+    assert not np.array_equal(gpu[..., :3], ind[..., :3])
+    assert np.array_equal(gpu[..., 3], ind[..., 3])
+
+
 def test_procedural_ggx_matches_oracle(ctx):
     sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=4, target_tris=30_000, num_instances=12, num_lights=6, subdiv=3)
     W = H = 48
@@ -257,7 +273,7 @@ def test_renderer_seam_renders_a_p2_file(ctx, tmp_path):
     assert data.startswith(header)
     img = np.frombuffer(data[len(header):], "<f4").reshape(48, 64, 3)
     sc, st = phg.Scene.parse_p2(path=scene_path)
-    film = gpu_render(ctx, sc, 64, 48, 16, st["max_bounces"], 17)
+    film = gpu_render(ctx, sc, 64, 48, 16, st["max_bounces"], 17, sampler=st["sampler"])
     assert np.array_equal(img, phg.develop(film).astype(np.float32))
 
 

@@ -86,6 +86,7 @@ def test_materials_and_settings():
     sc, st = phg.Scene.parse_p2(path=SCENE)
     assert (st["width"], st["height"], st["spp"], st["max_bounces"]) == (64, 48, 16, 12)
     assert st["estimator"] == phg.PHG_ESTIMATOR_BNEEPT and st["filter"] == phg.PHG_FILTER_BOX
+    assert st["sampler"] == phg.SAMPLER_STRATIFIED   # sample-source(stratified)
     assert st["warnings"] == 0
     kinds = sorted(set(sc.materials["type"]))
     assert kinds == [phg.MAT_ABSORBER, phg.MAT_LAMBERT, phg.MAT_CONDUCTOR, phg.MAT_DIELECTRIC]
