@@ -779,12 +779,36 @@ static inline float pdfA_to_pdfW(float pdfA, V3 posToTarget, V3 N) {
     return std::isfinite(s) ? std::fabs(s) : 0.0f;
 }
 
+// Colour energy of linear-sRGB radiance, math::estimate_color_energy (color_spaces.ipp:743-764):
+// Smits' spectrum (color_spaces.ipp:341-476) times D65 normalised to unit sum, summed over
+// [350, 850] nm in 100 samples -- linear in the Smits basis weights, so seven basis energies
+// (white, cyan, magenta, yellow, red, green, blue; tests/golden/color_energy.json, derived from
+// the reference's tables by tests/golden/make_color_energy.py) and the colour's branch.
+static float color_energy(RGB c) {
+    static const double kBasis[7] = {0.9996779720137005, 0.5398497088854213, 0.7349389630228806, 0.7044563722431154, 0.4544199699880077, 0.2582578706735675, 0.3028703986116803};
+    const double r = c.r, g = c.g, b = c.b;
+    double lo, w1, w2;
+    int i1, i2;
+    if (r <= g && r <= b) {
+        lo = r; i1 = 1; w1 = std::min(g, b) - r;
+        if (g <= b) { i2 = 6; w2 = b - g; } else { i2 = 5; w2 = g - b; }
+    } else if (g <= r && g <= b) {
+        lo = g; i1 = 2; w1 = std::min(r, b) - g;
+        if (r <= b) { i2 = 6; w2 = b - r; } else { i2 = 4; w2 = r - b; }
+    } else {
+        lo = b; i1 = 3; w1 = std::min(r, g) - b;
+        if (r <= g) { i2 = 5; w2 = g - r; } else { i2 = 4; w2 = r - g; }
+    }
+    const double e = kBasis[0] * lo + kBasis[i1] * w1 + kBasis[i2] * w2;
+    return e > 0.0 ? (float)e : 0.0f;
+}
+
 static void emitters_prepare(Scene& S) {
     const orc_scene* s = S.s;
     S.tri_area.resize(s->num_tris);
     for (uint32_t t = 0; t < s->num_tris; ++t) S.tri_area[t] = tri_area(S.vtx(t, 0), S.vtx(t, 1), S.vtx(t, 2));
-    // weights = approximate radiant flux = luminance(radiance) * area * pi per triangle,
-    // summed over the group (GroupedDiffuseSurfaceEmitter.cpp:110-120)
+    // weights = approximate radiant flux = colour energy(radiance) * area * pi per triangle
+    // (DiffuseSurfaceEmitter.cpp:55-86), summed over the group (GroupedDiffuseSurfaceEmitter.cpp:110-120)
     const uint32_t n = s->num_emitters;
     std::vector<float> w(n, 0.0f);
     for (uint32_t e = 0; e < n; ++e) {
@@ -792,7 +816,7 @@ static void emitters_prepare(Scene& S) {
         float total = 0.0f;
         for (uint32_t k = 0; k < em.tri_count; ++k) {
             const uint32_t t = s->emitter_tris[em.tri_begin + k];
-            const float flux = luminance(RGB{em.radiance[0], em.radiance[1], em.radiance[2]}) * S.tri_area[t] * kPi;
+            const float flux = color_energy(RGB{em.radiance[0], em.radiance[1], em.radiance[2]}) * S.tri_area[t] * kPi;
             total += flux;
         }
         w[e] = total;
@@ -1148,6 +1172,8 @@ void orc_pcg32_generate(uint64_t seq, uint64_t init_state, int use_default, uint
 }
 
 uint64_t orc_moremur(uint64_t v) { return moremur(v); }
+
+float orc_color_energy(const float* rgb3) { return color_energy(RGB{rgb3[0], rgb3[1], rgb3[2]}); }
 
 // The stratified source's keyed bijection, and the first-vertex dimensions of one batch:
 // jitter5 holds the stream values u0, u2, u3, u4, u5 of every pixel; out5 receives them

@@ -206,6 +206,15 @@ def render_paths(scene, width, height, pixels, samples, max_bounces=10000, seed=
     return out
 
 
+def color_energy(rgb):
+    """Colour energy of a linear-sRGB radiance (the emitter-pick weight per unit area)."""
+    L = lib()
+    L.orc_color_energy.restype = C.c_float
+    L.orc_color_energy.argtypes = [C.c_void_p]
+    a = np.ascontiguousarray(rgb, np.float32).reshape(3)
+    return float(L.orc_color_energy(a.ctypes.data))
+
+
 def strat_perm(i, n, key):
     """The stratified source's keyed bijection of [0, n)."""
     L = lib()

@@ -15,6 +15,7 @@
 
 #include "../../include/phg.h"
 #include "bvh_build.h"
+#include "color_energy.h"
 #include "guide.h"
 #include "kernels.h"
 
@@ -143,9 +144,7 @@ static inline H3 hsub(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; 
 static inline H3 hcross(H3 a, H3 b) { return H3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 static inline float hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline float htri_area(H3 a, H3 b, H3 c) { return std::sqrt(hdot(hcross(hsub(b, a), hsub(c, a)), hcross(hsub(b, a), hsub(c, a)))) * 0.5f; }
-static inline float hluminance(const float* c) {
-    return c[0] * 0.21263900587151027f + c[1] * 0.715168678767756f + c[2] * 0.07219231536073371f;
-}
+static inline float hcolor_energy(const float* c) { return (float)color_energy(c[0], c[1], c[2]); }
 
 static phg_status set_err(Ctx* c, phg_status s, const std::string& msg) { c->err = msg; return s; }
 
@@ -402,7 +401,9 @@ static phg_status prepare_scene(const phg_scene_desc* s, const BvhBuildParams& b
         r.mat = s->tri_material[t];
         r.emitter = s->tri_emitter[t];
     }
-    // emitters: power-favouring CDF (ESPowerFavoring::update, TPwcDistribution1D ctor)
+    // emitters: power-favouring CDF (ESPowerFavoring::update, TPwcDistribution1D ctor); a
+    // triangle's weight is its approximate radiant flux, colour energy x area x pi
+    // (DiffuseSurfaceEmitter::calcRadiantFluxApprox, DiffuseSurfaceEmitter.cpp:55-86)
     const uint32_t ne = s->num_emitters;
     h.emitters.assign(ne, EmitterRec{});
     h.cdf.assign(ne + 1, 0.0f);
@@ -419,7 +420,7 @@ static phg_status prepare_scene(const phg_scene_desc* s, const BvhBuildParams& b
             const phg_emitter& em = s->emitters[e];
             float total = 0.0f;
             for (uint32_t k = 0; k < em.tri_count; ++k) {
-                const float flux = hluminance(em.radiance) * h.etris[em.tri_begin + k].area * 3.14159265358979323846f;
+                const float flux = hcolor_energy(em.radiance) * h.etris[em.tri_begin + k].area * 3.14159265358979323846f;
                 total += flux;
             }
             w[e] = total;

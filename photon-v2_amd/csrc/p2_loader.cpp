@@ -44,6 +44,7 @@
 #include <vector>
 
 #include "../../include/phg.h"
+#include "color_energy.h"
 #include "scene_buf.h"
 
 namespace {
@@ -604,12 +605,12 @@ struct Loader {
                     return 0.5 * std::sqrt((double)dot3(c, c));
                 };
                 const double area = tri_area(v[0], v[1], v[3]) + tri_area(v[1], v[2], v[3]);
-                // AAreaLight::buildEmitter: color scaled to the given power, spread over the
-                // area and the cosine-weighted hemisphere; power of an RGB color estimated by
-                // its relative luminance
-                const double lum = 0.2126390058715103 * color.x + 0.715168678767756 * color.y + 0.0721923153607337 * color.z;
-                if (!(lum > 0) || !(area > 0)) throw ParseError{"rectangle-light has no power or area"};
-                const float k = (float)(watts / (lum * area * 3.14159265358979323846));
+                // AAreaLight::buildEmitter (AAreaLight.cpp:55-58): the color put to the given
+                // energy (Spectrum::putEnergy: divided by its colour energy, times the watts),
+                // spread over the area and the cosine-weighted hemisphere
+                const double energy = phg::color_energy(color.x, color.y, color.z);
+                if (!(energy > 0) || !(area > 0)) throw ParseError{"rectangle-light has no power or area"};
+                const float k = (float)(watts / (energy * area * 3.14159265358979323846));
                 if (absorber == UINT32_MAX) absorber = s.add_material(PHG_MAT_ABSORBER, f3(0, 0, 0), 0, 1, 1);
                 const int32_t e = (int32_t)s.emits.size();
                 phg_emitter em{};

@@ -650,3 +650,37 @@ def test_native_build_renders_the_same_film():
     a, _ = orc.render(sc, 24, 24, 4, max_bounces=5, seed=3)
     b, _ = orc.render(sc, 24, 24, 4, max_bounces=5, seed=3, native=True)
     assert np.array_equal(a, b)
+
+
+# ---- emitter-pick weights: the reference's colour-energy estimate
+def test_color_energy_matches_the_reference_estimate():
+    """math::estimate_color_energy of emitted linear-sRGB radiance (DiffuseSurfaceEmitter.cpp:
+    55-86) against tests/golden/color_energy.json, which make_color_energy.py computes from the
+    reference's Smits and D65 tables (resampled as the reference does): the oracle's seven basis
+    energies and Smits branches reproduce every vector to float rounding."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "color_energy.json")))
+    for r, g, b, e in gold["vectors"]:
+        got = orc.color_energy([r, g, b])
+        assert abs(got - e) <= 2e-7 * max(1.0, abs(e)), (r, g, b, got, e)
+
+
+def test_color_energy_invariants():
+    """What the construction fixes: white (1, 1, 1) has unit energy (the bases reproduce
+    illuminant E and D65 is normalised), complementary bases add up to white, the estimate is
+    positively homogeneous and continuous across Smits' branches, and never negative."""
+    e = orc.color_energy
+    assert abs(e([1, 1, 1]) - 1.0) < 1e-3
+    for c, p in (([0, 1, 1], [1, 0, 0]), ([1, 0, 1], [0, 1, 0]), ([1, 1, 0], [0, 0, 1])):
+        assert abs(e(c) + e(p) - e([1, 1, 1])) < 0.01
+    rng = np.random.default_rng(8)
+    for c in rng.uniform(0, 3, (50, 3)):
+        assert abs(e(2.5 * c) - 2.5 * e(c)) <= 1e-6 * e(c) * 2.5 + 1e-7
+        # continuity: a colour on a branch boundary and a nudge either side
+        for i, j in ((0, 1), (1, 2), (0, 2)):
+            d = c.copy(); d[j] = d[i]
+            for eps in (1e-6, -1e-6):
+                dd = d.copy(); dd[j] += eps
+                assert abs(e(dd) - e(d)) < 1e-5
+    assert e([-1, 0, 0]) == 0.0 and e([0, 0, 0]) == 0.0

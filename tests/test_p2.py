@@ -75,8 +75,12 @@ geometry(rectangle) @g = [real width 1] [real height 1];
 actor(model) @floor = [geometry geometry @g];"""
     sc, st = phg.Scene.parse_p2(text)
     assert len(sc.emitters) == 1 and list(sc.emitter_tris) == [0, 1]
-    lum = 0.2126390058715103 * 1 + 0.715168678767756 * 0.5 + 0.0721923153607337 * 0.25
-    k = 200 / (lum * 6 * np.pi)     # watts over luminance x area x pi (AAreaLight::buildEmitter)
+    # watts over colour energy x area x pi (AAreaLight::buildEmitter, Spectrum::putEnergy); the
+    # colour energy is the reference's estimate (tests/golden/color_energy.json constants)
+    import json
+    E = json.load(open(os.path.join(HERE, "golden", "color_energy.json")))["constants"]
+    energy = E["white"] * 0.25 + E["yellow"] * (0.5 - 0.25) + E["red"] * (1 - 0.5)   # b < g < r
+    k = 200 / (energy * 6 * np.pi)
     assert np.allclose(sc.emitters["radiance"][0], [k, 0.5 * k, 0.25 * k], rtol=1e-5)
     assert np.allclose(tri_normals(sc.verts[:2]), [0, -1, 0], atol=1e-6)   # faces down
     assert sc.materials["type"][sc.tri_material[0]] == phg.MAT_ABSORBER
@@ -152,7 +156,10 @@ def test_reference_cornell_scene_loads():
     assert (st["width"], st["height"], st["spp"]) == (300, 300, 40)
     assert st["estimator"] == phg.PHG_ESTIMATOR_BNEEPT and st["warnings"] == 0
     assert len(sc.emitters) == 1 and sc.num_tris == 2 + 5 * 2 + 20480
-    assert np.allclose(sc.emitters["radiance"][0], 500 / (9 * np.pi), rtol=1e-5)
+    # a white light's colour energy is the white basis energy (tests/golden/color_energy.json)
+    import json
+    e_white = json.load(open(os.path.join(HERE, "golden", "color_energy.json")))["constants"]["white"]
+    assert np.allclose(sc.emitters["radiance"][0], 500 / (e_white * 9 * np.pi), rtol=1e-5)
     v = sc.verts.reshape(-1, 3)
     assert np.allclose(v.min(axis=0), -5) and np.allclose(v.max(axis=0), 5)
 
