@@ -174,12 +174,17 @@ def batch_leg(args):
 SCALE_VIEWS = [(2048, 2048), (1920, 1080), (1600, 900), (1280, 720), (1024, 1024), (960, 540), (800, 800), (640, 360)]
 
 
-# Algorithmic multiply-add work of the guiding MLP (64-64-64-64-64-40, 2 flops per multiply-add):
+# Algorithmic multiply-add work of the guiding MLP (64-64-64-64-64-41, 2 flops per multiply-add):
 # inference = forward; a training record = forward + dW for every layer + dZ propagation
 # through layers 4..1.
-MLP_FWD_FLOPS = 2 * (4 * 64 * 64 + 64 * 40)
-MLP_TRAIN_FLOPS = 2 * MLP_FWD_FLOPS + 2 * (64 * 40 + 3 * 64 * 64)
+MLP_FWD_FLOPS = 2 * (4 * 64 * 64 + 64 * 41)
+MLP_TRAIN_FLOPS = 2 * MLP_FWD_FLOPS + 2 * (64 * 41 + 3 * 64 * 64)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def guide_prob_label(p: float):
+    """The lobes' selection probability as reported: the fixed value, or "learned" (negative)."""
+    return "learned per query (network output 40, in [0.05, 0.95])" if p < 0 else p
 
 
 def ordinal(n: int) -> str:
@@ -283,7 +288,8 @@ def scale_convergence(args, ctx, jobs):
     plain = cv.progressive_batch(ctx, jobs, args.scale_conv_max_spp, refs, seed_base=100000, graph=True, target=0.005)
     gtot = {}
     guided = cv.progressive_batch(ctx, jobs, args.scale_conv_max_spp, refs, seed_base=200000, guided=True,
-                                  train_spp=16, guide_steps=32, guide_depth=1, target=0.005, totals=gtot, graph=True)
+                                  train_spp=16, guide_steps=32, guide_depth=1, guide_prob=args.guide_prob,
+                                  target=0.005, totals=gtot, graph=True)
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
     for name, cur in (("plain", plain), ("guided", guided)):
         print(f"[bench] scale {name}: " + " ".join(f"{s}spp/{t:.2f}s/{e:.2e}" for s, t, e in cur), file=sys.stderr,
@@ -291,8 +297,8 @@ def scale_convergence(args, ctx, jobs):
     return {"reference_spp": args.scale_ref_spp, "reference_s": round(ref_s, 2), "relmse_target": 0.01,
             "plain_s": None if tp is None else round(tp, 4), "guided_s": None if tg is None else round(tg, 4),
             "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
-            "guided": {"guide_prob": 0.5, "guide_depth": 1, "train_spp": 16, "steps_per_pass": 32,
-                       "mlp": mlp_utilisation(gtot)},
+            "guided": {"guide_prob": guide_prob_label(args.guide_prob), "guide_depth": 1, "train_spp": 16,
+                       "steps_per_pass": 32, "mlp": mlp_utilisation(gtot)},
             "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
             "guided_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in guided]}
 
@@ -324,12 +330,12 @@ def convergence_leg(args):
         ctx.upload(scene)
         gtot = {}
         guided = cv.progressive(ctx, res, res, args.conv_max_spp, R, vR, seed=1, guided=True,
-                                train_spp=args.conv_train_spp, guide_prob=0.5, guide_steps=32, target=0.005,
+                                train_spp=args.conv_train_spp, guide_prob=args.guide_prob, guide_steps=32, target=0.005,
                                 guide_depth=1, totals=gtot, graph=True)
         # the captured chunks carry no per-kernel times: the trained network's inference rate
         # comes from one eager guided render of the same image
         ctx.film_clear(res, res)
-        ctx.render(16, seed=99, guided=True, guide_depth=1)
+        ctx.render(16, seed=99, guided=True, guide_depth=1, guide_prob=args.guide_prob)
         ist = ctx.stats()
         gtot["guide_infer_ms"], gtot["guide_queries"] = ist["guide_infer_ms"], ist["guide_queries"]
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
@@ -350,7 +356,8 @@ def convergence_leg(args):
             "relmse_target": 0.01, "plain_s": None if tp is None else round(tp, 4),
             "guided_s": None if tg is None else round(tg, 4),
             "guided_speedup": None if (tp is None or tg is None) else round(tp / tg, 3),
-            "guided": {"guide_prob": 0.5, "guide_depth": 1, "train_spp": args.conv_train_spp, "steps_per_pass": 32,
+            "guided": {"guide_prob": guide_prob_label(args.guide_prob), "guide_depth": 1,
+                       "train_spp": args.conv_train_spp, "steps_per_pass": 32,
                        "training_pass_paths": "<= 16M (16 spp at 1024^2)", "records": "first path vertex", "lr": 1e-3,
                        "batch": 1 << 18, "mlp": mlp_utilisation(gtot)},
             "per_sample": per_sample,
@@ -371,6 +378,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-conv", action="store_true", help="skip the guided-vs-plain convergence leg")
+    ap.add_argument("--guide-prob", type=float, default=-1.0,
+                    help="guided legs: probability of sampling the learned lobes (negative: the network's own "
+                         "per-query probability)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs leg")
     ap.add_argument("--batch-jobs", type=int, default=48)
     ap.add_argument("--no-scale", action="store_true", help="skip the 20M-triangle multi-view leg")

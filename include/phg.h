@@ -157,7 +157,8 @@ typedef struct phg_render_opts {
     uint32_t spp_per_pass;     /* samples per pixel per wavefront pass; 0 = automatic */
     uint64_t seed;
     uint32_t flags;            /* PHG_RENDER_* */
-    float guide_prob;          /* PHG_RENDER_GUIDED: probability of sampling the learned lobes (one-sample MIS) */
+    float guide_prob;          /* PHG_RENDER_GUIDED: probability of sampling the learned lobes (one-sample MIS);
+                                  < 0: the network's own per-query probability (output 40, in [0.05, 0.95]) */
     float guide_lr;            /* PHG_RENDER_TRAIN: Adam learning rate */
     uint32_t guide_batch;      /* PHG_RENDER_TRAIN: records per training step (0 = 2^18) */
     uint32_t guide_steps;      /* PHG_RENDER_TRAIN: Adam steps after each pass */
@@ -303,7 +304,7 @@ typedef struct phg_batch_opts {
     uint32_t order;            /* PHG_BATCH_ORDER_*: which jobs the passes take first */
     uint32_t flags;            /* PHG_BATCH_* */
     uint32_t guide_steps;      /* PHG_BATCH_TRAIN: Adam steps after each pass (0 = 4) */
-    float guide_prob;          /* PHG_BATCH_GUIDED: probability of sampling the learned lobes */
+    float guide_prob;          /* PHG_BATCH_GUIDED: probability of sampling the learned lobes; < 0: learned per query */
     float guide_lr;            /* PHG_BATCH_TRAIN: Adam learning rate (0 = 1e-3) */
     uint32_t guide_batch;      /* PHG_BATCH_TRAIN: records per step (0 = 2^18) */
     uint32_t guide_depth;      /* PHG_BATCH_GUIDED: path vertices that sample the lobes (0 = all); training
@@ -365,21 +366,28 @@ phg_status phg_autotune_batch(void* ctx, const phg_scene_desc* scenes, uint32_t 
 phg_status phg_get_stats(void* ctx, phg_stats* out);
 
 /* ---- Path-guiding network (no counterpart in the reference, which has no guiding).
- * A 64-64-64-64-64-40 ReLU MLP over a frequency encoding of (position, normal, outgoing
- * direction) producing 8 von Mises-Fisher lobes over the incident direction; BF16 on the
- * matrix cores with FP32 accumulation.  Parameters: one FP32 blob of PHG_GUIDE_PARAMS values,
- * W0 b0 W1 b1 W2 b2 W3 b3 W4 b4 with each W stored [in][out].
+ * A 64-64-64-64-64-41 ReLU MLP over a frequency encoding of (position, normal, outgoing
+ * direction) producing 8 von Mises-Fisher lobes over the incident direction and the logit z of
+ * the lobes' selection probability a = 0.05 + 0.9 sigmoid(z); BF16 on the matrix cores with
+ * FP32 accumulation.  Parameters: one FP32 blob of PHG_GUIDE_PARAMS values, W0 b0 W1 b1 W2 b2
+ * W3 b3 W4 b4 with each W stored [in][out].
  * Training record: 16 floats = 9 features in [0,1] (position in the scene box, normal and
  * outgoing direction mapped from [-1,1]), incident direction xyz, weight (target / sampling
- * density), BSDF density b of the direction, selection probability a, and one unused float.
- * The loss is -weight * log(a q + (1 - a) b) with q the lobes' density: the lobes are fitted
- * as one component of the one-sample MIS mixture that samples the direction; with b = 0 (or
- * a = 0) it is the plain weighted negative log-likelihood of q. */
-enum { PHG_GUIDE_PARAMS = 19240, PHG_GUIDE_OUTPUTS = 40, PHG_GUIDE_RECORD = 16 };
+ * density), BSDF density b of the direction, the selection probability the direction was
+ * sampled with (0: the BSDF alone), and, where that probability is to be learned, the density
+ * p_s the direction was drawn with (0: a fixed probability).  The lobe outputs minimise
+ * -weight * log(a q + (1 - a) b) with q the lobes' density and a the record's fixed
+ * probability or the network's own: the lobes are fitted as the one-sample MIS mixture that
+ * samples the direction; with b = 0 (or a = 0) it is the plain weighted negative
+ * log-likelihood of q.  The selection logit (a head on the last hidden layer whose gradient
+ * does not reach the hidden layers) minimises weight^2 p_s / (a q + (1 - a) b), the
+ * one-sample estimate of the mixture estimator's second moment. */
+enum { PHG_GUIDE_PARAMS = 19305, PHG_GUIDE_OUTPUTS = 41, PHG_GUIDE_RECORD = 16 };
 
 phg_status phg_guide_set_params(void* ctx, const float* params, uint32_t n);
 phg_status phg_guide_get_params(void* ctx, float* params, uint32_t n);
-/* raw network outputs (40 per query) for 9 features per query */
+/* raw network outputs (41 per query: 8 lobes x (mean direction 3, log kappa, weight logit), selection
+ * logit) for 9 features per query */
 phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out);
 /* gradient of the mean of the records' losses (above) */
 phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, float* grads);

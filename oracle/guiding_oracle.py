@@ -1,11 +1,12 @@
 """CPU oracle of the path-guiding network — TEST INFRASTRUCTURE ONLY.
 
 Restates, in numpy FP64, the guiding model the GPU trains and queries: frequency encoding of
-(position, normal, outgoing direction), a 64-64-64-64-64-40 ReLU MLP, and a mixture of 8
-von Mises-Fisher lobes over the incident direction, with the weighted negative
-log-likelihood used for online training, -w log(a q + (1 - a) b) (q = the lobes' density, b =
-the BSDF density, a = the lobes' selection probability in the one-sample MIS mixture), and
-its analytic gradients.  The reference renderer
+(position, normal, outgoing direction), a 64-64-64-64-64-41 ReLU MLP, a mixture of 8
+von Mises-Fisher lobes over the incident direction (outputs 0-39) and the lobes' selection
+probability a = 0.05 + 0.9 sigmoid(output 40), with the weighted negative log-likelihood used
+for online training, -w log(a q + (1 - a) b) (q = the lobes' density, b = the BSDF density, a =
+the selection probability of the one-sample MIS mixture, a record's fixed value or the
+network's own), and its analytic gradients.  The reference renderer
 has no path guiding; this is the specification the GPU kernels are checked against.  The
 analytic gradients are themselves pinned by central finite differences in the tests.
 
@@ -20,8 +21,10 @@ N_IN = 64
 N_HID = 64
 N_LAYERS = 5            # 4 hidden layers of 64, then the output layer
 N_LOBES = 8
-N_OUT = 5 * N_LOBES     # per lobe: mean direction (3, unnormalised), log kappa, weight logit
+N_LOBE_VALS = 5 * N_LOBES   # per lobe: mean direction (3, unnormalised), log kappa, weight logit
+N_OUT = N_LOBE_VALS + 1      # + the logit of the lobes' selection probability
 KAPPA_LOG_MIN, KAPPA_LOG_MAX = -8.0, 7.0
+SEL_MIN, SEL_MAX = 0.05, 0.95
 LAYER_SHAPES = [(N_IN, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_OUT)]
 
 
@@ -83,9 +86,15 @@ def forward(params, e, bf16=False):
     return acts, h
 
 
+def selection(o):
+    """The lobes' selection probability from raw outputs (output 40)."""
+    z = np.asarray(o, np.float64).reshape(-1, N_OUT)[:, N_LOBE_VALS]
+    return SEL_MIN + (SEL_MAX - SEL_MIN) / (1.0 + np.exp(-z))
+
+
 def decode(o):
     """Mixture parameters from raw outputs: mu (n,8,3), kappa (n,8), weights (n,8)."""
-    o = np.asarray(o, np.float64).reshape(-1, N_LOBES, 5)
+    o = np.asarray(o, np.float64).reshape(-1, N_OUT)[:, :N_LOBE_VALS].reshape(-1, N_LOBES, 5)
     r = o[:, :, 0:3]
     rl = np.linalg.norm(r, axis=2, keepdims=True)
     mu = np.where(rl > 0, r / np.where(rl > 0, rl, 1), np.array([0.0, 0.0, 1.0]))
@@ -112,32 +121,62 @@ def mixture_pdf(o, omega):
     return (w * p).sum(axis=1)
 
 
-def _sel_share(q, bsdf_pdf, sel):
+def _mix_prob(o, q, sel, samp_pdf):
+    """The selection probability a of each record: the network's own where samp_pdf > 0 (the
+    record's probability is learned) and the record's sel > 0, else sel (None: 0)."""
+    if sel is None:
+        return np.zeros_like(q)
+    a = np.broadcast_to(np.asarray(sel, np.float64), q.shape).copy()
+    if samp_pdf is not None:
+        use = (np.broadcast_to(np.asarray(samp_pdf, np.float64), q.shape) > 0) & (a > 0)
+        a = np.where(use, selection(o), a)
+    return a
+
+
+def _sel_share(q, bsdf_pdf, a):
     """a q / (a q + (1 - a) b): the lobes' share of the one-sample MIS density (1 where b or a is 0)."""
-    if bsdf_pdf is None or sel is None:
+    if bsdf_pdf is None:
         return np.ones_like(q)
     b = np.broadcast_to(np.asarray(bsdf_pdf, np.float64), q.shape)
-    a = np.broadcast_to(np.asarray(sel, np.float64), q.shape)
     use = (a > 0) & (b > 0)
     den = np.where(use, a * q + (1 - a) * b, 1.0)
     return np.where(use, a * q / den, 1.0)
 
 
-def record_density(q, bsdf_pdf, sel):
+def record_density(q, bsdf_pdf, a):
     """Density whose log the record loss takes: a q + (1 - a) b, or q where b or a is 0."""
-    if bsdf_pdf is None or sel is None:
+    if bsdf_pdf is None:
         return q
     b = np.broadcast_to(np.asarray(bsdf_pdf, np.float64), q.shape)
-    a = np.broadcast_to(np.asarray(sel, np.float64), q.shape)
     use = (a > 0) & (b > 0)
     return np.where(use, a * q + (1 - a) * b, q)
 
 
-def loss(params, e, omega, weight, bf16=False, bsdf_pdf=None, sel=None):
-    """Mean of -weight * log(a q + (1 - a) b) (a q's constant factor dropped where b = 0)."""
+def loss(params, e, omega, weight, bf16=False, bsdf_pdf=None, sel=None, samp_pdf=None):
+    """Mean of -weight * log(a q + (1 - a) b) (a q's constant factor dropped where b = 0): the
+    objective of the lobe outputs 0..39."""
     _, o = forward(params, e, bf16)
     q = mixture_pdf(o, omega)
-    return -(np.asarray(weight, np.float64) * np.log(record_density(q, bsdf_pdf, sel))).mean()
+    a = _mix_prob(o, q, sel, samp_pdf)
+    return -(np.asarray(weight, np.float64) * np.log(record_density(q, bsdf_pdf, a))).mean()
+
+
+def selection_objective(o, omega, weight, bsdf_pdf, sel, samp_pdf):
+    """Per record, weight^2 samp_pdf / (a q + (1 - a) b) with a the network's own selection
+    probability: for a direction drawn with density samp_pdf and weight = integrand /
+    samp_pdf, the one-sample estimate of the second moment of integrand / (a q + (1 - a) b),
+    i.e. of the one-sample MIS estimator's variance plus a constant.  Records not learned
+    (samp_pdf = 0), or with a = 0 or b = 0: 0.  Its derivative in output 40 is output_grad's
+    column 40."""
+    o = np.asarray(o, np.float64).reshape(-1, N_OUT)
+    q = mixture_pdf(o, omega)
+    a = _mix_prob(o, q, sel, samp_pdf)
+    b = np.broadcast_to(np.asarray(bsdf_pdf, np.float64), q.shape)
+    ps = np.broadcast_to(np.asarray(samp_pdf, np.float64), q.shape)
+    lrn = (ps > 0) & (a > 0) & (b > 0)
+    w = np.broadcast_to(np.asarray(weight, np.float64), q.shape)
+    dens = a * q + (1 - a) * b
+    return np.where(lrn, w * w * ps / np.where(lrn, dens, 1.0), 0.0)
 
 
 def dlogp_dkappa(k):
@@ -151,19 +190,24 @@ def dlogp_dkappa(k):
     return np.where(small, series, exact)
 
 
-def output_grad(o, omega, weight, bsdf_pdf=None, sel=None):
-    """d(-weight * log(a q + (1 - a) b)) / d(raw outputs) per sample (not yet divided by the
-    batch size); without b and a (or where either is 0) it is d(-weight * log q)."""
-    o = np.asarray(o, np.float64).reshape(-1, N_LOBES, 5)
-    n = o.shape[0]
+def output_grad(o, omega, weight, bsdf_pdf=None, sel=None, samp_pdf=None):
+    """Output gradients per sample (not yet divided by the batch size).  Outputs 0..39:
+    d(-weight * log(a q + (1 - a) b)); without b and a (or where either is 0) d(-weight * log q).
+    a is the network's selection probability where samp_pdf > 0 (and sel > 0); then output 40
+    gets -weight^2 samp_pdf (q - b) / (a q + (1 - a) b)^2 da/dz, the derivative of
+    selection_objective; otherwise a = sel and output 40 gets 0."""
+    oa = np.asarray(o, np.float64).reshape(-1, N_OUT)
+    n = oa.shape[0]
+    o = oa[:, :N_LOBE_VALS].reshape(-1, N_LOBES, 5)
     r = o[:, :, 0:3]
     rl = np.linalg.norm(r, axis=2, keepdims=True)
-    mu, kappa, w = decode(o.reshape(n, -1))
+    mu, kappa, w = decode(oa)
     om = np.asarray(omega, np.float64).reshape(-1, 1, 3)
     c = (mu * om).sum(axis=2)
     p = vmf_norm(kappa) * np.exp(kappa * (c - 1.0))
     q = (w * p).sum(axis=1, keepdims=True)
-    share = _sel_share(q[:, 0], bsdf_pdf, sel).reshape(-1, 1)
+    a = _mix_prob(oa, q[:, 0], sel, samp_pdf)
+    share = _sel_share(q[:, 0], bsdf_pdf, a).reshape(-1, 1)
     scale = -np.asarray(weight, np.float64).reshape(-1, 1) * share / q  # d loss / d q
     g = np.zeros_like(o)
     wp = w * p
@@ -175,14 +219,28 @@ def output_grad(o, omega, weight, bsdf_pdf=None, sel=None):
     g[:, :, 3] = scale * np.where(inside, dk, 0.0)
     # softmax logits
     g[:, :, 4] = scale * (wp - w * q)
-    return g.reshape(n, N_OUT)
+    out = np.zeros((n, N_OUT))
+    out[:, :N_LOBE_VALS] = g.reshape(n, N_LOBE_VALS)
+    if samp_pdf is not None and bsdf_pdf is not None and sel is not None:
+        b = np.broadcast_to(np.asarray(bsdf_pdf, np.float64), (n,))
+        ps = np.broadcast_to(np.asarray(samp_pdf, np.float64), (n,))
+        lrn = (ps > 0) & \
+            (np.broadcast_to(np.asarray(sel, np.float64), (n,)) > 0) & (b > 0)
+        dens = a * q[:, 0] + (1 - a) * b
+        dadz = (a - SEL_MIN) * (SEL_MAX - a) / (SEL_MAX - SEL_MIN)
+        wt = np.broadcast_to(np.asarray(weight, np.float64), (n,))
+        out[:, N_LOBE_VALS] = np.where(lrn, -wt * wt * ps * (q[:, 0] - b) / np.where(lrn, dens, 1.0) ** 2 * dadz,
+                                       0.0)
+    return out
 
 
-def backward(params, e, omega, weight, bf16=False, bsdf_pdf=None, sel=None):
-    """Gradients of the mean loss w.r.t. every (W, b), in the parameter layout."""
+def backward(params, e, omega, weight, bf16=False, bsdf_pdf=None, sel=None, samp_pdf=None):
+    """Gradients of the mean record objectives w.r.t. every (W, b), in the parameter layout.
+    Output 40 (the selection logit) is a head on the last hidden layer: its gradient reaches
+    its own weights and bias but not the hidden layers."""
     acts, o = forward(params, e, bf16)
     n = acts[0].shape[0]
-    g = output_grad(o, omega, weight, bsdf_pdf, sel) / n
+    g = output_grad(o, omega, weight, bsdf_pdf, sel, samp_pdf) / n
     grads = [None] * N_LAYERS
     for li in range(N_LAYERS - 1, -1, -1):
         W, b = params[li]
@@ -190,6 +248,9 @@ def backward(params, e, omega, weight, bf16=False, bsdf_pdf=None, sel=None):
         x = acts[li]
         grads[li] = (x.T @ g, g.sum(axis=0))
         if li > 0:
+            if li == N_LAYERS - 1:
+                Wq = Wq.copy()
+                Wq[:, N_LOBE_VALS] = 0.0
             gx = g @ Wq.T
             g = gx * (acts[li] > 0)
     return grads

@@ -176,7 +176,7 @@ static GuideNetDev guide_net(Ctx* c) {
 // He-uniform initialisation (fan-in), zero biases, from a splitmix64 stream.
 static std::vector<float> guide_default_params(uint64_t seed) {
     std::vector<float> p(kGuideParamCount, 0.0f);
-    const int offs[11] = {0, 4096, 4160, 8256, 8320, 12416, 12480, 16576, 16640, 19200, 19240};
+    const int offs[11] = {0, 4096, 4160, 8256, 8320, 12416, 12480, 16576, 16640, 19264, 19305};
     uint64_t st = seed * 0x9E3779B97F4A7C15ULL + 0xD1B54A32D192ED03ULL;
     auto next = [&]() {
         uint64_t z = (st += 0x9E3779B97F4A7C15ULL);
@@ -189,6 +189,8 @@ static std::vector<float> guide_default_params(uint64_t seed) {
         for (int i = offs[2 * l]; i < offs[2 * l + 1]; ++i)
             p[i] = (float)(((double)(next() >> 11) * 0x1p-53) * 2.0 - 1.0) * lim;
     }
+    // the selection logit starts at 0 for every query (selection probability halfway)
+    for (int i = 0; i < 64; ++i) p[offs[8] + i * 41 + 40] = 0.0f;
     return p;
 }
 
@@ -782,6 +784,9 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         if (pl.segs.empty() || pl.segs.size() > kMaxSegs) return set_err(c, PHG_ERR_INVALID_ARG, "bad pass plan");
     }
     if (cap == 0) return PHG_OK;
+    // a fixed selection probability lies in [0, 1]; a negative one selects the learned one
+    if (m.guided && !(m.guide_prob <= 1.0f))
+        return set_err(c, PHG_ERR_INVALID_ARG, "guide_prob must be at most 1 (negative: learned per query)");
     PHG_HIP(c, hipSetDevice(c->device));
     PHG_HIP(c, c->rng.ensure(cap));
     PHG_HIP(c, c->T.ensure(cap)); PHG_HIP(c, c->Wem.ensure(cap)); PHG_HIP(c, c->prevP.ensure(cap)); PHG_HIP(c, c->L.ensure(cap));
@@ -980,7 +985,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
                 for (uint32_t k = 0; k < kChunkBounces; ++k) {
                     const uint32_t b = m0 + k;
                     const bool in_depth = m.guide_depth == 0 || b < m.guide_depth;
-                    if (guided && pp.guide_prob > 0.0f && b < pl.max_bounces && in_depth) infer_mask |= 1u << k;
+                    if (guided && pp.guide_prob != 0.0f && b < pl.max_bounces && in_depth) infer_mask |= 1u << k;
                     if ((train || guided) && in_depth) guided_mask |= 1u << k;
                 }
                 hipGraphExec_t ex = nullptr;
@@ -1006,7 +1011,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         PHG_HIP(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1])); t_shade += ms;
         for (uint32_t b = 0;; ++b) {
-            const bool use_guide = guided && pp.guide_prob > 0.0f && b < pl.max_bounces &&
+            const bool use_guide = guided && pp.guide_prob != 0.0f && b < pl.max_bounces &&
                                    (m.guide_depth == 0 || b < m.guide_depth);
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
             PHG_HIP(c, launch_extend(sc, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
@@ -1052,7 +1057,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             PHG_HIP(c, hipEventRecord(c->ev[0], c->stream));
             PHG_HIP(c, c->g_blockcnt.ensure(2 * (size_t)guide_collect_blocks(pp.paths)));
             PHG_HIP(c, guide_collect(pb, pp, c->g_scene, c->g_train.p, train_cap, c->g_counters.p, c->g_blockcnt.p,
-                                     c->g_keep, 0.0f, ++c->g_pass_salt, c->stream));
+                                     c->g_keep, ++c->g_pass_salt, c->stream));
             uint32_t cnt[2] = {0, 0};
             PHG_HIP(c, hipMemcpyAsync(cnt, c->g_counters.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             PHG_HIP(c, hipStreamSynchronize(c->stream));

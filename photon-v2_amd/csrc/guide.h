@@ -15,7 +15,7 @@ struct GuideScene {
 
 // bf16 arrays are carried as uint16_t bit patterns on the host side
 struct GuideNetDev {
-    const float* params;     // fp32 master parameters (19240)
+    const float* params;     // fp32 master parameters (kGuideParamCount)
     const uint16_t* wt;      // 5 x [64 out][64 in] bf16
     const uint16_t* wn;      // 5 x [64 in][64 out] bf16
     const float* bias;       // 5 x 64 fp32
@@ -29,9 +29,11 @@ struct GuideNetDev {
     float* bp_mut;
 };
 
-constexpr uint32_t kGuideParamCount = 19240;
+constexpr uint32_t kGuideParamCount = 19305;
+// record: 9 features, sampled direction (3), weight, BSDF density, selection probability used,
+// and the sampling density of the direction if that probability was the network's own (else 0)
 constexpr uint32_t kGuideRecordFloats = 16;
-constexpr uint32_t kGuideOutputs = 40;
+constexpr uint32_t kGuideOutputs = 41;      // raw outputs: 8 lobes x 5, selection logit
 constexpr uint32_t kGuideNetLayers = 5;     // dense layers of the network (scratch sizing)
 
 hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st);
@@ -56,7 +58,7 @@ struct GuideWorkspace {
     uint32_t max_blocks;     // blocks of the forward kernel
     uint32_t slices;         // first-level reduction slices
 };
-// Gradient of the mean record loss over nrec records, written to grads (19240 floats).
+// Gradient of the mean record loss over nrec records, written to grads (kGuideParamCount floats).
 hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nrec, float* grads,
                            const GuideWorkspace& ws, hipStream_t st);
 hipError_t guide_adam(const GuideNetDev& net, float* grads, float* m, float* v, float lr, int step, hipStream_t st);
@@ -67,7 +69,7 @@ hipError_t guide_fill(float* p, uint32_t n, float v, hipStream_t st);
 // guide_collect_blocks(paths) entries of scratch.
 uint32_t guide_collect_blocks(uint32_t paths);
 hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
-                         uint32_t capacity, uint32_t* counters, uint32_t* block_counts, float keep_prob, float sel,
+                         uint32_t capacity, uint32_t* counters, uint32_t* block_counts, float keep_prob,
                          uint32_t salt, hipStream_t st);
 
 }  // namespace phg

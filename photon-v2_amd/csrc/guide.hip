@@ -34,12 +34,14 @@ __host__ __device__ __forceinline__ int layer_out(int l) { return l == kGuideLay
 // layout) feeds it to the next layer as the B operand.  The input (k) order of the next layer
 // is therefore permuted: position j (0..7) of lane group g in k-chunk kc is input
 // 16 (2 kc + j / 4) + 4 g + j % 4, and the weights are stored in that order.  The last
-// layer's 40 outputs are moved to rows where lane group g holds lobes 2g and 2g + 1 whole
-// (slots s = 0..9 at rows 16 (s / 4) + 4 g + s % 4; 8 of the 48 rows stay zero).
+// layer's 40 lobe outputs are moved to rows where lane group g holds lobes 2g and 2g + 1 whole
+// (slots s = 0..9 at rows 16 (s / 4) + 4 g + s % 4), and the selection logit (output 40) to
+// slot 10 of lane group 0; the other 7 of the 48 rows stay zero.
 __host__ __device__ inline int infer_in_index(int kc, int g, int j) { return 16 * (2 * kc + (j >> 2)) + 4 * g + (j & 3); }
-// logical output (0..39) at padded output row R of the last layer, or -1
+// logical output (0..40) at padded output row R of the last layer, or -1
 __host__ __device__ inline int infer_out_logical(int R) {
     const int g = (R & 15) >> 2, s = 4 * (R >> 4) + (R & 3);
+    if (s == 10 && g == 0) return kGuideSel;
     if (s >= 10) return -1;
     const int lobe = 2 * g + (s >= 5 ? 1 : 0), p = s >= 5 ? s - 5 : s;
     return 5 * lobe + p;
@@ -74,8 +76,11 @@ __global__ void k_guide_prepare(const float* __restrict__ params, uint16_t* __re
         const float* W = params + kGuideParamOffsets[2 * l];
         const float v = a < nout ? W[b * nout + a] : 0.0f;
         wt[i] = (__bf16)v;
-        // wn: [in][out]: a = in, b = out
-        const float v2 = b < nout ? W[a * nout + b] : 0.0f;
+        // wn: [in][out]: a = in, b = out.  The selection output is a head on the last hidden
+        // layer: its column is zero here, so its gradient does not flow back into the hidden
+        // layers (they learn from the lobes' gradients alone).
+        const bool head = l == kGuideLayers - 1 && b == kGuideSel;
+        const float v2 = b < nout && !head ? W[a * nout + b] : 0.0f;
         wn[i] = (__bf16)v2;
     }
     if (i < kGuideLayers * 64) {
@@ -201,17 +206,22 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
                         }
                 }
             }
-            // slots s = 0..9 of this lane: logical outputs 10 g + s (lobes 2g, 2g + 1)
-            float o[10];
+            // slots s = 0..9 of this lane: logical outputs 10 g + s (lobes 2g, 2g + 1); slot 10
+            // of lane group 0: the selection logit
+            float o[11];
 #pragma unroll
-            for (int s2 = 0; s2 < 10; ++s2)
+            for (int s2 = 0; s2 < 11; ++s2)
                 o[s2] = acc[s2 >> 2][s2 & 3] + s_b[(kGuideLayers - 1) * 64 + 16 * (s2 >> 2) + 4 * g + (s2 & 3)];
-            float* dst = out + (size_t)q * kGuideOut + 10 * g;
             if (raw) {
-                if (q < n)
+                float* dst = out + (size_t)q * kGuideOut + 10 * g;
+                if (q < n) {
                     for (int s2 = 0; s2 < 10; ++s2) dst[s2] = o[s2];
+                    if (g == 0) out[(size_t)q * kGuideOut + kGuideSel] = o[10];
+                }
                 continue;
             }
+            float* dst = out + (size_t)q * kGuideDec + 10 * g;
+            if (q < n && g == 0) out[(size_t)q * kGuideDec + kGuideSel] = guide_sel(o[10]);
             float mx = fmaxf(o[4], o[9]);
             mx = fmaxf(mx, __shfl_xor(mx, 16));
             mx = fmaxf(mx, __shfl_xor(mx, 32));
@@ -282,7 +292,7 @@ __global__ __launch_bounds__(256) void k_train_forward(GuideNetDev net, const fl
             float gr[kGuideOut];
             if (q < nrec) {
                 const float* R = rec + kGuideRecordFloats * (size_t)q;
-                guide_output_grad(s_out[wave] + r * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], R[13], R[14], gr);
+                guide_output_grad(s_out[wave] + r * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], R[13], R[14], R[15], gr);
             } else {
                 for (int k = 0; k < kGuideOut; ++k) gr[k] = 0.0f;
             }
@@ -509,7 +519,7 @@ __global__ __launch_bounds__(256) void k_guide_collect_scan(uint32_t* __restrict
 __global__ __launch_bounds__(256) void k_guide_collect_write(PathBuffers pb, PassParams pp, GuideScene gs,
                                                              float* __restrict__ out, uint32_t capacity,
                                                              const uint32_t* __restrict__ block_offsets,
-                                                             float keep_prob, float sel, uint32_t salt) {
+                                                             float keep_prob, uint32_t salt) {
     __shared__ uint32_t s[256];
     const uint32_t pid = blockIdx.x * 256 + threadIdx.x;
     uint32_t valid = 0, mask = 0;
@@ -532,21 +542,24 @@ __global__ __launch_bounds__(256) void k_guide_collect_write(PathBuffers pb, Pas
         float* o = out + (size_t)k * kGuideRecordFloats;
         for (int q = 0; q < 9; ++q) o[q] = x[q];
         o[9] = d4.x; o[10] = d4.y; o[11] = d4.z; o[12] = w;
-        o[13] = n4.w; o[14] = sel; o[15] = 0.0f;
+        // BSDF density of the direction, the lobes' selection probability used there (0 where
+        // the BSDF sampled alone) and, where that was the network's own, the density the
+        // direction was drawn with (else 0)
+        o[13] = n4.w; o[14] = v4.w; o[15] = d4.w;
     }
 }
 
 uint32_t guide_collect_blocks(uint32_t paths) { return (paths + 255u) / 256u; }
 
 hipError_t guide_collect(const PathBuffers& pb, const PassParams& pp, const GuideScene& gs, float* out_rec,
-                         uint32_t capacity, uint32_t* counters, uint32_t* block_counts, float keep_prob, float sel,
+                         uint32_t capacity, uint32_t* counters, uint32_t* block_counts, float keep_prob,
                          uint32_t salt, hipStream_t st) {
     const uint32_t nb = guide_collect_blocks(pp.paths);
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(k_guide_collect_count, dim3(nb), dim3(256), 0, st, pb, pp, keep_prob, salt, block_counts);
     hipLaunchKernelGGL(k_guide_collect_scan, dim3(1), dim3(256), 0, st, block_counts, nb, counters);
     hipLaunchKernelGGL(k_guide_collect_write, dim3(nb), dim3(256), 0, st, pb, pp, gs, out_rec, capacity, block_counts,
-                       keep_prob, sel, salt);
+                       keep_prob, salt);
     return hipGetLastError();
 }
 

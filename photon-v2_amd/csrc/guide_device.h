@@ -15,13 +15,19 @@ constexpr int kGuideIn = 64;
 constexpr int kGuideHid = 64;
 constexpr int kGuideLayers = 5;
 constexpr int kGuideLobes = 8;
-constexpr int kGuideOut = 5 * kGuideLobes;   // 40
-constexpr int kGuideOutPad = 48;             // output rows padded to a multiple of 16
+constexpr int kGuideLobeVals = 5 * kGuideLobes;  // 40: per lobe mean direction (3), log kappa, weight logit
+constexpr int kGuideSel = kGuideLobeVals;        // output 40: logit of the lobes' selection probability
+constexpr int kGuideOut = kGuideLobeVals + 1;    // 41 raw outputs per query
+constexpr int kGuideDec = kGuideLobeVals + 1;    // decoded: 8 x (mu, kappa, weight), then the selection probability
+constexpr int kGuideOutPad = 48;                 // output rows padded to a multiple of 16
 constexpr float kKappaLogMin = -8.0f, kKappaLogMax = 7.0f;
+// learned selection probability of the lobes in the one-sample MIS: kSelMin + (kSelMax -
+// kSelMin) sigmoid(output 40), so the BSDF always keeps a share of the samples
+constexpr float kSelMin = 0.05f, kSelMax = 0.95f;
 
 // flat FP32 parameter blob: W0 b0 W1 b1 W2 b2 W3 b3 W4 b4 with W stored [in][out]
-constexpr int kGuideParamOffsets[11] = {0, 4096, 4160, 8256, 8320, 12416, 12480, 16576, 16640, 19200, 19240};
-constexpr int kGuideParams = 19240;
+constexpr int kGuideParamOffsets[11] = {0, 4096, 4160, 8256, 8320, 12416, 12480, 16576, 16640, 19264, 19305};
+constexpr int kGuideParams = 19305;
 
 // 9 feature scalars in [0, 1]: position in the scene box, normal, outgoing direction
 PHG_HD void guide_features(const GuideScene& gs, V3 P, V3 N, V3 V, float* x) {
@@ -54,7 +60,12 @@ PHG_HD float vmf_norm(float k) {
     return k / (2.0f * kPi * (-expm1f(-2.0f * k)));
 }
 
-// decode 40 raw outputs into 8 x (mu xyz, kappa, weight)
+// the lobes' selection probability from raw output 40
+PHG_HD float guide_sel(float z) {
+    return kSelMin + (kSelMax - kSelMin) / (1.0f + expf(-z));
+}
+
+// decode the 40 lobe outputs into 8 x (mu xyz, kappa, weight)
 PHG_HD void guide_decode(const float* o, float* lobes) {
     float mx = -INFINITY;
     for (int j = 0; j < kGuideLobes; ++j) mx = fmaxf(mx, o[5 * j + 4]);
@@ -75,7 +86,7 @@ PHG_HD void guide_decode(const float* o, float* lobes) {
 // samples the BSDF alone at this vertex, which keeps the estimator unbiased
 PHG_HD bool guide_lobes_ok(const float* lobes) {
     float acc = 0.0f;
-    for (int k = 0; k < kGuideOut; ++k) acc += lobes[k];
+    for (int k = 0; k < kGuideDec; ++k) acc += lobes[k];
     return __builtin_isfinite(acc);
 }
 
@@ -111,11 +122,17 @@ PHG_HD V3 guide_sample(const float* lobes, float u0, float u1, float u2) {
     return normalize(bx * (s * cosf(phi)) + mu * w + bz * (s * sinf(phi)));
 }
 
-// d(-weight * log(a q + (1 - a) b))/d(raw outputs) for one sample, q = the lobes' density at
-// w, b = the BSDF density there, a = the selection probability of the lobes; b = 0 or a = 0
-// gives d(-weight * log q) (matches guiding_oracle.output_grad)
-PHG_HD void guide_output_grad(const float* o, V3 w, float weight, float bsdf_pdf, float sel, float* g) {
-    float lobes[kGuideOut];
+// Output gradients for one record (matches guiding_oracle.output_grad): direction w drawn
+// with density samp_pdf, weight = integrand / that density.  q = the lobes' density at w,
+// b = the BSDF density there, a = the selection probability of the lobes: the network's own
+// (output 40) where samp_pdf > 0 (the record's selection probability was learned), else sel.
+//   outputs 0..39: d(-weight * log(a q + (1 - a) b)), the radiance-weighted log-likelihood of
+//     the one-sample MIS density (b = 0 or a = 0 gives d(-weight * log q));
+//   output 40 (learned, a > 0, b > 0; else 0): -weight^2 samp_pdf (q - b) / (a q + (1 - a) b)^2
+//     da/dz, the one-sample estimate of d/dz of the second moment of integrand / (a q + (1 - a)
+//     b), which is the one-sample MIS estimator's variance plus a constant.
+PHG_HD void guide_output_grad(const float* o, V3 w, float weight, float bsdf_pdf, float sel, float samp_pdf, float* g) {
+    float lobes[kGuideLobeVals];
     guide_decode(o, lobes);
     float p[kGuideLobes], c[kGuideLobes];
     float q = 0.0f;
@@ -126,8 +143,17 @@ PHG_HD void guide_output_grad(const float* o, V3 w, float weight, float bsdf_pdf
         q += L[4] * p[j];
     }
     // d log(a q + (1 - a) b) = (a q / (a q + (1 - a) b)) d log q
-    const float share = (sel > 0.0f && bsdf_pdf > 0.0f) ? sel * q / (sel * q + (1.0f - sel) * bsdf_pdf) : 1.0f;
+    const bool learned = samp_pdf > 0.0f && sel > 0.0f;
+    const float a = learned ? guide_sel(o[kGuideSel]) : sel;
+    const bool mix = a > 0.0f && bsdf_pdf > 0.0f;
+    const float dens = mix ? a * q + (1.0f - a) * bsdf_pdf : q;
+    const float share = mix ? a * q / dens : 1.0f;
     const float scale = q > 0.0f ? -weight * share / q : 0.0f;
+    // selection logit: da/dz = (a - kSelMin) (kSelMax - a) / (kSelMax - kSelMin)
+    g[kGuideSel] = (learned && mix && dens > 0.0f)
+                       ? -weight * weight * samp_pdf * (q - bsdf_pdf) / (dens * dens) *
+                             ((a - kSelMin) * (kSelMax - a) / (kSelMax - kSelMin))
+                       : 0.0f;
     for (int j = 0; j < kGuideLobes; ++j) {
         const float* L = lobes + 5 * j;
         const float k = L[3];

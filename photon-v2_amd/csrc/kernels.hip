@@ -505,11 +505,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                     u2 = strat_2d(pixel, job.npix, job.seed, sample, 0u, 1u, u2);
                     u3 = strat_2d(pixel, job.npix, job.seed, sample, 1u, 0u, u3);
                 }
-                const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideOut : nullptr;
+                const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideDec : nullptr;
                 const bool guide_here = pp.guide_depth == 0 || bounce < pp.guide_depth;
                 const phg_material& m = sc.mats[mat];
-                const float alpha = (GUIDED && guide_here && pp.guide_prob > 0.0f && guide_samples_material(m) &&
-                                     guide_lobes_ok(lob)) ? pp.guide_prob : 0.0f;
+                // the lobes' selection probability: guide_prob, or the network's own for this
+                // query where guide_prob < 0
+                const float alpha = (GUIDED && guide_here && pp.guide_prob != 0.0f && guide_samples_material(m) &&
+                                     guide_lobes_ok(lob)) ? (pp.guide_prob > 0.0f ? pp.guide_prob : lob[kGuideSel]) : 0.0f;
                 if (m.type == PHG_MAT_ABSORBER) stop = true;
                 if (!stop) {
                     const RGB T = rgb3(T4.x, T4.y, T4.z);
@@ -642,8 +644,9 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                                 float4* rc = pb.rec + ((size_t)bounce * pp.paths + pid) * kRecFloat4;
                                 rc[0] = make_float4(X.P.x, X.P.y, X.P.z, pdfc);
                                 rc[1] = make_float4(X.Ns.x, X.Ns.y, X.Ns.z, bsdf_pdf_ls);
-                                rc[2] = make_float4(V.x, V.y, V.z, 0.0f);
-                                rc[3] = make_float4(ud.x, ud.y, ud.z, 0.0f);
+                                rc[2] = make_float4(V.x, V.y, V.z, alpha);
+                                // the sampling density where the selection probability is learned
+                                rc[3] = make_float4(ud.x, ud.y, ud.z, pp.guide_prob < 0.0f ? pdfc : 0.0f);
                                 rc[4] = make_float4(T.r, T.g, T.b, 0.0f);  // throughput before this vertex
                                 const float4 Lnow = pb.L[pid];   // radiance so far, this vertex's emission included
                                 rc[5] = make_float4(Lnow.x, Lnow.y, Lnow.z, 0.0f);

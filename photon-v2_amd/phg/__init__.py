@@ -36,7 +36,7 @@ JOB_CAMERA = 1
 PHG_ESTIMATOR_BNEEPT, PHG_ESTIMATOR_BVPT, PHG_ESTIMATOR_BVPTDL = 0, 1, 2
 PHG_FILTER_BOX, PHG_FILTER_GAUSSIAN, PHG_FILTER_MN, PHG_FILTER_BH = 0, 1, 2, 3
 SAMPLER_INDEPENDENT, SAMPLER_STRATIFIED = 0, 1
-GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19240, 40, 16
+GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19305, 41, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 
 

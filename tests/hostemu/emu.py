@@ -19,7 +19,8 @@ def lib():
         vp = C.c_void_p
         _lib.emu_intersect.argtypes = [vp, C.c_uint32, vp, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                        vp, vp, vp, vp, vp, vp]
-        _lib.emu_guide_output_grad.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp]
+        _lib.emu_guide_output_grad.argtypes = [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]
+        _lib.emu_guide_sel.argtypes = [vp, C.c_uint32, vp]
         _lib.emu_guide_pdf.argtypes = [vp, C.c_uint32, vp, vp]
         _lib.emu_guide_sample.argtypes = [vp, C.c_uint32, vp, vp]
         _lib.emu_guide_encode.argtypes = [vp, C.c_uint32, vp]
@@ -47,19 +48,27 @@ def _f(a, shape):
     return np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))
 
 
-def guide_output_grad(o, w3, weight, bsdf_pdf=None, sel=None):
-    o = _f(o, (-1, 40)); n = o.shape[0]
+def guide_output_grad(o, w3, weight, bsdf_pdf=None, sel=None, samp_pdf=None):
+    o = _f(o, (-1, 41)); n = o.shape[0]
     w3 = _f(w3, (n, 3)); weight = _f(weight, (n,))
     b = _f(np.zeros(n) if bsdf_pdf is None else bsdf_pdf, (n,))
     a = _f(np.zeros(n) if sel is None else sel, (n,))
-    out = np.zeros((n, 40), np.float32)
+    lrn = _f(np.zeros(n) if samp_pdf is None else samp_pdf, (n,))
+    out = np.zeros((n, 41), np.float32)
     lib().emu_guide_output_grad(o.ctypes.data, n, w3.ctypes.data, weight.ctypes.data, b.ctypes.data, a.ctypes.data,
-                                out.ctypes.data)
+                                lrn.ctypes.data, out.ctypes.data)
+    return out
+
+
+def guide_sel(o):
+    o = _f(o, (-1, 41)); n = o.shape[0]
+    out = np.zeros(n, np.float32)
+    lib().emu_guide_sel(o.ctypes.data, n, out.ctypes.data)
     return out
 
 
 def guide_pdf(o, w3):
-    o = _f(o, (-1, 40)); n = o.shape[0]
+    o = _f(o, (-1, 41)); n = o.shape[0]
     w3 = _f(w3, (n, 3))
     out = np.zeros(n, np.float32)
     lib().emu_guide_pdf(o.ctypes.data, n, w3.ctypes.data, out.ctypes.data)
@@ -67,7 +76,7 @@ def guide_pdf(o, w3):
 
 
 def guide_sample(o40, u3):
-    o40 = _f(o40, (40,)); u3 = _f(u3, (-1, 3)); n = u3.shape[0]
+    o40 = _f(o40, (-1,))[:40].copy(); u3 = _f(u3, (-1, 3)); n = u3.shape[0]
     out = np.zeros((n, 3), np.float32)
     lib().emu_guide_sample(o40.ctypes.data, n, u3.ctypes.data, out.ctypes.data)
     return out

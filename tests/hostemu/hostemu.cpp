@@ -47,22 +47,26 @@ extern "C" int emu_intersect(const float* verts, uint32_t ntri, const float* ray
 #include "../../photon-v2_amd/csrc/guide_device.h"
 
 extern "C" void emu_guide_output_grad(const float* o, uint32_t n, const float* w3, const float* weight,
-                                      const float* bsdf_pdf, const float* sel, float* out) {
+                                      const float* bsdf_pdf, const float* sel, const float* samp_pdf, float* out) {
     for (uint32_t i = 0; i < n; ++i)
-        guide_output_grad(o + 40 * (size_t)i, mk(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]), weight[i], bsdf_pdf[i], sel[i],
-                          out + 40 * (size_t)i);
+        guide_output_grad(o + kGuideOut * (size_t)i, mk(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]), weight[i], bsdf_pdf[i],
+                          sel[i], samp_pdf[i], out + kGuideOut * (size_t)i);
+}
+
+extern "C" void emu_guide_sel(const float* o, uint32_t n, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = guide_sel(o[kGuideOut * (size_t)i + kGuideSel]);
 }
 
 extern "C" void emu_guide_pdf(const float* o, uint32_t n, const float* w3, float* out) {
     for (uint32_t i = 0; i < n; ++i) {
-        float lobes[40];
-        guide_decode(o + 40 * (size_t)i, lobes);
+        float lobes[kGuideLobeVals];
+        guide_decode(o + kGuideOut * (size_t)i, lobes);
         out[i] = guide_pdf(lobes, mk(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]));
     }
 }
 
 extern "C" void emu_guide_sample(const float* o40, uint32_t n, const float* u3, float* out3) {
-    float lobes[40];
+    float lobes[kGuideLobeVals];
     guide_decode(o40, lobes);
     for (uint32_t i = 0; i < n; ++i) {
         const V3 d = guide_sample(lobes, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2]);

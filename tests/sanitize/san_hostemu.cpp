@@ -44,17 +44,18 @@ int main() {
                 }
     }
     // guiding-model math on random network outputs
-    std::vector<float> o(40 * 64), w3(3 * 64), weight(64), bp(64), sel(64), grad(40 * 64), pdf(64), out3(3 * 64), x9(9 * 64),
-        enc(64 * 64);
+    std::vector<float> o(41 * 64), w3(3 * 64), weight(64), bp(64), sel(64), lrn(64), grad(41 * 64), pdf(64), out3(3 * 64),
+        x9(9 * 64), enc(64 * 64), sp(64);
     for (float& x : o) x = 3.0f * U(rng);
     for (uint32_t i = 0; i < 64; ++i) {
         float d[3] = {U(rng), U(rng), U(rng)};
         const float l = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + 1e-6f;
         for (int k = 0; k < 3; ++k) w3[3 * i + k] = d[k] / l;
-        weight[i] = 0.5f * (U(rng) + 1.0f); bp[i] = 0.5f * (U(rng) + 1.0f); sel[i] = 0.5f;
+        weight[i] = 0.5f * (U(rng) + 1.0f); bp[i] = 0.5f * (U(rng) + 1.0f); sel[i] = 0.5f; lrn[i] = (i & 1u) ? bp[i] : 0.0f;
     }
     for (float& x : x9) x = 0.5f * (U(rng) + 1.0f);
-    emu_guide_output_grad(o.data(), 64, w3.data(), weight.data(), bp.data(), sel.data(), grad.data());
+    emu_guide_output_grad(o.data(), 64, w3.data(), weight.data(), bp.data(), sel.data(), lrn.data(), grad.data());
+    emu_guide_sel(o.data(), 64, sp.data());
     emu_guide_pdf(o.data(), 64, w3.data(), pdf.data());
     std::vector<float> u3(3 * 64);
     for (float& x : u3) x = 0.5f * (U(rng) + 1.0f);

@@ -37,8 +37,9 @@ def ctx():
     c.close()
 
 
-def random_records(n, seed, mixture=True):
-    """Records of 16 floats: features, direction, weight, BSDF density, selection probability, pad."""
+def random_records(n, seed, mixture=True, learned=False):
+    """Records of 16 floats: features, direction, weight, BSDF density, selection probability,
+    sampling density where the selection probability is learned (0: fixed)."""
     rng = np.random.default_rng(seed)
     x = rng.random((n, 9))
     d = rng.normal(size=(n, 3))
@@ -46,12 +47,14 @@ def random_records(n, seed, mixture=True):
     w = rng.exponential(1.0, n)
     b = rng.exponential(0.5, n) * (rng.random(n) < 0.8) if mixture else np.zeros(n)
     a = rng.choice([0.3, 0.5, 0.7], n) if mixture else np.zeros(n)
-    return np.concatenate([x, d, w[:, None], b[:, None], a[:, None], np.zeros((n, 1))], axis=1).astype(np.float32)
+    ps = rng.exponential(0.5, n) * (rng.random(n) < 0.7) if learned else np.zeros(n)
+    return np.concatenate([x, d, w[:, None], b[:, None], a[:, None], ps[:, None]], axis=1).astype(np.float32)
 
 
 def oracle_grads(params, rec):
     e = go.encode(rec[:, :9].astype(np.float64))
-    return go.backward(params, e, rec[:, 9:12], rec[:, 12], bf16=True, bsdf_pdf=rec[:, 13], sel=rec[:, 14])
+    return go.backward(params, e, rec[:, 9:12], rec[:, 12], bf16=True, bsdf_pdf=rec[:, 13], sel=rec[:, 14],
+                       samp_pdf=rec[:, 15])
 
 
 def test_forward_matches_oracle(ctx):
@@ -66,10 +69,11 @@ def test_forward_matches_oracle(ctx):
     assert err <= 0.02 * max(1.0, scale)
 
 
-def test_gradients_match_oracle(ctx):
+@pytest.mark.parametrize("learned", [False, True])
+def test_gradients_match_oracle(ctx, learned):
     params = go.init_params(5)
     ctx.guide_set_params(flat(params))
-    rec = random_records(4096, 6)
+    rec = random_records(4096, 6, learned=learned)
     gpu = ctx.guide_gradients(rec).astype(np.float64)
     grads = oracle_grads(unflat(flat(params)), rec)
     ref = np.concatenate([np.concatenate([gW.ravel(), gb.ravel()]) for gW, gb in grads])
@@ -89,7 +93,7 @@ def test_gradients_match_oracle(ctx):
 def test_adam_steps_follow_oracle(ctx):
     params = go.init_params(7)
     ctx.guide_set_params(flat(params))
-    rec = random_records(2048, 8)
+    rec = random_records(2048, 8, learned=True)
     ctx.guide_train(rec, lr=1e-3, steps=3)
     gpu = ctx.guide_get_params().astype(np.float64)
     p = unflat(flat(params))
@@ -139,7 +143,7 @@ def test_training_fits_a_known_distribution(ctx):
     assert ll > np.log(1 / (4 * np.pi)) + 1.5
 
 
-def slit_records(n, rng):
+def slit_records(n, rng, learned=False):
     """Floor points of a room (y = 0, x < 2.9) lit only through a slit at x = 3, z in
     [1.95, 2.05]: cosine-sampled directions, record weight 5 through the slit and 0.01
     elsewhere (the ratio f cos L_i / pdf for a white Lambertian floor)."""
@@ -156,7 +160,13 @@ def slit_records(n, rng):
     through = ok & (np.abs(P[:, 2] + t * D[:, 2] - 2.0) < 0.05) & (t * D[:, 1] < 3.0)
     w = np.where(through, 5.0, 0.01)
     x = go.features(P, N, V, [0, 0, 0], [4, 3, 4])
-    rec = np.concatenate([x, D, w[:, None], np.zeros((n, 3))], 1).astype(np.float32)
+    if learned:
+        # records whose selection probability is learned: the BSDF (cosine) density, a nominal
+        # selection 0.5 and the density the directions were drawn with (cosine)
+        tail = np.stack([D[:, 1] / np.pi, np.full(n, 0.5), D[:, 1] / np.pi], 1)
+    else:
+        tail = np.zeros((n, 3))
+    rec = np.concatenate([x, D, w[:, None], tail], 1).astype(np.float32)
     return rec, through
 
 
@@ -188,17 +198,48 @@ def test_training_gain_on_a_slit_lit_floor(ctx):
     assert var_bsdf / var_mix > 1.25
 
 
-@pytest.mark.parametrize("albedo,radiance", [(0.5, 0.5)])
-def test_guided_furnace_stays_unbiased(albedo, radiance):
-    """Guided BNEEPT with online training: the furnace's closed form Le / (1 - rho) holds."""
+def test_learned_selection_on_a_slit_lit_floor(ctx):
+    """Training on records of the one-sample MIS mixture with the learned selection
+    probability (its head trained on the estimator's second moment): the lobes still gather
+    mass through the slit, and the mixture at the learned probability estimates the floor's
+    irradiance integral with less variance than cosine sampling alone and no more than the
+    mixture at the fixed 0.5 (same reweighting as the fixed-probability test)."""
+    rng = np.random.default_rng(13)
+    ctx.guide_set_params(flat(go.init_params(12)))
+    test, through = slit_records(200_000, rng, learned=True)
+    for _ in range(8):
+        ctx.guide_train(slit_records(1 << 16, rng, learned=True)[0], lr=1e-3, steps=64)
+    o = ctx.guide_forward(test[:, :9]).astype(np.float64)
+    q = go.mixture_pdf(o, test[:, 9:12].astype(np.float64))
+    a = go.selection(o)
+    c = test[:, 10].astype(np.float64) / np.pi
+    w = test[:, 12].astype(np.float64)
+    mean = w.mean()
+    var_bsdf = (w * w).mean() - mean ** 2
+    var_half = (w * w * c / (0.5 * q + 0.5 * c)).mean() - mean ** 2
+    var_learned = (w * w * c / (a * q + (1 - a) * c)).mean() - mean ** 2
+    mass_q = np.mean(np.where(through, q / c, 0.0))
+    print(f"[guide] learned selection: mean a={a.mean():.3f} range=[{a.min():.3f}, {a.max():.3f}] slit mass "
+          f"cosine={through.mean():.4f} trained={mass_q:.4f} variance cosine={var_bsdf:.4f} a=0.5 {var_half:.4f} "
+          f"learned {var_learned:.4f} gain={var_bsdf / var_learned:.2f}x", flush=True)
+    assert np.all((a >= go.SEL_MIN - 1e-6) & (a <= go.SEL_MAX + 1e-6))
+    assert mass_q > 5 * through.mean()
+    assert var_learned < var_bsdf
+    assert var_learned <= 1.05 * var_half
+
+
+@pytest.mark.parametrize("albedo,radiance,guide_prob", [(0.5, 0.5, 0.5), (0.5, 0.5, -1.0)])
+def test_guided_furnace_stays_unbiased(albedo, radiance, guide_prob):
+    """Guided BNEEPT with online training, at a fixed and at the learned selection
+    probability: the furnace's closed form Le / (1 - rho) holds."""
     sc = furnace_scene(albedo=albedo, radiance=radiance)
     with phg.Context(0) as c:
         c.upload(sc)
         c.film_clear(32, 32)
         c.render(8, max_bounces=10000, seed=1, spp_per_pass=1, guided=True, train=True, guide_batch=4096,
-                 guide_steps=4)
+                 guide_steps=4, guide_prob=guide_prob)
         c.film_clear(32, 32)
-        c.render(64, max_bounces=10000, seed=2, spp_per_pass=1, guided=True, train=False)
+        c.render(64, max_bounces=10000, seed=2, spp_per_pass=1, guided=True, train=False, guide_prob=guide_prob)
         film = c.film()
         st = c.stats()
     expect = radiance / (1 - albedo)
@@ -211,12 +252,14 @@ def test_guided_furnace_stays_unbiased(albedo, radiance):
     assert abs(mean - expect) < 4 * se + 1e-4
 
 
-@pytest.mark.parametrize("kind,guide_depth", [("cornell", 0), ("cornell", 2), ("ggx", 0)])
-def test_guided_matches_plain_in_expectation(kind, guide_depth):
-    """Guided renders (lobes at every vertex, or only at the first two) agree with plain ones
-    per pixel within 99% confidence intervals: the Lambertian Cornell box, and a scene of GGX
-    conductors and dielectrics of roughness 0.05..0.6, where the lobes are sampled at the
-    rougher conductors and the other surfaces sample their BSDF alone."""
+@pytest.mark.parametrize("kind,guide_depth,guide_prob", [("cornell", 0, 0.5), ("cornell", 2, 0.5), ("ggx", 0, 0.5),
+                                                         ("cornell", 0, -1.0), ("ggx", 0, -1.0)])
+def test_guided_matches_plain_in_expectation(kind, guide_depth, guide_prob):
+    """Guided renders (lobes at every vertex, or only at the first two; a fixed or the
+    learned per-query selection probability) agree with plain ones per pixel within 99%
+    confidence intervals: the Lambertian Cornell box, and a scene of GGX conductors and
+    dielectrics of roughness 0.05..0.6, where the lobes are sampled at the rougher conductors
+    and the other surfaces sample their BSDF alone."""
     if kind == "cornell":
         sc = phg.Scene.generate(phg.GEN_CORNELL)
     else:
@@ -228,9 +271,11 @@ def test_guided_matches_plain_in_expectation(kind, guide_depth):
     with phg.Context(0) as c:
         c.upload(sc)
         c.film_clear(W, H)
-        c.render(16, max_bounces=10000, seed=3, spp_per_pass=1, guided=True, train=True, guide_batch=8192)
+        c.render(16, max_bounces=10000, seed=3, spp_per_pass=1, guided=True, train=True, guide_batch=8192,
+                 guide_prob=guide_prob)
         c.film_clear(W, H)
-        c.render(256, max_bounces=10000, seed=4, spp_per_pass=4, guided=True, guide_depth=guide_depth)
+        c.render(256, max_bounces=10000, seed=4, spp_per_pass=4, guided=True, guide_depth=guide_depth,
+                 guide_prob=guide_prob)
         guided = c.film()
         st = c.stats()
         c.film_clear(W, H)
@@ -240,8 +285,8 @@ def test_guided_matches_plain_in_expectation(kind, guide_depth):
     frac = np.mean(np.abs(z) > 2.5758)
     mg = guided[..., :3].sum() / guided[..., 3].sum()
     mp = plain[..., :3].sum() / plain[..., 3].sum()
-    print(f"[guide] {kind} guided vs plain: outside_99CI={frac:.4f} mean_guided={mg:.5f} mean_plain={mp:.5f}",
-          flush=True)
+    print(f"[guide] {kind} prob={guide_prob} guided vs plain: outside_99CI={frac:.4f} mean_guided={mg:.5f} "
+          f"mean_plain={mp:.5f}", flush=True)
     zm = image_mean_z(guided, plain)
     print(f"[guide] {kind} image-mean z per channel {np.round(zm, 2)}", flush=True)
     assert frac < 0.02

@@ -13,8 +13,8 @@ def test_encoding_matches_oracle():
 
 def test_mixture_pdf_and_gradient_match_oracle():
     rng = np.random.default_rng(2)
-    o = rng.normal(0, 1.5, (2000, 40))
-    o[:, 3::5] = rng.uniform(-9, 8, (2000, 8))     # log kappa, including clamped values
+    o = rng.normal(0, 1.5, (2000, 41))
+    o[:, 3:40:5] = rng.uniform(-9, 8, (2000, 8))     # log kappa, including clamped values
     w = rng.normal(size=(2000, 3))
     w /= np.linalg.norm(w, axis=1, keepdims=True)
     weight = rng.exponential(1.0, 2000)
@@ -34,12 +34,21 @@ def test_mixture_pdf_and_gradient_match_oracle():
     g = emu.guide_output_grad(o, w, weight, b, a)
     scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-12
     assert np.all(np.abs(g - g_ref) <= 2e-3 * scale + 1e-6)
+    # with the network's own selection probability (a sampling density on most records)
+    ps = rng.exponential(0.5, 2000) * (rng.random(2000) < 0.7)
+    ps32 = ps.astype(np.float32).astype(np.float64)
+    g_ref = go.output_grad(o32, w32, weight.astype(np.float32), b32, a32, ps32)
+    g = emu.guide_output_grad(o, w, weight, b, a, ps)
+    scale = np.abs(g_ref).max(axis=1, keepdims=True) + 1e-12
+    assert np.all(np.abs(g - g_ref) <= 2e-3 * scale + 1e-6)
+    assert np.allclose(emu.guide_sel(o), go.selection(o32), rtol=1e-6, atol=1e-7)
 
 
 def test_lobe_sampling_follows_mixture_pdf():
     rng = np.random.default_rng(3)
-    o = rng.normal(0, 1.0, 40)
-    o[3::5] = np.log([0.5, 3.0, 10.0, 40.0, 0.01, 5.0, 100.0, 1.0])
+    o = rng.normal(0, 1.0, 41)
+    o[3:40:5] = np.log([0.5, 3.0, 10.0, 40.0, 0.01, 5.0, 100.0, 1.0])
+    o[40] = 0.0
     u = rng.random((200_000, 3))
     d = emu.guide_sample(o, u)
     assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-5)
