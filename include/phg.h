@@ -140,14 +140,33 @@ enum { PHG_ESTIMATOR_BNEEPT = 0, PHG_ESTIMATOR_BVPT = 1, PHG_ESTIMATOR_BVPTDL = 
  * filter the film's sums are weighted: sum w r, g, b, sum w, sum w r^2, g^2, b^2. */
 enum { PHG_FILTER_BOX = 0, PHG_FILTER_GAUSSIAN = 1, PHG_FILTER_MN = 2, PHG_FILTER_BH = 3 };
 
-/* Sample source of a job (EngineEnv/SampleSource: sample-source(uniform-random | stratified)).
+/* Sample source of a job (EngineEnv/SampleSource: sample-source(uniform-random | stratified |
+ * halton)).
  * INDEPENDENT: every path draws its dimensions from its own PCG32 stream
  * (SGUniformRandom). STRATIFIED: SGStratified (Core/SampleGenerator/SGStratified.cpp:16-154):
  * sample s of every pixel forms one batch, and the five dimensions the estimator reads first at
  * the first surface are stratified across the batch's pixels (2-D, 2-D and 1-D jittered strata
  * in keyed shuffled order); the raster offset and all later dimensions are as INDEPENDENT.
- * Per-pixel distributions are the same; the pixels of one batch are negatively correlated. */
-enum { PHG_SAMPLER_INDEPENDENT = 0, PHG_SAMPLER_STRATIFIED = 1 };
+ * Per-pixel distributions are the same; the pixels of one batch are negatively correlated.
+ * HALTON: SGHalton (Core/SampleGenerator/SGHalton.cpp:22-206): the sample of pixel p in batch s
+ * is Halton point s * W * H + p; dimensions 0, 1 (bases 2, 3) place it over the whole sample
+ * window (it lands in any pixel, and the film splats it there), dimensions 2..6 (bases 5..17)
+ * are the first surface's five; later dimensions are as INDEPENDENT.  OR in a permutation
+ * (HaltonSampleSource "permutation"; default fixed) and a sequence ("sequence"; default
+ * original).  The reference draws the fixed and per-digit permutations, the random leap and
+ * the random start from its random source; here they are keyed by the job's seed.  Owen
+ * scrambling is the reference's own (keyed by the dimension only). */
+enum { PHG_SAMPLER_INDEPENDENT = 0, PHG_SAMPLER_STRATIFIED = 1, PHG_SAMPLER_HALTON = 2 };
+enum {
+    PHG_HALTON_PERM_FIXED = 0 << 8,      /* one permutation of the digit values per dimension */
+    PHG_HALTON_PERM_NONE = 1 << 8,       /* plain radical inverse */
+    PHG_HALTON_PERM_PER_DIGIT = 2 << 8,  /* one permutation per dimension and digit position */
+    PHG_HALTON_PERM_OWEN = 3 << 8,       /* each digit permuted by a hash of the digits before it */
+    PHG_HALTON_SEQ_ORIGINAL = 0 << 12,   /* points 0, 1, 2, ... */
+    PHG_HALTON_SEQ_LEAP = 1 << 12,       /* every 19961st point (the 257th good prime) */
+    PHG_HALTON_SEQ_RANDOM_LEAP = 2 << 12,  /* a leap drawn from the good primes 152..512 */
+    PHG_HALTON_SEQ_RANDOM_START = 3 << 12  /* a 64-bit start offset per dimension */
+};
 
 typedef struct phg_render_opts {
     uint32_t width, height;

@@ -940,6 +940,9 @@ struct Strat {
     bool on = false;
     uint32_t pixel = 0, npix = 0, sample = 0;
     uint64_t seed = 0;
+    // Halton source (below): the job's sampler value (kind, permutation, sequence) and leap
+    bool halton = false;
+    uint32_t hsampler = 0, hleap = 1;
 };
 
 // 8-round Feistel network over the smallest even bit count holding n, round function
@@ -985,7 +988,180 @@ static uint32_t floor_sqrt(uint32_t n) {
     return (uint32_t)lo;
 }
 
+// ---------------------------------------------------------------------------------------
+// Halton sample source (SGHalton.cpp:22-206 with Halton/halton.h and
+// Math/Random/Quasi/radical_inverse.h), as this renderer defines it.  Batch s (one raster
+// sample per pixel, N = W H) gives its sample i the point index s N + i; the index of
+// dimension d is start_d + (s N + i) * leap.  Dimensions 0, 1 (bases 2, 3) are plain radical
+// inverses and place the raster sample over the sample window; dimensions 2..6 (bases 5, 7,
+// 11, 13, 17) are the first surface's u[0], u[2], u[3], u[4], u[5] and go through the digit
+// permutation: none; fixed (one permutation of the digit values per dimension) and per-digit
+// (one per dimension and digit position), keyed by the seed where the reference draws them
+// from its random source; Owen (halton.h OwenScrambledPermuter: each digit permuted with a
+// hash of the dimension and of the digits already emitted).  Sequences: original (leap 1,
+// start 0), leap (the 257th good prime), random leap (a good prime of index 151..511 keyed by
+// the seed), random start (a keyed 64-bit start per dimension).  A value that rounds to 1 is
+// taken as the largest float below 1.
+// ---------------------------------------------------------------------------------------
+static const uint32_t kHaltonPrimes[7] = {2, 3, 5, 7, 11, 13, 17};
+
+// math::permuted_index (Math/hash.ipp:206-243): Kensler, "Correlated Multi-Jittered
+// Sampling" (2013), the hashed permutation of [0, l) with key p
+static uint32_t kensler_permute(uint32_t i, uint32_t l, uint32_t p) {
+    uint32_t w = l - 1;
+    for (uint32_t sh = 1; sh <= 16; sh <<= 1) w |= w >> sh;
+    while (true) {
+        i ^= p;           i *= 0xe170893du;
+        i ^= p >> 16;     i ^= (i & w) >> 4;
+        i ^= p >> 8;      i *= 0x0929eb3fu;
+        i ^= p >> 23;     i ^= (i & w) >> 1;
+        i *= 1 | p >> 27; i *= 0x6935fa69u;
+        i ^= (i & w) >> 11; i *= 0x74dcb303u;
+        i ^= (i & w) >> 2;  i *= 0x9e501cc3u;
+        i ^= (i & w) >> 2;  i *= 0xc860a3dfu;
+        i &= w;           i ^= i >> 5;
+        if (i < l) break;
+    }
+    return (i + p) % l;
+}
+
+// num_meaningful_digits<float> (radical_inverse.h:25-38)
+static uint32_t halton_float_digits(uint32_t base) {
+    const float rcp = 1.0f / (float)base;
+    uint32_t n = 0;
+    float scaler = 1.0f;
+    while (1.0f - (float)(base - 1) * scaler < 1.0f) {
+        ++n;
+        scaler *= rcp;
+    }
+    return n;
+}
+
+static float halton_clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+// The reversed digits of a radical inverse before the float conversion: `digits` holds them as
+// an integer (most significant = the index's least significant digit), `count` how many.
+struct Reversed { uint64_t digits = 0; uint32_t count = 0; bool bits64 = false; };
+
+// radical_inverse<BASE, float> (radical_inverse.h:69-104): base 2 reverses all 64 bits; other
+// bases reverse the index's digits until it is exhausted (or the next digit could overflow)
+static Reversed reverse_plain(uint32_t base, uint64_t value) {
+    Reversed r;
+    if (base == 2) {
+        for (int b = 0; b < 64; ++b) r.digits |= ((value >> b) & 1u) << (63 - b);
+        r.count = 64;
+        r.bits64 = true;
+        return r;
+    }
+    const uint64_t limit = (UINT64_MAX - (base - 1)) / base;
+    for (uint64_t cur = value; cur > 0 && r.digits <= limit; cur /= base) {
+        r.digits = r.digits * base + cur % base;
+        ++r.count;
+    }
+    return r;
+}
+
+static uint32_t halton_digit_key(uint64_t seed, uint32_t dim, uint32_t pos) {
+    return (uint32_t)(moremur((seed << 24) ^ ((uint64_t)dim << 12) ^ (uint64_t)pos ^ 0xA0761D6478BD642FULL) >> 32);
+}
+
+// radical_inverse_permuted<BASE, float> (radical_inverse.h:113-150) with the permuters of
+// halton.h (perm 0 fixed, 2 per-digit, 3 Owen): always the float's meaningful digit count
+static Reversed reverse_permuted(uint32_t dim, uint64_t value, uint32_t perm, uint64_t seed) {
+    const uint32_t base = kHaltonPrimes[dim];
+    const uint64_t limit = (UINT64_MAX - (base - 1)) / base;
+    const uint32_t ndigits = halton_float_digits(base);
+    Reversed r;
+    uint64_t cur = value, owen_hash = moremur(((uint64_t)dim << 4) + 1), owen_prefix = 0;
+    for (uint32_t pos = 0; pos < ndigits && r.digits <= limit; ++pos) {
+        const uint32_t digit = (uint32_t)(cur % base);
+        cur /= base;
+        uint32_t key;
+        if (perm == 3) {
+            owen_hash = moremur(owen_hash ^ owen_prefix);
+            key = (uint32_t)owen_hash;
+        } else {
+            key = halton_digit_key(seed, dim, perm == 2 ? pos : 0xFFFu);
+        }
+        const uint32_t permuted = kensler_permute(digit, base, key);
+        if (perm == 3) owen_prefix = owen_prefix * base + permuted;
+        r.digits = r.digits * base + permuted;
+        ++r.count;
+    }
+    return r;
+}
+
+// the float the reference makes of them: digits * 2^-64 for base 2, else digits times a scale
+// multiplied by 1/base (rounded to float) once per digit, clamped to [0, 1]
+static float reversed_to_float(uint32_t base, const Reversed& r) {
+    if (r.bits64) return (float)r.digits * 0x1p-64f;
+    const float rcp = 1.0f / (float)base;
+    float scaler = 1.0f;
+    for (uint32_t k = 0; k < r.count; ++k) scaler *= rcp;
+    return halton_clamp01((float)r.digits * scaler);
+}
+
+static float radical_inverse(uint32_t base, uint64_t value) { return reversed_to_float(base, reverse_plain(base, value)); }
+
+static Reversed halton_reversed(uint32_t dim, uint64_t index, uint32_t sampler, uint32_t leap, uint64_t seed) {
+    const uint32_t perm = (sampler >> 8) & 3, seq = (sampler >> 12) & 3;
+    uint64_t v = index * leap;
+    if (seq == 3) v += moremur((seed << 24) ^ ((uint64_t)dim << 8) ^ 0x2545F4914F6CDD1DULL);
+    return (dim < 2 || perm == 1) ? reverse_plain(kHaltonPrimes[dim], v) : reverse_permuted(dim, v, perm, seed);
+}
+
+// dimension `dim` of Halton point `index` under the job's sampler value and leap
+static float halton_dim(uint32_t dim, uint64_t index, uint32_t sampler, uint32_t leap, uint64_t seed) {
+    const float x = reversed_to_float(kHaltonPrimes[dim], halton_reversed(dim, index, sampler, leap, seed));
+    return x < 1.0f ? x : std::nextafter(1.0f, 0.0f);
+}
+
+// The n-th good prime (0-based): p_k with p_k^2 > p_{k-i} p_{k+i} for all 1 <= i <= k
+// (math_table.h GOOD_PRIME).
+static uint32_t good_prime(uint32_t n) {
+    static const std::vector<uint32_t> good = [] {
+        std::vector<uint32_t> primes, out;
+        for (uint32_t v = 2; primes.size() < 16384; ++v) {
+            bool is_p = true;
+            for (uint32_t q = 2; q * q <= v; ++q)
+                if (v % q == 0) { is_p = false; break; }
+            if (is_p) primes.push_back(v);
+        }
+        for (size_t k = 1; 2 * k < primes.size() && out.size() < 512; ++k) {
+            bool ok = true;
+            for (size_t i = 1; i <= k; ++i)
+                if ((uint64_t)primes[k] * primes[k] <= (uint64_t)primes[k - i] * primes[k + i]) { ok = false; break; }
+            if (ok) out.push_back(primes[k]);
+        }
+        return out;
+    }();
+    return n < good.size() ? good[n] : 0;
+}
+
+static uint32_t halton_leap_of(uint32_t sampler, uint64_t seed) {
+    const uint32_t seq = (sampler >> 12) & 3;
+    if (seq == 1) return good_prime(256);
+    if (seq == 2) return good_prime(151 + (uint32_t)(moremur(seed ^ 0x51ED270B27ACE8CDULL) % 361));
+    return 1;
+}
+
+// raster position of the Halton sample of pixel slot `pixel` in batch `sample` over the sample
+// window of half size `half` (0.5: the image)
+static void halton_raster_of(uint32_t pixel, uint32_t sample, uint32_t W, uint32_t H, double half, uint32_t sampler,
+                             uint32_t leap, uint64_t seed, double* x, double* y) {
+    const uint64_t index = (uint64_t)sample * ((uint64_t)W * H) + pixel;
+    const double lo = 0.5 - half;
+    *x = lo + (double)halton_dim(0, index, sampler, leap, seed) * (((double)W - 0.5 + half) - lo);
+    *y = lo + (double)halton_dim(1, index, sampler, leap, seed) * (((double)H - 0.5 + half) - lo);
+}
+
 static void strat_apply(const Strat& t, float* u) {
+    if (t.halton) {
+        const uint64_t index = (uint64_t)t.sample * t.npix + t.pixel;
+        const int slots[5] = {0, 2, 3, 4, 5};
+        for (uint32_t d = 2; d < 7; ++d) u[slots[d - 2]] = halton_dim(d, index, t.hsampler, t.hleap, t.seed);
+        return;
+    }
     const uint32_t k = floor_sqrt(t.npix), cells = k * k, whole = (t.npix / cells) * cells;
     const int dims[2][2] = {{0, 2}, {3, 4}};
     for (uint32_t stage = 0; stage < 2; ++stage) {
@@ -1148,8 +1324,10 @@ static RGB estimate_any(const Scene& S, V3 org, V3 dir, Stream& sf, uint32_t max
 
 static Strat strat_of(const orc_render_opts* opt, uint64_t pix, uint32_t sample, uint64_t npix) {
     Strat t;
-    t.on = opt->sampler == 1;
+    t.halton = (opt->sampler & 0xFF) == 2;
+    t.on = opt->sampler == 1 || t.halton;
     t.pixel = (uint32_t)pix; t.npix = (uint32_t)npix; t.sample = sample; t.seed = opt->seed;
+    if (t.halton) { t.hsampler = opt->sampler; t.hleap = halton_leap_of(opt->sampler, opt->seed); }
     return t;
 }
 
@@ -1189,6 +1367,31 @@ void orc_strat_batch(uint32_t npix, uint64_t seed, uint32_t sample, const float*
         out5[5 * p] = u[0]; out5[5 * p + 1] = u[2]; out5[5 * p + 2] = u[3]; out5[5 * p + 3] = u[4];
         out5[5 * p + 4] = u[5];
     }
+}
+
+// The Halton source's parts: dimension values, Kensler's permutation, the float digit count,
+// the plain radical inverse, the good primes and a job's leap; and the raster positions of a
+// batch (half: the sample window's half filter size, 0.5 for the image).
+float orc_halton_dim(uint32_t dim, uint64_t index, uint32_t sampler, uint32_t leap, uint64_t seed) {
+    return halton_dim(dim, index, sampler, leap, seed);
+}
+uint32_t orc_kensler_permute(uint32_t i, uint32_t l, uint32_t p) { return kensler_permute(i, l, p); }
+uint32_t orc_halton_float_digits(uint32_t base) { return halton_float_digits(base); }
+float orc_radical_inverse(uint32_t base, uint64_t value) { return radical_inverse(base, value); }
+// the reversed digits behind orc_halton_dim: returns them, their count in *count
+uint64_t orc_halton_reversed(uint32_t dim, uint64_t index, uint32_t sampler, uint32_t leap, uint64_t seed,
+                             uint32_t* count) {
+    const Reversed r = halton_reversed(dim, index, sampler, leap, seed);
+    *count = r.count;
+    return r.digits;
+}
+uint32_t orc_good_prime(uint32_t n) { return good_prime(n); }
+uint32_t orc_halton_leap(uint32_t sampler, uint64_t seed) { return halton_leap_of(sampler, seed); }
+void orc_halton_raster(uint32_t W, uint32_t H, uint32_t sample, double half, uint32_t sampler, uint64_t seed,
+                       double* xy) {
+    const uint32_t leap = halton_leap_of(sampler, seed);
+    for (uint32_t p = 0; p < W * H; ++p) halton_raster_of(p, sample, W, H, half, sampler, leap, seed, xy + 2 * (size_t)p,
+                                                         xy + 2 * (size_t)p + 1);
 }
 
 // Test-only: plant one of the mistakes listed at g_fault (0 restores the restatement).
@@ -1436,7 +1639,21 @@ void orc_camera_rays(const orc_camera* cam, uint32_t W, uint32_t H, const float*
 // pixel: sum r, g, b, sample count, sum r^2, g^2, b^2, 0.  Box filter: every sample falls in
 // its own pixel (TSamplingFilm.ipp:103-134 with a unit-size filter).  Work is split into
 // 128x128 tiles over nthreads threads (EqualSamplingRenderer.cpp:61).
+int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int filter, double* film);
+
 int orc_render(const orc_scene* scene, const orc_render_opts* opt, double* film, orc_render_stats* stats) {
+    if ((opt->sampler & 0xFF) == 2) {
+        // Halton samples land anywhere in the image: rendered and splatted by the filtered path
+        // with the unit box
+        const auto t0 = std::chrono::steady_clock::now();
+        orc_render_filtered(scene, opt, 0, film);
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->paths = (uint64_t)opt->width * opt->height * opt->spp;
+            stats->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return 0;
+    }
     Scene S; S.s = scene;
     bvh_build(S);
     emitters_prepare(S);
@@ -1497,8 +1714,14 @@ void orc_render_paths(const orc_scene* scene, const orc_render_opts* opt, const 
     PathStats st;
     for (uint32_t i = 0; i < n; ++i) {
         Stream sf(path_key(opt->seed, pix[i], smp[i], npix));
-        const float rx = (float)(pix[i] % W) + sf.u();
-        const float ry = (float)(pix[i] / W) + sf.u();
+        float rx = (float)(pix[i] % W) + sf.u();
+        float ry = (float)(pix[i] / W) + sf.u();
+        if ((opt->sampler & 0xFF) == 2) {
+            double hx, hy;
+            halton_raster_of(pix[i], smp[i], W, H, 0.5, opt->sampler, halton_leap_of(opt->sampler, opt->seed), opt->seed,
+                             &hx, &hy);
+            rx = (float)hx; ry = (float)hy;
+        }
         V3 o, d;
         camera_ray(scene->camera, W, H, rx, ry, &o, &d);
         const RGB L = estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st,
@@ -1590,6 +1813,8 @@ int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int 
     std::vector<float> rgb(3 * ns);
     std::atomic<uint64_t> next{0};
     const double hs = filter_half(filter);
+    const bool halton = (opt->sampler & 0xFF) == 2;
+    const uint32_t leap = halton ? orc::halton_leap_of(opt->sampler, opt->seed) : 1;
     auto worker = [&]() {
         orc::PathStats st;
         for (uint64_t pix = next.fetch_add(1); pix < npix; pix = next.fetch_add(1)) {
@@ -1599,7 +1824,12 @@ int orc_render_filtered(const orc_scene* scene, const orc_render_opts* opt, int 
                 orc::Stream sf(orc::path_key(opt->seed, pix, opt->spp_begin + k, npix));
                 const float ux = sf.u(), uy = sf.u();
                 double xr = (double)((float)x + ux), yr = (double)((float)y + uy);
-                if (hs != 0.5) { xr = window_raster(x, ux, W, hs); yr = window_raster(y, uy, H, hs); }
+                if (halton) {
+                    orc::halton_raster_of((uint32_t)pix, opt->spp_begin + k, W, H, hs, opt->sampler, leap, opt->seed,
+                                          &xr, &yr);
+                } else if (hs != 0.5) {
+                    xr = window_raster(x, ux, W, hs); yr = window_raster(y, uy, H, hs);
+                }
                 orc::V3 o, d;
                 orc::camera_ray(scene->camera, W, H, (float)xr, (float)yr, &o, &d);
                 const orc::RGB L = orc::estimate_any(S, o, d, sf, opt->max_bounces, opt->estimator, &st,

@@ -63,7 +63,8 @@ typedef struct orc_render_opts {
     uint64_t seed;
     uint32_t row_step;  /* 0 or 1 = all rows; k = only rows y with y % k == 0 */
     uint32_t estimator; /* 0 BNEEPT, 1 BVPT, 2 BVPTDL */
-    uint32_t sampler;   /* 0 independent, 1 stratified first-vertex dimensions */
+    uint32_t sampler;   /* 0 independent, 1 stratified first-vertex dimensions, 2 Halton (with the
+                           permutation in bits 8-9 and the sequence in bits 12-13) */
     uint32_t pad;
 } orc_render_opts;
 

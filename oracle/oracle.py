@@ -183,7 +183,9 @@ def render_filtered(scene, width, height, spp, filter, max_bounces=10000, seed=0
 def render(scene, width, height, spp, max_bounces=10000, seed=0, spp_begin=0, nthreads=None, row_step=1,
            estimator=0, native=False, sampler=0):
     """FP64 film (H, W, 8) of the oracle render (estimator 0 BNEEPT, 1 BVPT, 2 BVPTDL; sampler 0
-    independent, 1 stratified), plus ray counts.  native=True runs the -O3 -march=native build."""
+    independent, 1 stratified, 2 Halton with its permutation / sequence bits), plus ray counts
+    (zero for Halton, whose samples are splatted).  native=True runs the -O3 -march=native
+    build."""
     L = lib(native)
     sv = SceneView(scene)
     film = np.zeros((height, width, 8), np.float64)
@@ -213,6 +215,62 @@ def color_energy(rgb):
     L.orc_color_energy.argtypes = [C.c_void_p]
     a = np.ascontiguousarray(rgb, np.float32).reshape(3)
     return float(L.orc_color_energy(a.ctypes.data))
+
+
+def halton_dim(dim, index, sampler, leap=1, seed=0):
+    """Dimension dim (0..6) of Halton point `index` under a sampler value (kind 2 with its
+    permutation and sequence bits) and leap."""
+    L = lib()
+    L.orc_halton_dim.restype = C.c_float
+    L.orc_halton_dim.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64]
+    return L.orc_halton_dim(dim, index, sampler, leap, seed)
+
+
+def kensler_permute(i, n, key):
+    """Kensler's hashed permutation of [0, n) with key `key` (math::permuted_index)."""
+    L = lib()
+    L.orc_kensler_permute.restype = C.c_uint32
+    L.orc_kensler_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    return L.orc_kensler_permute(i, n, key)
+
+
+def halton_float_digits(base):
+    L = lib()
+    L.orc_halton_float_digits.restype = C.c_uint32
+    L.orc_halton_float_digits.argtypes = [C.c_uint32]
+    return L.orc_halton_float_digits(base)
+
+
+def radical_inverse(base, value):
+    L = lib()
+    L.orc_radical_inverse.restype = C.c_float
+    L.orc_radical_inverse.argtypes = [C.c_uint32, C.c_uint64]
+    return L.orc_radical_inverse(base, value)
+
+
+def good_prime(n):
+    L = lib()
+    L.orc_good_prime.restype = C.c_uint32
+    L.orc_good_prime.argtypes = [C.c_uint32]
+    return L.orc_good_prime(n)
+
+
+def halton_leap(sampler, seed):
+    L = lib()
+    L.orc_halton_leap.restype = C.c_uint32
+    L.orc_halton_leap.argtypes = [C.c_uint32, C.c_uint64]
+    return L.orc_halton_leap(sampler, seed)
+
+
+def halton_raster(width, height, sample, sampler, seed=0, half=0.5):
+    """Raster positions (FP64) of the Halton source's batch `sample`: (W * H, 2), pixel order."""
+    L = lib()
+    L.orc_halton_raster.restype = None
+    L.orc_halton_raster.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32, C.c_uint64,
+                                    C.c_void_p]
+    out = np.zeros((width * height, 2), np.float64)
+    L.orc_halton_raster(width, height, sample, half, sampler, seed, out.ctypes.data)
+    return out
 
 
 def strat_perm(i, n, key):

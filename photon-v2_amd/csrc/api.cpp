@@ -1106,11 +1106,62 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     return PHG_OK;
 }
 
+// A sample-source value the renderer takes: a kind, and for Halton a permutation and a sequence.
+static bool sampler_ok(uint32_t sampler) {
+    const uint32_t kind = sampler & 0xFFu;
+    if (kind == PHG_SAMPLER_HALTON) return (sampler & ~0x33FFu) == 0u;
+    return kind <= PHG_SAMPLER_STRATIFIED && (sampler >> 8) == 0u;
+}
+
+// The n-th good prime (0-based): a prime p_k with p_k^2 > p_{k-i} p_{k+i} for every
+// 1 <= i <= k (math_table.h GOOD_PRIME lists the first 512, for the Halton leaps).
+static uint32_t good_prime(uint32_t n) {
+    static const std::vector<uint32_t> good = [] {
+        std::vector<uint32_t> primes;
+        for (uint32_t v = 2; primes.size() < 16384; ++v) {
+            bool p = true;
+            for (uint32_t q : primes) {
+                if (q * q > v) break;
+                if (v % q == 0) { p = false; break; }
+            }
+            if (p) primes.push_back(v);
+        }
+        std::vector<uint32_t> out;
+        for (size_t k = 1; k < primes.size() / 2 && out.size() < 512; ++k) {
+            bool ok = true;
+            for (size_t i = 1; i <= k && ok; ++i)
+                ok = (uint64_t)primes[k] * primes[k] > (uint64_t)primes[k - i] * primes[k + i];
+            if (ok) out.push_back(primes[k]);
+        }
+        return out;
+    }();
+    return good[n];
+}
+
+// moremur bit mixer (Math/hash.ipp:104-115), for host-side keyed choices
+static uint64_t mix64(uint64_t v) {
+    v ^= v >> 27; v *= 0x3C79AC492BA7B653ULL;
+    v ^= v >> 33; v *= 0x1C69B3F74AC4AE35ULL;
+    v ^= v >> 27;
+    return v;
+}
+
+// Index step of a Halton job's sequence (SGHalton.cpp:73-98): 1; the 257th good prime for
+// "leap"; for "random-leap" one of the good primes 152..512, keyed by the seed here.
+static uint32_t halton_leap(uint32_t sampler, uint64_t seed) {
+    if ((sampler & 0xFFu) != PHG_SAMPLER_HALTON) return 1u;
+    const uint32_t seq = sampler & 0x3000u;
+    if (seq == PHG_HALTON_SEQ_LEAP) return good_prime(256);
+    if (seq == PHG_HALTON_SEQ_RANDOM_LEAP) return good_prime(151u + (uint32_t)(mix64(seed ^ 0x51ED270B27ACE8CDULL) % 361u));
+    return 1u;
+}
+
 static JobDev make_job(const SceneInfo& si, uint32_t width, uint32_t height, uint32_t max_bounces, uint64_t seed,
                        uint64_t film_base, uint32_t estimator, uint32_t filter, uint32_t sampler) {
     JobDev j{};
     j.filter = filter;
     j.sampler = sampler;
+    j.halton_leap = halton_leap(sampler, seed);
     // BVPTDL is BVPT capped at one bounce
     j.estimator = estimator == PHG_ESTIMATOR_BNEEPT ? PHG_ESTIMATOR_BNEEPT : PHG_ESTIMATOR_BVPT;
     if (estimator == PHG_ESTIMATOR_BVPTDL) max_bounces = std::min(max_bounces, 1u);
@@ -1154,7 +1205,7 @@ phg_status phg_render(void* ctx, const phg_render_opts* o) {
         if (o->estimator != PHG_ESTIMATOR_BNEEPT && (o->flags & (PHG_RENDER_GUIDED | PHG_RENDER_TRAIN)))
             return set_err(c, PHG_ERR_UNSUPPORTED, "guiding needs the BNEEPT estimator");
         if (o->filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown filter");
-        if (o->sampler > PHG_SAMPLER_STRATIFIED) return set_err(c, PHG_ERR_INVALID_ARG, "unknown sampler");
+        if (!sampler_ok(o->sampler)) return set_err(c, PHG_ERR_INVALID_ARG, "unknown sampler");
         std::vector<JobDev> jobs{
             make_job(c->scenes[0], o->width, o->height, o->max_bounces, o->seed, 0, o->estimator, o->filter, o->sampler)};
         std::vector<PassPlan> passes;
@@ -1202,7 +1253,7 @@ phg_status phg_render_batch(void* ctx, const phg_batch_job* in, uint32_t n, cons
             if (b.spp == 0 || (uint64_t)b.spp > (1ull << 40) / npix) return set_err(c, PHG_ERR_INVALID_ARG, "job spp out of range");
             if (b.estimator > PHG_ESTIMATOR_BVPTDL) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job estimator");
             if (b.filter > PHG_FILTER_BH) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job filter");
-            if (b.sampler > PHG_SAMPLER_STRATIFIED) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job sampler");
+            if (!sampler_ok(b.sampler)) return set_err(c, PHG_ERR_INVALID_ARG, "unknown job sampler");
             jobs[j] = make_job(c->scenes[b.scene], b.width, b.height, b.max_bounces, b.seed, film_doubles, b.estimator,
                                b.filter, b.sampler);
             if (b.flags & PHG_JOB_CAMERA) jobs[j].cam = b.camera;

@@ -50,8 +50,8 @@ struct alignas(16) JobDev {
     uint64_t film_base;           // first double of the job's film (8 per pixel)
     uint32_t estimator;           // PHG_ESTIMATOR_BNEEPT, or BVPT (BVPTDL runs as BVPT capped at one bounce)
     uint32_t filter;              // PHG_FILTER_*: reconstruction filter of the film
-    uint32_t sampler;             // PHG_SAMPLER_*: sample source of the first vertex's dimensions
-    uint32_t pad;
+    uint32_t sampler;             // PHG_SAMPLER_* (Halton: with its permutation and sequence bits)
+    uint32_t halton_leap;         // Halton: index step of the sequence (1 but for the leap sequences)
 };
 static_assert(sizeof(JobDev) == 128, "job record is 128 bytes");
 

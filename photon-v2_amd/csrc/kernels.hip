@@ -154,7 +154,14 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         const uint32_t px = pixel % job.width, py = pixel / job.width;
         const float ux = s.u(), uy = s.u();
         float rx = (float)px + ux, ry = (float)py + uy;
-        if (job.filter != PHG_FILTER_BOX) {
+        if ((job.sampler & 0xFFu) == PHG_SAMPLER_HALTON) {
+            // Halton point sample N + pixel over the whole sample window (the stream's two
+            // numbers are drawn all the same, so later dimensions keep their stream positions)
+            double hx, hy;
+            halton_raster(job, pixel, sample, &hx, &hy);
+            rx = (float)hx;
+            ry = (float)hy;
+        } else if (job.filter != PHG_FILTER_BOX) {
             rx = (float)window_raster(px, ux, job.width, kFilterHalf);
             ry = (float)window_raster(py, uy, job.height, kFilterHalf);
         }
@@ -171,8 +178,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         pb.path_seg[i] = si;
         pb.rng[i] = s.state;
         pb.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        pb.Wem[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pb.prevP[i] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+        // the first vertex reads neither (no previous vertex): a Halton job's first-vertex
+        // dimensions 2..6 travel there to k_shade
+        float4 w0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p0 = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
+        if ((job.sampler & 0xFFu) == PHG_SAMPLER_HALTON) {
+            const uint64_t hidx = (uint64_t)sample * job.npix + pixel;
+            w0 = make_float4(halton_value(2u, hidx, job.sampler, job.halton_leap, job.seed),
+                             halton_value(3u, hidx, job.sampler, job.halton_leap, job.seed),
+                             halton_value(4u, hidx, job.sampler, job.halton_leap, job.seed),
+                             halton_value(5u, hidx, job.sampler, job.halton_leap, job.seed));
+            p0.x = halton_value(6u, hidx, job.sampler, job.halton_leap, job.seed);
+        }
+        pb.Wem[i] = w0;
+        pb.prevP[i] = p0;
         pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (pp.record)
             for (uint32_t b = 0; b < pp.rec_bounces; ++b)
@@ -505,6 +523,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                     u2 = strat_2d(pixel, job.npix, job.seed, sample, 0u, 1u, u2);
                     u3 = strat_2d(pixel, job.npix, job.seed, sample, 1u, 0u, u3);
                 }
+                // the Halton source's ray stage: dimensions 2..6 of the path's Halton point,
+                // computed by k_generate into the first vertex's (still unused) MIS state
+                const bool halton = bounce == 0u && (job.sampler & 0xFFu) == PHG_SAMPLER_HALTON;
+                if (halton) {
+                    const float4 hv = pb.Wem[pid];
+                    u0 = hv.x; u2 = hv.y; u3 = hv.z;
+                }
                 const float* lob = GUIDED ? pb.lobes + (size_t)i * kGuideDec : nullptr;
                 const bool guide_here = pp.guide_depth == 0 || bounce < pp.guide_depth;
                 const phg_material& m = sc.mats[mat];
@@ -572,6 +597,10 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                     if (strat) {
                         u4 = strat_2d(pixel, job.npix, job.seed, sample, 1u, 1u, u4);
                         u5 = strat_1d(pixel, job.npix, job.seed, sample, u5);
+                    }
+                    if (halton) {
+                        u4 = pb.Wem[pid].w;
+                        u5 = pb.prevP[pid].x;
                     }
                     float u8 = 0.0f, u9 = 0.0f, u10 = 0.0f, u11 = 0.0f;
                     if (GUIDED) { u8 = s.u(); u9 = s.u(); u10 = s.u(); u11 = s.u(); }
@@ -728,9 +757,40 @@ __device__ void film_gather(const JobDev& job, const PassSeg& sg, const PathBuff
     f[0] = f0; f[1] = f1; f[2] = f2; f[3] = f3;
 }
 
+// Film of a Halton job: the samples land anywhere in the sample window, so each path splats
+// its radiance into the pixels whose centres its filter covers, with FP64 atomics (the
+// reference's ranges, HdrRgbFilm.cpp:82-122: within the image, bounds inclusive, so a box
+// sample on a pixel edge counts in both pixels; the order of the additions, and so the last
+// bits of the sums, vary from run to run).
+__device__ void film_scatter_halton(const JobDev& job, const PassSeg& sg, const PathBuffers& pb, uint32_t x) {
+    const double hs = job.filter == PHG_FILTER_BOX ? 0.5 : kFilterHalf;
+    const uint32_t py = x / job.width;
+    const uint32_t slot = tile_slot(x - py * job.width, py, job.width, job.height);
+    for (uint32_t s = 0; s < sg.samples; ++s) {
+        double xp, yp;
+        halton_raster(job, x, sg.sample_begin + s, &xp, &yp);
+        const float4 L = pb.L[sg.path_offset + (size_t)s * sg.npix + slot];
+        const double fminx = fmax(xp - hs, 0.0), fminy = fmax(yp - hs, 0.0);
+        const double fmaxx = fmin(xp + hs, (double)job.width), fmaxy = fmin(yp + hs, (double)job.height);
+        // within [0, W) x [0, H): fmin >= 0 and fmax <= W, H
+        const int x0 = (int)ceil(fminx - 0.5), y0 = (int)ceil(fminy - 0.5);
+        const int x1 = (int)floor(fmaxx - 0.5) + 1, y1 = (int)floor(fmaxy - 0.5) + 1;
+        for (int yy = y0; yy < y1; ++yy)
+            for (int xx = x0; xx < x1; ++xx) {
+                const double w = job.filter == PHG_FILTER_BOX
+                                     ? 1.0 : film_filter(job.filter, (double)xx - (xp - 0.5), (double)yy - (yp - 0.5));
+                double* f = pb.film + job.film_base + 8 * ((size_t)yy * job.width + (size_t)xx);
+                atomicAdd(f + 0, w * L.x); atomicAdd(f + 1, w * L.y); atomicAdd(f + 2, w * L.z);
+                atomicAdd(f + 3, w);
+                atomicAdd(f + 4, w * (double)L.x * L.x); atomicAdd(f + 5, w * (double)L.y * L.y);
+                atomicAdd(f + 6, w * (double)L.z * L.z);
+            }
+    }
+}
+
 // FP64 film accumulation, owner computes: each pixel sums its own samples of this pass in
 // sample order (box filter, HdrRgbFilm::addRgbSample, HdrRgbFilm.cpp:82-122), or gathers the
-// filtered contributions of the samples around it.
+// filtered contributions of the samples around it.  Halton jobs scatter (film_scatter_halton).
 __global__ __launch_bounds__(kShadeBlock) void k_film(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb) {
     const uint32_t nseg = ppd->num_segs, total = ppd->film_threads;
     const PassSeg* __restrict__ segs = ppd->segs;
@@ -743,6 +803,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_film(SceneDev sc, const PassPar
         const PassSeg sg = segs[lo];
         const uint32_t x = t - sg.pix_offset;
         const JobDev& job = sc.jobs[sg.job];
+        if ((job.sampler & 0xFFu) == PHG_SAMPLER_HALTON) {
+            film_scatter_halton(job, sg, pb, x);
+            continue;
+        }
         if (job.filter != PHG_FILTER_BOX) {
             film_gather(job, sg, pb, x);
             continue;

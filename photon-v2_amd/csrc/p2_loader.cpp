@@ -538,11 +538,25 @@ struct Loader {
             if (!smp && r.category == "sample-source") smp = &r;
         }
         if (smp) st.spp = (uint32_t)std::max(1.0, real_or(*smp, "samples", 16));
-        // stratified (SGStratified) and uniform-random (SGUniformRandom) sources are rendered as
-        // such (PHG_SAMPLER_*); a Halton source places raster samples by a low-discrepancy
-        // sequence over the whole image (SGHalton.cpp) and is rendered with independent samples
-        st.sampler = smp && smp->type == "stratified" ? PHG_SAMPLER_STRATIFIED : PHG_SAMPLER_INDEPENDENT;
-        if (smp && smp->type == "halton") warn();
+        // stratified (SGStratified), Halton (SGHalton; HaltonSampleSource.h: permutation
+        // default fixed, sequence default original) and uniform-random (SGUniformRandom)
+        st.sampler = PHG_SAMPLER_INDEPENDENT;
+        if (smp && smp->type == "stratified") st.sampler = PHG_SAMPLER_STRATIFIED;
+        if (smp && smp->type == "halton") {
+            st.sampler = PHG_SAMPLER_HALTON;
+            if (const Clause* c = smp->find("permutation")) {
+                if (c->value == "none") st.sampler |= PHG_HALTON_PERM_NONE;
+                else if (c->value == "per-digit") st.sampler |= PHG_HALTON_PERM_PER_DIGIT;
+                else if (c->value == "owen") st.sampler |= PHG_HALTON_PERM_OWEN;
+                else if (!c->value.empty() && c->value != "fixed") warn();
+            }
+            if (const Clause* c = smp->find("sequence")) {
+                if (c->value == "leap") st.sampler |= PHG_HALTON_SEQ_LEAP;
+                else if (c->value == "random-leap") st.sampler |= PHG_HALTON_SEQ_RANDOM_LEAP;
+                else if (c->value == "random-start") st.sampler |= PHG_HALTON_SEQ_RANDOM_START;
+                else if (!c->value.empty() && c->value != "original") warn();
+            }
+        }
         if (vis) {
             if (vis->type != "path-tracing") warn();
             const Clause* e = vis->find("estimator");

@@ -168,6 +168,111 @@ PHG_HD float strat_1d(uint32_t pixel, uint32_t npix, uint64_t seed, uint32_t sam
     return strat_value(strat_permute(pixel, npix, strat_key(seed, sample, 2u)), npix, u);
 }
 
+// ---- Halton sample source (Core/SampleGenerator/SGHalton.cpp:22-206, Halton/halton.h,
+// Math/Random/Quasi/radical_inverse.h) -------------------------------------------------------
+// Sample si of batch b (a batch is one raster sample per pixel, si the pixel index) is Halton
+// point b N + si; its dimension d is the radical inverse in the d-th prime base of
+// start_d + (b N + si) leap.  Dimensions 0, 1 (bases 2, 3) place the raster sample over the
+// whole sample window; dimensions 2..6 (bases 5, 7, 11, 13, 17) are the ray stage's five, read
+// at the first surface as u0, u2, u3, u4, u5 (the stratified source's stages), each through the
+// chosen digit permutation.  Values are FP32 as the reference computes them (a float scale
+// that shrinks by 1/base per digit times the reversed digits converted to float, clamped to
+// [0, 1]); a value that rounds to 1 is used as the largest float below 1.
+constexpr uint32_t kHaltonBase[7] = {2u, 3u, 5u, 7u, 11u, 13u, 17u};
+enum : uint32_t { kHaltonPermFixed = 0, kHaltonPermNone = 1, kHaltonPermPerDigit = 2, kHaltonPermOwen = 3 };
+enum : uint32_t { kHaltonSeqOriginal = 0, kHaltonSeqLeap = 1, kHaltonSeqRandomLeap = 2, kHaltonSeqRandomStart = 3 };
+PHG_HD uint32_t halton_perm(uint32_t sampler) { return (sampler >> 8) & 3u; }
+PHG_HD uint32_t halton_seq(uint32_t sampler) { return (sampler >> 12) & 3u; }
+
+// Kensler's hashed permutation ("Correlated Multi-Jittered Sampling", 2013; the reference's
+// math::permuted_index, Math/hash.ipp:206-243): element i of a permutation of [0, l) keyed by p
+PHG_HD uint32_t halton_permute(uint32_t i, uint32_t l, uint32_t p) {
+    uint32_t w = l - 1u;
+    w |= w >> 1; w |= w >> 2; w |= w >> 4; w |= w >> 8; w |= w >> 16;
+    do {
+        i ^= p; i *= 0xe170893du; i ^= p >> 16; i ^= (i & w) >> 4;
+        i ^= p >> 8; i *= 0x0929eb3fu; i ^= p >> 23; i ^= (i & w) >> 1;
+        i *= 1u | p >> 27; i *= 0x6935fa69u; i ^= (i & w) >> 11; i *= 0x74dcb303u;
+        i ^= (i & w) >> 2; i *= 0x9e501cc3u; i ^= (i & w) >> 2; i *= 0xc860a3dfu;
+        i &= w; i ^= i >> 5;
+    } while (i >= l);
+    return (i + p) % l;
+}
+
+// digits of the base of each dimension that can still change a float in [0, 1): the count n
+// of the loop "while 1 - (b - 1) s < 1: n += 1, s *= 1 / b" in FP32 with every operation
+// rounded (radical_inverse.h num_meaningful_digits); a table, since a fused multiply-add
+// would change the loop's last comparison
+constexpr uint32_t kHaltonDigits[7] = {25u, 17u, 12u, 10u, 9u, 8u, 8u};
+
+PHG_HD float halton_below_one(float v) { return v < 1.0f ? v : 0x1.fffffep-1f; }
+
+// unpermuted radical inverse of v in base b (the raster dimensions, and every dimension with
+// permutation "none")
+PHG_HD float halton_ri(uint32_t b, uint64_t v) {
+    if (b == 2u) return (float)__builtin_bitreverse64(v) * 0x1p-64f;
+    const float rcp = 1.0f / (float)b;
+    const uint64_t safe = (~0ull - (b - 1u)) / b;
+    uint64_t rev = 0;
+    float scale = 1.0f;
+    while (v > 0u && rev <= safe) {
+        const uint64_t q = v / b;
+        rev = rev * b + (v - q * b);
+        scale *= rcp;
+        v = q;
+    }
+    return fminf(fmaxf((float)rev * scale, 0.0f), 1.0f);
+}
+
+// key of the keyed digit permutations: dimension d (and digit position k for per-digit) of a
+// job with stream seed `seed`
+PHG_HD uint32_t halton_key(uint64_t seed, uint32_t d, uint32_t k) {
+    return (uint32_t)(moremur((seed << 24) ^ ((uint64_t)d << 12) ^ (uint64_t)k ^ 0xA0761D6478BD642FULL) >> 32);
+}
+
+// permuted radical inverse of v in the base of dimension d (d >= 2): exactly kHaltonDigits[d]
+// digits, each passed through the permutation -- fixed: one keyed permutation of the digit
+// values per dimension; per-digit: one per dimension and digit position; owen: keyed by the
+// dimension and every digit already emitted (halton.h OwenScrambledPermuter)
+PHG_HD float halton_ri_perm(uint32_t d, uint64_t v, uint32_t perm, uint64_t seed) {
+    const uint32_t b = kHaltonBase[d];
+    const float rcp = 1.0f / (float)b;
+    const uint64_t safe = (~0ull - (b - 1u)) / b;
+    const uint32_t ndig = kHaltonDigits[d];
+    const uint32_t fixed_key = halton_key(seed, d, 0xFFFu);
+    uint64_t rev = 0, owen_hash = moremur(((uint64_t)d << 4) + 1u), owen_digits = 0;
+    float scale = 1.0f;
+    for (uint32_t k = 0; k < ndig && rev <= safe; ++k) {
+        const uint64_t q = v / b;
+        const uint32_t digit = (uint32_t)(v - q * b);
+        uint32_t pd;
+        if (perm == kHaltonPermOwen) {
+            owen_hash = moremur(owen_hash ^ owen_digits);
+            pd = halton_permute(digit, b, (uint32_t)owen_hash);
+            owen_digits = owen_digits * b + pd;
+        } else if (perm == kHaltonPermPerDigit) {
+            pd = halton_permute(digit, b, halton_key(seed, d, k));
+        } else {
+            pd = halton_permute(digit, b, fixed_key);
+        }
+        rev = rev * b + pd;
+        scale *= rcp;
+        v = q;
+    }
+    return fminf(fmaxf((float)rev * scale, 0.0f), 1.0f);
+}
+
+// dimension d (0..6) of Halton point `index` (b N + si) of a job: sampler carries the
+// permutation and sequence, leap the index step (1 but for the leap sequences)
+PHG_HD float halton_value(uint32_t d, uint64_t index, uint32_t sampler, uint32_t leap, uint64_t seed) {
+    uint64_t v = index * (uint64_t)leap;
+    if (halton_seq(sampler) == kHaltonSeqRandomStart)
+        v += moremur((seed << 24) ^ ((uint64_t)d << 8) ^ 0x2545F4914F6CDD1DULL);
+    const uint32_t perm = halton_perm(sampler);
+    const float x = (d < 2u || perm == kHaltonPermNone) ? halton_ri(kHaltonBase[d], v) : halton_ri_perm(d, v, perm, seed);
+    return halton_below_one(x);
+}
+
 // ---- watertight ray/triangle (Math/Geometry/TWatertightTriangle.ipp:16-149) -------------
 PHG_HD bool wt_intersect(V3 va, V3 vb, V3 vc, V3 org, V3 dir, float minT, float maxT,
                                              float* out_t, float* out_a, float* out_b, float* out_c) {
@@ -669,6 +774,19 @@ PHG_HD double window_raster(uint32_t x, float u, uint32_t n, double half) {
     const double lo = 0.5 - half;
     const double ext = ((double)n - 0.5 + half) - lo;
     return lo + (((double)x + (double)u) / (double)n) * ext;
+}
+
+// Raster position (FP64, as the film splats it) of the Halton source's sample of `pixel` in
+// batch `sample`: Halton dimensions 0 and 1 of point sample N + pixel over the sample window
+// (the image, or the soft-edged window of a filtered film: lo + v * extent).
+PHG_HD void halton_raster(const JobDev& job, uint32_t pixel, uint32_t sample, double* x, double* y) {
+    const uint64_t index = (uint64_t)sample * job.npix + pixel;
+    const double half = job.filter == PHG_FILTER_BOX ? 0.5 : kFilterHalf;
+    const float vx = halton_value(0u, index, job.sampler, job.halton_leap, job.seed);
+    const float vy = halton_value(1u, index, job.sampler, job.halton_leap, job.seed);
+    const double lo = 0.5 - half;
+    *x = lo + (double)vx * (((double)job.width - 0.5 + half) - lo);
+    *y = lo + (double)vy * (((double)job.height - 0.5 + half) - lo);
 }
 
 }  // namespace phg

@@ -35,7 +35,10 @@ BATCH_COUNT_TRAVERSAL, BATCH_GRAPH, BATCH_GUIDED, BATCH_TRAIN = 1, 2, 4, 8
 JOB_CAMERA = 1
 PHG_ESTIMATOR_BNEEPT, PHG_ESTIMATOR_BVPT, PHG_ESTIMATOR_BVPTDL = 0, 1, 2
 PHG_FILTER_BOX, PHG_FILTER_GAUSSIAN, PHG_FILTER_MN, PHG_FILTER_BH = 0, 1, 2, 3
-SAMPLER_INDEPENDENT, SAMPLER_STRATIFIED = 0, 1
+SAMPLER_INDEPENDENT, SAMPLER_STRATIFIED, SAMPLER_HALTON = 0, 1, 2
+# Halton permutation and sequence, OR-ed into SAMPLER_HALTON (phg.h PHG_HALTON_*)
+HALTON_PERM_FIXED, HALTON_PERM_NONE, HALTON_PERM_PER_DIGIT, HALTON_PERM_OWEN = 0 << 8, 1 << 8, 2 << 8, 3 << 8
+HALTON_SEQ_ORIGINAL, HALTON_SEQ_LEAP, HALTON_SEQ_RANDOM_LEAP, HALTON_SEQ_RANDOM_START = 0, 1 << 12, 2 << 12, 3 << 12
 GUIDE_PARAMS, GUIDE_OUTPUTS, GUIDE_RECORD = 19305, 41, 16
 FLT_MAX = float(np.finfo(np.float32).max)
 

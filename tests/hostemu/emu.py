@@ -25,6 +25,9 @@ def lib():
         _lib.emu_guide_sample.argtypes = [vp, C.c_uint32, vp, vp]
         _lib.emu_guide_encode.argtypes = [vp, C.c_uint32, vp]
         _lib.emu_pixel_order.argtypes = [C.c_uint32, C.c_uint32, vp, vp]
+        _lib.emu_halton_values.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, vp]
+        _lib.emu_halton_raster.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.c_uint64, vp]
     return _lib
 
 
@@ -95,3 +98,18 @@ def pixel_order(W, H):
     q = np.zeros(W * H, np.uint32)
     lib().emu_pixel_order(W, H, p.ctypes.data, q.ctypes.data)
     return p, q
+
+
+def halton_values(dims, index, sampler, leap=1, seed=0):
+    """The product's Halton dimension values (float32) for arrays of dimensions and indices."""
+    d = np.ascontiguousarray(dims, np.uint32)
+    i = np.ascontiguousarray(index, np.uint64)
+    out = np.zeros(len(d), np.float32)
+    lib().emu_halton_values(d.ctypes.data, i.ctypes.data, len(d), sampler, leap, seed, out.ctypes.data)
+    return out
+
+
+def halton_raster(W, H, sample, filter, sampler, leap=1, seed=0):
+    out = np.zeros((W * H, 2), np.float64)
+    lib().emu_halton_raster(W, H, sample, filter, sampler, leap, seed, out.ctypes.data)
+    return out

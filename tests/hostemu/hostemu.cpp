@@ -88,3 +88,19 @@ extern "C" void emu_pixel_order(uint32_t W, uint32_t H, uint32_t* pixel_of_slot,
     for (uint32_t y = 0; y < H; ++y)
         for (uint32_t x = 0; x < W; ++x) slot_of_pixel[y * W + x] = phg::tile_slot(x, y, W, H);
 }
+
+// The product's Halton source for the host: dimension dims[i] of point index[i] of a job with
+// the given sampler value, leap and seed (pt_device.h halton_value).
+extern "C" void emu_halton_values(const uint32_t* dims, const uint64_t* index, uint32_t n, uint32_t sampler,
+                                  uint32_t leap, uint64_t seed, float* out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = halton_value(dims[i], index[i], sampler, leap, seed);
+}
+
+// The product's Halton raster positions of batch `sample` of a W x H job (filter: PHG_FILTER_*)
+extern "C" void emu_halton_raster(uint32_t W, uint32_t H, uint32_t sample, uint32_t filter, uint32_t sampler,
+                                  uint32_t leap, uint64_t seed, double* xy) {
+    JobDev job{};
+    job.width = W; job.height = H; job.npix = W * H;
+    job.filter = filter; job.sampler = sampler; job.halton_leap = leap; job.seed = seed;
+    for (uint32_t p = 0; p < W * H; ++p) halton_raster(job, p, sample, xy + 2 * (size_t)p, xy + 2 * (size_t)p + 1);
+}

@@ -317,3 +317,55 @@ def test_p2_exact_conductor_matches_oracle(ctx):
     gpu = gpu_render(ctx, sc, W, H, spp, B, 19)
     ref, _ = orc.render(sc, W, H, spp, max_bounces=B, seed=19)
     check_parity(gpu, ref, "p2_exact_conductor", 0.5)
+
+
+HALTON_CASES = [phg.SAMPLER_HALTON,
+                phg.SAMPLER_HALTON | phg.HALTON_PERM_OWEN | phg.HALTON_SEQ_RANDOM_START,
+                phg.SAMPLER_HALTON | phg.HALTON_PERM_PER_DIGIT | phg.HALTON_SEQ_LEAP]
+
+
+@pytest.mark.parametrize("sampler", HALTON_CASES)
+def test_halton_source_matches_oracle(ctx, sampler):
+    """sample-source(halton) (SGHalton.cpp:22-206): raster samples from Halton dimensions 0, 1
+    over the whole image, splatted into the pixels they land in; the first surface's five
+    dimensions from dimensions 2..6.  Sample counts per pixel (which the raster positions alone
+    fix, edge samples counted twice) are exactly the oracle's; radiance sums agree where the
+    paths do (the film's additions are atomic, so only their order differs)."""
+    sc = phg.Scene.generate(phg.GEN_CORNELL)
+    W = H = 64
+    print(f"[render] case=cornell_halton sampler={sampler:#x} start", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, 16, 5, 11, sampler=sampler, spp_per_pass=5)
+    ref, _ = orc.render(sc, W, H, 16, max_bounces=5, seed=11, sampler=sampler)
+    check_parity(gpu, ref, f"cornell_halton_{sampler:#x}", 0.9)
+    if sampler & 0x3000 == phg.HALTON_SEQ_ORIGINAL:
+        # the original sequence's points sit on pixel edges often: those count in both pixels
+        assert gpu[..., 3].sum() > W * H * 16
+    ind = gpu_render(ctx, sc, W, H, 16, 5, 11)
+    assert not np.array_equal(gpu[..., 3], ind[..., 3])
+
+
+@pytest.mark.parametrize("filt", [phg.PHG_FILTER_BOX, phg.PHG_FILTER_GAUSSIAN])
+def test_halton_film_matches_oracle_exactly_on_lights(ctx, filt):
+    """Lights seen directly (no transcendental function on the path): every Halton sample's
+    radiance is the oracle's, so the splatted films agree to FP64 rounding, box and Gaussian.
+    An odd number of squares keeps their edges off the image centre: the original sequence
+    puts many points exactly on x = W / 2, where a ray on the edge between two squares is a
+    tie that the two BVHs may break differently."""
+    cb = checkerboard_lights(11)
+    W, H = 40, 30
+    for sampler in HALTON_CASES:
+        print(f"[render] case=lights_halton filter={filt} sampler={sampler:#x} start", flush=True)
+        gpu = gpu_render(ctx, cb, W, H, 8, 0, 31, filter=filt, sampler=sampler, spp_per_pass=3)
+        ref = orc.render_filtered(cb, W, H, 8, filt, max_bounces=0, seed=31, sampler=sampler)
+        assert np.allclose(gpu[..., :4], ref[..., :4], rtol=1e-12, atol=1e-12)
+        assert np.allclose(gpu[..., 4:7], ref[..., 4:7], rtol=1e-12, atol=1e-12)
+
+
+def test_halton_furnace_closed_form(ctx):
+    sc = furnace_scene(albedo=0.5, radiance=0.5)
+    film = gpu_render(ctx, sc, 32, 32, 64, 10000, 1, sampler=phg.SAMPLER_HALTON | phg.HALTON_PERM_OWEN)
+    n = film[..., 3].sum()
+    mean = film[..., 0].sum() / n
+    se = np.sqrt(max(film[..., 4].sum() / n - mean * mean, 0.0) / n) * 2.0
+    print(f"[render] case=furnace_halton mean={mean:.6f} se={se:.2e}", flush=True)
+    assert abs(mean - 1.0) < 4 * se + 1e-3

@@ -181,3 +181,21 @@ def test_pfm_writer_layout(tmp_path):
     px = np.frombuffer(data[len(header):], "<f4").reshape(H, W, 3)
     expect = np.where(film[..., 3:4] > 0, film[..., :3] / np.where(film[..., 3:4] > 0, film[..., 3:4], 1), 0)
     assert np.allclose(px, expect.astype(np.float32)) and np.all(px[1, 2] == 0)
+
+
+@pytest.mark.parametrize("clauses,want", [
+    ("", phg.SAMPLER_HALTON | phg.HALTON_PERM_FIXED | phg.HALTON_SEQ_ORIGINAL),
+    ("[enum permutation owen] [enum sequence random-start]", phg.SAMPLER_HALTON | phg.HALTON_PERM_OWEN
+     | phg.HALTON_SEQ_RANDOM_START),
+    ("[enum permutation per-digit] [enum sequence leap]", phg.SAMPLER_HALTON | phg.HALTON_PERM_PER_DIGIT
+     | phg.HALTON_SEQ_LEAP),
+    ("[enum permutation none] [enum sequence random-leap]", phg.SAMPLER_HALTON | phg.HALTON_PERM_NONE
+     | phg.HALTON_SEQ_RANDOM_LEAP),
+])
+def test_halton_sample_source(clauses, want):
+    """sample-source(halton) with HaltonSampleSource's permutation (default fixed) and sequence
+    (default original)."""
+    text = f"sample-source(halton) @s = [integer samples 8] {clauses};\n" + \
+        "geometry(rectangle) @g = [real width 1] [real height 1];\nactor(model) @m = [geometry geometry @g];"
+    _, st = phg.Scene.parse_p2(text)
+    assert st["sampler"] == want and st["spp"] == 8 and st["warnings"] == 0
