@@ -178,10 +178,23 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         pb.path_seg[i] = si;
         pb.rng[i] = s.state;
         pb.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        // the first vertex reads neither (no previous vertex): a Halton job's first-vertex
-        // dimensions 2..6 travel there to k_shade
+        // the first vertex reads neither (no previous vertex): the first surface's dimensions
+        // u0, u2, u3, u4 and u5 of a stratified or Halton job travel there to k_shade, which
+        // then holds none of this code (it would push the shading kernel past its registers)
         float4 w0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p0 = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
-        if ((job.sampler & 0xFFu) == PHG_SAMPLER_HALTON) {
+        if (job.sampler == PHG_SAMPLER_STRATIFIED) {
+            // the stream values k_shade draws at the first vertex (u0 .. u5, after the raster
+            // pair) are the strata's jitters: stages (u0, u2), (u3, u4), u5
+            Stream t = s;
+            const float j0 = t.u();
+            (void)t.u();
+            const float j2 = t.u(), j3 = t.u(), j4 = t.u(), j5 = t.u();
+            w0 = make_float4(strat_2d(pixel, job.npix, job.seed, sample, 0u, 0u, j0),
+                             strat_2d(pixel, job.npix, job.seed, sample, 0u, 1u, j2),
+                             strat_2d(pixel, job.npix, job.seed, sample, 1u, 0u, j3),
+                             strat_2d(pixel, job.npix, job.seed, sample, 1u, 1u, j4));
+            p0.x = strat_1d(pixel, job.npix, job.seed, sample, j5);
+        } else if ((job.sampler & 0xFFu) == PHG_SAMPLER_HALTON) {
             const uint64_t hidx = (uint64_t)sample * job.npix + pixel;
             w0 = make_float4(halton_value(2u, hidx, job.sampler, job.halton_leap, job.seed),
                              halton_value(3u, hidx, job.sampler, job.halton_leap, job.seed),
@@ -516,17 +529,10 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                 float u0 = s.u();
                 const float u1 = s.u();
                 float u2 = s.u(), u3 = s.u();
-                // the stratified source's first-vertex dimensions (stages (u0, u2), (u3, u4), u5)
-                const bool strat = bounce == 0u && job.sampler == PHG_SAMPLER_STRATIFIED;
-                if (strat) {
-                    u0 = strat_2d(pixel, job.npix, job.seed, sample, 0u, 0u, u0);
-                    u2 = strat_2d(pixel, job.npix, job.seed, sample, 0u, 1u, u2);
-                    u3 = strat_2d(pixel, job.npix, job.seed, sample, 1u, 0u, u3);
-                }
-                // the Halton source's ray stage: dimensions 2..6 of the path's Halton point,
+                // a stratified or Halton job's first-surface dimensions (u0, u2, u3, u4, u5),
                 // computed by k_generate into the first vertex's (still unused) MIS state
-                const bool halton = bounce == 0u && (job.sampler & 0xFFu) == PHG_SAMPLER_HALTON;
-                if (halton) {
+                const bool src0 = bounce == 0u && job.sampler != PHG_SAMPLER_INDEPENDENT;
+                if (src0) {
                     const float4 hv = pb.Wem[pid];
                     u0 = hv.x; u2 = hv.y; u3 = hv.z;
                 }
@@ -594,11 +600,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                     // ---- BSDF sample + extension (BNEEPTEstimator.cpp:151-255)
                     float u4 = s.u(), u5 = s.u();
                     const float u6 = s.u(), u7 = s.u();
-                    if (strat) {
-                        u4 = strat_2d(pixel, job.npix, job.seed, sample, 1u, 1u, u4);
-                        u5 = strat_1d(pixel, job.npix, job.seed, sample, u5);
-                    }
-                    if (halton) {
+                    if (src0) {
                         u4 = pb.Wem[pid].w;
                         u5 = pb.prevP[pid].x;
                     }
