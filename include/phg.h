@@ -408,6 +408,9 @@ phg_status phg_guide_get_params(void* ctx, float* params, uint32_t n);
 /* raw network outputs (41 per query: 8 lobes x (mean direction 3, log kappa, weight logit), selection
  * logit) for 9 features per query */
 phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out);
+/* decoded mixture (41 per query: 8 lobes x (unit mean direction 3, kappa, normalised weight), then
+ * the lobes' selection probability 0.05 + 0.9 sigmoid(logit)), as the guided shade stage reads it */
+phg_status phg_guide_lobes(void* ctx, const float* features, uint64_t n, float* out);
 /* gradient of the mean of the records' losses (above) */
 phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, float* grads);
 /* `steps` Adam steps on the same records (learning rate lr) */

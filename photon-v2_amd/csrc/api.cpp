@@ -1368,7 +1368,7 @@ phg_status phg_guide_get_params(void* ctx, float* params, uint32_t n) {
     });
 }
 
-phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out) {
+static phg_status guide_infer_host(void* ctx, const float* features, uint64_t n, float* out, int raw) {
     return guarded(ctx, [&]() -> phg_status {
         Ctx* c = static_cast<Ctx*>(ctx);
         if (!c || !features || !out || n >= (1ull << 31)) return PHG_ERR_INVALID_ARG;
@@ -1379,11 +1379,19 @@ phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float
         float* d_in = c->g_q.p;
         float* d_out = c->g_q.p + n * 9;
         PHG_HIP(c, hipMemcpyAsync(d_in, features, n * 9 * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        PHG_HIP(c, guide_infer_features(guide_net(c), d_in, (uint32_t)n, d_out, 1, c->cu_count * 2, c->stream));
+        PHG_HIP(c, guide_infer_features(guide_net(c), d_in, (uint32_t)n, d_out, raw, c->cu_count * 2, c->stream));
         PHG_HIP(c, hipMemcpyAsync(out, d_out, n * kGuideOutputs * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         PHG_HIP(c, hipStreamSynchronize(c->stream));
         return PHG_OK;
     });
+}
+
+phg_status phg_guide_forward(void* ctx, const float* features, uint64_t n, float* out) {
+    return guide_infer_host(ctx, features, n, out, 1);
+}
+
+phg_status phg_guide_lobes(void* ctx, const float* features, uint64_t n, float* out) {
+    return guide_infer_host(ctx, features, n, out, 0);
 }
 
 phg_status phg_guide_gradients(void* ctx, const float* records, uint64_t n, float* grads) {

@@ -38,6 +38,17 @@ __host__ __device__ __forceinline__ int layer_out(int l) { return l == kGuideLay
 // (slots s = 0..9 at rows 16 (s / 4) + 4 g + s % 4), and the selection logit (output 40) to
 // slot 10 of lane group 0; the other 7 of the 48 rows stay zero.
 __host__ __device__ inline int infer_in_index(int kc, int g, int j) { return 16 * (2 * kc + (j >> 2)) + 4 * g + (j & 3); }
+// The first layer's input is the encoding, so its k order is free: lane group g encodes
+// feature coordinates 2g (k-chunk 0) and 2g + 1 (k-chunk 1) whole — slot 0 the raw value,
+// slots 1..6 sin and cos at the three bands — and slot 7 holds raw coordinate 8 and the
+// constant (g = 0) or the sine and cosine of coordinate 8 at band g - 1 (g = 1..3).  A lane
+// then evaluates two coordinates' encodings instead of 16 scattered entries.
+__host__ __device__ inline int infer_in0_index(int kc, int g, int j) {
+    if (j == 7) return g == 0 ? (kc == 0 ? 8 : kGuideIn - 1) : 9 + 18 * (g - 1) + 16 + kc;
+    const int c = 2 * g + kc;
+    if (j == 0) return c;
+    return 9 + 18 * ((j - 1) >> 1) + 2 * c + ((j - 1) & 1);
+}
 // logical output (0..40) at padded output row R of the last layer, or -1
 __host__ __device__ inline int infer_out_logical(int R) {
     const int g = (R & 15) >> 2, s = 4 * (R >> 4) + (R & 3);
@@ -55,11 +66,13 @@ __global__ void k_guide_prepare(const float* __restrict__ params, uint16_t* __re
     __bf16* wp = reinterpret_cast<__bf16*>(wp_bits);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < kGuideLayers * 64 * 64) {
-        // wp index ((((l * 4 + mt) * 2 + kc) * 16 + m) * 4 + g) * 8 + j: output row 16 mt + m
-        // of layer l, input infer_in_index(kc, g, j)
+        // wp index ((((l * 4 + mt) * 2 + kc) * 4 + g) * 16 + m) * 8 + j: output row 16 mt + m
+        // of layer l, input infer_in0_index(kc, g, j) (l = 0) or infer_in_index(kc, g, j); lane
+        // 16 g + m reads its 16-byte A operand at 16 (16 g + m) bytes from the (mt, kc) block, so
+        // the lanes of a wave read consecutive bytes (no LDS bank conflicts)
         const int l = i / 4096, rem = i % 4096;
-        const int mt = rem >> 10, kc = (rem >> 9) & 1, m = (rem >> 5) & 15, g = (rem >> 3) & 3, j = rem & 7;
-        const int R = 16 * mt + m, in = infer_in_index(kc, g, j);
+        const int mt = rem >> 10, kc = (rem >> 9) & 1, g = (rem >> 7) & 3, m = (rem >> 3) & 15, j = rem & 7;
+        const int R = 16 * mt + m, in = l == 0 ? infer_in0_index(kc, g, j) : infer_in_index(kc, g, j);
         const int nout = layer_out(l);
         const int o = l == kGuideLayers - 1 ? infer_out_logical(R) : R;
         const float* W = params + kGuideParamOffsets[2 * l];
@@ -125,6 +138,49 @@ struct InferSource {
     GuideScene gs;
 };
 
+// The inputs of one query as loaded from memory, fetched an iteration ahead of their use:
+// a feature row (a = x0..3, b = x4..7, c = x8), or a queued hit (a) and its ray's direction (b).
+struct QueryRaw {
+    float4 a, b;
+    float c;
+};
+
+__device__ __forceinline__ QueryRaw query_fetch(const InferSource& src, uint32_t q, uint32_t n) {
+    QueryRaw r;
+    r.a = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
+    r.b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r.c = 0.0f;
+    if (q >= n) {
+        if (src.features) r.a.x = 0.0f;
+        return r;
+    }
+    if (src.features) {
+        const float* f = src.features + 9 * (size_t)q;
+        r.a = make_float4(f[0], f[1], f[2], f[3]);
+        r.b = make_float4(f[4], f[5], f[6], f[7]);
+        r.c = f[8];
+    } else {
+        r.a = src.hits[q];
+        r.b = reinterpret_cast<const float4*>(src.rays + q)[1];
+    }
+    return r;
+}
+
+// features of a fetched query (zeros for a miss or a query past the end)
+__device__ __forceinline__ void query_features_of(const InferSource& src, const QueryRaw& r, float* x) {
+    if (src.features) {
+        x[0] = r.a.x; x[1] = r.a.y; x[2] = r.a.z; x[3] = r.a.w;
+        x[4] = r.b.x; x[5] = r.b.y; x[6] = r.b.z; x[7] = r.b.w; x[8] = r.c;
+        return;
+    }
+    const int32_t tri = __float_as_int(r.a.x);
+    if (tri < 0) { for (int k = 0; k < 9; ++k) x[k] = 0.0f; return; }
+    V3 a, b, c;
+    load_tri(src.tris, (uint32_t)tri, &a, &b, &c);
+    const SurfaceHit X = make_hit(a, b, c, (uint32_t)tri, r.a.y, r.a.z, r.a.w);
+    guide_features(src.gs, X.P, X.Ns, mk(-r.b.x, -r.b.y, -r.b.z), x);
+}
+
 __device__ __forceinline__ void query_features(const InferSource& src, uint32_t q, float* x) {
     if (src.features) {
         for (int k = 0; k < 9; ++k) x[k] = src.features[9 * (size_t)q + k];
@@ -140,20 +196,45 @@ __device__ __forceinline__ void query_features(const InferSource& src, uint32_t 
     guide_features(src.gs, X.P, X.Ns, mk(-rd.x, -rd.y, -rd.z), x);
 }
 
-// Inference: decoded mixture (8 x (mu, kappa, weight)) per query, or raw outputs.
-// Each wave takes 64 queries per iteration: every lane computes one query's features, then
-// four tiles of 16 queries go through the 5 layers with the activations in registers (each
-// lane group g of a tile holds 16 of a query's 64 values, see infer_in_index); the permuted
-// weights (40 KB) are staged in LDS once per block.  The last layer leaves lobes 2g and 2g + 1
-// of query r in lane 16 g + r, which decodes them; the softmax over the 8 lobe weights
-// reduces across the 4 lanes of a query.
+// Inference: decoded mixture (8 x (mu, kappa, weight), selection probability) per query, or
+// raw outputs.  Each wave takes 64 queries per iteration: every lane computes one query's
+// 9 features (kept in LDS), then the four 16-query tiles go through the 5 layers together,
+// so that each weight operand read from LDS feeds four matrix-core products.  Activations
+// stay in registers: a layer is computed transposed (see infer_in_index), the bias enters
+// as the accumulator of the first product, and ReLU acts on the packed bf16 results (the
+// sign bit decides, so max(bf16(v), 0) as 16-bit integers equals bf16(max(v, 0))).  The last
+// layer leaves lobes 2g and 2g + 1 of query r in lane 16 g + r, which decodes them; the
+// softmax over the 8 lobe weights reduces across the 4 lanes of a query.
 constexpr int kInferWaves = 8;
+typedef __attribute__((ext_vector_type(8))) short s16x8;
+
+typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+
+// two floats to packed bf16, round to nearest even: converting them as one 2-vector makes the
+// compiler emit one v_cvt_pk_bf16_f32 per pair instead of two conversions and a re-pack
+__device__ __forceinline__ unsigned cvt_pk_bf16(float lo, float hi) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+__device__ __forceinline__ bf16x8 pack_bf16(f32x4 lo, f32x4 hi) {
+    const u32x4 u = {cvt_pk_bf16(lo[0], lo[1]), cvt_pk_bf16(lo[2], lo[3]), cvt_pk_bf16(hi[0], hi[1]),
+                     cvt_pk_bf16(hi[2], hi[3])};
+    return __builtin_bit_cast(bf16x8, u);
+}
+__device__ __forceinline__ bf16x8 relu_pack(f32x4 lo, f32x4 hi) {
+    s16x8 s = __builtin_bit_cast(s16x8, pack_bf16(lo, hi));
+    s = __builtin_elementwise_max(s, (s16x8)0);
+    return __builtin_bit_cast(bf16x8, s);
+}
+
 __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev net, InferSource src,
                                                                   const uint32_t* counts, const uint32_t* bounce_base,
                                                                   uint32_t bk, uint32_t count, float* __restrict__ out,
                                                                   int raw, unsigned long long* queries) {
     __shared__ __attribute__((aligned(16))) __bf16 s_w[kGuideLayers * 64 * 64];
-    __shared__ float s_b[kGuideLayers * 64];
+    __shared__ __attribute__((aligned(16))) float s_b[kGuideLayers * 64];
+    __shared__ float s_x[kInferWaves][9 * 64];   // features of the wave's 64 queries, [coordinate][query]
     // queued hits of bounce bounce_base[0] + bk (the extension-queue count of that bounce)
     const uint32_t n = counts ? counts[(bounce_base[0] + bk) % kCountSlots] : count;
     if (queries && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(queries, (unsigned long long)n);
@@ -163,55 +244,82 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
     for (int i = threadIdx.x; i < kGuideLayers * 64; i += 64 * kInferWaves) s_b[i] = net.bp[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    float* xs = s_x[wave];
+    // band g - 1 of coordinate 8 (slot 7 of lane groups 1..3), in revolutions
+    const float scale8 = g == 0 ? 0.0f : (float)(1 << (g - 1)) * 0.5f;
     const uint32_t step = gridDim.x * kInferWaves * 64u;
-    for (uint32_t base = (blockIdx.x * kInferWaves + wave) * 64u; base < n; base += step) {
-        float x[9];
-        const uint32_t qf = base + (uint32_t)lane;
-        if (qf < n) query_features(src, qf, x);
-        else for (int k = 0; k < 9; ++k) x[k] = 0.0f;
+    const uint32_t first = (blockIdx.x * kInferWaves + wave) * 64u;
+    QueryRaw next = query_fetch(src, first + (uint32_t)lane, n);
+    for (uint32_t base = first; base < n; base += step) {
+        {
+            // this iteration's query was fetched during the previous one; the next one is
+            // fetched after this one's features are made (loads complete in issue order, so
+            // waiting for the triangle would otherwise wait for it too) and stays in flight
+            // while this iteration computes
+            float x[9];
+            query_features_of(src, next, x);
+            next = query_fetch(src, base + step + (uint32_t)lane, n);
+            wave_sync();   // the previous iteration's reads of xs are done
+#pragma unroll
+            for (int k = 0; k < 9; ++k) xs[k * 64 + lane] = x[k];
+            wave_sync();
+        }
+        // layer-0 operands of the four tiles (infer_in0_index)
+        bf16x8 bop[4][2];
+#pragma unroll
+        for (int T = 0; T < 4; ++T) {
+            const int qi = 16 * T + r;
+            const float v8 = xs[8 * 64 + qi];
+            const float r8 = scale8 * v8;
+            const float e0 = g == 0 ? v8 : __builtin_amdgcn_sinf(r8);
+            const float e1 = g == 0 ? 1.0f : __builtin_amdgcn_cosf(r8);
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc) {
+                const float v = xs[(2 * g + kc) * 64 + qi];
+                float e[8];
+                e[0] = v;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const float rev = (float)(1 << b) * 0.5f * v;
+                    e[1 + 2 * b] = __builtin_amdgcn_sinf(rev);
+                    e[2 + 2 * b] = __builtin_amdgcn_cosf(rev);
+                }
+                e[7] = kc == 0 ? e0 : e1;
+                bop[T][kc] = pack_bf16(f32x4{e[0], e[1], e[2], e[3]}, f32x4{e[4], e[5], e[6], e[7]});
+            }
+        }
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int l = 0; l < kGuideLayers; ++l) {
+            const int nmt = l == kGuideLayers - 1 ? kGuideOutPad / 16 : 4;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                if (mt >= nmt) continue;
+                const f32x4 bias = *reinterpret_cast<const f32x4*>(s_b + l * 64 + 16 * mt + 4 * g);
+#pragma unroll
+                for (int kc = 0; kc < 2; ++kc) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(s_w + l * 4096 + ((mt * 2 + kc) * 64 + lane) * 8);
+#pragma unroll
+                    for (int T = 0; T < 4; ++T) acc[T][mt] = mfma16(a, bop[T][kc], kc == 0 ? bias : acc[T][mt]);
+                }
+            }
+            if (l < kGuideLayers - 1) {
+#pragma unroll
+                for (int T = 0; T < 4; ++T) {
+                    bop[T][0] = relu_pack(acc[T][0], acc[T][1]);
+                    bop[T][1] = relu_pack(acc[T][2], acc[T][3]);
+                }
+            }
+        }
+#pragma unroll
         for (int T = 0; T < 4; ++T) {
             if (base + 16u * T >= n) break;  // whole wave
             const uint32_t q = base + 16u * T + (uint32_t)r;
-            float xf[9];
-#pragma unroll
-            for (int k = 0; k < 9; ++k) xf[k] = __shfl(x[k], 16 * T + r);
-            bf16x8 bop[2];
-#pragma unroll
-            for (int kc = 0; kc < 2; ++kc)
-#pragma unroll
-                for (int j = 0; j < 8; ++j) bop[kc][j] = (__bf16)guide_encode(xf, infer_in_index(kc, g, j));
-            f32x4 acc[4];
-#pragma unroll
-            for (int l = 0; l < kGuideLayers; ++l) {
-                const int nmt = l == kGuideLayers - 1 ? kGuideOutPad / 16 : 4;
-#pragma unroll
-                for (int mt = 0; mt < 4; ++mt) {
-                    if (mt >= nmt) continue;
-                    acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                    for (int kc = 0; kc < 2; ++kc) {
-                        const bf16x8 a = *reinterpret_cast<const bf16x8*>(
-                            s_w + l * 4096 + (((mt * 2 + kc) * 16 + r) * 4 + g) * 8);
-                        acc[mt] = mfma16(a, bop[kc], acc[mt]);
-                    }
-                }
-                if (l < kGuideLayers - 1) {
-#pragma unroll
-                    for (int kc = 0; kc < 2; ++kc)
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const int mt = 2 * kc + (j >> 2), i = j & 3;
-                            const float v = acc[mt][i] + s_b[l * 64 + 16 * mt + 4 * g + i];
-                            bop[kc][j] = (__bf16)fmaxf(v, 0.0f);
-                        }
-                }
-            }
             // slots s = 0..9 of this lane: logical outputs 10 g + s (lobes 2g, 2g + 1); slot 10
             // of lane group 0: the selection logit
             float o[11];
 #pragma unroll
-            for (int s2 = 0; s2 < 11; ++s2)
-                o[s2] = acc[s2 >> 2][s2 & 3] + s_b[(kGuideLayers - 1) * 64 + 16 * (s2 >> 2) + 4 * g + (s2 & 3)];
+            for (int s2 = 0; s2 < 11; ++s2) o[s2] = acc[T][s2 >> 2][s2 & 3];
             if (raw) {
                 float* dst = out + (size_t)q * kGuideOut + 10 * g;
                 if (q < n) {
@@ -225,20 +333,27 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
             float mx = fmaxf(o[4], o[9]);
             mx = fmaxf(mx, __shfl_xor(mx, 16));
             mx = fmaxf(mx, __shfl_xor(mx, 32));
-            const float ea = expf(o[4] - mx), eb = expf(o[9] - mx);
+            const float ea = __expf(o[4] - mx), eb = __expf(o[9] - mx);
             float sum = ea + eb;
             sum += __shfl_xor(sum, 16);
             sum += __shfl_xor(sum, 32);
+            // hardware reciprocal and reciprocal square root (about 1 ulp): the lobes are stored
+            // once and the shade stage samples and evaluates them from the same values
+            const float inv_sum = __builtin_amdgcn_rcpf(sum);
             if (q < n) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const float* oh = o + 5 * h;
-                    const float rl = sqrtf(oh[0] * oh[0] + oh[1] * oh[1] + oh[2] * oh[2]);
+                    const float r2 = oh[0] * oh[0] + oh[1] * oh[1] + oh[2] * oh[2];
                     float* L = dst + 5 * h;
-                    if (rl > 0.0f) { L[0] = oh[0] / rl; L[1] = oh[1] / rl; L[2] = oh[2] / rl; }
-                    else { L[0] = 0.0f; L[1] = 0.0f; L[2] = 1.0f; }
-                    L[3] = expf(clampf(oh[3], kKappaLogMin, kKappaLogMax));
-                    L[4] = (h == 0 ? ea : eb) / sum;
+                    if (r2 > 0x1p-100f) {
+                        const float inv = __builtin_amdgcn_rsqf(r2);
+                        L[0] = oh[0] * inv; L[1] = oh[1] * inv; L[2] = oh[2] * inv;
+                    } else {
+                        L[0] = 0.0f; L[1] = 0.0f; L[2] = 1.0f;
+                    }
+                    L[3] = __expf(clampf(oh[3], kKappaLogMin, kKappaLogMax));
+                    L[4] = (h == 0 ? ea : eb) * inv_sum;
                 }
             }
         }

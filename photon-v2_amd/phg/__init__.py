@@ -146,7 +146,7 @@ class phg_scene_host(C.Structure):
 # every entry point declared in include/phg.h
 EXPORTED = ["phg_create", "phg_destroy", "phg_last_error", "phg_upload_scene", "phg_intersect",
             "phg_intersect_counted", "phg_film_clear", "phg_film_read", "phg_render", "phg_get_stats",
-            "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_gradients",
+            "phg_guide_set_params", "phg_guide_get_params", "phg_guide_forward", "phg_guide_lobes", "phg_guide_gradients",
             "phg_guide_train", "phg_scene_generate", "phg_scene_free", "phg_upload_scenes", "phg_render_batch",
             "phg_batch_film_read", "phg_set_tuning", "phg_get_tuning", "phg_autotune_batch", "phg_scene_parse_p2",
             "phg_scene_load_p2", "phg_film_develop", "phg_film_write_pfm", "phg_intersect_noalloc"]
@@ -178,6 +178,7 @@ def lib() -> C.CDLL:
     L.phg_guide_set_params.argtypes = [vp, vp, C.c_uint32]
     L.phg_guide_get_params.argtypes = [vp, vp, C.c_uint32]
     L.phg_guide_forward.argtypes = [vp, vp, C.c_uint64, vp]
+    L.phg_guide_lobes.argtypes = [vp, vp, C.c_uint64, vp]
     L.phg_guide_gradients.argtypes = [vp, vp, C.c_uint64, vp]
     L.phg_guide_train.argtypes = [vp, vp, C.c_uint64, C.c_float, C.c_uint32]
     L.phg_scene_generate.argtypes = [C.POINTER(phg_gen_opts), C.POINTER(C.POINTER(phg_scene_host))]
@@ -519,6 +520,13 @@ class Context:
         f = np.ascontiguousarray(features, np.float32).reshape(-1, 9)
         out = np.zeros((f.shape[0], GUIDE_OUTPUTS), np.float32)
         self._check(self._L.phg_guide_forward(self._h, f.ctypes.data, f.shape[0], out.ctypes.data), "phg_guide_forward")
+        return out
+
+    def guide_lobes(self, features: np.ndarray) -> np.ndarray:
+        """Decoded mixture per query: 8 x (unit mean 3, kappa, weight), then the selection probability."""
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, 9)
+        out = np.zeros((f.shape[0], GUIDE_OUTPUTS), np.float32)
+        self._check(self._L.phg_guide_lobes(self._h, f.ctypes.data, f.shape[0], out.ctypes.data), "phg_guide_lobes")
         return out
 
     def guide_gradients(self, records: np.ndarray) -> np.ndarray:

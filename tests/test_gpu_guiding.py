@@ -69,6 +69,40 @@ def test_forward_matches_oracle(ctx):
     assert err <= 0.02 * max(1.0, scale)
 
 
+def test_decoded_lobes_match_oracle(ctx):
+    """The decoded mixture the guided shade stage reads: unit means, kappa = exp(clamped log
+    kappa), softmax weights and the selection probability.  Decoding the GPU's own raw outputs
+    with the oracle must give the GPU's lobes to float precision (hardware rcp / rsqrt / exp,
+    about 1 ulp each); against the oracle's forward they agree to the BF16 tolerance."""
+    params = go.init_params(11)
+    # larger output-layer weights spread kappa over its clamp range and the weights over
+    # several lobes
+    W4, b4 = params[-1]
+    params[-1] = (W4 * 8.0, b4)
+    ctx.guide_set_params(flat(params))
+    x = np.random.default_rng(12).random((1337, 9)).astype(np.float32)   # not a multiple of 64
+    raw = ctx.guide_forward(x)
+    lob = ctx.guide_lobes(x).astype(np.float64)
+    mu, kappa, w = go.decode(raw)
+    L = lob[:, :40].reshape(-1, 8, 5)
+    assert np.allclose(L[:, :, 0:3], mu, rtol=0, atol=2e-6)
+    assert np.allclose(L[:, :, 3], kappa, rtol=2e-6, atol=0)
+    assert np.allclose(L[:, :, 4], w, rtol=4e-6, atol=1e-7)
+    assert np.allclose(lob[:, 40], go.selection(raw), rtol=1e-6, atol=0)
+    assert np.allclose(np.linalg.norm(L[:, :, 0:3], axis=2), 1.0, atol=2e-6)
+    assert np.allclose(L[:, :, 4].sum(axis=1), 1.0, atol=2e-6)
+    # against the oracle's own forward pass (BF16 operands)
+    _, ref = go.forward(unflat(flat(params)), go.encode(x.astype(np.float64)), bf16=True)
+    rmu, rk, rw = go.decode(ref)
+    spread = np.ptp(np.log(kappa))
+    print(f"[guide] lobes: log-kappa spread {spread:.2f}, max weight {w.max():.3f}", flush=True)
+    assert spread > 4.0 and w.max() > 0.5
+    cos = np.sum(L[:, :, 0:3] * rmu, axis=2)
+    assert np.quantile(cos, 0.01) > 0.99
+    assert np.abs(np.log(L[:, :, 3]) - np.log(rk)).max() < 0.5
+    assert np.abs(L[:, :, 4] - rw).max() < 0.1
+
+
 @pytest.mark.parametrize("learned", [False, True])
 def test_gradients_match_oracle(ctx, learned):
     params = go.init_params(5)
