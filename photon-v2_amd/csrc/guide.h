@@ -45,7 +45,7 @@ hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const ph
                             const GuideScene& gs, const uint32_t* counts, const uint32_t* bounce_base, uint32_t bk,
                             float* out, unsigned long long* queries, uint32_t grid, hipStream_t st);
 // Scratch of a training step (device memory): layer inputs X (5 x cap x 64 bf16), two dZ
-// buffers (cap x 64 bf16 each), per-wave dW / db partials (5 x (max_parts + 4) x 4160 floats)
+// buffers (cap x 64 bf16 each), per-block dW / db partials (5 x (max_parts + 4) x 4160 floats)
 // and their first-level sums (5 x slices x 4160 floats).
 struct GuideWorkspace {
     uint16_t* X;

@@ -102,32 +102,6 @@ __global__ void k_guide_prepare(const float* __restrict__ params, uint16_t* __re
     }
 }
 
-// One dense layer for a 16-row tile held by one wave: act_out = act(act_in . W + b).
-// act_in/act_out: bf16 [16][64] in LDS; wt_l: bf16 [64 out][64 in]; out_f32 (last layer): [16][48].
-__device__ __forceinline__ void layer_forward(const __bf16* act_in, const __bf16* wt_l, const float* bias_l,
-                                              __bf16* act_out, float* out_f32, bool last) {
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(act_in + r * 64 + 8 * g);
-    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(act_in + r * 64 + 32 + 8 * g);
-    const int ntiles = last ? kGuideOutPad / 16 : 4;
-    for (int nt = 0; nt < ntiles; ++nt) {
-        const int col = nt * 16 + r;
-        const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(wt_l + col * 64 + 8 * g);
-        const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(wt_l + col * 64 + 32 + 8 * g);
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-        c = mfma16(a0, b0, c);
-        c = mfma16(a1, b1, c);
-        const float bv = bias_l[col];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = 4 * g + i;
-            const float v = c[i] + bv;
-            if (last) out_f32[row * kGuideOutPad + col] = v;
-            else act_out[row * 64 + col] = (__bf16)fmaxf(v, 0.0f);
-        }
-    }
-}
-
 // Features of one query: from a feature array, or from a queued hit (position, shading
 // normal and outgoing direction of the surface point, as the shade stage computes them).
 struct InferSource {
@@ -371,7 +345,97 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
 // the result does not depend on scheduling.
 
 // Forward over 16-record tiles (4 waves per block, weights in LDS), storing every layer's
-// input and the output-layer gradient.
+// input and the output-layer gradient.  Rows of 64 bf16 in LDS (activations, weights) keep
+// 16-byte chunk c of row r at position c ^ (r & 7) (lds_chunk), so that the lanes reading
+// chunk g of 16 consecutive rows fall on 8 different bank groups instead of one.
+__device__ __forceinline__ int lds_chunk(int row, int chunk) { return row * 8 + (chunk ^ (row & 7)); }
+
+// One dense layer for a 16-row tile held by one wave: act_out = act(act_in . W + b).
+// act_in/act_out: bf16 [16][64] (swizzled rows) in LDS; wt_l: bf16 [64 out][64 in] (swizzled
+// rows); out_f32 (last layer): [16][48].
+__device__ __forceinline__ void layer_forward(const __bf16* act_in, const __bf16* wt_l, const float* bias_l,
+                                              __bf16* act_out, float* out_f32, bool last) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const bf16x8* ain = reinterpret_cast<const bf16x8*>(act_in);
+    const bf16x8* win = reinterpret_cast<const bf16x8*>(wt_l);
+    const bf16x8 a0 = ain[lds_chunk(r, g)];
+    const bf16x8 a1 = ain[lds_chunk(r, 4 + g)];
+    const int ntiles = last ? kGuideOutPad / 16 : 4;
+    for (int nt = 0; nt < ntiles; ++nt) {
+        const int col = nt * 16 + r;
+        const bf16x8 b0 = win[lds_chunk(col, g)];
+        const bf16x8 b1 = win[lds_chunk(col, 4 + g)];
+        const float bv = bias_l[col];
+        f32x4 c = {bv, bv, bv, bv};
+        c = mfma16(a0, b0, c);
+        c = mfma16(a1, b1, c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * g + i;
+            if (last) out_f32[row * kGuideOutPad + col] = c[i];
+            else act_out[lds_chunk(row, col >> 3) * 8 + (col & 7)] = (__bf16)fmaxf(c[i], 0.0f);
+        }
+    }
+}
+
+// Output gradients of one record computed by its 4 lanes (lanes r, r + 16, r + 32, r + 48 of
+// the tile): lane group g takes lobes 2g and 2g + 1 (outputs 10 g .. 10 g + 9) and lane group 0
+// also the selection logit (output 40).  The arithmetic is guide_output_grad's; the softmax and
+// the mixture density, sums over the 8 lobes, are reduced across the 4 lanes.
+__device__ __forceinline__ void output_grad_lanes(const float* o, int g, V3 w, float weight, float bsdf_pdf, float sel,
+                                                  float samp_pdf, float* gl, float* gsel) {
+    const float* ol = o + 10 * g;
+    float mx = fmaxf(ol[4], ol[9]);
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float e[2] = {expf(ol[4] - mx), expf(ol[9] - mx)};
+    float sum = e[0] + e[1];
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    float mu[2][3], kap[2], wt[2], c[2], p[2], inv[2];
+    float qp = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float* oh = ol + 5 * h;
+        const float rl = sqrtf(oh[0] * oh[0] + oh[1] * oh[1] + oh[2] * oh[2]);
+        if (rl > 0.0f) { mu[h][0] = oh[0] / rl; mu[h][1] = oh[1] / rl; mu[h][2] = oh[2] / rl; }
+        else { mu[h][0] = 0.0f; mu[h][1] = 0.0f; mu[h][2] = 1.0f; }
+        inv[h] = rl > 0.0f ? 1.0f / rl : 1.0f;
+        kap[h] = expf(clampf(oh[3], kKappaLogMin, kKappaLogMax));
+        wt[h] = e[h] / sum;
+        c[h] = mu[h][0] * w.x + mu[h][1] * w.y + mu[h][2] * w.z;
+        p[h] = vmf_norm(kap[h]) * expf(kap[h] * (c[h] - 1.0f));
+        qp += wt[h] * p[h];
+    }
+    float q = qp + __shfl_xor(qp, 16);
+    q += __shfl_xor(q, 32);
+    const bool learned = samp_pdf > 0.0f && sel > 0.0f;
+    const float a = learned ? guide_sel(o[kGuideSel]) : sel;
+    const bool mix = a > 0.0f && bsdf_pdf > 0.0f;
+    const float dens = mix ? a * q + (1.0f - a) * bsdf_pdf : q;
+    const float share = mix ? a * q / dens : 1.0f;
+    const float scale = q > 0.0f ? -weight * share / q : 0.0f;
+    *gsel = (learned && mix && dens > 0.0f)
+                ? -weight * weight * samp_pdf * (q - bsdf_pdf) / (dens * dens) *
+                      ((a - kSelMin) * (kSelMax - a) / (kSelMax - kSelMin))
+                : 0.0f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float k = kap[h];
+        const float wp = wt[h] * p[h];
+        const float f = scale * wp * k * inv[h];
+        gl[5 * h + 0] = f * (w.x - c[h] * mu[h][0]);
+        gl[5 * h + 1] = f * (w.y - c[h] * mu[h][1]);
+        gl[5 * h + 2] = f * (w.z - c[h] * mu[h][2]);
+        const float lk = ol[5 * h + 3];
+        float dl;  // 1/k - 2/expm1(2k) - 1
+        if (k < 1e-2f) dl = -k / 3.0f + k * k * k / 45.0f;
+        else dl = 1.0f / k - 2.0f / expm1f(2.0f * k) - 1.0f;
+        gl[5 * h + 3] = (lk > kKappaLogMin && lk < kKappaLogMax) ? scale * wp * (dl + c[h]) * k : 0.0f;
+        gl[5 * h + 4] = scale * (wp - wt[h] * q);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_train_forward(GuideNetDev net, const float* __restrict__ rec, uint32_t nrec,
                                                        uint32_t npad, float inv_batch, __bf16* __restrict__ X,
                                                        __bf16* __restrict__ dz_out) {
@@ -380,40 +444,66 @@ __global__ __launch_bounds__(256) void k_train_forward(GuideNetDev net, const fl
     __shared__ __attribute__((aligned(16))) __bf16 s_act[4][2][16 * 64];
     __shared__ float s_out[4][16 * kGuideOutPad];
     for (int i = threadIdx.x; i < kGuideLayers * 64 * 64 / 8; i += 256)
-        reinterpret_cast<bf16x8*>(s_wt)[i] = reinterpret_cast<const bf16x8*>(net.wt)[i];
+        reinterpret_cast<bf16x8*>(s_wt)[(i >> 9) * 512 + lds_chunk((i >> 3) & 63, i & 7)] =
+            reinterpret_cast<const bf16x8*>(net.wt)[i];
     for (int i = threadIdx.x; i < kGuideLayers * 64; i += 256) s_bias[i] = net.bias[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     for (uint32_t tile = blockIdx.x * 4 + wave; tile * 16 < npad; tile += gridDim.x * 4) {
         const uint32_t q = tile * 16 + r;
+        const bool valid = q < nrec;
         float x[9];
-        if (q < nrec) for (int k = 0; k < 9; ++k) x[k] = rec[kGuideRecordFloats * (size_t)q + k];
+        if (valid) for (int k = 0; k < 9; ++k) x[k] = rec[kGuideRecordFloats * (size_t)q + k];
         else for (int k = 0; k < 9; ++k) x[k] = 0.0f;
-        __bf16* a0 = s_act[wave][0];
-        for (int j = 16 * g; j < 16 * g + 16; ++j) a0[r * 64 + j] = q < nrec ? (__bf16)guide_encode(x, j) : (__bf16)0.0f;
+        // encoding entries 16 g .. 16 g + 15 of record r: chunks 2 g and 2 g + 1 of its row
+        bf16x8* a0 = reinterpret_cast<bf16x8*>(s_act[wave][0]);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            bf16x8 v;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = valid ? (__bf16)guide_encode(x, 16 * g + 8 * h + j) : (__bf16)0.0f;
+            a0[lds_chunk(r, 2 * g + h)] = v;
+        }
         wave_sync();
         for (int l = 0; l < kGuideLayers; ++l) {
             // store this layer's input tile: 16 rows x 64 bf16 = 128 x 16 bytes, two per lane
             const __bf16* src = s_act[wave][l & 1];
+            const bf16x8* sv = reinterpret_cast<const bf16x8*>(src);
             __bf16* dst = X + ((size_t)l * npad + (size_t)tile * 16) * 64;
-            reinterpret_cast<bf16x8*>(dst)[lane] = reinterpret_cast<const bf16x8*>(src)[lane];
-            reinterpret_cast<bf16x8*>(dst)[lane + 64] = reinterpret_cast<const bf16x8*>(src)[lane + 64];
+            reinterpret_cast<bf16x8*>(dst)[lane] = sv[lds_chunk(lane >> 3, lane & 7)];
+            reinterpret_cast<bf16x8*>(dst)[lane + 64] = sv[lds_chunk(8 + (lane >> 3), lane & 7)];
             const bool last = l == kGuideLayers - 1;
             layer_forward(src, s_wt + l * 4096, s_bias + l * 64, s_act[wave][(l + 1) & 1], s_out[wave], last);
             wave_sync();
         }
-        // output-layer gradient of each record, one lane per record
-        if (g == 0) {
-            float gr[kGuideOut];
-            if (q < nrec) {
-                const float* R = rec + kGuideRecordFloats * (size_t)q;
-                guide_output_grad(s_out[wave] + r * kGuideOutPad, mk(R[9], R[10], R[11]), R[12], R[13], R[14], R[15], gr);
-            } else {
-                for (int k = 0; k < kGuideOut; ++k) gr[k] = 0.0f;
-            }
-            __bf16* d = dz_out + (size_t)q * 64;
-            for (int k = 0; k < 64; ++k) d[k] = (__bf16)(k < kGuideOut ? gr[k] * inv_batch : 0.0f);
+        // output-layer gradients: the 4 lanes of record r share its 8 lobes; the results go back
+        // into the record's row of s_out (outputs 0..40, zeros up to 47) and leave as bf16 rows
+        // of 64 (two 16-byte stores per lane)
+        float* orow = s_out[wave] + r * kGuideOutPad;
+        float gl[10], gsel;
+        {
+            const float* R = rec + kGuideRecordFloats * (size_t)(valid ? q : 0u);
+            const V3 w = valid ? mk(R[9], R[10], R[11]) : mk(0.0f, 0.0f, 1.0f);
+            output_grad_lanes(orow, g, w, valid ? R[12] : 0.0f, valid ? R[13] : 0.0f, valid ? R[14] : 0.0f,
+                              valid ? R[15] : 0.0f, gl, &gsel);
         }
+        wave_sync();
+#pragma unroll
+        for (int s2 = 0; s2 < 10; ++s2) orow[10 * g + s2] = valid ? gl[s2] * inv_batch : 0.0f;
+        if (g == 0) orow[kGuideSel] = valid ? gsel * inv_batch : 0.0f;
+        if (g == 1)
+            for (int k = kGuideOut; k < kGuideOutPad; ++k) orow[k] = 0.0f;
+        wave_sync();
+        bf16x8 d0, d1;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k0 = 16 * g + j, k1 = 16 * g + 8 + j;
+            d0[j] = (__bf16)(k0 < kGuideOutPad ? orow[k0] : 0.0f);
+            d1[j] = (__bf16)(k1 < kGuideOutPad ? orow[k1] : 0.0f);
+        }
+        bf16x8* d = reinterpret_cast<bf16x8*>(dz_out + (size_t)q * 64 + 16 * g);
+        d[0] = d0;
+        d[1] = d1;
         wave_sync();
     }
 }
@@ -421,17 +511,20 @@ __global__ __launch_bounds__(256) void k_train_forward(GuideNetDev net, const fl
 // Backward through layer l for a contiguous run of 32-record sub-tiles per wave: dW_l and db_l
 // partials in registers, and (l > 0) dZ_{l-1}.  Sub-tiles of X_l and dZ_l are staged in LDS
 // so that the transposed operands of X_l^T dZ_l are LDS gathers.
-__global__ __launch_bounds__(256) void k_train_backward(const __bf16* __restrict__ wn_l, const __bf16* __restrict__ X_l,
-                                                        const __bf16* __restrict__ dz, __bf16* __restrict__ dz_prev,
-                                                        uint32_t npad, uint32_t chunk, int l, float* __restrict__ partial) {
+constexpr int kBackWaves = 4;
+__global__ __launch_bounds__(64 * kBackWaves) void k_train_backward(const __bf16* __restrict__ wn_l,
+                                                                    const __bf16* __restrict__ X_l,
+                                                                    const __bf16* __restrict__ dz,
+                                                                    __bf16* __restrict__ dz_prev, uint32_t npad,
+                                                                    uint32_t chunk, int l, float* __restrict__ partial) {
     __shared__ __attribute__((aligned(16))) __bf16 s_w[64 * 64];
-    __shared__ __attribute__((aligned(16))) __bf16 s_x[4][32 * 64];
-    __shared__ __attribute__((aligned(16))) __bf16 s_dz[4][32 * 64];
-    for (int i = threadIdx.x; i < 64 * 64 / 8; i += 256)
+    __shared__ __attribute__((aligned(16))) __bf16 s_x[kBackWaves][32 * 64];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dz[kBackWaves][32 * 64];
+    for (int i = threadIdx.x; i < 64 * 64 / 8; i += 64 * kBackWaves)
         reinterpret_cast<bf16x8*>(s_w)[i] = reinterpret_cast<const bf16x8*>(wn_l)[i];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const uint32_t wid = blockIdx.x * 4 + wave;
+    const uint32_t wid = blockIdx.x * kBackWaves + wave;
     const int nout = layer_out(l), ntiles = (nout + 15) / 16;
     f32x4 acc[4][4];
 #pragma unroll
@@ -442,14 +535,26 @@ __global__ __launch_bounds__(256) void k_train_backward(const __bf16* __restrict
     __bf16* sx = s_x[wave];
     __bf16* sd = s_dz[wave];
     const uint32_t begin = wid * chunk, end = min(begin + chunk, npad);
+    // 32 records of X_l and dZ_l per sub-tile: 32 x 64 bf16 = 256 x 16 bytes each, 4 per lane;
+    // the next sub-tile is loaded into registers while this one is computed
+    bf16x8 px[4], pd[4];
+    if (begin < end) {
+        const bf16x8* gx = reinterpret_cast<const bf16x8*>(X_l + (size_t)begin * 64);
+        const bf16x8* gd = reinterpret_cast<const bf16x8*>(dz + (size_t)begin * 64);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { px[k] = gx[lane + 64 * k]; pd[k] = gd[lane + 64 * k]; }
+    }
     for (uint32_t base = begin; base < end; base += 32) {
-        // stage 32 records of X_l and dZ_l: 32 x 64 bf16 = 256 x 16 bytes each, 4 per lane
-        const bf16x8* gx = reinterpret_cast<const bf16x8*>(X_l + (size_t)base * 64);
-        const bf16x8* gd = reinterpret_cast<const bf16x8*>(dz + (size_t)base * 64);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            reinterpret_cast<bf16x8*>(sx)[lane + 64 * k] = gx[lane + 64 * k];
-            reinterpret_cast<bf16x8*>(sd)[lane + 64 * k] = gd[lane + 64 * k];
+            reinterpret_cast<bf16x8*>(sx)[lane + 64 * k] = px[k];
+            reinterpret_cast<bf16x8*>(sd)[lane + 64 * k] = pd[k];
+        }
+        if (base + 32 < end) {
+            const bf16x8* gx = reinterpret_cast<const bf16x8*>(X_l + (size_t)(base + 32) * 64);
+            const bf16x8* gd = reinterpret_cast<const bf16x8*>(dz + (size_t)(base + 32) * 64);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { px[k] = gx[lane + 64 * k]; pd[k] = gd[lane + 64 * k]; }
         }
         wave_sync();
         // dW_l += X^T dZ: A = X^T (16 inputs x 32 records), B = dZ (32 records x 16 outputs)
@@ -470,38 +575,54 @@ __global__ __launch_bounds__(256) void k_train_backward(const __bf16* __restrict
         // db_l: column sums of dZ (one column per lane)
         for (int s = 0; s < 32; ++s) acc_b += (float)sd[s * 64 + lane];
         if (l > 0) {
-            // dZ_{l-1} = (dZ W^T) masked by X_l > 0: rows = records (2 tiles), cols = inputs
+            // dZ_{l-1} = (dZ W^T) masked by X_l > 0, computed transposed (W dZ^T: rows = inputs,
+            // columns = records), so that a lane holds 4 consecutive inputs of one record: one
+            // 8-byte mask read and one 8-byte store per product
+            typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4;
             for (int t = 0; t < 2; ++t) {
-                const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(sd + (t * 16 + r) * 64 + 8 * g);
-                const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(sd + (t * 16 + r) * 64 + 32 + 8 * g);
+                const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(sd + (t * 16 + r) * 64 + 8 * g);
+                const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(sd + (t * 16 + r) * 64 + 32 + 8 * g);
+                const int rec_row = t * 16 + r;
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt) {
-                    const int col = nt * 16 + r;
-                    const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(s_w + col * 64 + 8 * g);
-                    const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(s_w + col * 64 + 32 + 8 * g);
+                    const int in_row = nt * 16 + r;
+                    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(s_w + in_row * 64 + 8 * g);
+                    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(s_w + in_row * 64 + 32 + 8 * g);
                     f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
                     c = mfma16(a0, b0, c);
                     c = mfma16(a1, b1, c);
+                    // inputs nt * 16 + 4 g + i of record rec_row
+                    const int col = nt * 16 + 4 * g;
+                    const bf16x4 xm = *reinterpret_cast<const bf16x4*>(sx + rec_row * 64 + col);
+                    bf16x4 o;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int row = t * 16 + 4 * g + i;
-                        const bool active = (float)sx[row * 64 + col] > 0.0f;
-                        dz_prev[(size_t)(base + row) * 64 + col] = (__bf16)(active ? c[i] : 0.0f);
-                    }
+                    for (int i = 0; i < 4; ++i) o[i] = (__bf16)((float)xm[i] > 0.0f ? c[i] : 0.0f);
+                    *reinterpret_cast<bf16x4*>(dz_prev + (size_t)(base + rec_row) * 64 + col) = o;
                 }
             }
         }
         wave_sync();
     }
-    // this wave's partial: dW [64 in][64 out] then db [64]
-    float* P = partial + (size_t)wid * (64 * 64 + 64);
+    // the block's partial, dW [64 in][64 out] then db [64]: the waves' sums added in LDS in
+    // wave order (a fixed order, so the result does not depend on scheduling), then written once
+    __shared__ float s_part[64 * 64 + 64];
+    for (int w = 0; w < kBackWaves; ++w) {
+        if (wave == w) {
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+            for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
+                for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) P[(mt * 16 + 4 * g + i) * 64 + nt * 16 + r] = acc[mt][nt][i];
-    P[64 * 64 + lane] = acc_b;
+                    for (int i = 0; i < 4; ++i) {
+                        const int e = (mt * 16 + 4 * g + i) * 64 + nt * 16 + r;
+                        s_part[e] = w == 0 ? acc[mt][nt][i] : s_part[e] + acc[mt][nt][i];
+                    }
+            s_part[64 * 64 + lane] = w == 0 ? acc_b : s_part[64 * 64 + lane] + acc_b;
+        }
+        __syncthreads();
+    }
+    float* P = partial + (size_t)blockIdx.x * (64 * 64 + 64);
+    for (int e = threadIdx.x; e < 64 * 64 + 64; e += 64 * kBackWaves) P[e] = s_part[e];
 }
 
 // First-level sums of the waves' partials, all layers in one launch: grid (17, slices, 5);
@@ -721,17 +842,17 @@ hipError_t guide_gradients(const GuideNetDev& net, const float* rec, uint32_t nr
     const uint32_t waves = std::max<uint32_t>(1u, std::min<uint32_t>(ws.max_parts, tiles / 4));
     const uint32_t chunk = ((tiles + waves - 1) / waves) * 32;
     const uint32_t used = (npad + chunk - 1) / chunk;
-    const uint32_t blocks = (used + 3) / 4;
+    const uint32_t blocks = (used + kBackWaves - 1) / kBackWaves;
     int cur = 0;
     for (int l = kGuideLayers - 1; l >= 0; --l) {
-        hipLaunchKernelGGL(k_train_backward, dim3(blocks), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_train_backward, dim3(blocks), dim3(64 * kBackWaves), 0, st,
                            reinterpret_cast<const __bf16*>(net.wn) + l * 4096, X + (size_t)l * npad * 64, dz[cur],
                            dz[cur ^ 1], npad, chunk, l, ws.partial + (size_t)l * (ws.max_parts + 4) * (64 * 64 + 64));
         cur ^= 1;
     }
     const uint32_t eblocks = (64 * 64 + 64 + 255) / 256;
     hipLaunchKernelGGL(k_train_reduce_parts, dim3(eblocks, ws.slices, kGuideLayers), dim3(256), 0, st, ws.partial,
-                       (size_t)(ws.max_parts + 4) * (64 * 64 + 64), blocks * 4, ws.partial2);
+                       (size_t)(ws.max_parts + 4) * (64 * 64 + 64), blocks, ws.partial2);
     hipLaunchKernelGGL(k_train_reduce, dim3(eblocks, 1, kGuideLayers), dim3(256), 0, st, ws.partial2, ws.slices, grads);
     return hipGetLastError();
 }
