@@ -686,6 +686,7 @@ __device__ __forceinline__ uint32_t collect_mask(const PathBuffers& pb, const Pa
     for (uint32_t b = 0; b < pp.rec_bounces; ++b) {
         const float4* rc = pb.rec + ((size_t)b * pp.paths + pid) * kRecFloat4;
         const float4 r0 = rc[0];
+        if (r0.w < 0.0f) continue;       // a vertex without a record (a material the lobes do not serve)
         if (!(r0.w > 0.0f)) break;
         // (L - Lb) / T = f cos L_i / pdf along the sampled direction: the sample's weight for
         // fitting the lobes to the product of BSDF, cosine and incident radiance

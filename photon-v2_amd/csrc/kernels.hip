@@ -671,7 +671,12 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                             const uint32_t slot_e = wave_push(&pb.counts[(bounce + 1) % kCountSlots], true);
                             queue_ray(pb.ext[qout], slot_e, X.P + off, job.root, ud, kFltMax);
                             pb.ext_pid[qout][slot_e] = pid;
-                            if (GUIDED && pp.record && bounce < pp.rec_bounces) {
+                            if (GUIDED && pp.record && bounce < pp.rec_bounces && !guide_samples_material(m)) {
+                                // no record where the lobes are never sampled (their fit there
+                                // would only take capacity from the surfaces they serve); the
+                                // marker lets the collection skip to the path's next vertex
+                                pb.rec[((size_t)bounce * pp.paths + pid) * kRecFloat4] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+                            } else if (GUIDED && pp.record && bounce < pp.rec_bounces) {
                                 float4* rc = pb.rec + ((size_t)bounce * pp.paths + pid) * kRecFloat4;
                                 rc[0] = make_float4(X.P.x, X.P.y, X.P.z, pdfc);
                                 rc[1] = make_float4(X.Ns.x, X.Ns.y, X.Ns.z, bsdf_pdf_ls);
