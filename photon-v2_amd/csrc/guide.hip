@@ -179,7 +179,8 @@ __device__ __forceinline__ void query_features(const InferSource& src, uint32_t 
 // sign bit decides, so max(bf16(v), 0) as 16-bit integers equals bf16(max(v, 0))).  The last
 // layer leaves lobes 2g and 2g + 1 of query r in lane 16 g + r, which decodes them; the
 // softmax over the 8 lobe weights reduces across the 4 lanes of a query.
-constexpr int kInferWaves = 8;
+constexpr int kInferWaves = 8;   // waves per block
+constexpr int kInferGroup = 4;   // 16-query tiles that go through the layers together
 typedef __attribute__((ext_vector_type(8))) short s16x8;
 
 typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
@@ -223,111 +224,126 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
     const float scale8 = g == 0 ? 0.0f : (float)(1 << (g - 1)) * 0.5f;
     const uint32_t step = gridDim.x * kInferWaves * 64u;
     const uint32_t first = (blockIdx.x * kInferWaves + wave) * 64u;
+    // Software pipeline over the wave's iterations.  Loads and stores retire in issue order, so
+    // a wait for a load also waits for every store issued before it: the next iteration's
+    // features (the triangle loads) are therefore made after this iteration's layers and before
+    // its stores, and the raw inputs of the iteration after that are fetched one iteration ahead.
     QueryRaw next = query_fetch(src, first + (uint32_t)lane, n);
+    if (first < n) {
+        float x[9];
+        query_features_of(src, next, x);
+        next = query_fetch(src, first + step + (uint32_t)lane, n);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) xs[k * 64 + lane] = x[k];
+        wave_sync();
+    }
     for (uint32_t base = first; base < n; base += step) {
-        {
-            // this iteration's query was fetched during the previous one; the next one is
-            // fetched after this one's features are made (loads complete in issue order, so
-            // waiting for the triangle would otherwise wait for it too) and stays in flight
-            // while this iteration computes
-            float x[9];
-            query_features_of(src, next, x);
-            next = query_fetch(src, base + step + (uint32_t)lane, n);
-            wave_sync();   // the previous iteration's reads of xs are done
+        // the four 16-query tiles in groups of kInferGroup: the layer-0 operands
+        // (infer_in0_index), the five layers, then the decode
 #pragma unroll
-            for (int k = 0; k < 9; ++k) xs[k * 64 + lane] = x[k];
-            wave_sync();
-        }
-        // layer-0 operands of the four tiles (infer_in0_index)
-        bf16x8 bop[4][2];
+        for (int T0 = 0; T0 < 4; T0 += kInferGroup) {
+            if (base + 16u * T0 >= n) break;  // whole wave
+            bf16x8 bop[kInferGroup][2];
 #pragma unroll
-        for (int T = 0; T < 4; ++T) {
-            const int qi = 16 * T + r;
-            const float v8 = xs[8 * 64 + qi];
-            const float r8 = scale8 * v8;
-            const float e0 = g == 0 ? v8 : __builtin_amdgcn_sinf(r8);
-            const float e1 = g == 0 ? 1.0f : __builtin_amdgcn_cosf(r8);
-#pragma unroll
-            for (int kc = 0; kc < 2; ++kc) {
-                const float v = xs[(2 * g + kc) * 64 + qi];
-                float e[8];
-                e[0] = v;
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    const float rev = (float)(1 << b) * 0.5f * v;
-                    e[1 + 2 * b] = __builtin_amdgcn_sinf(rev);
-                    e[2 + 2 * b] = __builtin_amdgcn_cosf(rev);
-                }
-                e[7] = kc == 0 ? e0 : e1;
-                bop[T][kc] = pack_bf16(f32x4{e[0], e[1], e[2], e[3]}, f32x4{e[4], e[5], e[6], e[7]});
-            }
-        }
-        f32x4 acc[4][4];
-#pragma unroll
-        for (int l = 0; l < kGuideLayers; ++l) {
-            const int nmt = l == kGuideLayers - 1 ? kGuideOutPad / 16 : 4;
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-                if (mt >= nmt) continue;
-                const f32x4 bias = *reinterpret_cast<const f32x4*>(s_b + l * 64 + 16 * mt + 4 * g);
+            for (int t = 0; t < kInferGroup; ++t) {
+                const int qi = 16 * (T0 + t) + r;
+                const float v8 = xs[8 * 64 + qi];
+                const float r8 = scale8 * v8;
+                const float e0 = g == 0 ? v8 : __builtin_amdgcn_sinf(r8);
+                const float e1 = g == 0 ? 1.0f : __builtin_amdgcn_cosf(r8);
 #pragma unroll
                 for (int kc = 0; kc < 2; ++kc) {
-                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(s_w + l * 4096 + ((mt * 2 + kc) * 64 + lane) * 8);
+                    const float v = xs[(2 * g + kc) * 64 + qi];
+                    float e[8];
+                    e[0] = v;
 #pragma unroll
-                    for (int T = 0; T < 4; ++T) acc[T][mt] = mfma16(a, bop[T][kc], kc == 0 ? bias : acc[T][mt]);
-                }
-            }
-            if (l < kGuideLayers - 1) {
-#pragma unroll
-                for (int T = 0; T < 4; ++T) {
-                    bop[T][0] = relu_pack(acc[T][0], acc[T][1]);
-                    bop[T][1] = relu_pack(acc[T][2], acc[T][3]);
-                }
-            }
-        }
-#pragma unroll
-        for (int T = 0; T < 4; ++T) {
-            if (base + 16u * T >= n) break;  // whole wave
-            const uint32_t q = base + 16u * T + (uint32_t)r;
-            // slots s = 0..9 of this lane: logical outputs 10 g + s (lobes 2g, 2g + 1); slot 10
-            // of lane group 0: the selection logit
-            float o[11];
-#pragma unroll
-            for (int s2 = 0; s2 < 11; ++s2) o[s2] = acc[T][s2 >> 2][s2 & 3];
-            if (raw) {
-                float* dst = out + (size_t)q * kGuideOut + 10 * g;
-                if (q < n) {
-                    for (int s2 = 0; s2 < 10; ++s2) dst[s2] = o[s2];
-                    if (g == 0) out[(size_t)q * kGuideOut + kGuideSel] = o[10];
-                }
-                continue;
-            }
-            float* dst = out + (size_t)q * kGuideDec + 10 * g;
-            if (q < n && g == 0) out[(size_t)q * kGuideDec + kGuideSel] = guide_sel(o[10]);
-            float mx = fmaxf(o[4], o[9]);
-            mx = fmaxf(mx, __shfl_xor(mx, 16));
-            mx = fmaxf(mx, __shfl_xor(mx, 32));
-            const float ea = __expf(o[4] - mx), eb = __expf(o[9] - mx);
-            float sum = ea + eb;
-            sum += __shfl_xor(sum, 16);
-            sum += __shfl_xor(sum, 32);
-            // hardware reciprocal and reciprocal square root (about 1 ulp): the lobes are stored
-            // once and the shade stage samples and evaluates them from the same values
-            const float inv_sum = __builtin_amdgcn_rcpf(sum);
-            if (q < n) {
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const float* oh = o + 5 * h;
-                    const float r2 = oh[0] * oh[0] + oh[1] * oh[1] + oh[2] * oh[2];
-                    float* L = dst + 5 * h;
-                    if (r2 > 0x1p-100f) {
-                        const float inv = __builtin_amdgcn_rsqf(r2);
-                        L[0] = oh[0] * inv; L[1] = oh[1] * inv; L[2] = oh[2] * inv;
-                    } else {
-                        L[0] = 0.0f; L[1] = 0.0f; L[2] = 1.0f;
+                    for (int b = 0; b < 3; ++b) {
+                        const float rev = (float)(1 << b) * 0.5f * v;
+                        e[1 + 2 * b] = __builtin_amdgcn_sinf(rev);
+                        e[2 + 2 * b] = __builtin_amdgcn_cosf(rev);
                     }
-                    L[3] = __expf(clampf(oh[3], kKappaLogMin, kKappaLogMax));
-                    L[4] = (h == 0 ? ea : eb) * inv_sum;
+                    e[7] = kc == 0 ? e0 : e1;
+                    bop[t][kc] = pack_bf16(f32x4{e[0], e[1], e[2], e[3]}, f32x4{e[4], e[5], e[6], e[7]});
+                }
+            }
+            f32x4 acc[kInferGroup][4];
+#pragma unroll
+            for (int l = 0; l < kGuideLayers; ++l) {
+                const int nmt = l == kGuideLayers - 1 ? kGuideOutPad / 16 : 4;
+#pragma unroll
+                for (int mt = 0; mt < 4; ++mt) {
+                    if (mt >= nmt) continue;
+                    const f32x4 bias = *reinterpret_cast<const f32x4*>(s_b + l * 64 + 16 * mt + 4 * g);
+#pragma unroll
+                    for (int kc = 0; kc < 2; ++kc) {
+                        const bf16x8 a = *reinterpret_cast<const bf16x8*>(s_w + l * 4096 + ((mt * 2 + kc) * 64 + lane) * 8);
+#pragma unroll
+                        for (int t = 0; t < kInferGroup; ++t)
+                            acc[t][mt] = mfma16(a, bop[t][kc], kc == 0 ? bias : acc[t][mt]);
+                    }
+                }
+                if (l < kGuideLayers - 1) {
+#pragma unroll
+                    for (int t = 0; t < kInferGroup; ++t) {
+                        bop[t][0] = relu_pack(acc[t][0], acc[t][1]);
+                        bop[t][1] = relu_pack(acc[t][2], acc[t][3]);
+                    }
+                }
+            }
+            if (T0 + kInferGroup >= 4 && base + step < n) {
+                // the next iteration's features, staged in LDS once every lane has encoded
+                float x[9];
+                query_features_of(src, next, x);
+                next = query_fetch(src, base + 2u * step + (uint32_t)lane, n);
+                wave_sync();
+#pragma unroll
+                for (int k = 0; k < 9; ++k) xs[k * 64 + lane] = x[k];
+                wave_sync();
+            }
+#pragma unroll
+            for (int t = 0; t < kInferGroup; ++t) {
+                if (base + 16u * (T0 + t) >= n) break;  // whole wave
+                const uint32_t q = base + 16u * (T0 + t) + (uint32_t)r;
+                // slots s = 0..9 of this lane: logical outputs 10 g + s (lobes 2g, 2g + 1); slot 10
+                // of lane group 0: the selection logit
+                float o[11];
+#pragma unroll
+                for (int s2 = 0; s2 < 11; ++s2) o[s2] = acc[t][s2 >> 2][s2 & 3];
+                if (raw) {
+                    float* dst = out + (size_t)q * kGuideOut + 10 * g;
+                    if (q < n) {
+                        for (int s2 = 0; s2 < 10; ++s2) dst[s2] = o[s2];
+                        if (g == 0) out[(size_t)q * kGuideOut + kGuideSel] = o[10];
+                    }
+                    continue;
+                }
+                float* dst = out + (size_t)q * kGuideDec + 10 * g;
+                if (q < n && g == 0) out[(size_t)q * kGuideDec + kGuideSel] = guide_sel(o[10]);
+                float mx = fmaxf(o[4], o[9]);
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                mx = fmaxf(mx, __shfl_xor(mx, 32));
+                const float ea = __expf(o[4] - mx), eb = __expf(o[9] - mx);
+                float sum = ea + eb;
+                sum += __shfl_xor(sum, 16);
+                sum += __shfl_xor(sum, 32);
+                // hardware reciprocal and reciprocal square root (about 1 ulp): the lobes are
+                // stored once and the shade stage samples and evaluates them from the same values
+                const float inv_sum = __builtin_amdgcn_rcpf(sum);
+                if (q < n) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float* oh = o + 5 * h;
+                        const float r2 = oh[0] * oh[0] + oh[1] * oh[1] + oh[2] * oh[2];
+                        float* L = dst + 5 * h;
+                        if (r2 > 0x1p-100f) {
+                            const float inv = __builtin_amdgcn_rsqf(r2);
+                            L[0] = oh[0] * inv; L[1] = oh[1] * inv; L[2] = oh[2] * inv;
+                        } else {
+                            L[0] = 0.0f; L[1] = 0.0f; L[2] = 1.0f;
+                        }
+                        L[3] = __expf(clampf(oh[3], kKappaLogMin, kKappaLogMax));
+                        L[4] = (h == 0 ? ea : eb) * inv_sum;
+                    }
                 }
             }
         }
