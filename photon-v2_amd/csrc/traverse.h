@@ -125,7 +125,10 @@ struct Traversal {
                 // with the box's parametric span on that axis, not with t itself: widen the
                 // test against the current closest hit by 2^-19 of that span.
                 const float slack = fmaxf(fabsf(m0), fabsf(M0)) * 0x1p-19f;
-                const bool hit = (t0 <= t1 * mult + slack) && (t0 <= tb + slack) && cc[k] != kEmptyChild;
+                // t0 <= t1 * mult + slack and t0 <= tb + slack, as one comparison: adding the
+                // slack rounds monotonically, so min(a + s, b + s) = min(a, b) + s exactly, and
+                // t1 is never NaN (the fminf chain starts at FLT_MAX)
+                const bool hit = (t0 <= fminf(t1 * mult, tb) + slack) && cc[k] != kEmptyChild;
                 key[k] = hit ? t0 : INFINITY;
                 kid[k] = cc[k];
             }
