@@ -177,10 +177,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         pb.ext_pid[0][i] = i;
         pb.path_seg[i] = si;
         pb.rng[i] = s.state;
-        pb.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        // the first vertex reads neither (no previous vertex): the first surface's dimensions
-        // u0, u2, u3, u4 and u5 of a stratified or Halton job travel there to k_shade, which
-        // then holds none of this code (it would push the shading kernel past its registers)
+        // The throughput (one) is not stored: k_shade takes it as one at the camera vertex.
+        // The MIS state and previous vertex are not read there either (no previous vertex),
+        // so they are stored only for a stratified or Halton job, whose first surface's
+        // dimensions u0, u2, u3, u4 and u5 travel in them to k_shade, which then holds none of
+        // this code (it would push the shading kernel past its registers).
         float4 w0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p0 = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0u));
         if (job.sampler == PHG_SAMPLER_STRATIFIED) {
             // the stream values k_shade draws at the first vertex (u0 .. u5, after the raster
@@ -202,8 +203,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
                              halton_value(5u, hidx, job.sampler, job.halton_leap, job.seed));
             p0.x = halton_value(6u, hidx, job.sampler, job.halton_leap, job.seed);
         }
-        pb.Wem[i] = w0;
-        pb.prevP[i] = p0;
+        if (job.sampler != PHG_SAMPLER_INDEPENDENT) {
+            pb.Wem[i] = w0;
+            pb.prevP[i] = p0;
+        }
         pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (pp.record)
             for (uint32_t b = 0; b < pp.rec_bounces; ++b)
@@ -462,7 +465,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
             const PassSeg sg = pp.segs[pb.path_seg[pid]];
             const JobDev& job = sc.jobs[sg.job];
             const bool bvpt = job.estimator != PHG_ESTIMATOR_BNEEPT;
-            float4 T4 = pb.T[pid];
+            // a camera vertex's throughput is one (k_generate does not store it)
+            float4 T4 = bounce == 0u ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : pb.T[pid];
             if (!stop && emitter >= 0) {
                 // the radiance so far is read and written here only (not held through the
                 // NEE and BSDF stages)
