@@ -99,6 +99,10 @@ enum : uint32_t { kFlagDieAfterEmission = 1u };
 
 
 // Pass segment holding pass path p: the last segment whose path_offset is <= p.
+// Passes of at most this many segments find a path's segment by searching the (cached) segment
+// table instead of storing and re-reading a per-path index (three search steps at most).
+constexpr uint32_t kSegSearchMax = 8;
+
 __device__ __forceinline__ uint32_t find_seg(const PassSeg* __restrict__ segs, uint32_t nseg, uint32_t p) {
     uint32_t lo = 0, hi = nseg;
     while (hi - lo > 1u) {
@@ -175,7 +179,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const Pas
         const V3 d = normalize(right * cx + up * cy + fwd);
         queue_ray(pb.ext[0], i, mk(c.pos[0], c.pos[1], c.pos[2]), job.root, d, kFltMax);
         pb.ext_pid[0][i] = i;
-        pb.path_seg[i] = si;
+        if (pp.num_segs > kSegSearchMax) pb.path_seg[i] = si;
         pb.rng[i] = s.state;
         // The throughput (one) is not stored: k_shade takes it as one at the camera vertex.
         // The MIS state and previous vertex are not read there either (no previous vertex),
@@ -462,7 +466,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                 X = make_hit(tv.a, tv.b, tv.c, (uint32_t)tri, h.y, h.z, h.w);
                 if (!side_agreed(X, dir)) stop = true;  // SurfaceTracer.h:126-143
             }
-            const PassSeg sg = pp.segs[pb.path_seg[pid]];
+            const uint32_t nseg = pp.num_segs;
+            const PassSeg sg = pp.segs[nseg <= kSegSearchMax ? find_seg(pp.segs, nseg, pid) : pb.path_seg[pid]];
             const JobDev& job = sc.jobs[sg.job];
             const bool bvpt = job.estimator != PHG_ESTIMATOR_BNEEPT;
             // a camera vertex's throughput is one (k_generate does not store it)
