@@ -98,11 +98,11 @@ enum : uint32_t { kFlagDieAfterEmission = 1u };
 
 
 
-// Pass segment holding pass path p: the last segment whose path_offset is <= p.
 // Passes of at most this many segments find a path's segment by searching the (cached) segment
 // table instead of storing and re-reading a per-path index (three search steps at most).
 constexpr uint32_t kSegSearchMax = 8;
 
+// Pass segment holding pass path p: the last segment whose path_offset is <= p.
 __device__ __forceinline__ uint32_t find_seg(const PassSeg* __restrict__ segs, uint32_t nseg, uint32_t p) {
     uint32_t lo = 0, hi = nseg;
     while (hi - lo > 1u) {
