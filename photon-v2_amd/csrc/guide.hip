@@ -155,21 +155,6 @@ __device__ __forceinline__ void query_features_of(const InferSource& src, const 
     guide_features(src.gs, X.P, X.Ns, mk(-r.b.x, -r.b.y, -r.b.z), x);
 }
 
-__device__ __forceinline__ void query_features(const InferSource& src, uint32_t q, float* x) {
-    if (src.features) {
-        for (int k = 0; k < 9; ++k) x[k] = src.features[9 * (size_t)q + k];
-        return;
-    }
-    const float4 h = src.hits[q];
-    const int32_t tri = __float_as_int(h.x);
-    if (tri < 0) { for (int k = 0; k < 9; ++k) x[k] = 0.0f; return; }
-    V3 a, b, c;
-    load_tri(src.tris, (uint32_t)tri, &a, &b, &c);
-    const SurfaceHit X = make_hit(a, b, c, (uint32_t)tri, h.y, h.z, h.w);
-    const float4 rd = reinterpret_cast<const float4*>(src.rays + q)[1];
-    guide_features(src.gs, X.P, X.Ns, mk(-rd.x, -rd.y, -rd.z), x);
-}
-
 // Inference: decoded mixture (8 x (mu, kappa, weight), selection probability) per query, or
 // raw outputs.  Each wave takes 64 queries per iteration: every lane computes one query's
 // 9 features (kept in LDS), then the four 16-query tiles go through the 5 layers together,
