@@ -61,6 +61,28 @@ def test_batch_films_equal_single_renders(paths_per_pass, order):
         assert st["batch_passes"] >= 2
 
 
+@pytest.mark.parametrize("njobs", [8, 9])
+def test_segment_lookup_either_side_of_the_search_limit(njobs):
+    """A pass of at most 8 job segments finds a path's segment by searching the segment table;
+    a pass of more stores a per-path index.  One pass of 8 and one of 9 small jobs (stratified
+    and Halton ones among them, whose first-vertex dimensions travel through the path state):
+    every job's film equals its single render."""
+    sc = scenes()
+    samplers = [0, phg.SAMPLER_STRATIFIED, phg.SAMPLER_HALTON | phg.HALTON_PERM_FIXED]
+    jobs = [dict(scene=k % 3, width=24 + 4 * k, height=16 + 2 * k, spp=2 + k % 4, max_bounces=3 + k % 5,
+                 seed=40 + k, sampler=samplers[k % 3]) for k in range(njobs)]
+    print(f"[batch] case segments={njobs}", flush=True)
+    with phg.Context(0) as c:
+        c.upload_many(sc)
+        res = c.render_batch(jobs)
+        films = [c.batch_film(i) for i in range(njobs)]
+        st = c.stats()
+    assert st["batch_passes"] == 1, "all jobs must share one pass"
+    for i, job in enumerate(jobs):
+        assert res[i][1] == job["width"] * job["height"] * job["spp"]
+        assert same_film(films[i], single(sc[job["scene"]], job), job), f"job {i} differs from its single render"
+
+
 def test_batch_reports_completion_order():
     sc = scenes()[:2]
     jobs = [dict(scene=1, width=256, height=256, spp=8, max_bounces=8, seed=1),
