@@ -103,6 +103,32 @@ def test_decoded_lobes_match_oracle(ctx):
     assert np.abs(L[:, :, 4] - rw).max() < 0.1
 
 
+def test_inference_over_many_iterations_per_wave(ctx):
+    """2^20 + 777 queries: each wave of the inference kernel runs several iterations of its
+    software pipeline (next query fetched, next features made between the layers and the
+    stores).  Decoding the raw outputs gives the lobes everywhere, and sampled rows (first,
+    last, across the iteration boundaries) match the oracle's forward pass."""
+    params = go.init_params(13)
+    ctx.guide_set_params(flat(params))
+    n = (1 << 20) + 777
+    x = np.random.default_rng(14).random((n, 9)).astype(np.float32)
+    raw = ctx.guide_forward(x)
+    lob = ctx.guide_lobes(x).astype(np.float64)
+    mu, kappa, w = go.decode(raw)
+    L = lob[:, :40].reshape(-1, 8, 5)
+    assert np.allclose(L[:, :, 0:3], mu, rtol=0, atol=2e-6)
+    assert np.allclose(L[:, :, 3], kappa, rtol=2e-6, atol=0)
+    assert np.allclose(L[:, :, 4], w, rtol=4e-6, atol=1e-7)
+    rng = np.random.default_rng(15)
+    rows = np.unique(np.concatenate([np.arange(64), np.arange(n - 64, n), (1 << 18) + np.arange(-32, 32),
+                                     rng.integers(0, n, 4000)]))
+    _, ref = go.forward(unflat(flat(params)), go.encode(x[rows].astype(np.float64)), bf16=True)
+    scale = np.abs(ref).max()
+    err = np.abs(raw[rows] - ref).max()
+    print(f"[guide] {n} queries: max_abs_err={err:.3e} on {rows.size} rows, scale={scale:.3e}", flush=True)
+    assert err <= 0.02 * max(1.0, scale)
+
+
 @pytest.mark.parametrize("learned", [False, True])
 def test_gradients_match_oracle(ctx, learned):
     params = go.init_params(5)
