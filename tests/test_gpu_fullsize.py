@@ -1,14 +1,15 @@
 """Parity at the full workload sizes -- the 1M-triangle procedural scene at 1024^2 and the
 20M-triangle terrain world with its 8 views -- through properties that do not depend on size:
 closest hits equal brute force on sampled rays, progressive chunks add up to one render bit for
-bit, the GPU image agrees with the oracle per pixel on sampled rows, and a job rendered inside
-the 8-view batch equals the same job rendered alone."""
+bit, the GPU image agrees with the oracle per pixel on sampled rows, a job rendered inside the
+8-view batch equals the same job rendered alone, and guided batches of the 8 views (one network,
+trained online) agree with plain ones per pixel and in each view's image mean."""
 import numpy as np
 import pytest
 
 import phg
 from oracle import oracle as orc
-from helpers import pixel_z, random_rays, rays_towards
+from helpers import image_mean_z, pixel_z, random_rays, rays_towards
 
 pytestmark = pytest.mark.gpu
 
@@ -90,8 +91,15 @@ def terrain_jobs(spp):
     return jobs
 
 
-def test_terrain_20m_batch_job_equals_job_alone_and_brute_force_hits(ctx):
+@pytest.fixture(scope="module")
+def terrain_20m():
     sc = phg.Scene.generate(phg.GEN_TERRAIN)
+    assert sc.num_tris >= 19_000_000
+    return sc
+
+
+def test_terrain_20m_batch_job_equals_job_alone_and_brute_force_hits(ctx, terrain_20m):
+    sc = terrain_20m
     print(f"[fullsize] case=terrain tris={sc.num_tris}", flush=True)
     assert sc.num_tris >= 19_000_000
     rays = np.concatenate([rays_towards(sc.verts, 96, 51), random_rays(sc.verts, 32, 52)])
@@ -102,7 +110,6 @@ def test_terrain_20m_batch_job_equals_job_alone_and_brute_force_hits(ctx):
     assert np.all(same | ((flags & 1) != 0))
     both = same & (tri >= 0)
     assert np.array_equal(hits["t"][both].view(np.uint32), t[both].view(np.uint32))
-    del sc
     jobs = terrain_jobs(2)
     ctx.render_batch(jobs)
     in_batch = [ctx.batch_film(k) for k in (0, 7)]
@@ -112,3 +119,39 @@ def test_terrain_20m_batch_job_equals_job_alone_and_brute_force_hits(ctx):
         print(f"[fullsize] case=terrain_view{k} {jobs[k]['width']}x{jobs[k]['height']} "
               f"mean={alone[..., 0].sum() / alone[..., 3].sum():.6f}", flush=True)
         assert np.array_equal(in_batch[idx], alone)
+
+
+def test_terrain_20m_guided_batch_matches_plain(ctx, terrain_20m):
+    """The guided half of the at-scale workload on the full 20M-triangle terrain: one network
+    shared by the 8 views, trained online by a guided batch of them, then guided (the network's
+    own selection probability, lobes at the first vertex) and plain batches of the 8 views at an
+    eighth of their resolution, with independent seeds, agree per pixel (99% intervals) and in
+    every view's image mean."""
+
+    def small(spp, seed0):
+        jobs = terrain_jobs(spp)
+        for k, j in enumerate(jobs):
+            j["width"] //= 8
+            j["height"] //= 8
+            j["seed"] = seed0 + k
+        return jobs
+
+    ctx.upload_many([terrain_20m])
+    print(f"[fullsize] case=terrain_guided tris={terrain_20m.num_tris}", flush=True)
+    ctx.render_batch(small(16, 300), guided=True, train=True, guide_steps=16, guide_depth=1)
+    steps = ctx.stats()["guide_steps"]
+    jobs_g, jobs_p = small(256, 400), small(256, 500)
+    ctx.render_batch(jobs_g, guided=True, guide_depth=1, guide_prob=-1.0)
+    queries = ctx.stats()["guide_queries"]
+    guided = [ctx.batch_film(k) for k in range(len(jobs_g))]
+    ctx.render_batch(jobs_p)
+    plain = [ctx.batch_film(k) for k in range(len(jobs_p))]
+    assert steps > 0 and queries > 0
+    for k, (g, p) in enumerate(zip(guided, plain)):
+        z = pixel_z(g, p)
+        frac = np.mean(np.abs(z) > Z99)
+        zm = image_mean_z(g, p)
+        print(f"[fullsize] terrain view {k} {jobs_g[k]['width']}x{jobs_g[k]['height']}: outside_99CI={frac:.4f} "
+              f"image-mean z={np.round(zm, 2)}", flush=True)
+        assert frac < 0.02
+        assert np.all(np.abs(zm) < 4.0)
