@@ -206,6 +206,25 @@ def mlp_utilisation(st):
     return out
 
 
+def adam_step_utilisation(ctx, np, n=1 << 18):
+    """Matrix-core utilisation of one training step (forward, backward, reductions, Adam) on n
+    random records, timed as the difference of a 33-step and a 1-step phg_guide_train call."""
+    rng = np.random.default_rng(0)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rec = np.concatenate([rng.random((n, 9)), d, rng.exponential(1.0, (n, 1)), rng.exponential(0.5, (n, 1)),
+                          np.full((n, 1), 0.5), rng.exponential(0.5, (n, 1))], axis=1).astype(np.float32)
+    t0 = time.perf_counter()
+    ctx.guide_train(rec, lr=1e-3, steps=1)
+    t1 = time.perf_counter()
+    ctx.guide_train(rec, lr=1e-3, steps=33)
+    t2 = time.perf_counter()
+    ms = max(((t2 - t1) - (t1 - t0)) / 32 * 1e3, 1e-6)
+    t = n * MLP_TRAIN_FLOPS / (ms / 1e3) / 1e12
+    return {"records": n, "ms": round(ms, 4), "tflops": round(t, 2), "frac": round(t / MFMA_BF16_PEAK_TFLOPS, 4),
+            "timed": "one Adam step alone: host clock, 33 steps minus 1 step over the same uploaded records"}
+
+
 def scale_leg(args):
     """20M-triangle terrain world, 8 views of uneven resolution, guided with one shared network."""
     import numpy as np
@@ -338,6 +357,10 @@ def convergence_leg(args):
         ctx.render(16, seed=99, guided=True, guide_depth=1, guide_prob=args.guide_prob)
         ist = ctx.stats()
         gtot["guide_infer_ms"], gtot["guide_queries"] = ist["guide_infer_ms"], ist["guide_queries"]
+        # one Adam step on 2^18 records alone (after the measurements above; random records of
+        # the same layout): host clock over 33 steps minus 1 step, each call uploading the same
+        # records and synchronising at its end
+        train_step = adam_step_utilisation(ctx, np)
     tp, tg = cv.time_to(plain, 0.01), cv.time_to(guided, 0.01)
     # per-sample error at the last checkpoint both curves reach: relMSE x spp (the variance of
     # one sample per pixel, relative), and seconds per sample
@@ -359,7 +382,7 @@ def convergence_leg(args):
             "guided": {"guide_prob": guide_prob_label(args.guide_prob), "guide_depth": 1,
                        "train_spp": args.conv_train_spp, "steps_per_pass": 32,
                        "training_pass_paths": "<= 16M (16 spp at 1024^2)", "records": "first path vertex", "lr": 1e-3,
-                       "batch": 1 << 18, "mlp": mlp_utilisation(gtot)},
+                       "batch": 1 << 18, "mlp": dict(mlp_utilisation(gtot), train_step=train_step)},
             "per_sample": per_sample,
             "mode": "both legs replay captured bounce chunks (8 bounces per graph) until the path queue drains",
             "plain_curve": [[s, round(t, 4), float(f"{e:.4e}")] for s, t, e in plain],
