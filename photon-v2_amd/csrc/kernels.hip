@@ -428,24 +428,56 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
     const uint32_t rsz = ((n + 7u) / 8u + 63u) & ~63u;
     uint32_t* fetch = pb.fetch + fetch_slot(kFetchShade, bounce);
     uint32_t reg = blockIdx.x & 7u, left = gridDim.x >= 8u ? 1u + kFetchSteal : 8u, cnext = 0u, cend = 0u;
-    while (left != 0u) {
-        if (cnext == cend) {
-            uint32_t got = 0u;
-            if (lane == 0u) got = atomicAdd(&fetch[reg * kFetchStride], kFetchChunk);
-            got = __builtin_amdgcn_readfirstlane(got);
-            const uint32_t rbeg = reg * rsz;
-            const uint32_t rend = rbeg + rsz < n ? rbeg + rsz : n;
-            if (rbeg >= rend || got >= rend - rbeg) {
-                reg = (reg + 1u) & 7u;
-                --left;
-                continue;
+    // Queue entries whose ray left the scene need no shading, and after the first bounce they
+    // are most of the queue (rays from the ground that go to the sky).  Each wave therefore
+    // gathers the queue indices of the entries that hit a surface into its own LDS list and
+    // shades them 64 at a time, so its lanes are not left idle beside the misses.  hl holds
+    // nl (wave-uniform) indices; a 64-entry group adds at most 64 to fewer than 64.
+    __shared__ uint32_t s_hitlist[kShadeBlock / 64u][128];
+    uint32_t* hl = s_hitlist[threadIdx.x >> 6];
+    const uint64_t lanes_below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+    uint32_t nl = 0u;
+    for (;;) {
+        while (nl < 64u && left != 0u) {
+            if (cnext == cend) {
+                uint32_t got = 0u;
+                if (lane == 0u) got = atomicAdd(&fetch[reg * kFetchStride], kFetchChunk);
+                got = __builtin_amdgcn_readfirstlane(got);
+                const uint32_t rbeg = reg * rsz;
+                const uint32_t rend = rbeg + rsz < n ? rbeg + rsz : n;
+                if (rbeg >= rend || got >= rend - rbeg) {
+                    reg = (reg + 1u) & 7u;
+                    --left;
+                    continue;
+                }
+                cnext = rbeg + got;
+                cend = rend - cnext < kFetchChunk ? rend : cnext + kFetchChunk;
             }
-            cnext = rbeg + got;
-            cend = rend - cnext < kFetchChunk ? rend : cnext + kFetchChunk;
+            const uint32_t j = cnext + lane;
+            cnext = cend - cnext < 64u ? cend : cnext + 64u;
+            const bool hit = j < n && __float_as_int(pb.hits[j].x) >= 0;
+            const uint64_t hm = __ballot(hit);
+            if (hit) hl[nl + (uint32_t)__popcll(hm & lanes_below)] = j;
+            nl += (uint32_t)__popcll(hm);
         }
-        const uint32_t i = cnext + lane;
-        cnext = cend - cnext < 64u ? cend : cnext + 64u;
-        if (i < n) {
+        if (nl == 0u) break;
+        // the list's stores and loads cross lanes of this wave only: LDS serves one wave's
+        // accesses in order, and the wave barrier keeps the compiler from reordering them
+        __builtin_amdgcn_wave_barrier();
+        const bool have = lane < nl;
+        const uint32_t i = have ? hl[lane] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (nl > 64u) {
+            const uint32_t rest = nl - 64u;
+            const uint32_t carry = lane < rest ? hl[64u + lane] : 0u;
+            __builtin_amdgcn_wave_barrier();
+            if (lane < rest) hl[lane] = carry;
+            nl = rest;
+        } else {
+            nl = 0u;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (have) {
             const uint32_t pid = pb.ext_pid[qin][i];
             const float4 h = pb.hits[i];
             const int32_t tri = __float_as_int(h.x);
