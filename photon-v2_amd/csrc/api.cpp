@@ -790,11 +790,15 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     PHG_HIP(c, hipSetDevice(c->device));
     PHG_HIP(c, c->rng.ensure(cap));
     PHG_HIP(c, c->T.ensure(cap)); PHG_HIP(c, c->Wem.ensure(cap)); PHG_HIP(c, c->prevP.ensure(cap)); PHG_HIP(c, c->L.ensure(cap));
-    PHG_HIP(c, c->hits.ensure(cap)); PHG_HIP(c, c->shadow_c.ensure(cap));
-    PHG_HIP(c, c->ext0.ensure(cap)); PHG_HIP(c, c->ext1.ensure(cap)); PHG_HIP(c, c->shadow.ensure(cap));
-    PHG_HIP(c, c->ext_pid0.ensure(cap)); PHG_HIP(c, c->ext_pid1.ensure(cap)); PHG_HIP(c, c->shadow_pid.ensure(cap));
+    // the ray queues: 8 regions of queue_region_cap(paths) slots (a pass of fewer paths uses
+    // smaller regions of the same buffers)
+    const size_t qcap = queue_capacity(cap);
+    PHG_HIP(c, c->hits.ensure(qcap)); PHG_HIP(c, c->shadow_c.ensure(qcap));
+    PHG_HIP(c, c->ext0.ensure(qcap)); PHG_HIP(c, c->ext1.ensure(qcap)); PHG_HIP(c, c->shadow.ensure(qcap));
+    PHG_HIP(c, c->ext_pid0.ensure(qcap)); PHG_HIP(c, c->ext_pid1.ensure(qcap)); PHG_HIP(c, c->shadow_pid.ensure(qcap));
     PHG_HIP(c, c->path_seg.ensure(cap));
-    PHG_HIP(c, c->counts.ensure(2 * kCountSlots + 1));   // + the bounce base of captured chunks
+    // + the bounce base of captured chunks and the pass's queue region size
+    PHG_HIP(c, c->counts.ensure(kCountSlots + 2));
     PHG_HIP(c, c->fetch.ensure(kFetchWords));
     PHG_HIP(c, c->tcount.ensure(8));
     PHG_HIP(c, c->errflags.ensure(1));
@@ -810,7 +814,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     PHG_HIP(c, hipMemcpyAsync(c->jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, c->stream));
     PHG_HIP(c, hipMemsetAsync(c->tcount.p, 0, 8 * sizeof(unsigned long long), c->stream));
     PHG_HIP(c, hipMemsetAsync(c->errflags.p, 0, sizeof(uint32_t), c->stream));
-    PHG_HIP(c, launch_clear_counts(c->counts.p, 2 * kCountSlots, c->stream));
+    PHG_HIP(c, launch_clear_counts(c->counts.p, kCountSlots, c->stream));
     PHG_HIP(c, launch_clear_counts(c->fetch.p, kFetchWords, c->stream));
     SceneDev sc = c->sc;
     sc.jobs = c->jobs.p;
@@ -820,7 +824,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
     pb.hits = c->hits.p; pb.shadow = c->shadow.p; pb.shadow_pid = c->shadow_pid.p; pb.shadow_c = c->shadow_c.p;
     pb.counts = c->counts.p; pb.traversal_counters = c->tcount.p; pb.error_flags = c->errflags.p; pb.film = film;
     pb.path_seg = c->path_seg.p;
-    pb.bounce_base = c->counts.p + 2 * kCountSlots;
+    pb.bounce_base = c->counts.p + kCountSlots;
     pb.fetch = c->fetch.p;
     {
         const phg_status st = ensure_spill(c);
@@ -838,7 +842,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             phg_status st = guide_load(c, p0.data());
             if (st != PHG_OK) return st;
         }
-        PHG_HIP(c, c->g_lobes.ensure((size_t)cap * kGuideOutputs));
+        PHG_HIP(c, c->g_lobes.ensure(queue_capacity(cap) * kGuideOutputs));
     }
     uint32_t max_b = 0;
     for (const PassPlan& pl : passes) max_b = std::max(max_b, pl.max_bounces);
@@ -888,7 +892,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
         for (uint32_t k = 0; k < kChunkBounces && ce == hipSuccess; ++k) {
             ce = launch_extend(sc, pb, k, grid_trace, c->trace_block, count_trav, c->stream);
             if (ce == hipSuccess && (infer_mask >> k & 1u))
-                ce = guide_infer_hits(guide_net(c), pb.hits, pb.ext[k & 1u], sc.tris, c->g_scene, c->counts.p,
+                ce = guide_infer_hits(guide_net(c), pb.hits, pb.ext[k & 1u], sc.tris, c->g_scene, c->fetch.p,
                                       pb.bounce_base, k, c->g_lobes.p, c->tcount.p + 6, c->cu_count * 2, c->stream);
             if (ce == hipSuccess) ce = launch_shade(sc, ppd, pb, k, grid_shade, (guided_mask >> k & 1u) != 0, c->stream);
             if (ce == hipSuccess) ce = launch_shadow(sc, ppd, pb, k, grid_trace, c->trace_block, count_trav, c->stream);
@@ -1017,7 +1021,7 @@ static phg_status render_passes(Ctx* c, const std::vector<JobDev>& jobs, const s
             PHG_HIP(c, launch_extend(sc, pb, b, grid_trace, c->trace_block, count_trav, c->stream));
             PHG_HIP(c, hipEventRecord(c->ev[1], c->stream));
             if (use_guide) {
-                PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], sc.tris, c->g_scene, c->counts.p,
+                PHG_HIP(c, guide_infer_hits(guide_net(c), pb.hits, pb.ext[b & 1], sc.tris, c->g_scene, c->fetch.p,
                                             pb.bounce_base, b, c->g_lobes.p, c->tcount.p + 6, c->cu_count * 2,
                                             c->stream));
             }

@@ -39,10 +39,11 @@ constexpr uint32_t kGuideNetLayers = 5;     // dense layers of the network (scra
 hipError_t guide_prepare(const GuideNetDev& net, hipStream_t st);
 hipError_t guide_infer_features(const GuideNetDev& net, const float* features, uint32_t n, float* out, int raw,
                                 uint32_t grid, hipStream_t st);
-// Lobes of the hits queued at bounce bounce_base[0] + bk (their count is that bounce's
-// extension-queue counter in `counts`); adds the number of queries to *queries.
+// Lobes of the hits queued at bounce bounce_base[0] + bk (the entries of that bounce's
+// extension-queue regions, whose fill counters are the kPushExt counters in `regions`; region
+// size bounce_base[1]), each written at its queue slot; adds the number of queries to *queries.
 hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const phg_ray* rays, const TriRec* tris,
-                            const GuideScene& gs, const uint32_t* counts, const uint32_t* bounce_base, uint32_t bk,
+                            const GuideScene& gs, const uint32_t* regions, const uint32_t* bounce_base, uint32_t bk,
                             float* out, unsigned long long* queries, uint32_t grid, hipStream_t st);
 // Scratch of a training step (device memory): layer inputs X (5 x cap x 64 bf16), two dZ
 // buffers (cap x 64 bf16 each), per-block dW / db partials (5 x (max_parts + 4) x 4160 floats)

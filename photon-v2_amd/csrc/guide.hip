@@ -119,7 +119,7 @@ struct QueryRaw {
     float c;
 };
 
-__device__ __forceinline__ QueryRaw query_fetch(const InferSource& src, uint32_t q, uint32_t n) {
+__device__ __forceinline__ QueryRaw query_fetch(const InferSource& src, uint32_t q, uint32_t n, const QueueMap& qm) {
     QueryRaw r;
     r.a = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
     r.b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -134,8 +134,9 @@ __device__ __forceinline__ QueryRaw query_fetch(const InferSource& src, uint32_t
         r.b = make_float4(f[4], f[5], f[6], f[7]);
         r.c = f[8];
     } else {
-        r.a = src.hits[q];
-        r.b = reinterpret_cast<const float4*>(src.rays + q)[1];
+        const uint32_t qs = qm.slot(q);
+        r.a = src.hits[qs];
+        r.b = reinterpret_cast<const float4*>(src.rays + qs)[1];
     }
     return r;
 }
@@ -189,14 +190,17 @@ __device__ __forceinline__ bf16x8 relu_pack(f32x4 lo, f32x4 hi) {
 }
 
 __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev net, InferSource src,
-                                                                  const uint32_t* counts, const uint32_t* bounce_base,
+                                                                  const uint32_t* regions, const uint32_t* bounce_base,
                                                                   uint32_t bk, uint32_t count, float* __restrict__ out,
                                                                   int raw, unsigned long long* queries) {
     __shared__ __attribute__((aligned(16))) __bf16 s_w[kGuideLayers * 64 * 64];
     __shared__ __attribute__((aligned(16))) float s_b[kGuideLayers * 64];
     __shared__ float s_x[kInferWaves][9 * 64];   // features of the wave's 64 queries, [coordinate][query]
-    // queued hits of bounce bounce_base[0] + bk (the extension-queue count of that bounce)
-    const uint32_t n = counts ? counts[(bounce_base[0] + bk) % kCountSlots] : count;
+    // queued hits of bounce bounce_base[0] + bk: the entries of the extension queue's regions
+    // (query d is the d-th of them, regions in order), or `count` features in a plain array
+    QueueMap qm;
+    if (regions) qm.load(regions + fetch_slot(kPushExt, bounce_base[0] + bk), bounce_base[1]);
+    const uint32_t n = regions ? qm.n : count;
     if (queries && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(queries, (unsigned long long)n);
     if (n == 0) return;   // whole block (a captured chunk runs past the last bounce of its paths)
     for (int i = threadIdx.x; i < kGuideLayers * 64 * 64 / 8; i += 64 * kInferWaves)
@@ -213,11 +217,11 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
     // a wait for a load also waits for every store issued before it: the next iteration's
     // features (the triangle loads) are therefore made after this iteration's layers and before
     // its stores, and the raw inputs of the iteration after that are fetched one iteration ahead.
-    QueryRaw next = query_fetch(src, first + (uint32_t)lane, n);
+    QueryRaw next = query_fetch(src, first + (uint32_t)lane, n, qm);
     if (first < n) {
         float x[9];
         query_features_of(src, next, x);
-        next = query_fetch(src, first + step + (uint32_t)lane, n);
+        next = query_fetch(src, first + step + (uint32_t)lane, n, qm);
 #pragma unroll
         for (int k = 0; k < 9; ++k) xs[k * 64 + lane] = x[k];
         wave_sync();
@@ -279,7 +283,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
                 // the next iteration's features, staged in LDS once every lane has encoded
                 float x[9];
                 query_features_of(src, next, x);
-                next = query_fetch(src, base + 2u * step + (uint32_t)lane, n);
+                next = query_fetch(src, base + 2u * step + (uint32_t)lane, n, qm);
                 wave_sync();
 #pragma unroll
                 for (int k = 0; k < 9; ++k) xs[k * 64 + lane] = x[k];
@@ -289,21 +293,22 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideNetDev ne
             for (int t = 0; t < kInferGroup; ++t) {
                 if (base + 16u * (T0 + t) >= n) break;  // whole wave
                 const uint32_t q = base + 16u * (T0 + t) + (uint32_t)r;
+                const size_t qs = qm.slot(q);   // the query's queue slot (its output row)
                 // slots s = 0..9 of this lane: logical outputs 10 g + s (lobes 2g, 2g + 1); slot 10
                 // of lane group 0: the selection logit
                 float o[11];
 #pragma unroll
                 for (int s2 = 0; s2 < 11; ++s2) o[s2] = acc[t][s2 >> 2][s2 & 3];
                 if (raw) {
-                    float* dst = out + (size_t)q * kGuideOut + 10 * g;
+                    float* dst = out + qs * kGuideOut + 10 * g;
                     if (q < n) {
                         for (int s2 = 0; s2 < 10; ++s2) dst[s2] = o[s2];
-                        if (g == 0) out[(size_t)q * kGuideOut + kGuideSel] = o[10];
+                        if (g == 0) out[qs * kGuideOut + kGuideSel] = o[10];
                     }
                     continue;
                 }
-                float* dst = out + (size_t)q * kGuideDec + 10 * g;
-                if (q < n && g == 0) out[(size_t)q * kGuideDec + kGuideSel] = guide_sel(o[10]);
+                float* dst = out + qs * kGuideDec + 10 * g;
+                if (q < n && g == 0) out[qs * kGuideDec + kGuideSel] = guide_sel(o[10]);
                 float mx = fmaxf(o[4], o[9]);
                 mx = fmaxf(mx, __shfl_xor(mx, 16));
                 mx = fmaxf(mx, __shfl_xor(mx, 32));
@@ -820,11 +825,11 @@ hipError_t guide_infer_features(const GuideNetDev& net, const float* features, u
 }
 
 hipError_t guide_infer_hits(const GuideNetDev& net, const float4* hits, const phg_ray* rays, const TriRec* tris,
-                            const GuideScene& gs, const uint32_t* counts, const uint32_t* bounce_base, uint32_t bk,
+                            const GuideScene& gs, const uint32_t* regions, const uint32_t* bounce_base, uint32_t bk,
                             float* out, unsigned long long* queries, uint32_t grid, hipStream_t st) {
     InferSource src{};
     src.hits = hits; src.rays = rays; src.tris = tris; src.gs = gs;
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, net, src, counts, bounce_base, bk, 0u,
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64 * kInferWaves), 0, st, net, src, regions, bounce_base, bk, 0u,
                        out, 0, queries);
     return hipGetLastError();
 }

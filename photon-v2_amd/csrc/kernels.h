@@ -103,7 +103,8 @@ struct PathBuffers {
     phg_ray* shadow;         // shadow-ray queue
     uint32_t* shadow_pid;
     float4* shadow_c;        // unoccluded contribution
-    uint32_t* counts;        // [0, kCountSlots) extension counts, [kCountSlots, 2*kCountSlots) shadow counts
+    uint32_t* counts;        // [0, kCountSlots): entries of each bounce's extension queue (k_generate,
+                             // then the shadow kernel of the bounce before sums its regions)
     unsigned long long* traversal_counters;  // node visits / tri tests: ext, ext, shadow, shadow
     uint32_t* error_flags;
     double* film;            // 8 doubles per pixel
@@ -113,20 +114,59 @@ struct PathBuffers {
     int32_t* spill;          // traversal-stack spill, spill_cap entries per traversal thread
     uint32_t spill_cap;
     uint32_t* path_seg;      // per path: its pass segment
-    uint32_t* bounce_base;   // first bounce of the running chunk of bounces (the stage kernels
-                             // take their bounce as bounce_base[0] + their chunk offset)
+    uint32_t* bounce_base;   // [0]: first bounce of the running chunk of bounces (the stage
+                             // kernels take their bounce as bounce_base[0] + their chunk
+                             // offset); [1]: the queue region size of the running pass
     uint32_t* fetch;         // work counters of the stage kernels (fetch_slot), kFetchWords
 };
 
-// Region counters of the dynamic work distribution of the stage kernels: 8 per queue pass
-// (extension-ray traversal, shadow-ray traversal, shading) and bounce, in a ring of
-// kCountSlots bounces, each counter on a 128-byte line of its own (kFetchStride words apart).
-enum : uint32_t { kFetchExtend = 0, kFetchShadow = 1, kFetchShade = 2, kFetchKinds = 3 };
+// Region counters, 8 per queue pass and bounce, in a ring of kCountSlots bounces, each on a
+// 128-byte line of its own (kFetchStride words apart): the work counters of the dynamic work
+// distribution of the stage kernels (extension-ray traversal, shadow-ray traversal, shading)
+// and the fill counts of the regions of the extension- and shadow-ray queues (kPushExt,
+// kPushShadow).
+//
+// A queue of a pass is 8 regions of queue_region_cap(paths) slots; region r is
+// [r cap, r cap + count_r).  The shading kernel pushes the rays of a vertex taken from region r
+// of its input queue into region r of the output queues, so no region receives more entries
+// than its input region held (at most one ray per queue and vertex), and the camera rays
+// fill the regions in path order.  Device-scope atomics on one address serialize (about 10 ns
+// each on MI355X), so one counter per queue limited the shading kernel to one 64-vertex push
+// per 10 ns; 8 counters take 8 at once, and each XCD's waves drain and fill mostly their own.
+enum : uint32_t { kFetchExtend = 0, kFetchShadow = 1, kFetchShade = 2, kPushExt = 3, kPushShadow = 4, kFetchKinds = 5 };
 constexpr uint32_t kFetchStride = 32;
 constexpr uint32_t kFetchWords = kFetchKinds * kCountSlots * 8u * kFetchStride;
 __host__ __device__ inline uint32_t fetch_slot(uint32_t kind, uint32_t bounce) {
     return (kind * kCountSlots + bounce % kCountSlots) * 8u * kFetchStride;
 }
+__host__ __device__ inline uint32_t queue_region_cap(uint32_t paths) { return ((paths + 7u) / 8u + 63u) & ~63u; }
+// slots a queue of a pass of `paths` paths spans (its 8 regions)
+__host__ __device__ inline uint64_t queue_capacity(uint32_t paths) { return 8ull * queue_region_cap(paths); }
+
+// Dense position d in [0, n) of a queue's entries -> its slot (regions in order).
+struct QueueMap {
+    uint32_t cap = 0, n = 0;
+    uint32_t pre[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // entries before region r
+    bool identity = true;                        // a plain array (no regions)
+    // the fill counts of the 8 regions (kPushExt / kPushShadow counters of one bounce)
+    __host__ __device__ void load(const uint32_t* region_counts, uint32_t region_cap) {
+        cap = region_cap;
+        identity = false;
+        uint32_t run = 0;
+        for (int r = 0; r < 8; ++r) {
+            pre[r] = run;
+            run += region_counts[r * kFetchStride];
+        }
+        n = run;
+    }
+    __host__ __device__ uint32_t slot(uint32_t d) const {
+        if (identity) return d;
+        uint32_t r = 0, p = 0;
+        for (int k = 1; k < 8; ++k)
+            if (d >= pre[k]) { r = (uint32_t)k; p = pre[k]; }
+        return r * cap + (d - p);
+    }
+};
 
 // Size of the device scratch array of the allocation-free ray query; its address on `device`
 // (looked up on the first call per device, which makes that device current).

@@ -30,15 +30,27 @@ __device__ __forceinline__ uint32_t xcd_block_id() {
     return (b & 7u) * (n >> 3) + (b >> 3);
 }
 
-__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
-    const uint64_t mask = __ballot(pred);
+// Queue pushes of a wave: each lane with pred pushes one entry into region reg (its own) of a
+// queue whose region fill counters are counters[r kFetchStride], and gets its slot
+// r cap + position.  The lanes of a region take consecutive positions (ballot + popcount) and
+// one atomic per region the wave pushes to (nearly always one) reserves them.
+__device__ __forceinline__ uint32_t wave_push_region(uint32_t* counters, uint32_t reg, uint32_t cap, bool pred) {
     const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
-    uint32_t base = 0;
-    if (pred && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    const uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64u - lane)));
-    return base + (uint32_t)__popcll(below);
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    uint64_t left = __ballot(pred);
+    uint32_t slot = 0u;
+    while (left != 0ull) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)left) - 1u;
+        const uint32_t r = __builtin_amdgcn_readlane(reg, (int)leader);
+        const bool mine = pred && reg == r;
+        const uint64_t grp = __ballot(mine);
+        uint32_t base = 0u;
+        if (lane == leader) base = atomicAdd(&counters[r * kFetchStride], (uint32_t)__popcll(grp));
+        base = __builtin_amdgcn_readlane(base, (int)leader);
+        if (mine) slot = r * cap + base + (uint32_t)__popcll(grp & below);
+        left &= ~grp;
+    }
+    return slot;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -143,11 +155,18 @@ __device__ __forceinline__ void queue_ray(phg_ray* q, uint32_t slot, V3 o, uint3
 __global__ __launch_bounds__(kShadeBlock) void k_generate(SceneDev sc, const PassParams* __restrict__ ppd, PathBuffers pb) {
     const PassParams pp = *ppd;
     const uint32_t stride = gridDim.x * kShadeBlock;
+    const uint32_t cap = queue_region_cap(pp.paths);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         pb.counts[0] = pp.paths;
         pb.bounce_base[0] = 0u;
+        pb.bounce_base[1] = cap;
     }
-    if (blockIdx.x == 0 && threadIdx.x < 8u) pb.fetch[fetch_slot(kFetchExtend, 0u) + threadIdx.x * kFetchStride] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 8u) {
+        // path i's camera ray is slot i: the regions fill in order
+        const uint32_t r = threadIdx.x, rbeg = r * cap;
+        pb.fetch[fetch_slot(kFetchExtend, 0u) + r * kFetchStride] = 0u;
+        pb.fetch[fetch_slot(kPushExt, 0u) + r * kFetchStride] = pp.paths <= rbeg ? 0u : (pp.paths - rbeg < cap ? pp.paths - rbeg : cap);
+    }
     for (uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x; i < pp.paths; i += stride) {
         const uint32_t si = find_seg(pp.segs, pp.num_segs, i);
         const PassSeg sg = pp.segs[si];
@@ -244,6 +263,7 @@ static_assert(kFetchChunk % 64 == 0, "chunks keep 64-ray groups aligned");
 
 template <int RING, uint32_t BLOCK, bool ANY, bool COUNT, typename Done>
 __device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* __restrict__ queue, uint32_t n,
+                                            const uint32_t* __restrict__ rcount, uint32_t cap,
                                             bool coherent, uint32_t* __restrict__ fetch, int32_t* stk,
                                             int32_t* spill, uint32_t spill_cap, uint32_t* nvis, uint32_t* ntri,
                                             uint32_t* ovf, Done done) {
@@ -276,8 +296,8 @@ __device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* _
         return;
     }
     const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
-    // regions of whole 64-ray groups; region r is [r rsz, min(n, (r + 1) rsz))
-    const uint32_t rsz = ((n + 7u) / 8u + 63u) & ~63u;
+    // the queue's regions: region r is [r cap, r cap + rcount[r kFetchStride]) (whole 64-ray
+    // groups but the last)
     // wave-uniform: the region being drained, the regions still to try, the wave's chunk
     // (a grid of fewer than 8 blocks leaves regions without home waves: its waves try them all)
     uint32_t reg = blockIdx.x & 7u, left = gridDim.x >= 8u ? 1u + kFetchSteal : 8u, cnext = 0u, cend = 0u;
@@ -293,8 +313,8 @@ __device__ __forceinline__ void trace_queue(const SceneDev& sc, const phg_ray* _
                     uint32_t got = 0u;
                     if (lane == first) got = atomicAdd(&fetch[reg * kFetchStride], kFetchChunk);
                     got = __builtin_amdgcn_readfirstlane(got);
-                    const uint32_t rbeg = reg * rsz;
-                    const uint32_t rend = rbeg + rsz < n ? rbeg + rsz : n;
+                    const uint32_t rbeg = reg * cap;
+                    const uint32_t rend = rbeg + rcount[reg * kFetchStride];
                     if (rbeg >= rend || got >= rend - rbeg) {
                         reg = (reg + 1u) & 7u;
                         --left;
@@ -330,28 +350,30 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
     const uint32_t bounce = pb.bounce_base[0] + bk;
     __shared__ int32_t stack_mem[RING * BLOCK];
     int32_t* spill = pb.spill + (size_t)(xcd_block_id() * BLOCK + threadIdx.x) * pb.spill_cap;
+    // entries queued for this bounce (k_generate's count, or the one the previous bounce's
+    // shadow kernel summed from the regions), and the queue's regions
     const uint32_t n = pb.counts[bounce % kCountSlots];
-    // the counters the shade stage of this bounce pushes to start from zero, and the region
-    // counters of this bounce's shadow rays and of the next bounce's extension rays
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.counts[(bounce + 1) % kCountSlots] = 0u;
-        pb.counts[kCountSlots + bounce % kCountSlots] = 0u;
-        if (bounce > 0) atomicAdd(&pb.traversal_counters[4], (unsigned long long)n);
-    }
+    const uint32_t cap = pb.bounce_base[1];
+    const uint32_t* rcount = pb.fetch + fetch_slot(kPushExt, bounce);
+    // the region counters the shade stage of this bounce pushes to (this bounce's shadow rays,
+    // the next bounce's extension rays) start from zero, as do the work counters
+    if (blockIdx.x == 0 && threadIdx.x == 0 && bounce > 0) atomicAdd(&pb.traversal_counters[4], (unsigned long long)n);
     if (blockIdx.x == 0 && threadIdx.x < 8u) {
         pb.fetch[fetch_slot(kFetchExtend, bounce + 1u) + threadIdx.x * kFetchStride] = 0u;
         pb.fetch[fetch_slot(kFetchShadow, bounce) + threadIdx.x * kFetchStride] = 0u;
         pb.fetch[fetch_slot(kFetchShade, bounce) + threadIdx.x * kFetchStride] = 0u;
+        pb.fetch[fetch_slot(kPushExt, bounce + 1u) + threadIdx.x * kFetchStride] = 0u;
+        pb.fetch[fetch_slot(kPushShadow, bounce) + threadIdx.x * kFetchStride] = 0u;
     }
     const uint32_t q = bounce & 1u;
     uint32_t nvis = 0, ntri = 0, ovf = 0;
     if (count_traversal)
-        trace_queue<RING, BLOCK, false, true>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(kFetchExtend, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, true>(sc, pb.ext[q], n, rcount, cap, bounce == 0u, pb.fetch + fetch_slot(kFetchExtend, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        [&](uint32_t i, const TraceResult& tr) {
                                            pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                        });
     else
-        trace_queue<RING, BLOCK, false, false>(sc, pb.ext[q], n, bounce == 0u, pb.fetch + fetch_slot(kFetchExtend, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, false, false>(sc, pb.ext[q], n, rcount, cap, bounce == 0u, pb.fetch + fetch_slot(kFetchExtend, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                         [&](uint32_t i, const TraceResult& tr) {
                                             pb.hits[i] = make_float4(__int_as_float(tr.tri), tr.a, tr.b, tr.c);
                                         });
@@ -369,8 +391,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
     const uint32_t rec_bounces = ppd->record ? ppd->rec_bounces : 0u, npaths = ppd->paths;
     __shared__ int32_t stack_mem[RING * BLOCK];
     int32_t* spill = pb.spill + (size_t)(xcd_block_id() * BLOCK + threadIdx.x) * pb.spill_cap;
-    const uint32_t n = pb.counts[kCountSlots + bounce % kCountSlots];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&pb.traversal_counters[5], (unsigned long long)n);
+    const uint32_t cap = pb.bounce_base[1];
+    const uint32_t* rcount = pb.fetch + fetch_slot(kPushShadow, bounce);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the shading of this bounce is complete: the totals of its shadow rays (statistics)
+        // and of the next bounce's extension rays (the next bounce's traversal, inference and
+        // the host's test for an empty queue)
+        uint32_t ns = 0u, ne = 0u;
+        const uint32_t* ecount = pb.fetch + fetch_slot(kPushExt, bounce + 1u);
+        for (uint32_t r = 0; r < 8u; ++r) {
+            ns += rcount[r * kFetchStride];
+            ne += ecount[r * kFetchStride];
+        }
+        pb.counts[(bounce + 1) % kCountSlots] = ne;
+        atomicAdd(&pb.traversal_counters[5], (unsigned long long)ns);
+    }
     uint32_t nvis = 0, ntri = 0, ovf = 0;
     auto unoccluded = [&](uint32_t i, const TraceResult& tr) {
         if (tr.tri < 0) {
@@ -389,10 +424,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void
         }
     };
     if (count_traversal)
-        trace_queue<RING, BLOCK, true, true>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(kFetchShadow, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, true>(sc, pb.shadow, 0u, rcount, cap, false, pb.fetch + fetch_slot(kFetchShadow, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                       unoccluded);
     else
-        trace_queue<RING, BLOCK, true, false>(sc, pb.shadow, n, false, pb.fetch + fetch_slot(kFetchShadow, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
+        trace_queue<RING, BLOCK, true, false>(sc, pb.shadow, 0u, rcount, cap, false, pb.fetch + fetch_slot(kFetchShadow, bounce), stack_mem + threadIdx.x, spill, pb.spill_cap, &nvis, &ntri, &ovf,
                                        unoccluded);
     if (count_traversal) {
         atomicAdd(&pb.traversal_counters[2], (unsigned long long)nvis);
@@ -419,22 +454,27 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                                                        uint32_t bk) {
     const uint32_t bounce = pb.bounce_base[0] + bk;
     const PassParams& pp = *ppd;   // fields read where used (uniform loads; no registers held)
-    const uint32_t n = pb.counts[bounce % kCountSlots];
+    const uint32_t cap = pb.bounce_base[1];
+    const uint32_t* rcount = pb.fetch + fetch_slot(kPushExt, bounce);
+    uint32_t* push_shadow = pb.fetch + fetch_slot(kPushShadow, bounce);
+    uint32_t* push_ext = pb.fetch + fetch_slot(kPushExt, bounce + 1u);
     const uint32_t qin = bounce & 1u, qout = qin ^ 1u;
     // 64-vertex groups handed out dynamically, as the traversal kernels' incoherent queues:
     // chunks of kFetchChunk vertices from the XCD's region of the queue, then from the next
     // kFetchSteal regions (wave-uniform state; one atomic per chunk)
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t rsz = ((n + 7u) / 8u + 63u) & ~63u;
     uint32_t* fetch = pb.fetch + fetch_slot(kFetchShade, bounce);
     uint32_t reg = blockIdx.x & 7u, left = gridDim.x >= 8u ? 1u + kFetchSteal : 8u, cnext = 0u, cend = 0u;
     // Queue entries whose ray left the scene need no shading, and after the first bounce they
     // are most of the queue (rays from the ground that go to the sky).  Each wave therefore
     // gathers the queue indices of the entries that hit a surface into its own LDS list and
     // shades them 64 at a time, so its lanes are not left idle beside the misses.  hl holds
-    // nl (wave-uniform) indices; a 64-entry group adds at most 64 to fewer than 64.
+    // nl (wave-uniform) indices, and hr the queue region of each (whose region of the output
+    // queues the vertex pushes to); a 64-entry group adds at most 64 to fewer than 64.
     __shared__ uint32_t s_hitlist[kShadeBlock / 64u][128];
+    __shared__ uint8_t s_hitreg[kShadeBlock / 64u][128];
     uint32_t* hl = s_hitlist[threadIdx.x >> 6];
+    uint8_t* hr = s_hitreg[threadIdx.x >> 6];
     const uint64_t lanes_below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
     uint32_t nl = 0u;
     for (;;) {
@@ -443,8 +483,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                 uint32_t got = 0u;
                 if (lane == 0u) got = atomicAdd(&fetch[reg * kFetchStride], kFetchChunk);
                 got = __builtin_amdgcn_readfirstlane(got);
-                const uint32_t rbeg = reg * rsz;
-                const uint32_t rend = rbeg + rsz < n ? rbeg + rsz : n;
+                const uint32_t rbeg = reg * cap;
+                const uint32_t rend = rbeg + rcount[reg * kFetchStride];
                 if (rbeg >= rend || got >= rend - rbeg) {
                     reg = (reg + 1u) & 7u;
                     --left;
@@ -454,10 +494,14 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                 cend = rend - cnext < kFetchChunk ? rend : cnext + kFetchChunk;
             }
             const uint32_t j = cnext + lane;
+            const bool hit = j < cend && __float_as_int(pb.hits[j].x) >= 0;
             cnext = cend - cnext < 64u ? cend : cnext + 64u;
-            const bool hit = j < n && __float_as_int(pb.hits[j].x) >= 0;
             const uint64_t hm = __ballot(hit);
-            if (hit) hl[nl + (uint32_t)__popcll(hm & lanes_below)] = j;
+            if (hit) {
+                const uint32_t pos = nl + (uint32_t)__popcll(hm & lanes_below);
+                hl[pos] = j;
+                hr[pos] = (uint8_t)reg;
+            }
             nl += (uint32_t)__popcll(hm);
         }
         if (nl == 0u) break;
@@ -466,12 +510,17 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
         __builtin_amdgcn_wave_barrier();
         const bool have = lane < nl;
         const uint32_t i = have ? hl[lane] : 0u;
+        const uint32_t ireg = have ? hr[lane] : 0u;
         __builtin_amdgcn_wave_barrier();
         if (nl > 64u) {
             const uint32_t rest = nl - 64u;
             const uint32_t carry = lane < rest ? hl[64u + lane] : 0u;
+            const uint8_t carry_r = lane < rest ? hr[64u + lane] : (uint8_t)0;
             __builtin_amdgcn_wave_barrier();
-            if (lane < rest) hl[lane] = carry;
+            if (lane < rest) {
+                hl[lane] = carry;
+                hr[lane] = carry_r;
+            }
             nl = rest;
         } else {
             nl = 0u;
@@ -628,8 +677,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                                     if (!rgb_zero(sh_c)) {
                                         // queued at once (its lanes ballot together), so the
                                         // shadow ray is not carried through the BSDF stage
-                                        const uint32_t slot_s =
-                                            wave_push(&pb.counts[kCountSlots + bounce % kCountSlots], true);
+                                        const uint32_t slot_s = wave_push_region(push_shadow, ireg, cap, true);
                                         queue_ray(pb.shadow, slot_s, so, job.root, sd, st);
                                         pb.shadow_pid[slot_s] = pid;
                                         pb.shadow_c[slot_s] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.0f);
@@ -709,7 +757,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(4))
                             pb.Wem[pid] = make_float4(wem.r, wem.g, wem.b, pdfc);
                             pb.prevP[pid] = make_float4(X.P.x, X.P.y, X.P.z, __uint_as_float(die ? kFlagDieAfterEmission : 0u));
                             pb.rng[pid] = s.state;
-                            const uint32_t slot_e = wave_push(&pb.counts[(bounce + 1) % kCountSlots], true);
+                            const uint32_t slot_e = wave_push_region(push_ext, ireg, cap, true);
                             queue_ray(pb.ext[qout], slot_e, X.P + off, job.root, ud, kFltMax);
                             pb.ext_pid[qout][slot_e] = pid;
                             if (GUIDED && pp.record && bounce < pp.rec_bounces && !guide_samples_material(m)) {
