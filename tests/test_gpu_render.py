@@ -78,6 +78,24 @@ def test_procedural_ggx_matches_oracle(ctx):
     check_parity(gpu, ref, "procedural", 0.5)
 
 
+@pytest.mark.parametrize("W,H,spp", [(5, 3, 1), (23, 7, 3), (9, 9, 40)])
+def test_small_passes_with_empty_queue_regions_match_oracle(ctx, W, H, spp):
+    """The ray queues are 8 regions of ceil(paths / 8) slots rounded up to 64: a pass of 15
+    paths fills only the first region, one of 483 leaves the last region part full, and a
+    pass of 3240 paths gives every region 448 slots.  Every path must still be traced and
+    shaded exactly once: the same sample counts as the oracle and, at these sizes, the
+    oracle's radiance in most pixels (paths differ only where libm and the device math
+    library round a transcendental differently)."""
+    sc = phg.Scene.generate(phg.GEN_PROCEDURAL, seed=4, target_tris=30_000, num_instances=12, num_lights=6, subdiv=3)
+    print(f"[render] case=small_pass W={W} H={H} spp={spp}", flush=True)
+    gpu = gpu_render(ctx, sc, W, H, spp, 8, 7)
+    ref, _ = orc.render(sc, W, H, spp, max_bounces=8, seed=7)
+    assert np.all(gpu[..., 3] == ref[..., 3])
+    ident = np.all(np.isclose(gpu[..., :3], ref[..., :3], rtol=1e-6, atol=1e-9), axis=-1).mean()
+    print(f"[render] case=small_pass identical_pixels={ident:.3f}", flush=True)
+    assert ident >= 0.5
+
+
 def test_unbounded_paths_match_oracle(ctx):
     # reference semantics: no bounce cap (MAX_RAY_BOUNCES 10000), Russian roulette ends paths
     sc = phg.Scene.generate(phg.GEN_CORNELL)
